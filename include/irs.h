@@ -1,0 +1,189 @@
+/*
+ * irs.h — C ABI of the MI355X line-by-line engine (libirs.so).
+ *
+ * Plain C99: pointers, sizes, and POD structs; no torch or HIP types.  Every
+ * entry point returns an int status (IRS_OK = 0, negative on error) and never
+ * lets a C++ exception cross the boundary; the message of the last failure on
+ * a context is available from irs_last_error().  Host arrays are owned by the
+ * caller; the context owns all device memory, its HIP stream and its caches.
+ * Calls on one context must be serialised by the caller; distinct contexts
+ * may be used from distinct threads.
+ *
+ * Layout conventions are the reference's: uniform FP64 wavenumber grids,
+ * layer arrays column-major with wavenumber fastest ([n_nu x n_layers]),
+ * levels surface-first, pressures in hPa at profile level and atm at
+ * line-shape level, radiance in mW m^-2 sr^-1 (cm^-1)^-1.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   irs_forward            forward_model            src/Sensor/sounder.jl:233-360
+ *   irs_xsec_layer         compute_voigt_cross_sections (+ _reject_weak_lines,
+ *                          compute_voigt_lm_cross_sections)
+ *                                                   src/CrossSections/voigt.jl:333-462,
+ *                                                   src/CrossSections/line_mixing.jl:1351-1369
+ *   irs_rte                schwarzschild_rte        src/Solver/schwarzschild.jl:255-334
+ *   irs_ils_kernel         ils_kernel               src/Sensor/ils.jl:80-114
+ *   irs_ils_channels       apply_ils + _resample_to_channels
+ *                                                   src/Sensor/ils.jl:131-153,
+ *                                                   src/Sensor/sounder.jl:372-377
+ *   irs_brightness_temperature  brightness_temperature  src/Solver/planck.jl:43-57
+ *   irs_linelist_upload    HITRANLinelist(lines)    src/HITRAN/linelist.jl:52-56
+ */
+#ifndef IRS_H
+#define IRS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRS_ABI_VERSION 1
+
+/* status codes */
+#define IRS_OK         0
+#define IRS_EINVAL    -1   /* a precondition of the reference operator failed */
+#define IRS_EHIP      -2   /* HIP runtime error */
+#define IRS_ENOMEM    -3   /* device allocation failed */
+#define IRS_ENOTFOUND -4   /* unknown handle */
+#define IRS_ENODEV    -5   /* no usable gfx950 device */
+
+/* enums mirroring the reference's keyword symbols */
+#define IRS_APOD_GAUSSIAN   0   /* :gaussian            ils.jl:90-92  */
+#define IRS_APOD_NB_WEAK    1   /* :norton_beer_weak    ils.jl:26-30  */
+#define IRS_APOD_NB_MEDIUM  2   /* :norton_beer_medium */
+#define IRS_APOD_NB_STRONG  3   /* :norton_beer_strong */
+
+#define IRS_SRC_TOON 0          /* source_function=:toon  schwarzschild.jl:286-298 */
+#define IRS_SRC_CIM  1          /* source_function=:cim   schwarzschild.jl:299-318 */
+
+#define IRS_TMETHOD_LOGP_AT_PCG   0   /* T_method=:logp_at_pcg   layers.jl:133-178 */
+#define IRS_TMETHOD_MASS_WEIGHTED 1   /* T_method=:mass_weighted */
+
+#define IRS_LM_NONE 0           /* line_mixing=nothing */
+#define IRS_LM_VPY  1           /* first-order (VP_Y) mixing on the CO2 list, Y per line */
+
+typedef struct irs_ctx irs_ctx;
+
+/* Line list, structure of arrays, sorted by unshifted wavenumber
+ * (HITRANLine fields, linelist.jl:20-31).  y_lm may be NULL. */
+typedef struct {
+    int64_t       n;
+    const int8_t *mol;
+    const int8_t *iso;
+    const double *nu;          /* cm^-1 */
+    const double *S;           /* cm^-1/(molec cm^-2) at 296 K */
+    const float  *gamma_air;   /* cm^-1/atm */
+    const float  *gamma_self;  /* cm^-1/atm */
+    const double *elower;      /* cm^-1 */
+    const float  *n_air;
+    const float  *delta_air;   /* cm^-1/atm */
+    const double *y_lm;        /* atm^-1, first-order line-mixing coefficient, or NULL */
+} irs_linelist;
+
+/* Sounder (sounder.jl:33-40) */
+typedef struct {
+    double  nu_min, nu_max, dnu;
+    double  opd_max, fwhm_gauss;
+    int64_t n_channels;
+} irs_sounder;
+
+/* One profile: levels surface first (profiles.jl:12-17) plus the scene
+ * scalars forward_model takes as keywords. */
+typedef struct {
+    int            n_levels;
+    const double  *p_hpa;       /* [n_levels] */
+    const double  *T;           /* [n_levels] */
+    int            n_species;
+    const int     *species;     /* HITRAN molecule id of each species */
+    const int32_t *linelist;    /* handle per species from irs_linelist_upload, -1 = none */
+    const double  *vmr;         /* [n_species][n_levels] */
+    double         T_sfc;       /* <= 0: use T[0] (sounder.jl:337) */
+    double         eps_sfc;
+    double         mu;          /* cos(viewing zenith angle) */
+} irs_profile;
+
+/* forward_model keywords (sounder.jl:233-252), same defaults via
+ * irs_default_options(). */
+typedef struct {
+    double internal_dnu;        /* 0.001 */
+    int    high_res_factor;     /* <= 0: derive from internal_dnu */
+    double cutoff;              /* 25 */
+    double lm_cutoff;           /* = cutoff */
+    double dptmn;               /* 1e-6 */
+    double x_far;               /* 122 */
+    int    with_ils;            /* 1 */
+    int    apodization;         /* IRS_APOD_GAUSSIAN */
+    int    line_mixing;         /* IRS_LM_NONE */
+    int    T_method;            /* IRS_TMETHOD_LOGP_AT_PCG */
+    int    source_function;     /* IRS_SRC_CIM */
+    int    reserved0;
+} irs_fm_options;
+
+/* Timing and work counters of the last irs_forward / irs_xsec_layer call. */
+typedef struct {
+    double  ms_total;           /* host wall time of the call */
+    double  ms_prep;            /* line-parameter + DPTMIN kernels */
+    double  ms_linesum;         /* line-sum kernel (HIP events on the context stream) */
+    double  ms_rte;
+    double  ms_ils;
+    int64_t n_hi;               /* internal grid points */
+    int64_t n_layers;
+    int64_t linesum_launches;
+    int64_t active_lines;       /* sum over (profile, layer, species) */
+    int64_t active_pairs;       /* nominal pairs |nu - nu0| <= cutoff over active lines */
+    int64_t dispersive_pairs;   /* nominal VP_Y pairs */
+    int64_t core_pairs_bound;   /* pairs in core ranges (evaluated with w(z)) */
+} irs_stats;
+
+int         irs_version(void);
+/* Fills buf with "name;gcnArch;CUs;clockMHz;memGB".  Returns IRS_ENODEV without a GPU. */
+int         irs_device_info(int device, char *buf, int buflen);
+int         irs_create(int device, irs_ctx **out);
+void        irs_destroy(irs_ctx *ctx);
+const char *irs_last_error(const irs_ctx *ctx);
+void        irs_default_options(irs_fm_options *opt);
+int         irs_last_stats(const irs_ctx *ctx, irs_stats *out);
+
+/* Copies a sorted line list to the device; returns a handle. */
+int irs_linelist_upload(irs_ctx *ctx, const irs_linelist *ll, int32_t *handle);
+int irs_linelist_free(irs_ctx *ctx, int32_t handle);
+
+/* Cross sections of one species in one layer on nu_i = grid node i of
+ * (nu0, dnu): sigma[n_nu] in cm^2/molec.  dptmn > 0 applies DPTMIN with the
+ * given coef (= vmr * dp * N_air); line_mixing = IRS_LM_VPY adds the VP_Y
+ * dispersive term of the lines with y_lm != 0 (and skips DPTMIN, as
+ * sounder.jl:295-299 does on the line-mixing CO2 path). */
+int irs_xsec_layer(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t n_nu,
+                   double T, double p_atm, double vmr_self, double coef, double dptmn,
+                   double cutoff, double lm_cutoff, double x_far, int line_mixing,
+                   double *sigma);
+
+/* Schwarzschild RTE on a host tau [n_nu x n_layers]; T_ave required for CIM. */
+int irs_rte(irs_ctx *ctx, double nu0, double dnu, int64_t n_nu, const double *tau, int n_layers,
+            const double *T_lev, double T_sfc, double mu, double eps_sfc, int source_function,
+            const double *T_ave, double *radiance);
+
+/* ILS taps: out[2*half+1], half = ceil(16/dnu) (ils.jl:80-114). */
+int irs_ils_half(double dnu, int64_t *half);
+int irs_ils_kernel(irs_ctx *ctx, double dnu, double opd_max, double fwhm, int apodization,
+                   int n_pts, double *taps);
+
+/* ILS convolution of a host spectrum on (nu0, dnu, n_nu) sampled at the sounder
+ * channels (exact-node or linear interpolation, Flat extrapolation). */
+int irs_ils_channels(irs_ctx *ctx, const double *spectrum, double nu0, double dnu, int64_t n_nu,
+                     const irs_sounder *snd, int apodization, double *r_ch);
+
+int irs_brightness_temperature(const double *nu, const double *radiance, int64_t n, double *bt);
+
+/* Geometry of the internal grid for (snd, opt). */
+int irs_forward_dims(const irs_sounder *snd, const irs_fm_options *opt, double *nu_lo,
+                     double *dnu_hi, int64_t *n_hi, int64_t *n_channels);
+
+/* forward_model over a batch of profiles: R, BT are [n_prof][n_channels]. */
+int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                const irs_fm_options *opt, double *R, double *BT);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,93 @@
+// Internal declarations shared by the HIP kernels and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define IRS_SQRT_LN2    0.83255461115769775635
+#define IRS_INV_SQRT_PI 0.56418958354775628695
+#define IRS_PI          3.14159265358979323846
+
+// Line-sum tile: 256 threads x P points, thread t owns points i0 + t + 256*p.
+#define IRS_LS_THREADS 256
+
+// One uploaded line list on the device (structure of arrays, sorted by nu).
+struct LineListDev {
+    int64_t n;
+    const int8_t *mol, *iso;
+    const double *nu, *S, *elower, *ylm;
+    const float *gair, *gself, *nair, *dair;
+    const int16_t *tips_row;       // row of the partition table per line, -1: none
+};
+
+// Per (profile, layer, species) work item.  Written by the host, read by the
+// prep and line-sum kernels with wave-uniform indices (scalar loads).
+struct ItemDev {
+    int64_t rec_off;        // first record of this item in rec0/rec1/rec2
+    int32_t n;              // lines in the list
+    int32_t key_off;        // offset of the list's key array
+    int32_t list;           // index into the line-list table
+    int32_t lm;             // 1: VP_Y dispersive term active
+    int32_t active;         // 0: skip (no lines or vmr == 0)
+    int32_t M;              // max |pressure shift| in grid steps, rounded up, + 2
+    float   C, Clm;         // cutoff / lm_cutoff in grid steps
+    double  kcore_per_nu;   // x_far * max Doppler coefficient * sqrt(T) / (sqrt(ln2) * dnu)
+    double  T, p_atm, vmr_self, coef, dptmn, cutoff, x_far;
+};
+
+struct PrepArgs {
+    const ItemDev *items;
+    const LineListDev *lists;
+    const double *tips_q;          // [n_rows][331]
+    int n_items;
+    double nu_lo, dnu;
+    float4 *rec0, *rec1, *rec2;
+    int32_t *keys;                 // per list, floor((nu - nu_lo)/dnu)
+    unsigned long long *counters;  // [0] active lines, [1] voigt pairs, [2] dispersive pairs
+    int64_t n_hi;
+};
+
+struct LinesumArgs {
+    const ItemDev *items;          // [n_prof][n_layers][n_species]
+    const float4 *rec0, *rec1, *rec2;
+    const int32_t *keys;
+    float *out;                    // [n_prof][n_layers][n_hi]
+    int64_t n_hi;
+    int n_layers, n_species;
+    double nu_lo, dnu;
+    int tiles;
+    const int32_t *tile_order;     // optional permutation of blockIdx.x (heavy tiles first)
+};
+
+struct RteArgs {
+    const float *tau32;            // [n_layers][n] (one of tau32/tau64)
+    const double *tau64;
+    int64_t n;
+    int n_layers;
+    const double *T_lev, *T_ave;   // device, [L+1], [L]
+    double T_sfc, mu, eps;
+    int source_function;
+    double an, sn, den;            // grid node i = (an + i*sn)/den
+    double *out;
+};
+
+struct IlsArgs {
+    const double *spec;            // [n]
+    int64_t n;
+    const double *taps;            // [2*half+1]
+    int64_t half;
+    double dnu;
+    const int64_t *ch_idx;         // left node of each channel
+    const double *ch_w;            // interpolation weight of the right node (0: exact)
+    const double *ch_nu;
+    int64_t n_ch;
+    double *r_out, *bt_out;
+};
+
+// launchers (k_*.hip)
+hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s);
+hipError_t launch_linesum(const LinesumArgs &a, int n_prof, int P, hipStream_t s);
+hipError_t launch_rte(const RteArgs &a, hipStream_t s);
+hipError_t launch_ils_taps(const double *A, const double *L, int n_pts, double dL, double dnu,
+                           int64_t half, double *taps, hipStream_t s);
+hipError_t launch_ils_channels(const IlsArgs &a, hipStream_t s);

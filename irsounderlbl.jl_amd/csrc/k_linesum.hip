@@ -1,0 +1,285 @@
+// FP32 tiled Voigt (+ VP_Y dispersive) line sum: tau[layer][nu] for a batch
+// of profiles.
+//
+// Semantics restated from voigt_cross_section_kernel! (src/CrossSections/
+// voigt.jl:243-277), lm_dispersive_kernel! (src/CrossSections/line_mixing.jl:
+// 1166-1186), compute_voigt_lm_cross_sections (:1351-1369) and the tau
+// accumulation of forward_model (src/Sensor/sounder.jl:283-306):
+//   sigma_s(nu) = max(sum_{|nu-nu0|<=cut} Snorm (H(x,y) - (2 - (dnu/cut)^2) Hc), 0)
+//   sigma_CO2   = max(max(sigma_v, 0) + sum_{|nu-nu0|<=lm_cut} YSn Im w, 0)   (VP_Y)
+//   tau         = sum_s coef_s sigma_s         (coef folded into the records)
+//
+// Design (MI355X): one workgroup = one spectral tile of 256*P grid points of
+// one (profile, layer); thread t owns points i0 + t + 256 p, so each wave
+// covers 64 consecutive points per p (coalesced stores, coherent branches).
+// Lines are read with wave-uniform indices (scalar loads, SGPR operands).  Per
+// species, the lines near the tile are split by five binary searches on the
+// sorted unshifted-centre keys into
+//   EDGE  : window boundary crosses the tile   -> per-pair window test + pedestal
+//   FAST  : far wing for every point of tile   -> A/(du^2+g^2): 3 VALU + 1 rcp
+//   CORE  : may have |x| <= x_far in the tile  -> exact w(z) (FP32) or far wing
+// The FAST lines' pedestal, a quadratic in the grid offset, is summed per
+// tile in FP64 (three moments) instead of per pair.  FAST partial sums run
+// over 32-line chunks and are folded into Kahan-compensated accumulators;
+// CORE/EDGE terms are Kahan-added one by one.  Offsets are formed from the
+// integer grid index and the line centre split into an integer index plus an
+// FP32 fraction (centre computed in FP64), so du carries one FP32 rounding.
+#include "irs_devmath.h"
+#include "irs_internal.h"
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__device__ __forceinline__ int lower_key(const int32_t *k, int n, int64_t v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)k[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int upper_key(const int32_t *k, int n, int64_t v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)k[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// FAST: far wing, window covers the whole tile, pedestal handled per tile.
+template <int P, bool LM>
+__device__ __forceinline__ void fast_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+                                           int jlo, int jhi, int tl, float (&acc)[P], float (&cmp)[P],
+                                           float (&accd)[P], float (&cmpd)[P]) {
+    for (int j = jlo; j < jhi; j += 32) {
+        const int je = min(j + 32, jhi);
+        float part[P], partd[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
+        for (int jj = j; jj < je; ++jj) {
+            const float4 q = r0p[jj];
+            const float du0 = (float)(tl - __float_as_int(q.x)) - q.y;
+            float B = 0.0f;
+            if (LM) B = r1p[jj].x;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float du = du0 + (float)(256 * p);
+                const float r = irs_rcpf(fmaf(du, du, q.z));
+                part[p] = fmaf(q.w, r, part[p]);
+                if (LM) partd[p] = fmaf(B * du, r, partd[p]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            kahan_add(acc[p], cmp[p], part[p]);
+            if (LM) kahan_add(accd[p], cmpd[p], partd[p]);
+        }
+    }
+}
+
+// EDGE: far wing with per-pair window test and pedestal.
+template <int P, bool LM>
+__device__ __forceinline__ void edge_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+                                           int jlo, int jhi, int tl, float C2, float C2lm, float iC2,
+                                           float (&acc)[P], float (&cmp)[P], float (&accd)[P],
+                                           float (&cmpd)[P]) {
+    for (int j = jlo; j < jhi; ++j) {
+        const float4 q0 = r0p[j];
+        const float4 q1 = r1p[j];
+        const float base = (float)(tl - __float_as_int(q0.x));
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const float du = (base + (float)(256 * p)) - q0.y;
+            const float d2 = du * du;
+            const float r = irs_rcpf(d2 + q0.z);
+            const float v = fmaf(q0.w, r, -q1.y * fmaf(-d2, iC2, 2.0f));
+            if (d2 <= C2) kahan_add(acc[p], cmp[p], v);
+            if (LM && d2 <= C2lm) kahan_add(accd[p], cmpd[p], q1.x * du * r);
+        }
+    }
+}
+
+// CORE (and the degenerate small-cutoff case): exact branch selection per pair.
+template <int P, bool LM>
+__device__ __forceinline__ void general_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+                                              const float4 *__restrict__ r2p, int jlo, int jhi, int tl,
+                                              float xfar, float C2, float C2lm, float iC2, float (&acc)[P],
+                                              float (&cmp)[P], float (&accd)[P], float (&cmpd)[P]) {
+    for (int j = jlo; j < jhi; ++j) {
+        const float4 q0 = r0p[j];
+        const float4 q1 = r1p[j];
+        const float4 q2 = r2p[j];
+        const float base = (float)(tl - __float_as_int(q0.x));
+        const bool dline = LM && (q1.x != 0.0f || q2.w != 0.0f);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const float du = (base + (float)(256 * p)) - q0.y;
+            const float d2 = du * du;
+            const float x = du * q1.z;
+            float h;
+            if (fabsf(x) > xfar) {
+                h = q0.w * irs_rcpf(d2 + q0.z);
+            } else {
+                float wr, wi;
+                w32(x, q1.w, wr, wi);
+                h = q2.x * wr;
+            }
+            const float v = h - q1.y * fmaf(-d2, iC2, 2.0f);
+            if (d2 <= C2) kahan_add(acc[p], cmp[p], v);
+            if (dline) {
+                const float xd = du * q2.y;
+                float dd;
+                if (fabsf(xd) > xfar) {
+                    dd = q1.x * du * irs_rcpf(d2 + q0.z);
+                } else {
+                    float wr, wi;
+                    w32(xd, q2.z, wr, wi);
+                    dd = q2.w * wi;
+                }
+                if (d2 <= C2lm) kahan_add(accd[p], cmpd[p], dd);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red, int t) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+template <int P, bool LM>
+__device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev &it, int t, int64_t i0,
+                                             int64_t i1, float (&tau)[P], int *s_rng, double *s_red) {
+    const int32_t *keys = a.keys + it.key_off;
+    const int n = it.n;
+    const double dnu = a.dnu;
+    const double Cd = it.cutoff / dnu;
+    const double Cmx = LM ? fmax((double)it.C, (double)it.Clm) : (double)it.C;
+    const double Cmn = LM ? fmin((double)it.C, (double)it.Clm) : (double)it.C;
+    const int64_t Cmax = (int64_t)ceil(Cmx), Cmin = (int64_t)floor(Cmn);
+    const int64_t M = it.M;
+    const double numax = a.nu_lo + (double)(i1 + Cmax + M) * dnu;
+    const int64_t K = (int64_t)ceil(it.kcore_per_nu * numax) + 2;
+    const int64_t vA = i0 - Cmax - M - 1;
+    const int64_t vB = i1 - 1 - Cmin + M + 1;
+    const int64_t vC = i0 - K - M - 1;
+    const int64_t vD = i1 - 1 + K + M + 1;
+    const int64_t vE = i0 + Cmin - M - 2;
+    const int64_t vF = i1 - 1 + Cmax + M + 1;
+    const bool nondeg = vA <= vB && vB <= vC && vC <= vD && vD <= vE && vE <= vF;
+    if (t < 6) {
+        int r;
+        switch (t) {
+            case 0: r = lower_key(keys, n, vA); break;
+            case 1: r = lower_key(keys, n, vB); break;
+            case 2: r = lower_key(keys, n, vC); break;
+            case 3: r = upper_key(keys, n, vD); break;
+            case 4: r = upper_key(keys, n, vE); break;
+            default: r = upper_key(keys, n, vF); break;
+        }
+        s_rng[t] = r;
+    }
+    __syncthreads();
+    const int jA = uni(s_rng[0]), jB = uni(s_rng[1]), jC = uni(s_rng[2]);
+    const int jD = uni(s_rng[3]), jE = uni(s_rng[4]), jF = uni(s_rng[5]);
+    __syncthreads();
+
+    const float4 *r0p = a.rec0 + it.rec_off;
+    const float4 *r1p = a.rec1 + it.rec_off;
+    const float4 *r2p = a.rec2 + it.rec_off;
+    const int64_t imid = i0 + 128 * P;
+    const int tl = (int)(i0 + t);
+    const float C2 = (float)((double)it.C * (double)it.C);
+    const float C2lm = (float)((double)it.Clm * (double)it.Clm);
+    const double iC2d = 1.0 / (Cd * Cd);
+    const float iC2 = (float)iC2d;
+    const float xfar = (float)it.x_far;
+
+    float acc[P], cmp[P], accd[P], cmpd[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) { acc[p] = cmp[p] = accd[p] = cmpd[p] = 0.0f; }
+
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+    if (nondeg) {
+        // pedestal moments of the FAST lines about the tile centre (FP64)
+        for (int rgi = 0; rgi < 2; ++rgi) {
+            const int lo = rgi == 0 ? jB : jD, hi = rgi == 0 ? jC : jE;
+            for (int j = lo + t; j < hi; j += IRS_LS_THREADS) {
+                const float4 r0 = r0p[j];
+                const double pj = (double)r1p[j].y;
+                const double w = (double)((int64_t)__float_as_int(r0.x) - imid) + (double)r0.y;
+                q0 += pj;
+                q1 += pj * w;
+                q2 += pj * w * w;
+            }
+        }
+        q0 = block_sum(q0, s_red, t);
+        q1 = block_sum(q1, s_red + 4, t);
+        q2 = block_sum(q2, s_red + 8, t);
+        fast_range<P, LM>(r0p, r1p, jB, jC, tl, acc, cmp, accd, cmpd);
+        general_range<P, LM>(r0p, r1p, r2p, jC, jD, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jD, jE, tl, acc, cmp, accd, cmpd);
+        edge_range<P, LM>(r0p, r1p, jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        edge_range<P, LM>(r0p, r1p, jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+    } else {
+        general_range<P, LM>(r0p, r1p, r2p, jA, jF, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const double v = (double)(i0 + t + 256 * p - imid);
+        const double ped = (2.0 - v * v * iC2d) * q0 + 2.0 * v * iC2d * q1 - iC2d * q2;
+        double sv = ((double)acc[p] - (double)cmp[p]) - ped;
+        sv = sv > 0.0 ? sv : 0.0;
+        if (LM) {
+            sv += (double)accd[p] - (double)cmpd[p];
+            sv = sv > 0.0 ? sv : 0.0;
+        }
+        tau[p] += (float)sv;
+    }
+}
+
+template <int P>
+__global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumArgs a) {
+    __shared__ int s_rng[8];
+    __shared__ double s_red[12];
+    const int t = threadIdx.x;
+    const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
+    const int layer = blockIdx.y, prof = blockIdx.z;
+    const int64_t i0 = (int64_t)tile * (IRS_LS_THREADS * P);
+    const int64_t i1 = min(i0 + (int64_t)(IRS_LS_THREADS * P), a.n_hi);
+    float tau[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) tau[p] = 0.0f;
+    const ItemDev *items = a.items + ((int64_t)prof * a.n_layers + layer) * a.n_species;
+    for (int s = 0; s < a.n_species; ++s) {
+        const ItemDev it = items[s];
+        if (!it.active || it.n == 0) continue;
+        if (it.lm) species_tile<P, true>(a, it, t, i0, i1, tau, s_rng, s_red);
+        else species_tile<P, false>(a, it, t, i0, i1, tau, s_rng, s_red);
+    }
+    float *out = a.out + ((int64_t)prof * a.n_layers + layer) * a.n_hi;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int64_t i = i0 + t + 256 * p;
+        if (i < a.n_hi) out[i] = tau[p];
+    }
+}
+
+hipError_t launch_linesum(const LinesumArgs &a, int n_prof, int P, hipStream_t s) {
+    dim3 grid(a.tiles, a.n_layers, n_prof);
+    if (P == 16) hipLaunchKernelGGL(linesum_kernel<16>, grid, dim3(IRS_LS_THREADS), 0, s, a);
+    else if (P == 4) hipLaunchKernelGGL(linesum_kernel<4>, grid, dim3(IRS_LS_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(linesum_kernel<8>, grid, dim3(IRS_LS_THREADS), 0, s, a);
+    return hipGetLastError();
+}

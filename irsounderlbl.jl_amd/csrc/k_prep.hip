@@ -1,0 +1,201 @@
+// Per-(layer, species, line) line parameters and DPTMIN decisions, FP64.
+//
+// One thread per (item, line).  Restates, per line, the host loop of
+// compute_voigt_cross_sections (src/CrossSections/voigt.jl:402-414,448-454),
+// the DPTMIN test of _reject_weak_lines (voigt.jl:333-355), the VP_Y line
+// weights of _append_dispersive_lines! (src/CrossSections/line_mixing.jl:1193-1220)
+// and the line-shape formulas of src/HITRAN/broadening.jl:40-160 and
+// src/HITRAN/partition.jl:10-41.  Results are packed into three float4
+// records per line, in grid-step units, for the FP32 line-sum kernel:
+//   rec0 = {centre index (int bits), centre fraction, g^2, A}
+//   rec1 = {B, P, fu, y}
+//   rec2 = {Sn, fu_d, y_d, YSn}
+// with g = gamma_L/dnu, A = coef S g/(pi dnu) (Lorentz wing amplitude),
+// P = coef Snorm H(x_cut, y) (pedestal), fu = dnu f, y = gamma_L f,
+// Sn = coef S f/sqrt(pi); the _d fields and B = coef Y p S_T/(pi dnu),
+// YSn = coef Y p S_T f_d/sqrt(pi) carry the dispersive term.
+#include "irs_devmath.h"
+#include "irs_internal.h"
+
+#define TIPS_T_MIN 70
+#define TIPS_NT 331
+#define T_REF 296.0
+#define C2_LINE 1.4387752   // broadening.jl:10
+#define C2_LM 1.4388        // line_mixing.jl:31
+
+__device__ __forceinline__ double tips_lookup(const double *v, double T) {
+    long lo = (long)floor(T) - TIPS_T_MIN;
+    long hi = lo + 1;
+    lo = lo < 0 ? 0 : (lo > TIPS_NT - 1 ? TIPS_NT - 1 : lo);
+    hi = hi < 0 ? 0 : (hi > TIPS_NT - 1 ? TIPS_NT - 1 : hi);
+    const double frac = T - floor(T);
+    return v[lo] * (1.0 - frac) + v[hi] * frac;
+}
+
+__device__ __forceinline__ double mol_mass(int mol) {   // broadening.jl:131-148
+    switch (mol) {
+        case 1: return 18.010;  case 2: return 44.010;  case 3: return 47.998;
+        case 4: return 44.013;  case 5: return 28.010;  case 6: return 16.043;
+        case 7: return 31.999;  case 8: return 34.015;  case 9: return 64.065;
+        case 10: return 36.461; case 11: return 17.031;
+        default: return 30.0;
+    }
+}
+
+__device__ __forceinline__ double lm_mass(int mol, int iso) {   // line_mixing.jl:743-753
+    if (mol == 2) {
+        switch (iso) {
+            case 1: return 43.98983; case 2: return 44.99318; case 3: return 45.99398;
+            case 4: return 44.99403; case 5: return 45.99706; case 6: return 44.99706;
+            case 7: return 47.99832; case 8: return 47.00134; case 9: return 48.00196;
+            case 0: case 10: return 47.00427;
+            default: return 44.0;
+        }
+    }
+    if (mol == 6) {
+        switch (iso) {
+            case 1: return 16.031300; case 2: return 17.034655;
+            case 3: return 17.037475; case 4: return 18.040830;
+            default: return 44.0;
+        }
+    }
+    return 44.0;
+}
+
+__global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
+    const int item = blockIdx.y;
+    const ItemDev it = a.items[item];
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    unsigned long long my_active = 0, my_pairs = 0, my_dpairs = 0;
+    if (it.active && j < it.n) {
+        const LineListDev L = a.lists[it.list];
+        const double T = it.T, p = it.p_atm, vs = it.vmr_self, coef = it.coef;
+        const double nu = L.nu[j];
+        const int mol = L.mol[j], iso = L.iso[j];
+        const int row = L.tips_row[j];
+        double qr;
+        if (row >= 0) {
+            const double *v = a.tips_q + (int64_t)row * TIPS_NT;
+            qr = tips_lookup(v, T_REF) / tips_lookup(v, T);
+        } else {
+            qr = T_REF / T;
+        }
+        // temperature_scaled_intensity, broadening.jl:89-99
+        const double stim = (1.0 - exp(-C2_LINE * nu / T)) / (1.0 - exp(-C2_LINE * nu / T_REF));
+        const double boltz = exp(-C2_LINE * L.elower[j] * (1.0 / T - 1.0 / T_REF));
+        const double S = L.S[j] * qr * boltz * stim;
+        // pressure_broadened_width, broadening.jl:40-53
+        const double n_air = (double)L.nair[j];
+        const double n_self = mol == 1 ? 0.95 : (mol == 2 ? 0.75 : n_air);
+        const double tr = T_REF / T;
+        const double gl = pow(tr, n_air) * (double)L.gair[j] * (1.0 - vs) * p +
+                          pow(tr, n_self) * (double)L.gself[j] * vs * p;
+        double gd = nu * 3.58126e-7 * sqrt(T / mol_mass(mol));
+        gd = fmax(gd, 1e-10);
+        const double f = IRS_SQRT_LN2 / gd;
+        const double y = gl * f;
+        const double snorm = S * f * IRS_INV_SQRT_PI;
+        // pressure_shift, broadening.jl:156-160
+        const double dself = mol == 1 ? 0.002 : (mol == 2 ? -0.003 : 0.0);
+        const double c = nu + ((double)L.dair[j] * (1.0 - vs) + dself * vs) * p;
+        // DPTMIN: peak = Snorm * erfcx(y) (voigt.jl:340-350); skipped on the VP_Y path
+        bool keep = true;
+        if (it.dptmn > 0.0 && !it.lm) keep = (snorm * erfcx(y)) * coef > it.dptmn;
+        // pedestal value at the cutoff (voigt.jl:448-454)
+        const double xc = it.cutoff * f;
+        double hc;
+        if (fabs(xc) > it.x_far) hc = y * IRS_INV_SQRT_PI / (xc * xc + y * y);
+        else { double wi; w64(xc, y, hc, wi); }
+        const double u = (c - a.nu_lo) / a.dnu;
+        const double cn = floor(u);
+        const double g = gl / a.dnu;
+        float4 r0, r1, r2;
+        r0.x = __int_as_float((int)cn);
+        r0.y = (float)(u - cn);
+        if (keep) {
+            r0.z = (float)(g * g);
+            r0.w = (float)(coef * S * g / (IRS_PI * a.dnu));
+            r1.y = (float)(coef * snorm * hc);
+            r1.z = (float)(a.dnu * f);
+            r1.w = (float)y;
+            r2.x = (float)(coef * snorm);
+        } else {
+            r0.z = 1.0f; r0.w = 0.0f; r1.y = 0.0f; r1.z = 1.0f; r1.w = 1.0f; r2.x = 0.0f;
+        }
+        r1.x = 0.0f; r2.y = 1.0f; r2.z = 1.0f; r2.w = 0.0f;
+        // VP_Y dispersive weights with the S-file conventions (line_mixing.jl:1193-1220)
+        bool disp = false;
+        if (it.lm && L.ylm && L.ylm[j] != 0.0) {
+            const double Y = L.ylm[j];
+            const double md = lm_mass(mol, iso);
+            const double gdd = 3.5812e-7 * nu * sqrt(T / md);
+            const double gld = (double)L.gair[j] * pow(296.0 / T, n_air) * p;
+            const double fd = IRS_SQRT_LN2 / fmax(gdd, 1e-10);
+            const double st0 = 1.0 - exp(-C2_LM * nu / 296.0);
+            const double stt = 1.0 - exp(-C2_LM * nu / T);
+            const double st = L.S[j] * qr * exp(-C2_LM * L.elower[j] * (1.0 / T - 1.0 / 296.0)) * stt / st0;
+            if (st > 0.0) {
+                r1.x = (float)(coef * Y * p * st / (IRS_PI * a.dnu));
+                r2.y = (float)(a.dnu * fd);
+                r2.z = (float)(gld * fd);
+                r2.w = (float)(coef * Y * p * st * fd * IRS_INV_SQRT_PI);
+                disp = true;
+            }
+        }
+        const int64_t o = it.rec_off + j;
+        a.rec0[o] = r0;
+        a.rec1[o] = r1;
+        a.rec2[o] = r2;
+        if (keep) {
+            my_active = 1;
+            const double lo = ceil((c - it.cutoff - a.nu_lo) / a.dnu);
+            const double hi = floor((c + it.cutoff - a.nu_lo) / a.dnu);
+            const double l2 = fmax(lo, 0.0), h2 = fmin(hi, (double)(a.n_hi - 1));
+            if (h2 >= l2) my_pairs = (unsigned long long)(h2 - l2 + 1.0);
+        }
+        if (disp) {
+            const double lc = (double)it.Clm * a.dnu;
+            const double lo = ceil((c - lc - a.nu_lo) / a.dnu);
+            const double hi = floor((c + lc - a.nu_lo) / a.dnu);
+            const double l2 = fmax(lo, 0.0), h2 = fmin(hi, (double)(a.n_hi - 1));
+            if (h2 >= l2) my_dpairs = (unsigned long long)(h2 - l2 + 1.0);
+        }
+    }
+    // block reduction of the work counters
+    for (int o = 32; o > 0; o >>= 1) {
+        my_active += __shfl_xor(my_active, o, 64);
+        my_pairs += __shfl_xor(my_pairs, o, 64);
+        my_dpairs += __shfl_xor(my_dpairs, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && a.counters && (my_active | my_pairs | my_dpairs)) {
+        atomicAdd(&a.counters[0], my_active);
+        atomicAdd(&a.counters[1], my_pairs);
+        atomicAdd(&a.counters[2], my_dpairs);
+    }
+}
+
+// floor((nu_j - nu_lo)/dnu) for one list: the sorted search key of the line sum.
+__global__ void __launch_bounds__(256) keys_kernel(const double *nu, int64_t n, double nu_lo, double dnu,
+                                                   int32_t *keys) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j < n) {
+        double k = floor((nu[j] - nu_lo) / dnu);
+        k = fmin(fmax(k, -2.0e9), 2.0e9);
+        keys[j] = (int32_t)k;
+    }
+}
+
+hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s) {
+    if (a.n_items <= 0 || max_lines <= 0) return hipSuccess;
+    dim3 grid((max_lines + 255) / 256, a.n_items);
+    hipLaunchKernelGGL(prep_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys(const double *nu, int64_t n, double nu_lo, double dnu, int32_t *keys,
+                       hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nu, n, nu_lo,
+                       dnu, keys);
+    return hipGetLastError();
+}

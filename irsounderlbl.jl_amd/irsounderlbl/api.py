@@ -1,0 +1,408 @@
+"""Host mirror of the reference operators on top of libirs (include/irs.h).
+
+Names, argument meaning, defaults and error behaviour follow the reference:
+  forward_model                   src/Sensor/sounder.jl:233-360
+  compute_voigt_cross_sections    src/CrossSections/voigt.jl:375-462 (+ DPTMIN :333-355)
+  compute_voigt_lm_cross_sections src/CrossSections/line_mixing.jl:1351-1369
+  schwarzschild_rte               src/Solver/schwarzschild.jl:255-334
+  ils_kernel / apply_ils          src/Sensor/ils.jl:80-153
+  planck_radiance / brightness_temperature   src/Solver/planck.jl:11-57
+Every numeric call runs in the HIP library; there is no CPU fallback.  A
+missing library or GPU raises IRSError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .types import (AtmosphericProfile, GasSpecies, HITRANLinelist, Sounder, ViewingGeometry,
+                    WavenumberGrid, IASIInstrument, nadir_geometry, sounder_grid)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "csrc", "libirs.so"))
+
+APODIZATION = {"gaussian": 0, "norton_beer_weak": 1, "norton_beer_medium": 2, "norton_beer_strong": 3}
+SOURCE_FUNCTION = {"toon": 0, "cim": 1}
+T_METHOD = {"logp_at_pcg": 0, "mass_weighted": 1}
+
+IRS_OK, IRS_EINVAL, IRS_EHIP, IRS_ENOMEM, IRS_ENOTFOUND, IRS_ENODEV = 0, -1, -2, -3, -4, -5
+
+
+class IRSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+i64, f64, i32 = C.c_int64, C.c_double, C.c_int
+P = C.c_void_p
+
+
+class IrsLinelist(C.Structure):
+    _fields_ = [("n", i64), ("mol", P), ("iso", P), ("nu", P), ("S", P), ("gamma_air", P),
+                ("gamma_self", P), ("elower", P), ("n_air", P), ("delta_air", P), ("y_lm", P)]
+
+
+class IrsSounder(C.Structure):
+    _fields_ = [("nu_min", f64), ("nu_max", f64), ("dnu", f64), ("opd_max", f64),
+                ("fwhm_gauss", f64), ("n_channels", i64)]
+
+
+class IrsProfile(C.Structure):
+    _fields_ = [("n_levels", i32), ("p_hpa", P), ("T", P), ("n_species", i32), ("species", P),
+                ("linelist", P), ("vmr", P), ("T_sfc", f64), ("eps_sfc", f64), ("mu", f64)]
+
+
+class IrsOptions(C.Structure):
+    _fields_ = [("internal_dnu", f64), ("high_res_factor", i32), ("cutoff", f64),
+                ("lm_cutoff", f64), ("dptmn", f64), ("x_far", f64), ("with_ils", i32),
+                ("apodization", i32), ("line_mixing", i32), ("T_method", i32),
+                ("source_function", i32), ("reserved0", i32)]
+
+
+class IrsStats(C.Structure):
+    _fields_ = [("ms_total", f64), ("ms_prep", f64), ("ms_linesum", f64), ("ms_rte", f64),
+                ("ms_ils", f64), ("n_hi", i64), ("n_layers", i64), ("linesum_launches", i64),
+                ("active_lines", i64), ("active_pairs", i64), ("dispersive_pairs", i64),
+                ("core_pairs_bound", i64)]
+
+
+EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_last_error",
+           "irs_default_options", "irs_last_stats", "irs_linelist_upload", "irs_linelist_free",
+           "irs_xsec_layer", "irs_rte", "irs_ils_half", "irs_ils_kernel", "irs_ils_channels",
+           "irs_brightness_temperature", "irs_forward_dims", "irs_forward")
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libirs.so and declare the C ABI.  Raises IRSError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise IRSError(IRS_ENODEV, f"libirs.so not built at {path} (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    L.irs_version.restype = i32
+    L.irs_device_info.argtypes = [i32, C.c_char_p, i32]
+    L.irs_create.argtypes = [i32, C.POINTER(P)]
+    L.irs_destroy.argtypes = [P]
+    L.irs_destroy.restype = None
+    L.irs_last_error.argtypes = [P]
+    L.irs_last_error.restype = C.c_char_p
+    L.irs_default_options.argtypes = [C.POINTER(IrsOptions)]
+    L.irs_default_options.restype = None
+    L.irs_last_stats.argtypes = [P, C.POINTER(IrsStats)]
+    L.irs_linelist_upload.argtypes = [P, C.POINTER(IrsLinelist), C.POINTER(C.c_int32)]
+    L.irs_linelist_free.argtypes = [P, C.c_int32]
+    L.irs_xsec_layer.argtypes = [P, C.c_int32, f64, f64, i64, f64, f64, f64, f64, f64, f64, f64,
+                                 f64, i32, P]
+    L.irs_rte.argtypes = [P, f64, f64, i64, P, i32, P, f64, f64, f64, i32, P, P]
+    L.irs_ils_half.argtypes = [f64, C.POINTER(i64)]
+    L.irs_ils_kernel.argtypes = [P, f64, f64, f64, i32, i32, P]
+    L.irs_ils_channels.argtypes = [P, P, f64, f64, i64, C.POINTER(IrsSounder), i32, P]
+    L.irs_brightness_temperature.argtypes = [P, P, i64, P]
+    L.irs_forward_dims.argtypes = [C.POINTER(IrsSounder), C.POINTER(IrsOptions), C.POINTER(f64),
+                                   C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]
+    L.irs_forward.argtypes = [P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
+                              C.POINTER(IrsOptions), P, P]
+    for name in EXPORTS:
+        if name not in ("irs_destroy", "irs_default_options", "irs_last_error"):
+            getattr(L, name).restype = getattr(L, name).restype or i32
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def _sounder(s: Sounder) -> IrsSounder:
+    return IrsSounder(s.nu_min, s.nu_max, s.dnu, s.opd_max, s.fwhm_gauss, s.n_channels)
+
+
+def _apod(apodization) -> int:
+    key = apodization.lstrip(":") if isinstance(apodization, str) else apodization
+    if isinstance(key, int):
+        return key
+    if key not in APODIZATION:
+        raise ValueError(f"Unknown apodization {apodization}; expected :gaussian or one of "
+                         f"{[k for k in APODIZATION if k != 'gaussian']}")
+    return APODIZATION[key]
+
+
+class Context:
+    """One device context (HIP stream + device buffers + uploaded line lists)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = P()
+        rc = self.lib.irs_create(device, C.byref(h))
+        if rc != IRS_OK:
+            raise IRSError(rc, f"irs_create failed ({rc}): no usable gfx950 device {device}")
+        self.h = h
+        self._lists: Dict[int, tuple] = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.irs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != IRS_OK:
+            msg = self.lib.irs_last_error(self.h).decode(errors="replace")
+            if rc == IRS_EINVAL:
+                raise ValueError(msg)
+            raise IRSError(rc, msg)
+
+    def stats(self) -> dict:
+        s = IrsStats()
+        self.check(self.lib.irs_last_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in IrsStats._fields_}
+
+    def upload(self, ll: HITRANLinelist) -> int:
+        key = id(ll)
+        if key in self._lists and self._lists[key][1] is ll:
+            return self._lists[key][0]
+        s = IrsLinelist(len(ll), _p(ll.mol_id), _p(ll.iso_id), _p(ll.wavenumber), _p(ll.intensity),
+                        _p(ll.air_broad), _p(ll.self_broad), _p(ll.lower_energy), _p(ll.temp_depend),
+                        _p(ll.pressure_shift), _p(ll.y_lm))
+        h = C.c_int32()
+        self.check(self.lib.irs_linelist_upload(self.h, C.byref(s), C.byref(h)))
+        self._lists[key] = (h.value, ll)
+        return h.value
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("IRS_DEVICE", "0")))
+    return _default_ctx
+
+
+def compute_voigt_cross_sections(nu_grid: WavenumberGrid, linelist: HITRANLinelist, T: float,
+                                 p_atm: float, *, vmr_self: float = 0.0, cutoff: float = 25.0,
+                                 x_far: float = 122.0, dptmn: float = 0.0, coef: float = 1.0,
+                                 ctx: Optional[Context] = None) -> np.ndarray:
+    """Voigt cross sections (cm^2/molec) on nu_grid; dptmn > 0 applies DPTMIN with coef."""
+    ctx = ctx or default_context()
+    h = ctx.upload(linelist)
+    out = np.zeros(nu_grid.n)
+    ctx.check(ctx.lib.irs_xsec_layer(ctx.h, h, nu_grid.nu_min, nu_grid.dnu, nu_grid.n, float(T),
+                                     float(p_atm), float(vmr_self), float(coef), float(dptmn),
+                                     float(cutoff), float(cutoff), float(x_far), 0, _p(out)))
+    return out
+
+
+def compute_voigt_lm_cross_sections(nu_grid: WavenumberGrid, ll_co2: HITRANLinelist, T: float,
+                                     p_atm: float, *, cutoff: float = 25.0,
+                                     lm_cutoff: Optional[float] = None, x_far: float = 122.0,
+                                     ctx: Optional[Context] = None) -> np.ndarray:
+    """max(max(sigma_voigt, 0) + sigma_disp, 0) with per-line Y (VP_Y)."""
+    ctx = ctx or default_context()
+    h = ctx.upload(ll_co2)
+    out = np.zeros(nu_grid.n)
+    lc = cutoff if lm_cutoff is None else lm_cutoff
+    ctx.check(ctx.lib.irs_xsec_layer(ctx.h, h, nu_grid.nu_min, nu_grid.dnu, nu_grid.n, float(T),
+                                     float(p_atm), 0.0, 1.0, 0.0, float(cutoff), float(lc),
+                                     float(x_far), 1, _p(out)))
+    return out
+
+
+def schwarzschild_rte(nu_grid: WavenumberGrid, tau_layers: np.ndarray, T_levels, T_sfc: float, *,
+                      mu: float = 1.0, eps_sfc: float = 1.0, source_function: str = "toon",
+                      T_ave=None, ctx: Optional[Context] = None) -> np.ndarray:
+    """TOA radiance; tau_layers is (n_nu, n_layers) (schwarzschild.jl:255-334)."""
+    tau = np.asfortranarray(tau_layers, np.float64)
+    n_nu, L = tau.shape
+    if n_nu != nu_grid.n:
+        raise ValueError(f"τ_layers rows {n_nu} ≠ ν_grid.n {nu_grid.n}")
+    T_levels = np.ascontiguousarray(T_levels, np.float64)
+    if len(T_levels) != L + 1:
+        raise ValueError(f"T_levels must have n_layers+1 = {L + 1} entries")
+    if not (0.0 < mu <= 1.0):
+        raise ValueError("μ must be in (0, 1]")
+    sf = source_function.lstrip(":")
+    if sf not in SOURCE_FUNCTION:
+        raise ValueError(f"source_function must be :toon or :cim, got :{sf}")
+    ta = None
+    if sf == "cim":
+        if T_ave is None:
+            raise ValueError(":cim source_function requires T_ave keyword")
+        ta = np.ascontiguousarray(T_ave, np.float64)
+        if len(ta) != L:
+            raise ValueError(f"T_ave must have n_layers = {L} entries, got {len(ta)}")
+    ctx = ctx or default_context()
+    out = np.zeros(n_nu)
+    ctx.check(ctx.lib.irs_rte(ctx.h, nu_grid.nu_min, nu_grid.dnu, n_nu, _p(tau), L, _p(T_levels),
+                              float(T_sfc), float(mu), float(eps_sfc), SOURCE_FUNCTION[sf], _p(ta),
+                              _p(out)))
+    return out
+
+
+def ils_kernel(dnu: float, opd_max: float, fwhm_gauss: float, *, apodization="gaussian",
+               n_pts: int = 2048, ctx: Optional[Context] = None):
+    """(delta_nu array, taps) area-normalised (ils.jl:80-114)."""
+    a = _apod(apodization)
+    ctx = ctx or default_context()
+    half = i64()
+    ctx.check(ctx.lib.irs_ils_half(float(dnu), C.byref(half)))
+    taps = np.zeros(2 * half.value + 1)
+    ctx.check(ctx.lib.irs_ils_kernel(ctx.h, float(dnu), float(opd_max), float(fwhm_gauss), a, n_pts,
+                                     _p(taps)))
+    return np.arange(-half.value, half.value + 1) * dnu, taps
+
+
+def apply_ils_channels(nu_grid: WavenumberGrid, spectrum, sounder: Sounder, *,
+                       apodization="gaussian", ctx: Optional[Context] = None) -> np.ndarray:
+    """ILS-convolved spectrum sampled at the sounder channels (ils.jl:131-153 +
+    sounder.jl:372-377), evaluated only at the channel nodes."""
+    ctx = ctx or default_context()
+    spec = np.ascontiguousarray(spectrum, np.float64)
+    if len(spec) != nu_grid.n:
+        raise ValueError("spectrum length mismatch")
+    snd = _sounder(sounder)
+    out = np.zeros(sounder_grid(sounder).n)
+    ctx.check(ctx.lib.irs_ils_channels(ctx.h, _p(spec), nu_grid.nu_min, nu_grid.dnu, nu_grid.n,
+                                       C.byref(snd), _apod(apodization), _p(out)))
+    return out
+
+
+apply_ils = apply_ils_channels
+
+
+def planck_radiance(nu, T):
+    """B(nu, T) = C1 nu^3 / expm1(C2 nu / T) (planck.jl:22-24); host arithmetic."""
+    nu = np.asarray(nu, np.float64)
+    return 1.191042953e-5 * nu * nu * nu / np.expm1(1.4387769 * nu / T)
+
+
+def brightness_temperature(nu, R):
+    """Inverse Planck through the C ABI (planck.jl:43-57)."""
+    lib = load_library()
+    nu = np.ascontiguousarray(np.atleast_1d(nu), np.float64)
+    R = np.ascontiguousarray(np.atleast_1d(R), np.float64)
+    if nu.shape != R.shape:
+        raise ValueError(f"R length {len(R)} ≠ grid size {len(nu)}")
+    out = np.zeros_like(R)
+    rc = lib.irs_brightness_temperature(_p(nu), _p(R), len(R), _p(out))
+    if rc != IRS_OK:
+        raise ValueError("bad arguments")
+    return out
+
+
+def _options(internal_dnu=0.001, high_res_factor=None, cutoff=25.0, lm_cutoff=None, dptmn=1e-6,
+             x_far=122.0, with_ils=True, apodization="gaussian", line_mixing=None,
+             T_method="logp_at_pcg", source_function="cim"):
+    o = IrsOptions()
+    load_library().irs_default_options(C.byref(o))
+    o.internal_dnu = 0.0 if internal_dnu is None else float(internal_dnu)
+    o.high_res_factor = 0 if high_res_factor is None else int(high_res_factor)
+    o.cutoff = float(cutoff)
+    o.lm_cutoff = float(cutoff if lm_cutoff is None else lm_cutoff)
+    o.dptmn = float(dptmn)
+    o.x_far = float(x_far)
+    o.with_ils = int(bool(with_ils))
+    o.apodization = _apod(apodization)
+    lm = line_mixing
+    if lm in (None, False, 0, "none"):
+        o.line_mixing = 0
+    elif lm in ("vpy", "VPY", 1, True):
+        o.line_mixing = 1
+    else:
+        raise ValueError(f"unsupported line_mixing {line_mixing!r}")
+    tm = T_method.lstrip(":")
+    if tm not in T_METHOD:
+        raise ValueError(f"T_method must be :logp_at_pcg or :mass_weighted, got :{tm}")
+    o.T_method = T_METHOD[tm]
+    sf = source_function.lstrip(":")
+    if sf not in SOURCE_FUNCTION:
+        raise ValueError(f"source_function must be :toon or :cim, got :{sf}")
+    o.source_function = SOURCE_FUNCTION[sf]
+    return o
+
+
+class _Batch:
+    """ctypes view of a batch of profiles (keeps the numpy buffers alive)."""
+
+    def __init__(self, ctx, profs, linelists, T_sfcs, eps, mus):
+        species = list(profs[0].vmr.keys())
+        self.species = np.array([int(s) for s in species], np.int32)
+        self.handles = np.array([ctx.upload(linelists[s]) if s in linelists and len(linelists[s]) else -1
+                                 for s in species], np.int32)
+        self.arr = (IrsProfile * len(profs))()
+        self.keep = []
+        for q, pr in enumerate(profs):
+            if list(pr.vmr.keys()) != species:
+                raise ValueError("all profiles of a batch need the same species")
+            vmr = np.ascontiguousarray(np.stack([pr.vmr[s] for s in species]), np.float64)
+            p = np.ascontiguousarray(pr.pressure, np.float64)
+            T = np.ascontiguousarray(pr.temperature, np.float64)
+            self.keep += [vmr, p, T]
+            self.arr[q] = IrsProfile(len(p), _p(p), _p(T), len(species), _p(self.species),
+                                     _p(self.handles), _p(vmr),
+                                     float(T_sfcs[q]) if T_sfcs[q] is not None else 0.0,
+                                     float(eps[q]), float(mus[q]))
+
+
+def forward_model(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
+                  sounder: Optional[Sounder] = None, iasi: Optional[Sounder] = None,
+                  geom: Optional[ViewingGeometry] = None, T_sfc=None, eps_sfc: float = 1.0,
+                  internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
+                  cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
+                  apply_continuum: bool = False, with_ils: bool = True,
+                  apodization="gaussian", line_mixing=None, T_method="logp_at_pcg",
+                  source_function="cim", x_far: float = 122.0, ctx: Optional[Context] = None):
+    """forward_model(prof, linelists; ...) -> (nu, R, BT)   (src/Sensor/sounder.jl:233-360).
+
+    `prof` may be one AtmosphericProfile or a list of them (a batch on one
+    device launch sequence); geom, T_sfc and eps_sfc may then be lists too.
+    Continuum absorption is not part of this engine: apply_continuum=True
+    raises (the reference default is True; pass False explicitly)."""
+    if apply_continuum:
+        raise NotImplementedError("continuum absorption is not implemented in the GPU engine; "
+                                  "pass apply_continuum=False")
+    inst = iasi if iasi is not None else (sounder if sounder is not None else IASIInstrument())
+    batch = isinstance(prof, (list, tuple))
+    profs = list(prof) if batch else [prof]
+    nb = len(profs)
+    geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
+    geoms = [g if g is not None else nadir_geometry() for g in geoms]
+    tsf = T_sfc if isinstance(T_sfc, (list, tuple, np.ndarray)) else [T_sfc] * nb
+    eps = eps_sfc if isinstance(eps_sfc, (list, tuple, np.ndarray)) else [eps_sfc] * nb
+    ctx = ctx or default_context()
+    opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
+                   apodization, line_mixing, T_method, source_function)
+    b = _Batch(ctx, profs, linelists, tsf, eps, [g.mu for g in geoms])
+    snd = _sounder(inst)
+    grid = sounder_grid(inst)
+    R = np.zeros((nb, grid.n))
+    BT = np.zeros((nb, grid.n))
+    ctx.check(ctx.lib.irs_forward(ctx.h, b.arr, nb, C.byref(snd), C.byref(opt), _p(R), _p(BT)))
+    if batch:
+        return grid, R, BT
+    return grid, R[0], BT[0]
+
+
+def forward_dims(sounder: Sounder, **kw):
+    o = _options(**kw)
+    snd = _sounder(sounder)
+    lo, d, n, nch = f64(), f64(), i64(), i64()
+    rc = load_library().irs_forward_dims(C.byref(snd), C.byref(o), C.byref(lo), C.byref(d),
+                                         C.byref(n), C.byref(nch))
+    if rc != IRS_OK:
+        raise ValueError("bad sounder/options")
+    return lo.value, d.value, n.value, nch.value

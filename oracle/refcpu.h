@@ -1,0 +1,175 @@
+/*
+ * refcpu — FP64 CPU restatement of the reference's forward-model hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle and the
+ * reported CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker; the product (the HIP
+ * library under irsounderlbl.jl_amd/) never links, loads or calls it.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  The reference is Julia; it cannot run in
+ * this environment, so parity is pinned by the reference's own test
+ * invariants, closed forms and independent library golden vectors (scipy /
+ * mpmath for w(z)); see tests/test_oracle_*.py.
+ *
+ * Conventions mirror the reference: grids are uniform FP64, layer arrays are
+ * column-major (nu fastest), levels are surface-first, pressures in hPa at the
+ * profile level and atm at the line-shape level, radiance in
+ * mW m^-2 sr^-1 (cm^-1)^-1.
+ */
+#ifndef IRS_REFCPU_H
+#define IRS_REFCPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* HITRAN line list in structure-of-arrays form, sorted by unshifted nu
+ * (src/HITRAN/linelist.jl:20-56).  gamma_air/gamma_self/n_air/delta_air are
+ * FP32 exactly as in `HITRANLine`.  y_lm (may be NULL) is the first-order
+ * line-mixing coefficient Y (atm^-1) used by the VP_Y dispersive term. */
+typedef struct {
+    int64_t n;
+    const int8_t *mol;
+    const int8_t *iso;
+    const double *nu;
+    const double *S;
+    const float  *gamma_air;
+    const float  *gamma_self;
+    const double *elower;
+    const float  *n_air;
+    const float  *delta_air;
+    const double *y_lm;
+} orc_linelist;
+
+/* ---- special functions ---------------------------------------------------- */
+/* Faddeeva w(x+iy), y >= 0 (x any sign).  Midpoint-shifted trapezoidal rule
+ * with exact pole term for |x|+y < 8, Laplace continued fraction elsewhere. */
+void   orc_faddeeva(double x, double y, double *re, double *im);
+double orc_erfcx(double y);
+
+/* ---- spectroscopy (src/HITRAN/partition.jl, broadening.jl) ---------------- */
+double orc_partition(int mol, int iso, double T);
+double orc_q_ratio(int mol, int iso, double T);
+double orc_partition_deriv(int mol, int iso, double T);
+double orc_scaled_intensity(const orc_linelist *ll, int64_t j, double T);
+void   orc_broadened_width(const orc_linelist *ll, int64_t j, double p_atm, double T,
+                           double vmr_self, double *gamma_l, double *gamma_d);
+double orc_pressure_shift(const orc_linelist *ll, int64_t j, double p_atm, double vmr_self);
+
+/* ---- cross sections (src/CrossSections/voigt.jl, line_mixing.jl) --------- */
+/* DPTMIN decisions (voigt.jl:333-355): keep[j] = 1 if the line survives. */
+int64_t orc_reject_weak_lines(const orc_linelist *ll, double T, double p_atm, double coef,
+                              double vmr_self, double dptmn, uint8_t *keep);
+/* Voigt cross sections (voigt.jl:375-462) on the grid nu_i = grid_nu(i).
+ * `keep` (may be NULL) restricts the sum to the active lines. */
+void orc_voigt_xsec(const orc_linelist *ll, const uint8_t *keep,
+                    double nu0, double dnu, int64_t n_nu,
+                    double T, double p_atm, double vmr_self,
+                    double cutoff, double x_far, double *sigma);
+/* VP_Y dispersive correction (line_mixing.jl:1166-1236,1303-1333) over the
+ * lines with y_lm != 0 (no clamp; may be negative). */
+void orc_lm_dispersive(const orc_linelist *ll, double nu0, double dnu, int64_t n_nu,
+                       double T, double p_atm, double cutoff, double x_far, double *dsigma);
+/* sigma = max(max(voigt,0) + dispersive, 0)  (line_mixing.jl:1351-1369) */
+void orc_voigt_lm_xsec(const orc_linelist *ll, double nu0, double dnu, int64_t n_nu,
+                       double T, double p_atm, double cutoff, double lm_cutoff,
+                       double x_far, double *sigma);
+
+/* ---- grid (src/Utils/wavenumber_grid.jl:23-27, Julia float range) --------- */
+int64_t orc_grid_len(double a, double step, double b);
+double  orc_grid_node(double a, double step, int64_t i);
+
+/* ---- atmosphere (src/Atmosphere/layers.jl, src/Utils/interpolation.jl) ---- */
+double orc_cg_column_vmr(double v1, double v2, double p1, double p2);
+double orc_cg_pressure(double v1, double v2, double p1, double p2);
+double orc_cg_temperature_mass(double T1, double T2, double p1, double p2);
+void   orc_interp_logp(const double *p_ref, const double *x_ref, int n_ref,
+                       const double *p_tgt, int n_tgt, double *out);
+
+/* ---- solver (src/Solver/planck.jl, transmittance.jl, schwarzschild.jl) ---- */
+double orc_planck(double nu, double T);
+double orc_bt(double nu, double R);
+double orc_dbdt(double nu, double T);
+double orc_cim_correction(double s);
+double orc_lit_correction(double s);
+double orc_cim_correction_deriv(double s);
+double orc_lit_correction_deriv(double s);
+/* source_function: 0 = toon, 1 = cim.  tau is (n_nu x n_layers), nu fastest. */
+int orc_schwarzschild_rte(double nu0, double dnu, int64_t n_nu, const double *tau, int n_layers,
+                          const double *T_lev, double T_sfc, double mu, double eps,
+                          int source_function, const double *T_ave, double *out_I);
+
+/* ---- sensor (src/Sensor/ils.jl, geometry.jl) ------------------------------ */
+/* apodization: 0 gaussian, 1 NB weak, 2 NB medium, 3 NB strong.  Writes
+ * 2*n_half+1 taps into out (caller sizes it from orc_ils_half). */
+int64_t orc_ils_half(double dnu);
+int     orc_ils_kernel(double dnu, double opd_max, double fwhm, int apodization, int n_pts,
+                       double *out);
+double  orc_norton_beer(double x, int style);
+/* direct convolution evaluated at the listed grid indices only (ils.jl:131-153) */
+void    orc_apply_ils_at(const double *spec, int64_t n_nu, double dnu, const double *kern,
+                         int64_t n_ils, const int64_t *idx, int64_t n_idx, double *out);
+double  orc_scan_to_local_zenith(double scan_deg, double h_sat_km, double r_earth_km);
+
+/* ---- forward model (src/Sensor/sounder.jl:233-360) ------------------------ */
+typedef struct {
+    int n_levels;
+    const double *p_hpa;       /* [n_levels] surface first */
+    const double *T;           /* [n_levels] */
+    int n_species;
+    const int *species;        /* HITRAN molecule id per species */
+    const double *vmr;         /* [n_species][n_levels] */
+    const orc_linelist *lines; /* [n_species] (n == 0: no lines) */
+} orc_profile;
+
+typedef struct {
+    double nu_min, nu_max, dnu, opd_max, fwhm;
+} orc_sounder;
+
+typedef struct {
+    double internal_dnu;       /* used when high_res_factor <= 0 */
+    int    high_res_factor;
+    double cutoff, lm_cutoff, dptmn, x_far;
+    int    with_ils;
+    int    apodization;
+    int    line_mixing;        /* 0 none, 1 VP_Y on the CO2 list (per-line y_lm) */
+    int    T_method;           /* 0 logp_at_pcg, 1 mass_weighted */
+    int    source_function;    /* 0 toon, 1 cim */
+    double T_sfc;              /* <= 0: profile T[0] */
+    double eps_sfc;
+    double mu;
+    int    ils_direct_full;    /* 1: convolve every internal point (reference loop) */
+} orc_options;
+
+typedef struct {
+    double nu_lo, dnu;          /* internal grid */
+    int64_t n_hi;
+    int64_t n_ch;
+    double  seconds_xsec, seconds_rte, seconds_ils;
+    int64_t active_pairs;       /* sum over (layer, species) of pairs in the window */
+    int64_t active_lines;       /* sum over (layer, species) of active lines */
+} orc_forward_info;
+
+/* R, BT: [n_ch]; tau_out (may be NULL): [n_hi x n_layers]; layer_mask (may be NULL):
+ * only layers with mask[k] != 0 get line sums (sampling for timing). */
+int orc_forward(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                double *R, double *BT, double *tau_out, const uint8_t *layer_mask,
+                orc_forward_info *info);
+/* channel count and internal-grid geometry for the options */
+int orc_forward_dims(const orc_sounder *snd, const orc_options *opt, orc_forward_info *info);
+
+/* Layer properties (layers.jl:133-178) for the profile; arrays [n_layers] or
+ * [n_species][n_layers]. */
+void orc_layer_properties(const orc_profile *prof, int T_method, double *p_mid, double *T_mid,
+                          double *dp, double *vmr_cg, double *p_cg, double *T_cg,
+                          double *T_ave);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

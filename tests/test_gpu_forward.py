@@ -1,0 +1,65 @@
+"""End-to-end forward_model parity on the GPU (BT vs the FP64 oracle)."""
+import numpy as np
+import pytest
+
+import refcpu
+from irsounderlbl import synthetic as syn
+from irsounderlbl.types import GasSpecies, Sounder, filter_linelist, wavenumber_grid
+
+pytestmark = pytest.mark.gpu
+
+BT_TOL = 0.005   # K, plain LBL
+
+
+def test_rte_matches_oracle(ctx):
+    from irsounderlbl.api import schwarzschild_rte
+    rng = np.random.default_rng(3)
+    g = wavenumber_grid(645.0, 805.0, 0.01)
+    L = 7
+    tau = rng.uniform(0.0, 3.0, (g.n, L)) * rng.uniform(0, 1, (1, L))
+    T_lev = np.linspace(295, 215, L + 1)
+    T_ave = 0.5 * (T_lev[:-1] + T_lev[1:]) + 0.7
+    for sf, eps in [("cim", 1.0), ("toon", 1.0), ("cim", 0.9), ("toon", 0.95)]:
+        got = schwarzschild_rte(g, tau, T_lev, 300.0, mu=0.8, eps_sfc=eps, source_function=sf,
+                                T_ave=T_ave if sf == "cim" else None, ctx=ctx)
+        ref = refcpu.schwarzschild_rte(g.nu_min, g.dnu, tau, T_lev, 300.0, 0.8, eps,
+                                       1 if sf == "cim" else 0, T_ave)
+        assert np.max(np.abs(got - ref) / ref) < 1e-12
+
+
+def test_ils_kernel_matches_oracle(ctx):
+    from irsounderlbl.api import ils_kernel
+    for apod in range(4):
+        _, got = ils_kernel(0.001, 2.0, 0.5, apodization=apod, ctx=ctx)
+        _, ref = refcpu.ils_kernel(0.001, 2.0, 0.5, apod)
+        assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_forward_config0_bt(ctx):
+    from irsounderlbl.api import forward_model
+    prof = syn.base_profile(False)
+    ll = syn.linelist_config0()
+    snd = Sounder.make(645.0, 800.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    nu, R, BT = forward_model(prof, {GasSpecies.CO2: ll}, sounder=snd, ctx=ctx)
+    Ro, BTo, info = refcpu.forward(prof, {GasSpecies.CO2: ll}, snd)
+    d = np.abs(BT - BTo)
+    print(f"config0 BT max|d| {d.max():.2e} K rms {np.sqrt((d**2).mean()):.2e} K; "
+          f"stats {ctx.stats()}")
+    assert len(BT) == 621
+    assert d.max() < BT_TOL
+
+
+def test_forward_full_range_subset_bt(ctx):
+    """Config-1 line lists (3 species, VP_Y) on a window sub-range with a nonblack
+    surface and slant path, vs the oracle."""
+    from irsounderlbl.api import forward_model
+    lls = syn.linelists_config1()
+    prof = syn.base_profile(True)
+    snd = Sounder.make(700.0, 730.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    sub = {k: filter_linelist(v, 650.0, 780.0) for k, v in lls.items()}
+    for lm in (None, "vpy"):
+        nu, R, BT = forward_model(prof, sub, sounder=snd, line_mixing=lm, ctx=ctx)
+        Ro, BTo, info = refcpu.forward(prof, sub, snd, line_mixing=1 if lm else 0)
+        d = np.abs(BT - BTo)
+        print(f"subset lm={lm} BT max|d| {d.max():.2e} K")
+        assert d.max() < BT_TOL

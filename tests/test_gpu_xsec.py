@@ -1,0 +1,104 @@
+"""GPU line-sum parity against the FP64 oracle (cross sections, one layer)."""
+import numpy as np
+import pytest
+
+import refcpu
+from helpers import four_line_fixture, rel_err_masked
+from irsounderlbl import synthetic as syn
+from irsounderlbl.types import GasSpecies, HITRANLinelist, filter_linelist, wavenumber_grid
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4   # relative, where sigma > 1e-6 * peak
+
+
+def test_four_line_fixture_matches_oracle(ctx):
+    from irsounderlbl.api import compute_voigt_cross_sections
+    ll = four_line_fixture()
+    g = wavenumber_grid(2345.0, 2355.0, 0.001)
+    for T, p in [(230.0, 0.2), (296.0, 1.0), (200.0, 1e-4)]:
+        got = compute_voigt_cross_sections(g, ll, T, p, cutoff=25.0, ctx=ctx)
+        ref = refcpu.voigt_xsec(ll, g.nu_min, g.dnu, g.n, T, p, cutoff=25.0)
+        e, _ = rel_err_masked(got, ref)
+        print(f"4-line T={T} p={p}: max rel err {e:.2e}")
+        assert e < TOL
+        assert np.abs(got - ref).max() < 1e-5 * ref.max()
+
+
+def test_x_far_branch_live_and_lossless(ctx):
+    """runtests.jl:247-273 on the GPU path: x_far = 122 vs exact everywhere, and 0."""
+    from irsounderlbl.api import compute_voigt_cross_sections
+    ll = four_line_fixture()
+    g = wavenumber_grid(2345.0, 2355.0, 0.001)
+    exact = refcpu.voigt_xsec(ll, g.nu_min, g.dnu, g.n, 230.0, 0.2, x_far=np.inf)
+    opt_a = compute_voigt_cross_sections(g, ll, 230.0, 0.2, x_far=122.0, ctx=ctx)
+    allz = compute_voigt_cross_sections(g, ll, 230.0, 0.2, x_far=0.0, ctx=ctx)
+    peak = exact.max()
+    assert np.abs(opt_a - exact).max() < 1e-5 * peak
+    m = exact > 1e-6 * peak
+    assert (np.abs(opt_a[m] - exact[m]) / exact[m]).max() < 3e-4
+    assert np.abs(allz - exact).max() > 1e-3 * peak
+
+
+def _layers_config0():
+    prof = syn.base_profile(False)
+    lp = refcpu.layer_properties(prof)
+    return prof, lp
+
+
+@pytest.mark.parametrize("k", [0, 24, 48])
+def test_config0_layer_matches_oracle(ctx, k):
+    from irsounderlbl.api import compute_voigt_cross_sections
+    prof, lp = _layers_config0()
+    ll = syn.linelist_config0()
+    T = lp["T_cg"][GasSpecies.CO2][k]
+    p = lp["p_cg"][GasSpecies.CO2][k] / 1013.25
+    g = wavenumber_grid(629.0, 816.0, 0.001)
+    got = compute_voigt_cross_sections(g, ll, T, p, ctx=ctx)
+    ref = refcpu.voigt_xsec(ll, g.nu_min, g.dnu, g.n, T, p)
+    e, i = rel_err_masked(got, ref)
+    print(f"layer {k} T={T:.2f} p={p:.3e}: max rel err {e:.2e} at {g.nu_min + i * g.dnu:.3f}")
+    assert e < TOL
+
+
+def test_dptmin_live_matches_oracle(ctx):
+    from irsounderlbl.api import compute_voigt_cross_sections
+    prof, lp = _layers_config0()
+    ll = syn.linelist_config0()
+    k = 45
+    T = lp["T_cg"][GasSpecies.CO2][k]
+    p = lp["p_cg"][GasSpecies.CO2][k] / 1013.25
+    coef = lp["vmr_cg"][GasSpecies.CO2][k] * lp["dp"][k] * 2.1209e22
+    keep = refcpu.reject_weak_lines(ll, T, p, coef, dptmn=1e-6)
+    assert 0 < keep.sum() < len(ll)
+    g = wavenumber_grid(629.0, 816.0, 0.001)
+    got = compute_voigt_cross_sections(g, ll, T, p, dptmn=1e-6, coef=coef, ctx=ctx)
+    ref = refcpu.voigt_xsec(ll, g.nu_min, g.dnu, g.n, T, p, keep=keep)
+    e, _ = rel_err_masked(got, ref)
+    print(f"DPTMIN layer {k}: kept {keep.sum()}/{len(ll)}, max rel err {e:.2e}")
+    assert e < TOL
+
+
+def test_vpy_line_mixing_matches_oracle(ctx):
+    from irsounderlbl.api import compute_voigt_lm_cross_sections
+    lls = syn.linelists_config1()
+    co2 = filter_linelist(lls[GasSpecies.CO2], 680.0, 720.0)
+    assert co2.y_lm is not None and np.count_nonzero(co2.y_lm) > 1000
+    g = wavenumber_grid(695.0, 705.0, 0.001)
+    for T, p in [(288.0, 0.9), (217.0, 0.05), (217.0, 1e-4)]:
+        got = compute_voigt_lm_cross_sections(g, co2, T, p, ctx=ctx)
+        ref = refcpu.voigt_lm_xsec(co2, g.nu_min, g.dnu, g.n, T, p)
+        e, _ = rel_err_masked(got, ref)
+        print(f"VP_Y T={T} p={p}: max rel err {e:.2e}")
+        assert e < TOL
+
+
+def test_empty_and_single_line(ctx):
+    from irsounderlbl.api import compute_voigt_cross_sections
+    g = wavenumber_grid(700.0, 701.0, 0.001)
+    one = HITRANLinelist(1, 1, [700.5], [1e-20], [0.07], [0.3], [100.0], [0.7], [-0.001])
+    got = compute_voigt_cross_sections(g, one, 250.0, 0.5, ctx=ctx)
+    ref = refcpu.voigt_xsec(one, g.nu_min, g.dnu, g.n, 250.0, 0.5, vmr_self=0.0)
+    assert rel_err_masked(got, ref)[0] < TOL
+    far = HITRANLinelist(1, 1, [900.0], [1e-20], [0.07], [0.3], [100.0], [0.7], [0.0])
+    assert np.all(compute_voigt_cross_sections(g, far, 250.0, 0.5, ctx=ctx) == 0.0)
