@@ -1,0 +1,162 @@
+"""Benchmark: full-range IASI spectra/s on one MI355X (plain FP32 LBL).
+
+Workload: one 50-level profile (49 layers) and 300,000 synthetic lines in
+620–2785 cm^-1 (CO2-like 180k with first-order line mixing on the 15 um half,
+H2O-like 100k, O3-like 20k), internal grid 0.001 cm^-1 (2,147,001 points),
+IASI 645–2760 cm^-1 / 0.25 cm^-1 -> 8461 channels, Gaussian-apodized ILS.
+A step is one forward_model call for one profile with the line lists already
+resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-accuracy]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "irsounderlbl.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 peak
+FLOP_PER_VOIGT_PAIR = 14       # nominal, far-wing expression of the reference kernel
+FLOP_PER_DISP_PAIR = 9
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(prof, lls, snd, layer):
+    """The FP64 oracle on the host: one layer's line sum (all species) plus the
+    full RTE and channel ILS, scaled to one spectrum."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcpu
+    L = prof.n_levels - 1
+    mask = np.zeros(L, np.uint8)
+    mask[layer] = 1
+    t0 = time.perf_counter()
+    _, _, info = refcpu.forward(prof, lls, snd, layer_mask=mask, line_mixing=1)
+    wall = time.perf_counter() - t0
+    per_spec = info.seconds_xsec * L + info.seconds_rte + info.seconds_ils
+    return {"value": 1.0 / per_spec, "unit": "spectra/s", "cores": refcpu.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle, layer {layer} of {L} line sums (all 3 species, VP_Y) + full "
+                      f"RTE + channel ILS; {wall:.1f} s wall; per-spectrum time = layer time x {L} + "
+                      f"RTE + ILS"}
+
+
+def accuracy_sample(ctx, prof, lls):
+    """Errors vs the FP64 oracle on bounded samples of the same workload: BT on a
+    10 cm^-1 channel window (1205-1215 cm^-1), and the CO2 cross section with
+    line mixing of layer 24 on a 10 cm^-1 grid window in the 15 um band."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcpu
+    from irsounderlbl.api import compute_voigt_lm_cross_sections, forward_model
+    from irsounderlbl.types import GasSpecies, Sounder, filter_linelist, wavenumber_grid
+    out = {}
+    lo, hi = 1205.0, 1215.0
+    snd = Sounder.make(lo, hi, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls.items()}
+    _, _, bt = forward_model(prof, sub, sounder=snd, line_mixing="vpy", ctx=ctx)
+    _, bto, _ = refcpu.forward(prof, sub, snd, line_mixing=1)
+    d = np.abs(bt - bto)
+    out["bt_1205-1215"] = {"max_abs_K": float(d.max()), "rms_K": float(np.sqrt(np.mean(d * d)))}
+    lp = refcpu.layer_properties(prof)
+    T = lp["T_cg"][GasSpecies.CO2][24]
+    p = lp["p_cg"][GasSpecies.CO2][24] / 1013.25
+    g = wavenumber_grid(700.0, 710.0, 0.001)
+    co2 = filter_linelist(lls[GasSpecies.CO2], 674.0, 736.0)
+    got = compute_voigt_lm_cross_sections(g, co2, T, p, ctx=ctx)
+    ref = refcpu.voigt_lm_xsec(co2, g.nu_min, g.dnu, g.n, T, p)
+    m = ref > 1e-6 * ref.max()
+    out["sigma_co2_layer24_700-710"] = {"max_rel": float(np.max(np.abs(got[m] - ref[m]) / ref[m]))}
+    return out
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "linesum_full_iasi_pmc.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this engine shards profiles across GPUs with no "
+              "exchange, and every box here has one GPU; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from irsounderlbl import synthetic as syn
+    from irsounderlbl.api import Context, forward_model
+    from irsounderlbl.types import GasSpecies, IASIInstrument
+
+    log("bench: building workload (300k lines, 50 levels)")
+    lls = syn.linelists_config1()
+    prof = syn.base_profile(True)
+    snd = IASIInstrument()
+    ctx = Context(0)
+    for ll in lls.values():
+        ctx.upload(ll)
+
+    def step():
+        return forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx)
+
+    for i in range(args.warmup):
+        step()
+        log(f"bench: warmup {i + 1}/{args.warmup} done")
+    import ctypes
+    ls_ms = []
+    stats = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        nu, R, BT = step()
+        st = ctx.stats()
+        ls_ms.append(st["ms_linesum"])
+        stats = st
+    t1 = time.perf_counter()
+    log(f"bench: {args.steps} steps in {t1 - t0:.3f} s")
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = 1e3 / ms_step
+    flops = FLOP_PER_VOIGT_PAIR * stats["active_pairs"] + FLOP_PER_DISP_PAIR * stats["dispersive_pairs"]
+    ls_avg = float(np.mean(ls_ms))
+    achieved = flops / (ls_avg * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": traffic_from_profiles(),
+            "kernel": "linesum_kernel<8>", "flop_per_launch": flops,
+            "launch_ms_avg": round(ls_avg, 3),
+            "pairs": {"voigt": stats["active_pairs"], "dispersive": stats["dispersive_pairs"]}}
+    out = {"metric": "IASI spectra/s on 1 GPU (LBL vs emulator) at BT err ≤0.005/0.05 K vs FP64",
+           "value": round(value, 4), "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (seeded line lists and profile, see irsounderlbl/synthetic.py)",
+           "config": {"workload": "full-range IASI, 1 profile, 300k lines (3 species, VP_Y), "
+                                  "49 layers, 2147001-point grid, 8461 channels",
+                      "path": "plain FP32 LBL", "profiles_per_step": 1},
+           "roofline": roof,
+           "kernels_ms": {"prep": round(stats["ms_prep"], 3), "linesum": round(ls_avg, 3),
+                          "rte": round(stats["ms_rte"], 3), "ils": round(stats["ms_ils"], 3)}}
+    if not args.no_accuracy:
+        log("bench: accuracy sample vs FP64 oracle")
+        out["accuracy_vs_fp64"] = accuracy_sample(ctx, prof, lls)
+    if not args.no_cpu:
+        log("bench: cpu baseline (oracle, one layer)")
+        out["cpu_baseline"] = cpu_baseline(prof, lls, snd, layer=24)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
