@@ -68,7 +68,7 @@ struct irs_ctx {
     DevBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3, d_tmp4, d_tmp5, d_chidx, d_chw, d_chnu, d_rch, d_bt;
     std::map<TapsKey, DevBuf> taps;
     irs_stats stats{};
-    int ls_P = 8;
+    int ls_P = 4;
 };
 
 namespace {

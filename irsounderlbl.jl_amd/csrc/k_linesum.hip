@@ -13,15 +13,15 @@
 // one (profile, layer); thread t owns points i0 + t + 256 p, so each wave
 // covers 64 consecutive points per p (coalesced stores, coherent branches).
 // Lines are read with wave-uniform indices (scalar loads, SGPR operands).  Per
-// species, the lines near the tile are split by five binary searches on the
+// species, the lines near the tile are split by six binary searches on the
 // sorted unshifted-centre keys into
 //   EDGE  : window boundary crosses the tile   -> per-pair window test + pedestal
 //   FAST  : far wing for every point of tile   -> A/(du^2+g^2): 3 VALU + 1 rcp
-//   CORE  : may have |x| <= x_far in the tile  -> exact w(z) (FP32) or far wing
-// The FAST lines' pedestal, a quadratic in the grid offset, is summed per
-// tile in FP64 (three moments) instead of per pair.  FAST partial sums run
-// over 32-line chunks and are folded into Kahan-compensated accumulators;
-// CORE/EDGE terms are Kahan-added one by one.  Offsets are formed from the
+//   CORE  : may have |x| <= x_far in the tile  -> far wing, or exact FP32 w(z)
+// The interior (FAST + CORE) lines' pedestal, a quadratic in the grid offset,
+// is summed per tile in FP64 (three moments) instead of per pair.  Partial
+// sums run over 16- or 32-line chunks and are folded into Kahan-compensated
+// FP32 accumulators, combined in FP64 before the clamp.  Offsets are formed from the
 // integer grid index and the line centre split into an integer index plus an
 // FP32 fraction (centre computed in FP64), so du carries one FP32 rounding.
 #include "irs_devmath.h"
@@ -55,26 +55,126 @@ __device__ __forceinline__ int upper_key(const int32_t *k, int n, int64_t v) {
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // FAST: far wing, window covers the whole tile, pedestal handled per tile.
+// Line records stream through LDS in chunks of LS_CHUNK lines: each thread
+// loads LS_CHUNK/256 records with coalesced global loads one chunk ahead
+// (double buffer) and stages {c, g^2, A, B} with c the line centre relative to
+// the tile origin in grid steps, formed as float(cn - i0) + frac (one FP32
+// rounding: |error| <= ulp(|c|)/2, i.e. ~1e-7 of |du| for every FAST pair,
+// since |du| >= |c| - tile width and |du| > core half-width).  Every wave then
+// reads the chunk with broadcast LDS reads; a pair costs sub, fma, rcp, fma
+// (+ mul, fma for the dispersive term).  Partial sums run over 32 lines and
+// are folded into the Kahan accumulators.
+#define LS_CHUNK 512
+#define LS_PER_THREAD (LS_CHUNK / IRS_LS_THREADS)
+
+template <int P, bool LM>
+__device__ __forceinline__ void fast_line(const float4 q, const float (&tp)[P], float (&part)[P],
+                                          float (&partd)[P]) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const float du = tp[p] - q.x;
+        const float r = irs_rcpf(fmaf(du, du, q.y));
+        part[p] = fmaf(q.z, r, part[p]);
+        if (LM) partd[p] = fmaf(q.w * du, r, partd[p]);
+    }
+}
+
 template <int P, bool LM>
 __device__ __forceinline__ void fast_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
-                                           int jlo, int jhi, int tl, float (&acc)[P], float (&cmp)[P],
-                                           float (&accd)[P], float (&cmpd)[P]) {
-    for (int j = jlo; j < jhi; j += 32) {
-        const int je = min(j + 32, jhi);
+                                           int jlo, int jhi, int t, int i0, float4 *s_rec,
+                                           float (&acc)[P], float (&cmp)[P], float (&accd)[P],
+                                           float (&cmpd)[P]) {
+    const int n = jhi - jlo;
+    if (n <= 0) return;
+    const int nch = (n + LS_CHUNK - 1) / LS_CHUNK;
+    float tp[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) tp[p] = (float)(t + 256 * p);
+    float4 g[LS_PER_THREAD];
+    float gb[LS_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < LS_PER_THREAD; ++u) {
+        const int j = jlo + t + u * IRS_LS_THREADS;
+        g[u] = j < jhi ? r0p[j] : make_float4(__int_as_float(i0 + 1000000), 0.0f, 1.0f, 0.0f);
+        gb[u] = (LM && j < jhi) ? r1p[j].x : 0.0f;
+    }
+    for (int c = 0; c < nch; ++c) {
+        float4 *buf = s_rec + (c & 1) * LS_CHUNK;
+#pragma unroll
+        for (int u = 0; u < LS_PER_THREAD; ++u)
+            buf[t + u * IRS_LS_THREADS] = make_float4((float)(__float_as_int(g[u].x) - i0) + g[u].y, g[u].z,
+                                                      g[u].w, gb[u]);
+        __syncthreads();
+        if (c + 1 < nch) {
+#pragma unroll
+            for (int u = 0; u < LS_PER_THREAD; ++u) {
+                const int j = jlo + (c + 1) * LS_CHUNK + t + u * IRS_LS_THREADS;
+                g[u] = j < jhi ? r0p[j] : make_float4(__int_as_float(i0 + 1000000), 0.0f, 1.0f, 0.0f);
+                gb[u] = (LM && j < jhi) ? r1p[j].x : 0.0f;
+            }
+        }
+        const int cnt = min(LS_CHUNK, n - c * LS_CHUNK);
+        for (int k0 = 0; k0 < cnt; k0 += 32) {
+            const int ke = min(k0 + 32, cnt);
+            float part[P], partd[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
+            int k = k0;
+            for (; k + 2 <= ke; k += 2) {
+                const float4 qa = buf[k], qb = buf[k + 1];
+                fast_line<P, LM>(qa, tp, part, partd);
+                fast_line<P, LM>(qb, tp, part, partd);
+            }
+            for (; k < ke; ++k) fast_line<P, LM>(buf[k], tp, part, partd);
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                kahan_add(acc[p], cmp[p], part[p]);
+                if (LM) kahan_add(accd[p], cmpd[p], partd[p]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// CORE: interior lines whose Doppler core (|x| <= x_far) may fall in the tile.
+// The far-wing value is used unless a pair is in the core, where w(z) is
+// evaluated exactly; the branch is taken only by waves that hold a core pair.
+// The pedestal of these lines is part of the per-tile moments.
+template <int P, bool LM>
+__device__ __forceinline__ void core_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+                                           const float4 *__restrict__ r2p, int jlo, int jhi, int tl,
+                                           float xfar, float (&acc)[P], float (&cmp)[P], float (&accd)[P],
+                                           float (&cmpd)[P]) {
+    for (int j0 = jlo; j0 < jhi; j0 += 16) {
+        const int je = min(j0 + 16, jhi);
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
-        for (int jj = j; jj < je; ++jj) {
-            const float4 q = r0p[jj];
-            const float du0 = (float)(tl - __float_as_int(q.x)) - q.y;
-            float B = 0.0f;
-            if (LM) B = r1p[jj].x;
-#pragma unroll
+        for (int j = j0; j < je; ++j) {
+            const float4 q0 = r0p[j];
+            const float4 q1 = r1p[j];
+            const float4 q2 = r2p[j];
+            const float base = (float)(tl - __float_as_int(q0.x));
+#pragma unroll 1
             for (int p = 0; p < P; ++p) {
-                const float du = du0 + (float)(256 * p);
-                const float r = irs_rcpf(fmaf(du, du, q.z));
-                part[p] = fmaf(q.w, r, part[p]);
-                if (LM) partd[p] = fmaf(B * du, r, partd[p]);
+                const float du = (base + (float)(256 * p)) - q0.y;
+                const float r = irs_rcpf(fmaf(du, du, q0.z));
+                float h = q0.w * r;
+                if (fabsf(du * q1.z) <= xfar) {
+                    float wr, wi;
+                    w32(du * q1.z, q1.w, wr, wi);
+                    h = q2.x * wr;
+                }
+                part[p] += h;
+                if (LM) {
+                    float dd = q1.x * du * r;
+                    if (fabsf(du * q2.y) <= xfar) {
+                        float wr, wi;
+                        w32(du * q2.y, q2.z, wr, wi);
+                        dd = q2.w * wi;
+                    }
+                    partd[p] += dd;
+                }
             }
         }
 #pragma unroll
@@ -91,23 +191,35 @@ __device__ __forceinline__ void edge_range(const float4 *__restrict__ r0p, const
                                            int jlo, int jhi, int tl, float C2, float C2lm, float iC2,
                                            float (&acc)[P], float (&cmp)[P], float (&accd)[P],
                                            float (&cmpd)[P]) {
-    for (int j = jlo; j < jhi; ++j) {
-        const float4 q0 = r0p[j];
-        const float4 q1 = r1p[j];
-        const float base = (float)(tl - __float_as_int(q0.x));
+    for (int j0 = jlo; j0 < jhi; j0 += 16) {
+        const int je = min(j0 + 16, jhi);
+        float part[P], partd[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
+        for (int j = j0; j < je; ++j) {
+            const float4 q0 = r0p[j];
+            const float4 q1 = r1p[j];
+            const float base = (float)(tl - __float_as_int(q0.x));
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float du = (base + (float)(256 * p)) - q0.y;
+                const float d2 = du * du;
+                const float r = irs_rcpf(d2 + q0.z);
+                const float v = fmaf(q0.w, r, -q1.y * fmaf(-d2, iC2, 2.0f));
+                part[p] += d2 <= C2 ? v : 0.0f;
+                if (LM) partd[p] += d2 <= C2lm ? q1.x * du * r : 0.0f;
+            }
+        }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const float du = (base + (float)(256 * p)) - q0.y;
-            const float d2 = du * du;
-            const float r = irs_rcpf(d2 + q0.z);
-            const float v = fmaf(q0.w, r, -q1.y * fmaf(-d2, iC2, 2.0f));
-            if (d2 <= C2) kahan_add(acc[p], cmp[p], v);
-            if (LM && d2 <= C2lm) kahan_add(accd[p], cmpd[p], q1.x * du * r);
+            kahan_add(acc[p], cmp[p], part[p]);
+            if (LM) kahan_add(accd[p], cmpd[p], partd[p]);
         }
     }
 }
 
-// CORE (and the degenerate small-cutoff case): exact branch selection per pair.
+// Degenerate small-cutoff case (window narrower than tile + core): exact
+// branch, window test and pedestal for every pair.
 template <int P, bool LM>
 __device__ __forceinline__ void general_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
                                               const float4 *__restrict__ r2p, int jlo, int jhi, int tl,
@@ -160,7 +272,8 @@ __device__ __forceinline__ double block_sum(double v, double *red, int t) {
 
 template <int P, bool LM>
 __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev &it, int t, int64_t i0,
-                                             int64_t i1, float (&tau)[P], int *s_rng, double *s_red) {
+                                             int64_t i1, float (&tau)[P], int *s_rng, double *s_red,
+                                             float4 *s_rec) {
     const int32_t *keys = a.keys + it.key_off;
     const int n = it.n;
     const double dnu = a.dnu;
@@ -212,24 +325,21 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
 
     double q0 = 0.0, q1 = 0.0, q2 = 0.0;
     if (nondeg) {
-        // pedestal moments of the FAST lines about the tile centre (FP64)
-        for (int rgi = 0; rgi < 2; ++rgi) {
-            const int lo = rgi == 0 ? jB : jD, hi = rgi == 0 ? jC : jE;
-            for (int j = lo + t; j < hi; j += IRS_LS_THREADS) {
-                const float4 r0 = r0p[j];
-                const double pj = (double)r1p[j].y;
-                const double w = (double)((int64_t)__float_as_int(r0.x) - imid) + (double)r0.y;
-                q0 += pj;
-                q1 += pj * w;
-                q2 += pj * w * w;
-            }
+        // pedestal moments of the interior (FAST + CORE) lines about the tile centre, FP64
+        for (int j = jB + t; j < jE; j += IRS_LS_THREADS) {
+            const float4 r0 = r0p[j];
+            const double pj = (double)r1p[j].y;
+            const double w = (double)((int64_t)__float_as_int(r0.x) - imid) + (double)r0.y;
+            q0 += pj;
+            q1 += pj * w;
+            q2 += pj * w * w;
         }
         q0 = block_sum(q0, s_red, t);
         q1 = block_sum(q1, s_red + 4, t);
         q2 = block_sum(q2, s_red + 8, t);
-        fast_range<P, LM>(r0p, r1p, jB, jC, tl, acc, cmp, accd, cmpd);
-        general_range<P, LM>(r0p, r1p, r2p, jC, jD, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
-        fast_range<P, LM>(r0p, r1p, jD, jE, tl, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jB, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
+        core_range<P, LM>(r0p, r1p, r2p, jC, jD, tl, xfar, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jD, jE, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         edge_range<P, LM>(r0p, r1p, jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         edge_range<P, LM>(r0p, r1p, jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
     } else {
@@ -253,6 +363,7 @@ template <int P>
 __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumArgs a) {
     __shared__ int s_rng[8];
     __shared__ double s_red[12];
+    __shared__ float4 s_rec[2 * LS_CHUNK];
     const int t = threadIdx.x;
     const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
     const int layer = blockIdx.y, prof = blockIdx.z;
@@ -265,8 +376,8 @@ __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumAr
     for (int s = 0; s < a.n_species; ++s) {
         const ItemDev it = items[s];
         if (!it.active || it.n == 0) continue;
-        if (it.lm) species_tile<P, true>(a, it, t, i0, i1, tau, s_rng, s_red);
-        else species_tile<P, false>(a, it, t, i0, i1, tau, s_rng, s_red);
+        if (it.lm) species_tile<P, true>(a, it, t, i0, i1, tau, s_rng, s_red, s_rec);
+        else species_tile<P, false>(a, it, t, i0, i1, tau, s_rng, s_red, s_rec);
     }
     float *out = a.out + ((int64_t)prof * a.n_layers + layer) * a.n_hi;
 #pragma unroll
