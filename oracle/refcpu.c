@@ -871,3 +871,441 @@ int orc_forward(const orc_profile *prof, const orc_sounder *snd, const orc_optio
     free(p_mid); free(T_mid); free(dp); free(T_ave); free(vc); free(pc); free(tc);
     return rc;
 }
+
+/* ========================================================================== *
+ * Jacobians
+ * ========================================================================== */
+
+/* broadening.jl:112-128 */
+static void scaled_intensity_deriv(const orc_linelist *ll, int64_t j, double T, double *S, double *dS) {
+    double nu0 = ll->nu[j], e = ll->elower[j];
+    int mol = ll->mol[j], iso = ll->iso[j];
+    double qr = orc_q_ratio(mol, iso, T);
+    double qt = orc_partition(mol, iso, T);
+    double dqt = orc_partition_deriv(mol, iso, T);
+    double stim_num = 1.0 - exp(-C2_LINE * nu0 / T);
+    double stim = stim_num / (1.0 - exp(-C2_LINE * nu0 / T_REF));
+    double boltz = exp(-C2_LINE * e * (1.0 / T - 1.0 / T_REF));
+    double s = ll->S[j] * qr * boltz * stim;
+    double a = C2_LINE * nu0;
+    double dln = -dqt / qt + C2_LINE * e / (T * T) - (a / (T * T)) / (exp(a / T) - 1.0);
+    *S = s;
+    *dS = s * dln;
+}
+
+/* broadening.jl:62-78 */
+static void width_deriv(const orc_linelist *ll, int64_t j, double p, double T, double vs,
+                        double *gl, double *gd, double *dgl, double *dgd, double *acoef, double *bcoef) {
+    int mol = ll->mol[j];
+    double n_air = (double)ll->n_air[j];
+    double n_self = n_self_of(mol, n_air);
+    double tr = T_REF / T;
+    double A = pow(tr, n_air) * (double)ll->gamma_air[j] * (1.0 - vs) * p;
+    double B = pow(tr, n_self) * (double)ll->gamma_self[j] * vs * p;
+    *gl = A + B;
+    *dgl = -(n_air * A + n_self * B) / T;
+    *gd = ll->nu[j] * 3.58126e-7 * sqrt(T / mol_mass(mol));
+    *dgd = *gd / (2.0 * T);
+    *acoef = pow(tr, n_air) * (double)ll->gamma_air[j];
+    *bcoef = pow(tr, n_self) * (double)ll->gamma_self[j];
+}
+
+/* cross_section_jacobian.jl:43-56 */
+static void voigt_H_grad(double x, double y, double x_far, double *H, double *Hx, double *Hy) {
+    if (fabs(x) > x_far) {
+        double d = x * x + y * y, d2 = d * d;
+        *H = y * INV_SQRT_PI / d;
+        *Hx = -2.0 * x * y * INV_SQRT_PI / d2;
+        *Hy = INV_SQRT_PI * (x * x - y * y) / d2;
+        return;
+    }
+    double K, Lw;
+    orc_faddeeva(x, y, &K, &Lw);
+    *H = K;
+    *Hx = -2.0 * x * K + 2.0 * y * Lw;
+    *Hy = 2.0 * (x * Lw + y * K) - 2.0 * INV_SQRT_PI;
+}
+
+/* cross_section_jacobian.jl:68-114 + 155-240 */
+static void voigt_xsec_grad(const orc_linelist *ll, double nu0g, double dnu, int64_t n_nu, double T,
+                            double p, double vs, double cutoff, double x_far, double *sig, double *dT,
+                            double *dp, double *dv) {
+    int64_t n = ll->n;
+    size_t sz = sizeof(double) * (n ? n : 1);
+    double *c0 = malloc(sz), *fa = malloc(sz), *dfa = malloc(sz), *ya = malloc(sz), *dyT = malloc(sz);
+    double *dyp = malloc(sz), *dyv = malloc(sz), *d0p = malloc(sz), *d0v = malloc(sz), *sn = malloc(sz);
+    double *dsnT = malloc(sz), *hc = malloc(sz), *dhcT = malloc(sz), *dhcp = malloc(sz), *dhcv = malloc(sz);
+    for (int64_t j = 0; j < n; ++j) {
+        int mol = ll->mol[j];
+        double dair = (double)ll->delta_air[j], dself = delta_self_of(mol);
+        c0[j] = orc_pressure_shift(ll, j, p, vs);
+        d0p[j] = dair * (1.0 - vs) + dself * vs;
+        d0v[j] = (dself - dair) * p;
+        double S, dS, gl, gd, dgl, dgd, ac, bc;
+        scaled_intensity_deriv(ll, j, T, &S, &dS);
+        width_deriv(ll, j, p, T, vs, &gl, &gd, &dgl, &dgd, &ac, &bc);
+        gd = fmax(gd, 1e-10);
+        double f = SQRT_LN2 / gd;
+        double df = -f * (dgd / gd);
+        double dglp = gl / p;
+        double dglv = (bc - ac) * p;
+        fa[j] = f; dfa[j] = df; ya[j] = gl * f;
+        dyT[j] = dgl * f + gl * df;
+        dyp[j] = dglp * f;
+        dyv[j] = dglv * f;
+        sn[j] = S * f * INV_SQRT_PI;
+        dsnT[j] = (dS * f + S * df) * INV_SQRT_PI;
+        double Hv, Hx, Hy;
+        voigt_H_grad(cutoff * f, ya[j], x_far, &Hv, &Hx, &Hy);
+        hc[j] = Hv;
+        dhcT[j] = Hx * (cutoff * df) + Hy * dyT[j];
+        dhcp[j] = Hy * dyp[j];
+        dhcv[j] = Hy * dyv[j];
+    }
+    const grid_t gg = make_grid(nu0g, dnu);
+    const double invc = 1.0 / cutoff, invc2 = invc * invc;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n_nu; ++i) {
+        double nu = grid_at(&gg, i);
+        int64_t jlo = lower_bound(c0, nu - cutoff, n);
+        int64_t jhi = upper_bound(c0, nu + cutoff, n);
+        double acc = 0, aT = 0, ap = 0, av = 0;
+        for (int64_t j = jlo; j <= jhi; ++j) {
+            double dv_ = nu - c0[j], f = fa[j], x = dv_ * f;
+            double H, Hx, Hy;
+            voigt_H_grad(x, ya[j], x_far, &H, &Hx, &Hy);
+            double dHt = Hx * (dv_ * dfa[j]) + Hy * dyT[j];
+            double dHp = Hx * (-d0p[j] * f) + Hy * dyp[j];
+            double dHv = Hx * (-d0v[j] * f) + Hy * dyv[j];
+            double pedfac = 2.0 - (dv_ * invc) * (dv_ * invc);
+            double dpf_p = 2.0 * dv_ * d0p[j] * invc2;
+            double dpf_v = 2.0 * dv_ * d0v[j] * invc2;
+            double dpedT = pedfac * dhcT[j];
+            double dpedp = dpf_p * hc[j] + pedfac * dhcp[j];
+            double dpedv = dpf_v * hc[j] + pedfac * dhcv[j];
+            double hm = H - pedfac * hc[j];
+            acc += sn[j] * hm;
+            aT += dsnT[j] * hm + sn[j] * (dHt - dpedT);
+            ap += sn[j] * (dHp - dpedp);
+            av += sn[j] * (dHv - dpedv);
+        }
+        if (acc > 0.0) { sig[i] = acc; dT[i] = aT; dp[i] = ap; dv[i] = av; }
+        else { sig[i] = 0; dT[i] = 0; dp[i] = 0; dv[i] = 0; }
+    }
+    free(c0); free(fa); free(dfa); free(ya); free(dyT); free(dyp); free(dyv); free(d0p); free(d0v);
+    free(sn); free(dsnT); free(hc); free(dhcT); free(dhcp); free(dhcv);
+}
+
+void orc_species_grad(const orc_linelist *ll, int line_mixing, double nu0g, double dnu, int64_t n_nu,
+                      double T, double p, double vs, double cutoff, double lm_cutoff, double x_far,
+                      double *sig, double *dT, double *dp, double *dv) {
+    int lm = line_mixing && ll->y_lm && ll->mol[0] == 2;
+    voigt_xsec_grad(ll, nu0g, dnu, n_nu, T, p, lm ? 0.0 : vs, cutoff, x_far, sig, dT, dp, dv);
+    if (!lm) return;
+    /* cross_section_jacobian.jl:276-322: central differences of the dispersive term */
+    double *d0 = malloc(sizeof(double) * n_nu), *da = malloc(sizeof(double) * n_nu),
+           *db = malloc(sizeof(double) * n_nu);
+    orc_lm_dispersive(ll, nu0g, dnu, n_nu, T, p, lm_cutoff, x_far, d0);
+    const double hT = 0.02;
+    orc_lm_dispersive(ll, nu0g, dnu, n_nu, T + hT, p, lm_cutoff, x_far, da);
+    orc_lm_dispersive(ll, nu0g, dnu, n_nu, T - hT, p, lm_cutoff, x_far, db);
+    for (int64_t i = 0; i < n_nu; ++i) dT[i] += (da[i] - db[i]) / (2.0 * hT);
+    const double hp = fmax(p, 1e-6) * 1e-4;
+    orc_lm_dispersive(ll, nu0g, dnu, n_nu, T, p + hp, lm_cutoff, x_far, da);
+    orc_lm_dispersive(ll, nu0g, dnu, n_nu, T, p - hp, lm_cutoff, x_far, db);
+    for (int64_t i = 0; i < n_nu; ++i) {
+        dp[i] += (da[i] - db[i]) / (2.0 * hp);
+        sig[i] += d0[i];
+        if (sig[i] <= 0.0) { sig[i] = 0; dT[i] = 0; dp[i] = 0; }
+    }
+    free(d0); free(da); free(db);
+}
+
+/* schwarzschild_jacobian.jl:47-53 */
+double orc_cg_mass_frac(double p1, double p2) {
+    if (approx_eq(p1, p2)) return 0.5;
+    double a = log(p2 / p1);
+    if (fabs(a) < 1e-3) return 0.5 + a / 12.0;
+    double b = p2 / p1;
+    return ((a - 1.0) * b + 1.0) / (a * (b - 1.0));
+}
+
+/* vmr_jacobian.jl:81-107 */
+void orc_cg_column_vmr_grad(double v1, double v2, double p1, double p2, double *d1, double *d2) {
+    if (v1 <= 0.0 || v2 <= 0.0) { *d1 = 0.5; *d2 = 0.5; return; }
+    if (approx_eq(v1, v2)) { double fr = orc_cg_mass_frac(p1, p2); *d1 = 1.0 - fr; *d2 = fr; return; }
+    double Lg = log(p2 / p1), al = log(v2 / v1) / Lg, be = al + 1.0, dp = fabs(p1 - p2);
+    if (fabs(be) < 1e-6) {
+        double h1 = fmax(fabs(v1), 1e-300) * 1e-6, h2 = fmax(fabs(v2), 1e-300) * 1e-6;
+        *d1 = (orc_cg_column_vmr(v1 + h1, v2, p1, p2) - orc_cg_column_vmr(v1 - h1, v2, p1, p2)) / (2 * h1);
+        *d2 = (orc_cg_column_vmr(v1, v2 + h2, p1, p2) - orc_cg_column_vmr(v1, v2 - h2, p1, p2)) / (2 * h2);
+        return;
+    }
+    double r = p2 / p1, rb = pow(r, be);
+    double c = v1 * p1 * (1.0 - rb) / (be * dp);
+    double J = (v1 * p1 / Lg) * ((rb - 1.0) / (be * be) - rb * Lg / be);
+    *d1 = c / v1 - J / (v1 * dp);
+    *d2 = J / (v2 * dp);
+}
+
+/* vmr_jacobian.jl:120-125 */
+double orc_dbt_dr(double nu, double R) {
+    if (R <= 0.0) return 0.0;
+    double A = C1_PL * nu * nu * nu, Lg = log1p(A / R);
+    return C2_PL * nu * A / (Lg * Lg * R * (R + A));
+}
+
+int orc_rte_jacobian(double nu0g, double dnu, int64_t n_nu, const double *tau, int L,
+                     const double *T_lev, const double *p_lev, double T_sfc, double mu, double eps,
+                     int sf, double *Iout, double *dtau, double *dTlev, double *dTsfc, double *deps) {
+    if (!(mu > 0.0 && mu <= 1.0)) return -1;
+    if (sf != 0 && sf != 1) return -2;
+    if (sf == 1 && !p_lev) return -3;
+    double *Tave = malloc(sizeof(double) * L), *fr = malloc(sizeof(double) * L);
+    for (int k = 0; k < L; ++k) {
+        if (sf == 1) {
+            Tave[k] = orc_cg_temperature_mass(T_lev[k], T_lev[k + 1], p_lev[k], p_lev[k + 1]);
+            fr[k] = orc_cg_mass_frac(p_lev[k], p_lev[k + 1]);
+        } else { Tave[k] = 0; fr[k] = 0; }
+    }
+    memset(dTlev, 0, sizeof(double) * (size_t)(L + 1) * n_nu);
+    const grid_t gg = make_grid(nu0g, dnu);
+    const double invmu = 1.0 / mu, refl = 1.0 - eps;
+#pragma omp parallel
+    {
+        double *tr = malloc(sizeof(double) * (L + 1));
+        double *gk = malloc(sizeof(double) * L), *em = malloc(sizeof(double) * L), *dem = malloc(sizeof(double) * L);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n_nu; ++i) {
+            double nu = grid_at(&gg, i);
+            tr[L] = 1.0;
+            for (int k = L - 1; k >= 0; --k) tr[k] = tr[k + 1] * exp(-tau[(size_t)k * n_nu + i] / mu);
+            double bs = orc_planck(nu, T_sfc), t1 = tr[0];
+            double Irad = eps * bs * t1;
+            deps[i] = bs * t1;
+            dTsfc[i] = eps * orc_dbdt(nu, T_sfc) * t1;
+            double below = eps * bs * t1;
+            for (int m = 0; m < L; ++m) {
+                double a = tau[(size_t)m * n_nu + i] * invmu, w = exp(-a), u = 1.0 - w, ttop = tr[m + 1];
+                double emv, dem_da;
+                if (sf == 1) {
+                    double bavg = orc_planck(nu, Tave[m]), btop = orc_planck(nu, T_lev[m + 1]);
+                    double f = orc_cim_correction(a), fp = orc_cim_correction_deriv(a);
+                    double core = bavg + (btop - bavg) * f;
+                    emv = u * core;
+                    dem_da = w * core + u * (btop - bavg) * fp;
+                    double cavg = ttop * u * (1.0 - f) * orc_dbdt(nu, Tave[m]);
+                    dTlev[(size_t)m * n_nu + i] += cavg * (1.0 - fr[m]);
+                    dTlev[(size_t)(m + 1) * n_nu + i] += cavg * fr[m] + ttop * u * f * orc_dbdt(nu, T_lev[m + 1]);
+                } else {
+                    double bbot = orc_planck(nu, T_lev[m]), btop = orc_planck(nu, T_lev[m + 1]);
+                    double fl = orc_lit_correction(a), flp = orc_lit_correction_deriv(a);
+                    emv = btop * u + (bbot - btop) * fl;
+                    dem_da = btop * w + (bbot - btop) * flp;
+                    dTlev[(size_t)m * n_nu + i] += ttop * fl * orc_dbdt(nu, T_lev[m]);
+                    dTlev[(size_t)(m + 1) * n_nu + i] += ttop * (u - fl) * orc_dbdt(nu, T_lev[m + 1]);
+                }
+                dtau[(size_t)m * n_nu + i] = invmu * (-below + ttop * dem_da);
+                Irad += ttop * emv;
+                below += ttop * emv;
+            }
+            if (eps < 1.0) {   /* schwarzschild_jacobian.jl:189-266 */
+                double g = 1.0, down = 0.0;
+                for (int k = 0; k < L; ++k) {
+                    double a = tau[(size_t)k * n_nu + i] * invmu, w = exp(-a), u = 1.0 - w;
+                    gk[k] = g;
+                    if (sf == 1) {
+                        double bavg = orc_planck(nu, Tave[k]), bbot = orc_planck(nu, T_lev[k]);
+                        double f = orc_cim_correction(a), fp = orc_cim_correction_deriv(a);
+                        double core = bavg + (bbot - bavg) * f;
+                        em[k] = u * core;
+                        dem[k] = w * core + u * (bbot - bavg) * fp;
+                        double cavg = refl * t1 * g * u * (1.0 - f) * orc_dbdt(nu, Tave[k]);
+                        dTlev[(size_t)k * n_nu + i] += cavg * (1.0 - fr[k]) + refl * t1 * g * u * f * orc_dbdt(nu, T_lev[k]);
+                        dTlev[(size_t)(k + 1) * n_nu + i] += cavg * fr[k];
+                    } else {
+                        double bbot = orc_planck(nu, T_lev[k]), btop = orc_planck(nu, T_lev[k + 1]);
+                        double fl = orc_lit_correction(a), flp = orc_lit_correction_deriv(a);
+                        em[k] = bbot * u + (btop - bbot) * fl;
+                        dem[k] = bbot * w + (btop - bbot) * flp;
+                        dTlev[(size_t)k * n_nu + i] += refl * t1 * g * (u - fl) * orc_dbdt(nu, T_lev[k]);
+                        dTlev[(size_t)(k + 1) * n_nu + i] += refl * t1 * g * fl * orc_dbdt(nu, T_lev[k + 1]);
+                    }
+                    down += g * em[k];
+                    g *= w;
+                }
+                Irad += refl * down * t1;
+                deps[i] += -down * t1;
+                double above = 0.0;
+                for (int m = L - 1; m >= 0; --m) {
+                    dtau[(size_t)m * n_nu + i] += refl * invmu * t1 * (-above + gk[m] * dem[m] - down);
+                    above += gk[m] * em[m];
+                }
+            }
+            Iout[i] = Irad;
+        }
+        free(tr); free(gk); free(em); free(dem);
+    }
+    free(Tave); free(fr);
+    return 0;
+}
+
+int orc_state_size(const orc_state_spec *s, int nlev) {
+    int n = s->include_temperature ? nlev : 0;
+    for (int r = 0; r < s->n_retrieved; ++r) n += (s->column_scale && s->column_scale[r]) ? 1 : nlev;
+    return n + (s->include_tsfc ? 1 : 0) + (s->include_emissivity ? 1 : 0);
+}
+
+int orc_jacobian(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                 const orc_state_spec *spec, double *K, double *y0, double *x0) {
+    orc_forward_info inf;
+    memset(&inf, 0, sizeof(inf));
+    orc_forward_dims(snd, opt, &inf);
+    const double lo = inf.nu_lo, d = inf.dnu;
+    const int64_t n = inf.n_hi, nch = inf.n_ch;
+    const int nlev = prof->n_levels, L = nlev - 1, ns = prof->n_species;
+    const double *P = prof->p_hpa, *Tl = prof->T;
+    double *p_mid = malloc(sizeof(double) * L), *T_mid = malloc(sizeof(double) * L);
+    double *dpk = malloc(sizeof(double) * L), *T_ave = malloc(sizeof(double) * L);
+    double *vc = malloc(sizeof(double) * L * ns), *pc = malloc(sizeof(double) * L * ns);
+    double *tc = malloc(sizeof(double) * L * ns);
+    orc_layer_properties(prof, opt->T_method, p_mid, T_mid, dpk, vc, pc, tc, T_ave);
+    const size_t NL = (size_t)n * L;
+    double *tau = calloc(NL, sizeof(double));
+    double *DTlo = calloc(NL, sizeof(double)), *DThi = calloc(NL, sizeof(double));
+    const int nr = spec->n_retrieved;
+    double *Vlo = calloc(NL * (nr ? nr : 1), sizeof(double)), *Vhi = calloc(NL * (nr ? nr : 1), sizeof(double));
+    double *sg = malloc(sizeof(double) * n), *gT = malloc(sizeof(double) * n);
+    double *gp = malloc(sizeof(double) * n), *gv = malloc(sizeof(double) * n);
+    for (int k = 0; k < L; ++k) {
+        for (int s = 0; s < ns; ++s) {
+            const orc_linelist *ll = &prof->lines[s];
+            double vmr = vc[(size_t)s * L + k];
+            if (ll->n == 0 || vmr == 0.0) continue;
+            int sp = prof->species[s];
+            double p_atm = pc[(size_t)s * L + k] / 1013.25, T = tc[(size_t)s * L + k];
+            double vs = sp == 1 ? vmr : 0.0, coef = vmr * dpk[k] * 2.1209e22;
+            int lm = (sp == 2 && opt->line_mixing == 1);
+            orc_species_grad(ll, lm, lo, d, n, T, p_atm, vs, opt->cutoff, opt->lm_cutoff, opt->x_far,
+                             sg, gT, gp, gv);
+            double p1 = P[k], p2 = P[k + 1];
+            double wT = opt->T_method == 1 ? orc_cg_mass_frac(p1, p2) : log(pc[(size_t)s * L + k] / p1) / log(p2 / p1);
+            double *tk = tau + (size_t)k * n;
+            for (int64_t i = 0; i < n; ++i) {
+                tk[i] += sg[i] * coef;
+                DTlo[(size_t)k * n + i] += gT[i] * coef * (1.0 - wT);
+                DThi[(size_t)k * n + i] += gT[i] * coef * wT;
+            }
+            for (int r = 0; r < nr; ++r) {
+                if (spec->retrieved[r] != sp) continue;
+                const double *vl = prof->vmr + (size_t)s * nlev;
+                double v1 = vl[k], v2 = vl[k + 1], gV1, gV2, gP1 = 0, gP2 = 0, gT1 = 0, gT2 = 0;
+                orc_cg_column_vmr_grad(v1, v2, p1, p2, &gV1, &gV2);
+                if (spec->vmr_coupling) {
+                    double h1 = fmax(fabs(v1), 1e-300) * 1e-6, h2 = fmax(fabs(v2), 1e-300) * 1e-6;
+                    gP1 = (orc_cg_pressure(v1 + h1, v2, p1, p2) - orc_cg_pressure(v1 - h1, v2, p1, p2)) / (2 * h1);
+                    gP2 = (orc_cg_pressure(v1, v2 + h2, p1, p2) - orc_cg_pressure(v1, v2 - h2, p1, p2)) / (2 * h2);
+                    if (opt->T_method != 1) {
+                        double dtdp = (Tl[k + 1] - Tl[k]) / (pc[(size_t)s * L + k] * log(p2 / p1));
+                        gT1 = dtdp * gP1;
+                        gT2 = dtdp * gP2;
+                    }
+                }
+                double *lo_ = Vlo + ((size_t)r * L + k) * n, *hi_ = Vhi + ((size_t)r * L + k) * n;
+                for (int64_t i = 0; i < n; ++i) {
+                    double ts = sg[i] * coef;
+                    double base_lo = ts * (gV1 / vmr), base_hi = ts * (gV2 / vmr);
+                    if (spec->vmr_coupling) {
+                        double dtp = gp[i] * (coef / 1013.25), dtT = gT[i] * coef, dtv = gv[i] * coef;
+                        base_lo += dtp * gP1 + dtT * gT1 + (sp == 1 ? dtv * gV1 : 0.0);
+                        base_hi += dtp * gP2 + dtT * gT2 + (sp == 1 ? dtv * gV2 : 0.0);
+                    }
+                    lo_[i] = base_lo;
+                    hi_[i] = base_hi;
+                }
+            }
+        }
+    }
+    double T_sfc = opt->T_sfc > 0.0 ? opt->T_sfc : Tl[0];
+    double *Ihi = malloc(sizeof(double) * n), *dtau = malloc(sizeof(double) * NL);
+    double *dTl = malloc(sizeof(double) * (NL + n)), *dTs = malloc(sizeof(double) * n), *de = malloc(sizeof(double) * n);
+    int rc = orc_rte_jacobian(lo, d, n, tau, L, Tl, P, T_sfc, opt->mu, opt->eps_sfc, opt->source_function,
+                              Ihi, dtau, dTl, dTs, de);
+    if (rc) return rc;
+    /* channel-space linear tail: ILS at the channel nodes, x dBT/dR */
+    int64_t nh = orc_ils_half(d), nt = 2 * nh + 1;
+    double *kern = malloc(sizeof(double) * nt);
+    orc_ils_kernel(d, snd->opd_max, snd->fwhm, opt->apodization, 2048, kern);
+    int64_t *idx = malloc(sizeof(int64_t) * nch);
+    double *nuc = malloc(sizeof(double) * nch), *Rc = malloc(sizeof(double) * nch), *dbt = malloc(sizeof(double) * nch);
+    for (int64_t c = 0; c < nch; ++c) {
+        nuc[c] = orc_grid_node(snd->nu_min, snd->dnu, c);
+        int64_t i = (int64_t)llround((nuc[c] - lo) / d);
+        idx[c] = i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    }
+    if (opt->with_ils) orc_apply_ils_at(Ihi, n, d, kern, nt, idx, nch, Rc);
+    else for (int64_t c = 0; c < nch; ++c) Rc[c] = Ihi[idx[c]];
+    for (int64_t c = 0; c < nch; ++c) {
+        dbt[c] = spec->observable == 0 ? orc_dbt_dr(nuc[c], Rc[c]) : 1.0;
+        y0[c] = spec->observable == 0 ? orc_bt(nuc[c], Rc[c]) : Rc[c];
+    }
+    double *col = malloc(sizeof(double) * n), *ch = malloc(sizeof(double) * nch);
+    int ci = 0;
+#define EMIT(colptr)                                                                 \
+    do {                                                                             \
+        if (opt->with_ils) orc_apply_ils_at(colptr, n, d, kern, nt, idx, nch, ch);   \
+        else for (int64_t c_ = 0; c_ < nch; ++c_) ch[c_] = (colptr)[idx[c_]];        \
+        for (int64_t c_ = 0; c_ < nch; ++c_) K[(size_t)ci * nch + c_] = ch[c_] * dbt[c_]; \
+    } while (0)
+    if (spec->include_temperature) {
+        for (int j = 0; j < nlev; ++j) {
+            for (int64_t i = 0; i < n; ++i) {
+                double v = dTl[(size_t)j * n + i];
+                if (j < L) v += dtau[(size_t)j * n + i] * DTlo[(size_t)j * n + i];
+                if (j >= 1) v += dtau[(size_t)(j - 1) * n + i] * DThi[(size_t)(j - 1) * n + i];
+                col[i] = v;
+            }
+            EMIT(col);
+            x0[ci] = Tl[j];
+            ++ci;
+        }
+    }
+    for (int r = 0; r < nr; ++r) {
+        int sidx = -1;
+        for (int s = 0; s < ns; ++s) if (prof->species[s] == spec->retrieved[r]) sidx = s;
+        const double *vl = sidx >= 0 ? prof->vmr + (size_t)sidx * nlev : NULL;
+        int cs = spec->column_scale && spec->column_scale[r];
+        if (cs) memset(col, 0, sizeof(double) * n);
+        for (int j = 0; j < nlev; ++j) {
+            double vj = vl ? vl[j] : 0.0;
+            double *tmp = cs ? malloc(sizeof(double) * n) : col;
+            for (int64_t i = 0; i < n; ++i) {
+                double v = 0.0;
+                if (vl) {
+                    if (j < L && vc[(size_t)sidx * L + j] != 0.0) v += dtau[(size_t)j * n + i] * Vlo[((size_t)r * L + j) * n + i];
+                    if (j >= 1 && vc[(size_t)sidx * L + j - 1] != 0.0)
+                        v += dtau[(size_t)(j - 1) * n + i] * Vhi[((size_t)r * L + j - 1) * n + i];
+                }
+                tmp[i] = v;
+            }
+            if (cs) {
+                for (int64_t i = 0; i < n; ++i) col[i] += vj * tmp[i];
+                free(tmp);
+            } else {
+                if (spec->log_vmr) for (int64_t i = 0; i < n; ++i) col[i] *= vj;
+                EMIT(col);
+                x0[ci] = spec->log_vmr ? log(fmax(vj, 1e-30)) : vj;
+                ++ci;
+            }
+        }
+        if (cs) { EMIT(col); x0[ci] = 0.0; ++ci; }
+    }
+    if (spec->include_tsfc) { EMIT(dTs); x0[ci] = T_sfc; ++ci; }
+    if (spec->include_emissivity) { EMIT(de); x0[ci] = opt->eps_sfc; ++ci; }
+#undef EMIT
+    free(col); free(ch); free(kern); free(idx); free(nuc); free(Rc); free(dbt);
+    free(Ihi); free(dtau); free(dTl); free(dTs); free(de);
+    free(tau); free(DTlo); free(DThi); free(Vlo); free(Vhi); free(sg); free(gT); free(gp); free(gv);
+    free(p_mid); free(T_mid); free(dpk); free(T_ave); free(vc); free(pc); free(tc);
+    return 0;
+}

@@ -173,3 +173,46 @@ int orc_num_threads(void);
 }
 #endif
 #endif
+
+/* ==== Jacobians (src/CrossSections/cross_section_jacobian.jl,
+ *      src/Solver/schwarzschild_jacobian.jl, src/Estimation/vmr_jacobian.jl,
+ *      src/Estimation/state_vector.jl) ===================================== */
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* sigma and d sigma / d{T, p_atm, vmr_self} (cross_section_jacobian.jl:155-240);
+ * line_mixing = 1 adds the VP_Y term with central differences in T and p
+ * (cross_section_jacobian.jl:255-322). */
+void orc_species_grad(const orc_linelist *ll, int line_mixing, double nu0, double dnu, int64_t n_nu,
+                      double T, double p_atm, double vmr_self, double cutoff, double lm_cutoff,
+                      double x_far, double *sig, double *dT, double *dp, double *dv);
+/* schwarzschild_rte_jacobian (schwarzschild_jacobian.jl:66-266).  dI_dtau is
+ * [n_layers][n_nu], dI_dTlev [n_layers+1][n_nu]. p_lev needed for CIM. */
+int orc_rte_jacobian(double nu0, double dnu, int64_t n_nu, const double *tau, int L,
+                     const double *T_lev, const double *p_lev, double T_sfc, double mu, double eps,
+                     int source_function, double *I, double *dI_dtau, double *dI_dTlev,
+                     double *dI_dTsfc, double *dI_deps);
+void orc_cg_column_vmr_grad(double v1, double v2, double p1, double p2, double *d1, double *d2);
+double orc_cg_mass_frac(double p1, double p2);
+double orc_dbt_dr(double nu, double R);
+
+typedef struct {
+    int include_temperature;
+    int n_retrieved;
+    const int *retrieved;      /* HITRAN ids of retrieved species (full-profile blocks) */
+    const int *column_scale;   /* per retrieved species: 1 = single column scale (may be NULL) */
+    int include_tsfc;
+    int include_emissivity;
+    int log_vmr;
+    int vmr_coupling;
+    int observable;            /* 0 bt, 1 radiance */
+} orc_state_spec;
+
+int orc_state_size(const orc_state_spec *spec, int n_levels);
+/* analytic_jacobian (vmr_jacobian.jl:152-505) without continuum; K is
+ * column-major [n_state][n_ch]; y0 [n_ch]; x0 [n_state]. */
+int orc_jacobian(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                 const orc_state_spec *spec, double *K, double *y0, double *x0);
+#ifdef __cplusplus
+}
+#endif

@@ -282,3 +282,82 @@ def layer_properties(prof, T_method=0):
     out["p_cg"] = {k: pc[i] for i, k in enumerate(keys)}
     out["T_cg"] = {k: tc[i] for i, k in enumerate(keys)}
     return out
+
+
+# ---- Jacobians -------------------------------------------------------------------
+class OrcStateSpec(C.Structure):
+    _fields_ = [("include_temperature", i32), ("n_retrieved", i32), ("retrieved", P),
+                ("column_scale", P), ("include_tsfc", i32), ("include_emissivity", i32),
+                ("log_vmr", i32), ("vmr_coupling", i32), ("observable", i32)]
+
+
+def _declare_jac():
+    L = lib()
+    if getattr(L, "_jac_declared", False):
+        return L
+    L.orc_species_grad.argtypes = [C.POINTER(OrcLinelist), i32, f64, f64, i64, f64, f64, f64, f64,
+                                   f64, f64, P, P, P, P]
+    L.orc_rte_jacobian.argtypes = [f64, f64, i64, P, i32, P, P, f64, f64, f64, i32, P, P, P, P, P]
+    L.orc_rte_jacobian.restype = i32
+    L.orc_cg_column_vmr_grad.argtypes = [f64, f64, f64, f64, C.POINTER(f64), C.POINTER(f64)]
+    L.orc_cg_mass_frac.argtypes = [f64, f64]
+    L.orc_cg_mass_frac.restype = f64
+    L.orc_dbt_dr.argtypes = [f64, f64]
+    L.orc_dbt_dr.restype = f64
+    L.orc_state_size.argtypes = [C.POINTER(OrcStateSpec), i32]
+    L.orc_state_size.restype = i32
+    L.orc_jacobian.argtypes = [C.POINTER(OrcProfile), C.POINTER(OrcSounder), C.POINTER(OrcOptions),
+                               C.POINTER(OrcStateSpec), P, P, P]
+    L.orc_jacobian.restype = i32
+    L._jac_declared = True
+    return L
+
+
+def species_grad(ll, nu0, dnu, n, T, p_atm, vmr_self=0.0, line_mixing=0, cutoff=25.0,
+                 lm_cutoff=25.0, x_far=122.0):
+    L = _declare_jac()
+    out = [np.zeros(n) for _ in range(4)]
+    s = make_linelist(ll)
+    L.orc_species_grad(C.byref(s), line_mixing, nu0, dnu, n, T, p_atm, vmr_self, cutoff, lm_cutoff,
+                       x_far, *[_ptr(a) for a in out])
+    return out
+
+
+def rte_jacobian(nu0, dnu, tau, T_lev, p_lev, T_sfc, mu=1.0, eps=1.0, source_function=1):
+    L = _declare_jac()
+    tau = np.asfortranarray(tau, np.float64)
+    n, Ln = tau.shape
+    I = np.zeros(n); dtau = np.zeros((Ln, n)); dT = np.zeros((Ln + 1, n))
+    dts = np.zeros(n); de = np.zeros(n)
+    T_lev = np.ascontiguousarray(T_lev, np.float64)
+    pl = None if p_lev is None else np.ascontiguousarray(p_lev, np.float64)
+    rc = L.orc_rte_jacobian(nu0, dnu, n, _ptr(tau), Ln, _ptr(T_lev), _ptr(pl), T_sfc, mu, eps,
+                            source_function, _ptr(I), _ptr(dtau), _ptr(dT), _ptr(dts), _ptr(de))
+    if rc != 0:
+        raise ValueError(f"orc_rte_jacobian error {rc}")
+    return I, dtau.T, dT.T, dts, de
+
+
+def jacobian(prof, linelists, sounder, retrieved=None, include_temperature=True,
+             include_tsfc=True, include_emissivity=True, log_vmr=True, vmr_coupling=True,
+             observable="bt", column_scale=None, **kw):
+    """Analytic Jacobian (FP64 oracle): returns (K [n_ch x n_state], y0, x0)."""
+    L = _declare_jac()
+    retrieved = list(retrieved or [])
+    r = np.array([int(s) for s in retrieved] or [0], np.int32)
+    cs = None if column_scale is None else np.array([int(bool(c)) for c in column_scale], np.int32)
+    spec = OrcStateSpec(int(include_temperature), len(retrieved), _ptr(r), _ptr(cs), int(include_tsfc),
+                        int(include_emissivity), int(log_vmr), int(vmr_coupling),
+                        0 if observable == "bt" else 1)
+    opts = options(dptmn=0.0, **kw)
+    pb = Problem(prof, linelists)
+    snd = OrcSounder(sounder.nu_min, sounder.nu_max, sounder.dnu, sounder.opd_max, sounder.fwhm_gauss)
+    info = OrcInfo()
+    L.orc_forward_dims(C.byref(snd), C.byref(opts), C.byref(info))
+    ns = L.orc_state_size(C.byref(spec), prof.n_levels)
+    K = np.zeros((ns, info.n_ch)); y0 = np.zeros(info.n_ch); x0 = np.zeros(ns)
+    rc = L.orc_jacobian(C.byref(pb.prof), C.byref(snd), C.byref(opts), C.byref(spec), _ptr(K),
+                        _ptr(y0), _ptr(x0))
+    if rc != 0:
+        raise ValueError(f"orc_jacobian error {rc}")
+    return K.T.copy(), y0, x0
