@@ -26,6 +26,8 @@
  *                                                   src/Sensor/ils.jl:131-153,
  *                                                   src/Sensor/sounder.jl:372-377
  *   irs_brightness_temperature  brightness_temperature  src/Solver/planck.jl:43-57
+ *   irs_jacobian           analytic_jacobian        src/Estimation/vmr_jacobian.jl:152-505
+ *   irs_state_size         StateVectorSpec length   src/Estimation/state_vector.jl:253-300
  *   irs_linelist_upload    HITRANLinelist(lines)    src/HITRAN/linelist.jl:52-56
  */
 #ifndef IRS_H
@@ -178,6 +180,29 @@ int irs_brightness_temperature(const double *nu, const double *radiance, int64_t
 /* Geometry of the internal grid for (snd, opt). */
 int irs_forward_dims(const irs_sounder *snd, const irs_fm_options *opt, double *nu_lo,
                      double *dnu_hi, int64_t *n_hi, int64_t *n_channels);
+
+/* Retrieval state layout (StateVectorSpec, src/Estimation/state_vector.jl:253-300):
+ * [T levels][VMR block per retrieved species][T_sfc][emissivity]. */
+typedef struct {
+    int        include_temperature;
+    int        n_retrieved;        /* <= 4 */
+    const int *retrieved;          /* HITRAN ids, each must be one of the profile's species */
+    const int *column_scale;       /* per retrieved species: 1 = ColumnScale, 0 = FullProfile (may be NULL) */
+    int        include_tsfc;
+    int        include_emissivity;
+    int        log_vmr;
+    int        vmr_coupling;       /* CG pressure / temperature / self-broadening coupling */
+    int        observable;         /* 0 brightness temperature, 1 radiance */
+} irs_state_spec;
+
+int irs_state_size(const irs_state_spec *spec, int n_levels);
+
+/* analytic_jacobian (src/Estimation/vmr_jacobian.jl:152-505) for one profile,
+ * full line set (no DPTMIN).  K is column-major [n_state][n_channels]; y0 the
+ * observable at the linearisation point [n_channels]; x0 the packed state. */
+int irs_jacobian(irs_ctx *ctx, const irs_profile *profile, const irs_sounder *snd,
+                 const irs_fm_options *opt, const irs_state_spec *spec, double *K, double *y0,
+                 double *x0);
 
 /* forward_model over a batch of profiles: R, BT are [n_prof][n_channels]. */
 int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,

@@ -21,6 +21,7 @@
 #include "../../include/irs.h"
 #include "../data/tips2024_table.h"
 #include "irs_internal.h"
+#include "irs_jac.h"
 
 hipError_t launch_keys(const double *nu, int64_t n, double nu_lo, double dnu, int32_t *keys,
                        hipStream_t s);
@@ -66,6 +67,7 @@ struct irs_ctx {
     std::vector<ListHost> lists;
     DevBuf d_lists, d_tips, d_items, d_rec0, d_rec1, d_rec2, d_keys, d_tau, d_rhi, d_counters;
     DevBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3, d_tmp4, d_tmp5, d_chidx, d_chw, d_chnu, d_rch, d_bt;
+    DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
     irs_stats stats{};
     int ls_P = 4;
@@ -471,6 +473,37 @@ void channel_positions(const Grid &gi, const Grid &gc, std::vector<int64_t> &idx
     }
 }
 
+double cg_mass_frac(double p1, double p2) {   // schwarzschild_jacobian.jl:47-53
+    if (approx_eq(p1, p2)) return 0.5;
+    double al = std::log(p2 / p1);
+    if (std::fabs(al) < 1e-3) return 0.5 + al / 12.0;
+    double be = p2 / p1;
+    return ((al - 1.0) * be + 1.0) / (al * (be - 1.0));
+}
+
+void cg_column_vmr_grad(double v1, double v2, double p1, double p2, double &d1, double &d2) {   // vmr_jacobian.jl:81-107
+    if (v1 <= 0.0 || v2 <= 0.0) { d1 = 0.5; d2 = 0.5; return; }
+    if (approx_eq(v1, v2)) { double fr = cg_mass_frac(p1, p2); d1 = 1.0 - fr; d2 = fr; return; }
+    double Lg = std::log(p2 / p1), al = std::log(v2 / v1) / Lg, be = al + 1.0, dp = std::fabs(p1 - p2);
+    if (std::fabs(be) < 1e-6) {
+        double h1 = std::max(std::fabs(v1), 1e-300) * 1e-6, h2 = std::max(std::fabs(v2), 1e-300) * 1e-6;
+        d1 = (cg_column_vmr(v1 + h1, v2, p1, p2) - cg_column_vmr(v1 - h1, v2, p1, p2)) / (2 * h1);
+        d2 = (cg_column_vmr(v1, v2 + h2, p1, p2) - cg_column_vmr(v1, v2 - h2, p1, p2)) / (2 * h2);
+        return;
+    }
+    double r = p2 / p1, rb = std::pow(r, be);
+    double c = v1 * p1 * (1.0 - rb) / (be * dp);
+    double J = (v1 * p1 / Lg) * ((rb - 1.0) / (be * be) - rb * Lg / be);
+    d1 = c / v1 - J / (v1 * dp);
+    d2 = J / (v2 * dp);
+}
+
+double dbt_dr(double nu, double R) {   // vmr_jacobian.jl:120-125
+    if (R <= 0.0) return 0.0;
+    double A = 1.191042953e-5 * nu * nu * nu, Lg = std::log1p(A / R);
+    return 1.4387769 * nu * A / (Lg * Lg * R * (R + A));
+}
+
 }  // namespace
 
 // =============================================================================
@@ -545,7 +578,9 @@ void irs_destroy(irs_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->d_lists, &ctx->d_tips, &ctx->d_items, &ctx->d_rec0, &ctx->d_rec1, &ctx->d_rec2,
                       &ctx->d_keys, &ctx->d_tau, &ctx->d_rhi, &ctx->d_counters, &ctx->d_tmp0, &ctx->d_tmp1,
                       &ctx->d_tmp2, &ctx->d_tmp3, &ctx->d_tmp4, &ctx->d_tmp5, &ctx->d_chidx, &ctx->d_chw,
-                      &ctx->d_chnu, &ctx->d_rch, &ctx->d_bt};
+                      &ctx->d_chnu, &ctx->d_rch, &ctx->d_bt, &ctx->j_items, &ctx->j_g, &ctx->j_tau,
+                      &ctx->j_dt, &ctx->j_v, &ctx->j_rte, &ctx->j_cols, &ctx->j_desc, &ctx->j_w, &ctx->j_K,
+                      &ctx->j_fac};
     for (DevBuf *b : bufs) b->release();
     for (auto &kv : ctx->taps) kv.second.release();
     for (auto &e : ctx->ev)
@@ -1006,6 +1041,282 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
     ctx->stats.n_layers = L;
     ctx->stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return IRS_OK;
+}
+
+int irs_state_size(const irs_state_spec *sp, int n_levels) {
+    if (!sp || n_levels <= 0) return -1;
+    int n = sp->include_temperature ? n_levels : 0;
+    for (int r = 0; r < sp->n_retrieved; ++r) n += (sp->column_scale && sp->column_scale[r]) ? 1 : n_levels;
+    return n + (sp->include_tsfc ? 1 : 0) + (sp->include_emissivity ? 1 : 0);
+}
+
+int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, const irs_fm_options *opt_in,
+                 const irs_state_spec *sp, double *K, double *y0, double *x0) {
+    if (!ctx) return IRS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (!pr || !snd || !sp || !K || !y0 || !x0) return fail(ctx, IRS_EINVAL, "null argument");
+    irs_fm_options opt;
+    if (opt_in) opt = *opt_in; else irs_default_options(&opt);
+    if (sp->observable != 0 && sp->observable != 1) return fail(ctx, IRS_EINVAL, "observable must be :bt or :radiance");
+    if (sp->n_retrieved < 0 || sp->n_retrieved > IRS_JAC_MAXR) return fail(ctx, IRS_EINVAL, "at most 4 retrieved species");
+    if (!(pr->mu > 0.0 && pr->mu <= 1.0)) return fail(ctx, IRS_EINVAL, "μ must be in (0, 1]");
+    if (opt.source_function != IRS_SRC_TOON && opt.source_function != IRS_SRC_CIM)
+        return fail(ctx, IRS_EINVAL, "source_function must be :toon or :cim");
+    auto t0 = std::chrono::steady_clock::now();
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    const Grid g = internal_grid(snd, &opt, nullptr);
+    const Grid gc = make_grid(snd->nu_min, snd->dnu, snd->nu_max);
+    const int64_t n = g.n, nch = gc.n;
+    const double nu_lo = g.at(0), dnu = g.step;
+    const int nlev = pr->n_levels, L = nlev - 1, ns = pr->n_species, nr = sp->n_retrieved;
+    std::vector<int> retr_sidx(nr, -1);
+    for (int r = 0; r < nr; ++r)
+        for (int s = 0; s < ns; ++s)
+            if (pr->species[s] == sp->retrieved[r]) retr_sidx[r] = s;
+    for (int r = 0; r < nr; ++r)
+        if (retr_sidx[r] < 0) return fail(ctx, IRS_EINVAL, "profile is missing a retrieved species");
+    for (int s = 0; s < ns; ++s)
+        if (pr->linelist[s] >= 0) {
+            int rc = check_list(ctx, pr->linelist[s]);
+            if (rc) return rc;
+        }
+    LayerProps lp;
+    int rc = layer_properties(ctx, pr, opt.T_method, lp);
+    if (rc) return rc;
+    rc = ensure_common(ctx);
+    if (rc) return rc;
+    std::vector<int32_t> used;
+    for (int s = 0; s < ns; ++s) if (pr->linelist[s] >= 0) used.push_back(pr->linelist[s]);
+    std::map<int32_t, int32_t> koff;
+    rc = build_keys(ctx, used, nu_lo, dnu, koff);
+    if (rc) return rc;
+    const double *P = pr->p_hpa, *Tl = pr->T;
+    // ---- work items (full line set: the cutoff-freezing policy of jacobian_fd.jl:10-24)
+    std::vector<JacItemDev> items((size_t)L * ns);
+    int64_t rec = 0;
+    int max_lines = 0;
+    for (int k = 0; k < L; ++k)
+        for (int s = 0; s < ns; ++s) {
+            JacItemDev &it = items[(size_t)k * ns + s];
+            std::memset(&it, 0, sizeof(it));
+            it.retr = -1;
+            const int32_t h = pr->linelist[s];
+            const double vmr = lp.vmr_cg[s][k];
+            if (h < 0 || vmr == 0.0 || ctx->lists[h].n == 0) continue;
+            const ListHost &lh = ctx->lists[h];
+            const int spid = pr->species[s];
+            const bool lm = opt.line_mixing == IRS_LM_VPY && spid == 2 && lh.has_lm;
+            const double p_atm = lp.p_cg[s][k] / 1013.25, T = lp.T_cg[s][k];
+            const double vs = (spid == 1 && !lm) ? vmr : 0.0;
+            ItemDev base = make_item(lh, h, T, p_atm, vs, vmr * lp.dp[k] * 2.1209e22, 0.0, opt.cutoff,
+                                     opt.lm_cutoff, opt.x_far, lm, dnu);
+            it.rec_off = rec;
+            it.n = base.n; it.key_off = koff[h]; it.list = h; it.lm = base.lm; it.active = 1; it.M = base.M;
+            it.C = base.C; it.Clm = base.Clm;
+            it.T = T; it.p_atm = p_atm; it.vmr_self = vs; it.coef = base.coef; it.cutoff = opt.cutoff;
+            it.x_far = opt.x_far;
+            const double p1 = P[k], p2 = P[k + 1];
+            const double wT = opt.T_method == IRS_TMETHOD_MASS_WEIGHTED ? cg_mass_frac(p1, p2)
+                                                                         : std::log(lp.p_cg[s][k] / p1) / std::log(p2 / p1);
+            it.wT_lo = 1.0 - wT;
+            it.wT_hi = wT;
+            for (int r = 0; r < nr; ++r) {
+                if (retr_sidx[r] != s) continue;
+                it.retr = r;
+                const double *vl = pr->vmr + (size_t)s * nlev;
+                const double v1 = vl[k], v2 = vl[k + 1];
+                double gV1, gV2, gP1 = 0, gP2 = 0, gT1 = 0, gT2 = 0;
+                cg_column_vmr_grad(v1, v2, p1, p2, gV1, gV2);
+                if (sp->vmr_coupling) {
+                    const double h1 = std::max(std::fabs(v1), 1e-300) * 1e-6, h2 = std::max(std::fabs(v2), 1e-300) * 1e-6;
+                    gP1 = (cg_pressure(v1 + h1, v2, p1, p2) - cg_pressure(v1 - h1, v2, p1, p2)) / (2 * h1);
+                    gP2 = (cg_pressure(v1, v2 + h2, p1, p2) - cg_pressure(v1, v2 - h2, p1, p2)) / (2 * h2);
+                    if (opt.T_method != IRS_TMETHOD_MASS_WEIGHTED) {
+                        const double dtdp = (Tl[k + 1] - Tl[k]) / (lp.p_cg[s][k] * std::log(p2 / p1));
+                        gT1 = dtdp * gP1;
+                        gT2 = dtdp * gP2;
+                    }
+                    if (spid == 1) { it.gs_lo = gV1; it.gs_hi = gV2; }
+                }
+                it.gv_lo = gV1 / vmr; it.gv_hi = gV2 / vmr;
+                it.gp_lo = gP1 / 1013.25; it.gp_hi = gP2 / 1013.25;
+                it.gt_lo = gT1; it.gt_hi = gT2;
+            }
+            rec += it.n;
+            max_lines = std::max(max_lines, it.n);
+        }
+    HIPCHK(ensure(ctx->j_items, sizeof(JacItemDev) * items.size()));
+    HIPCHK(ensure(ctx->j_g, sizeof(float4) * 6 * std::max<int64_t>(rec, 1)));
+    HIPCHK(hipMemcpyAsync(ctx->j_items.p, items.data(), sizeof(JacItemDev) * items.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    float4 *gb = (float4 *)ctx->j_g.p;
+    const int64_t R6 = std::max<int64_t>(rec, 1);
+    JacPrepArgs pa{};
+    pa.items = (const JacItemDev *)ctx->j_items.p;
+    pa.lists = (const LineListDev *)ctx->d_lists.p;
+    pa.tips_q = (const double *)ctx->d_tips.p;
+    pa.n_items = (int)items.size();
+    pa.nu_lo = nu_lo;
+    pa.dnu = dnu;
+    pa.g0 = gb; pa.g1 = gb + R6; pa.g2 = gb + 2 * R6; pa.g3 = gb + 3 * R6; pa.g4 = gb + 4 * R6; pa.g5 = gb + 5 * R6;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(launch_prep_grad(pa, max_lines, ctx->stream));
+    const size_t NL = (size_t)n * L;
+    HIPCHK(ensure(ctx->j_tau, sizeof(float) * NL));
+    HIPCHK(ensure(ctx->j_dt, sizeof(float) * NL * 2));
+    HIPCHK(ensure(ctx->j_v, sizeof(float) * NL * 2 * std::max(nr, 1)));
+    JacSumArgs sa{};
+    sa.items = pa.items;
+    sa.g0 = pa.g0; sa.g1 = pa.g1; sa.g2 = pa.g2; sa.g3 = pa.g3; sa.g4 = pa.g4; sa.g5 = pa.g5;
+    sa.keys = (const int32_t *)ctx->d_keys.p;
+    sa.n_hi = n;
+    sa.n_layers = L;
+    sa.n_species = ns;
+    sa.n_retr = nr;
+    sa.tau = (float *)ctx->j_tau.p;
+    sa.dtlo = (float *)ctx->j_dt.p;
+    sa.dthi = sa.dtlo + NL;
+    sa.vlo = (float *)ctx->j_v.p;
+    sa.vhi = sa.vlo + NL * std::max(nr, 1);
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    HIPCHK(launch_grad_linesum(sa, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    // ---- RTE Jacobian
+    std::vector<double> tlev(Tl, Tl + nlev), tave(L), frac(L);
+    for (int k = 0; k < L; ++k) {
+        tave[k] = cg_temperature_mass(Tl[k], Tl[k + 1], P[k], P[k + 1]);
+        frac[k] = cg_mass_frac(P[k], P[k + 1]);
+    }
+    HIPCHK(ensure(ctx->d_tmp3, sizeof(double) * (3 * nlev + 8)));
+    double *dT = (double *)ctx->d_tmp3.p;
+    HIPCHK(hipMemcpyAsync(dT, tlev.data(), sizeof(double) * nlev, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dT + nlev, tave.data(), sizeof(double) * L, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dT + 2 * nlev, frac.data(), sizeof(double) * L, hipMemcpyHostToDevice, ctx->stream));
+    const size_t rte_doubles = (size_t)n * (3 + L + (L + 1));
+    HIPCHK(ensure(ctx->j_rte, sizeof(double) * rte_doubles));
+    double *rb = (double *)ctx->j_rte.p;
+    RteJacArgs ra{};
+    ra.tau = sa.tau;
+    ra.n = n;
+    ra.n_layers = L;
+    ra.T_lev = dT;
+    ra.T_ave = dT + nlev;
+    ra.frac = dT + 2 * nlev;
+    ra.T_sfc = pr->T_sfc > 0.0 ? pr->T_sfc : Tl[0];
+    ra.mu = pr->mu;
+    ra.eps = pr->eps_sfc;
+    ra.source_function = opt.source_function;
+    ra.an = g.an; ra.sn = g.sn; ra.den = g.den;
+    ra.I = rb; ra.dtsfc = rb + n; ra.deps = rb + 2 * n; ra.dtau = rb + 3 * n; ra.dTlev = rb + 3 * n + NL;
+    HIPCHK(hipMemsetAsync(ra.dTlev, 0, sizeof(double) * (size_t)n * (L + 1), ctx->stream));
+    HIPCHK(launch_rte_jac(ra, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    // ---- columns
+    std::vector<JacColDesc> cols;
+    std::vector<double> lw((size_t)std::max(nr, 1) * nlev, 0.0);
+    int ci = 0;
+    if (sp->include_temperature)
+        for (int j = 0; j < nlev; ++j) { cols.push_back({0, j, j, 0}); x0[ci++] = Tl[j]; }
+    for (int r = 0; r < nr; ++r) {
+        const double *vl = pr->vmr + (size_t)retr_sidx[r] * nlev;
+        const bool cs = sp->column_scale && sp->column_scale[r];
+        for (int j = 0; j < nlev; ++j) lw[(size_t)r * nlev + j] = (cs || sp->log_vmr) ? vl[j] : 1.0;
+        if (cs) { cols.push_back({1, 0, L, r}); x0[ci++] = 0.0; }
+        else
+            for (int j = 0; j < nlev; ++j) {
+                cols.push_back({1, j, j, r});
+                x0[ci++] = sp->log_vmr ? std::log(std::max(vl[j], 1e-30)) : vl[j];
+            }
+    }
+    if (sp->include_tsfc) { cols.push_back({2, 0, 0, 0}); x0[ci++] = ra.T_sfc; }
+    if (sp->include_emissivity) { cols.push_back({3, 0, 0, 0}); x0[ci++] = pr->eps_sfc; }
+    const int ncol = (int)cols.size();
+    if (ncol == 0) return fail(ctx, IRS_EINVAL, "empty state vector");
+    HIPCHK(ensure(ctx->j_desc, sizeof(JacColDesc) * ncol));
+    HIPCHK(ensure(ctx->j_w, sizeof(double) * lw.size()));
+    HIPCHK(ensure(ctx->j_cols, sizeof(double) * (size_t)n * ncol));
+    HIPCHK(hipMemcpyAsync(ctx->j_desc.p, cols.data(), sizeof(JacColDesc) * ncol, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->j_w.p, lw.data(), sizeof(double) * lw.size(), hipMemcpyHostToDevice, ctx->stream));
+    JacColArgs ca{};
+    ca.cols = (const JacColDesc *)ctx->j_desc.p;
+    ca.n = n;
+    ca.n_layers = L;
+    ca.dtau = ra.dtau; ca.dTlev = ra.dTlev; ca.dtsfc = ra.dtsfc; ca.deps = ra.deps;
+    ca.level_w = (const double *)ctx->j_w.p;
+    ca.dtlo = sa.dtlo; ca.dthi = sa.dthi; ca.vlo = sa.vlo; ca.vhi = sa.vhi;
+    ca.out = (double *)ctx->j_cols.p;
+    // level_w is indexed [r][L+1]
+    HIPCHK(launch_jac_columns(ca, ncol, ctx->stream));
+    // ---- channel tail: I -> R at channels (for y0 and dBT/dR), then every column
+    double *taps = nullptr;
+    int64_t half = 0;
+    std::vector<double> one(1, 1.0);
+    if (opt.with_ils) {
+        rc = get_taps(ctx, dnu, snd->opd_max, snd->fwhm_gauss, opt.apodization, 2048, &taps, &half);
+        if (rc) return rc;
+    } else {
+        HIPCHK(ensure(ctx->d_tmp0, sizeof(double)));
+        HIPCHK(hipMemcpyAsync(ctx->d_tmp0.p, one.data(), sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        taps = (double *)ctx->d_tmp0.p;
+    }
+    std::vector<int64_t> cidx;
+    std::vector<double> cw, cnu;
+    channel_positions(g, gc, cidx, cw, cnu);
+    HIPCHK(ensure(ctx->d_chidx, sizeof(int64_t) * nch));
+    HIPCHK(ensure(ctx->d_chw, sizeof(double) * nch));
+    HIPCHK(ensure(ctx->d_chnu, sizeof(double) * nch));
+    HIPCHK(ensure(ctx->d_rch, sizeof(double) * nch));
+    HIPCHK(ensure(ctx->d_bt, sizeof(double) * nch));
+    HIPCHK(hipMemcpyAsync(ctx->d_chidx.p, cidx.data(), sizeof(int64_t) * nch, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_chw.p, cw.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_chnu.p, cnu.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
+    IlsArgs ia{};
+    ia.spec = ra.I;
+    ia.n = n;
+    ia.taps = taps;
+    ia.half = opt.with_ils ? half : 0;
+    ia.dnu = opt.with_ils ? dnu : 1.0;
+    ia.ch_idx = (const int64_t *)ctx->d_chidx.p;
+    ia.ch_w = (const double *)ctx->d_chw.p;
+    ia.ch_nu = (const double *)ctx->d_chnu.p;
+    ia.n_ch = nch;
+    ia.r_out = (double *)ctx->d_rch.p;
+    ia.bt_out = (double *)ctx->d_bt.p;
+    HIPCHK(launch_ils_channels(ia, ctx->stream));
+    std::vector<double> Rc(nch), BTc(nch), fac(nch);
+    HIPCHK(hipMemcpyAsync(Rc.data(), ctx->d_rch.p, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(BTc.data(), ctx->d_bt.p, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int64_t c = 0; c < nch; ++c) {
+        fac[c] = sp->observable == 0 ? dbt_dr(cnu[c], Rc[c]) : 1.0;
+        y0[c] = sp->observable == 0 ? BTc[c] : Rc[c];
+    }
+    HIPCHK(ensure(ctx->j_fac, sizeof(double) * nch));
+    HIPCHK(ensure(ctx->j_K, sizeof(double) * (size_t)nch * ncol));
+    HIPCHK(hipMemcpyAsync(ctx->j_fac.p, fac.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
+    JacIlsArgs ja{};
+    ja.cols = ca.out;
+    ja.n = n;
+    ja.taps = taps;
+    ja.half = ia.half;
+    ja.dnu = ia.dnu;
+    ja.ch_idx = ia.ch_idx;
+    ja.ch_w = ia.ch_w;
+    ja.factor = (const double *)ctx->j_fac.p;
+    ja.n_ch = nch;
+    ja.K = (double *)ctx->j_K.p;
+    HIPCHK(launch_jac_ils(ja, ncol, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    HIPCHK(hipMemcpyAsync(K, ctx->j_K.p, sizeof(double) * (size_t)nch * ncol, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->stats.ms_prep = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.ms_linesum = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+    ctx->stats.ms_rte = elapsed_ms(ctx->ev[2], ctx->ev[3]);
+    ctx->stats.ms_ils = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+    ctx->stats.n_hi = n;
+    ctx->stats.n_layers = L;
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return IRS_OK;
 }
 

@@ -1,0 +1,503 @@
+// Jacobian kernels: per-line derivative parameters (FP64), the FP32 gradient
+// line sum, the FP64 RTE Jacobian scan, column assembly and the multi-column
+// channel ILS.
+//
+// Restates compute_voigt_cross_sections_grad / voigt_grad_kernel!
+// (src/CrossSections/cross_section_jacobian.jl:43-240), the LM-aware gradient
+// (:255-322, here with the analytic derivative of the dispersive term instead
+// of central differences), schwarzschild_rte_jacobian and its reflected term
+// (src/Solver/schwarzschild_jacobian.jl:66-266), and the column assembly of
+// analytic_jacobian (src/Estimation/vmr_jacobian.jl:383-504).
+#include "irs_devmath.h"
+#include "irs_internal.h"
+#include "irs_jac.h"
+
+#define TIPS_T_MIN 70
+#define TIPS_NT 331
+#define T_REF 296.0
+#define C2_LINE 1.4387752
+#define C2_LM 1.4388
+
+__device__ __forceinline__ double tips_at(const double *v, double T) {
+    long lo = (long)floor(T) - TIPS_T_MIN, hi = lo + 1;
+    lo = lo < 0 ? 0 : (lo > TIPS_NT - 1 ? TIPS_NT - 1 : lo);
+    hi = hi < 0 ? 0 : (hi > TIPS_NT - 1 ? TIPS_NT - 1 : hi);
+    const double fr = T - floor(T);
+    return v[lo] * (1.0 - fr) + v[hi] * fr;
+}
+
+__device__ __forceinline__ double tips_slope(const double *v, double T) {   // partition.jl:59-66
+    const long lo = (long)floor(T) - TIPS_T_MIN, hi = lo + 1;
+    if (lo < 0 || hi > TIPS_NT - 1) return 0.0;
+    return v[hi] - v[lo];
+}
+
+__device__ __forceinline__ double molmass(int mol) {
+    const double m[11] = {18.010, 44.010, 47.998, 44.013, 28.010, 16.043, 31.999, 34.015, 64.065, 36.461, 17.031};
+    return (mol < 1 || mol > 11) ? 30.0 : m[mol - 1];
+}
+
+__device__ __forceinline__ double lmmass(int mol, int iso) {
+    const double co2[11] = {47.00427, 43.98983, 44.99318, 45.99398, 44.99403, 45.99706,
+                            44.99706, 47.99832, 47.00134, 48.00196, 47.00427};
+    const double ch4[5] = {44.0, 16.031300, 17.034655, 17.037475, 18.040830};
+    if (mol == 2 && iso >= 0 && iso <= 10) return co2[iso];
+    if (mol == 6 && iso >= 1 && iso <= 4) return ch4[iso];
+    return 44.0;
+}
+
+// (H, dH/dx, dH/dy) with the far-wing branch (cross_section_jacobian.jl:43-56)
+__device__ __forceinline__ void hgrad64(double x, double y, double xfar, double &H, double &Hx, double &Hy) {
+    if (fabs(x) > xfar) {
+        const double d = x * x + y * y, d2 = d * d;
+        H = y * IRS_INV_SQRT_PI / d;
+        Hx = -2.0 * x * y * IRS_INV_SQRT_PI / d2;
+        Hy = IRS_INV_SQRT_PI * (x * x - y * y) / d2;
+        return;
+    }
+    double K, Lw;
+    w64(x, y, K, Lw);
+    H = K;
+    Hx = -2.0 * x * K + 2.0 * y * Lw;
+    Hy = 2.0 * (x * Lw + y * K) - 2.0 * IRS_INV_SQRT_PI;
+}
+
+// One thread per (item, line): six float4 records, coef folded into amplitudes.
+//   g0 = {cn, cf, fu, y}        g1 = {Sn, dSn/dT, dfu/dT, dy/dT}
+//   g2 = {dy/dp, dy/dvs, dnu0/dp / dnu, dnu0/dvs / dnu}
+//   g3 = {Hc, dHc/dT, dHc/dp, dHc/dvs}
+//   g4 = {fu_d, y_d, YSn, dYSn/dT}     g5 = {dfu_d/dT, dy_d/dT, YSn/p, y_d/p}
+__global__ void __launch_bounds__(256) prep_grad_kernel(const JacPrepArgs a) {
+    const JacItemDev it = a.items[blockIdx.y];
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (!it.active || j >= it.n) return;
+    const LineListDev L = a.lists[it.list];
+    const double T = it.T, p = it.p_atm, vs = it.vmr_self, coef = it.coef, dnu = a.dnu;
+    const double nu = L.nu[j];
+    const int mol = L.mol[j], iso = L.iso[j], row = L.tips_row[j];
+    double qr, dlnq;
+    if (row >= 0) {
+        const double *v = a.tips_q + (int64_t)row * TIPS_NT;
+        const double qt = tips_at(v, T);
+        qr = tips_at(v, T_REF) / qt;
+        dlnq = tips_slope(v, T) / qt;
+    } else {
+        qr = T_REF / T;
+        dlnq = 1.0 / T;     // Q = T / T_REF
+    }
+    // temperature_scaled_intensity_deriv (broadening.jl:112-128)
+    const double a2 = C2_LINE * nu;
+    const double stim = (1.0 - exp(-a2 / T)) / (1.0 - exp(-a2 / T_REF));
+    const double boltz = exp(-C2_LINE * L.elower[j] * (1.0 / T - 1.0 / T_REF));
+    const double S = L.S[j] * qr * boltz * stim;
+    const double dS = S * (-dlnq + C2_LINE * L.elower[j] / (T * T) - (a2 / (T * T)) / (exp(a2 / T) - 1.0));
+    // pressure_broadened_width_deriv (broadening.jl:62-78)
+    const double n_air = (double)L.nair[j];
+    const double n_self = mol == 1 ? 0.95 : (mol == 2 ? 0.75 : n_air);
+    const double tr = T_REF / T;
+    const double ac = pow(tr, n_air) * (double)L.gair[j], bc = pow(tr, n_self) * (double)L.gself[j];
+    const double A_ = ac * (1.0 - vs) * p, B_ = bc * vs * p;
+    const double gl = A_ + B_;
+    const double dgl = -(n_air * A_ + n_self * B_) / T;
+    double gd = nu * 3.58126e-7 * sqrt(T / molmass(mol));
+    const double dgd = gd / (2.0 * T);
+    gd = fmax(gd, 1e-10);
+    const double f = IRS_SQRT_LN2 / gd, df = -f * (dgd / gd);
+    const double y = gl * f;
+    const double dyT = dgl * f + gl * df, dyp = (gl / p) * f, dyv = ((bc - ac) * p) * f;
+    const double sn = S * f * IRS_INV_SQRT_PI, dsnT = (dS * f + S * df) * IRS_INV_SQRT_PI;
+    const double dself = mol == 1 ? 0.002 : (mol == 2 ? -0.003 : 0.0);
+    const double dair = (double)L.dair[j];
+    const double c = nu + (dair * (1.0 - vs) + dself * vs) * p;
+    const double d0p = dair * (1.0 - vs) + dself * vs, d0v = (dself - dair) * p;
+    double Hc, Hcx, Hcy;
+    hgrad64(it.cutoff * f, y, it.x_far, Hc, Hcx, Hcy);
+    const double u = (c - a.nu_lo) / dnu, cn = floor(u);
+    const int64_t o = it.rec_off + j;
+    a.g0[o] = make_float4(__int_as_float((int)cn), (float)(u - cn), (float)(dnu * f), (float)y);
+    a.g1[o] = make_float4((float)(coef * sn), (float)(coef * dsnT), (float)(dnu * df), (float)dyT);
+    a.g2[o] = make_float4((float)dyp, (float)dyv, (float)(d0p / dnu), (float)(d0v / dnu));
+    a.g3[o] = make_float4((float)Hc, (float)(Hcx * (it.cutoff * df) + Hcy * dyT), (float)(Hcy * dyp),
+                          (float)(Hcy * dyv));
+    float4 g4 = make_float4(1.0f, 1.0f, 0.0f, 0.0f), g5 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (it.lm && L.ylm && L.ylm[j] != 0.0) {
+        const double Y = L.ylm[j];
+        const double md = lmmass(mol, iso);
+        const double gdd = 3.5812e-7 * nu * sqrt(T / md);
+        const double gld = (double)L.gair[j] * pow(296.0 / T, n_air) * p;
+        const double dgld = -n_air * gld / T;
+        const double fd = IRS_SQRT_LN2 / fmax(gdd, 1e-10), dfd = -fd / (2.0 * T);
+        const double st0 = 1.0 - exp(-C2_LM * nu / 296.0);
+        const double e1 = exp(-C2_LM * nu / T), stt = 1.0 - e1;
+        const double st = L.S[j] * qr * exp(-C2_LM * L.elower[j] * (1.0 / T - 1.0 / 296.0)) * stt / st0;
+        if (st > 0.0) {
+            const double dlnst = -dlnq + C2_LM * L.elower[j] / (T * T) - (C2_LM * nu / (T * T)) * e1 / stt;
+            const double ysn = coef * Y * p * st * fd * IRS_INV_SQRT_PI;
+            const double yd = gld * fd;
+            g4 = make_float4((float)(dnu * fd), (float)yd, (float)ysn, (float)(ysn * (dlnst + dfd / fd)));
+            g5 = make_float4((float)(dnu * dfd), (float)(dgld * fd + gld * dfd), (float)(ysn / p),
+                             (float)(yd / p));
+        }
+    }
+    a.g4[o] = g4;
+    a.g5[o] = g5;
+}
+
+hipError_t launch_prep_grad(const JacPrepArgs &a, int max_lines, hipStream_t s) {
+    if (a.n_items <= 0 || max_lines <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prep_grad_kernel, dim3((max_lines + 255) / 256, a.n_items), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gradient line sum: per (tile, layer) block, every line in the window, every
+// pair with the exact branch selection, window test and pedestal.  Per species
+// the clamped (sigma, dT, dp, dvs) (coef folded) are combined with the item's
+// chain-rule weights into tau, DTlo, DThi and, for retrieved species, Vlo/Vhi.
+#define JP 4
+
+__device__ __forceinline__ void kadd(float &s, float &c, float v) {
+    const float yv = v - c;
+    const float t = s + yv;
+    c = (t - s) - yv;
+    s = t;
+}
+
+__device__ __forceinline__ int lower_key_j(const int32_t *k, int n, int64_t v) {
+    int lo = 0, hi = n;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int64_t)k[mid] < v) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+
+__device__ __forceinline__ int upper_key_j(const int32_t *k, int n, int64_t v) {
+    int lo = 0, hi = n;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int64_t)k[mid] <= v) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) grad_linesum_kernel(const JacSumArgs a) {
+    __shared__ int s_rng[2];
+    const int t = threadIdx.x;
+    const int layer = blockIdx.y;
+    const int64_t i0 = (int64_t)blockIdx.x * (256 * JP);
+    const int64_t i1 = min(i0 + (int64_t)(256 * JP), a.n_hi);
+    float tau[JP], dtlo[JP], dthi[JP], vlo[IRS_JAC_MAXR][JP], vhi[IRS_JAC_MAXR][JP];
+#pragma unroll
+    for (int p = 0; p < JP; ++p) {
+        tau[p] = dtlo[p] = dthi[p] = 0.0f;
+#pragma unroll
+        for (int r = 0; r < IRS_JAC_MAXR; ++r) vlo[r][p] = vhi[r][p] = 0.0f;
+    }
+    for (int s = 0; s < a.n_species; ++s) {
+        const JacItemDev it = a.items[(int64_t)layer * a.n_species + s];
+        if (!it.active || it.n == 0) continue;
+        const int32_t *keys = a.keys + it.key_off;
+        const double Cmx = it.lm ? fmax((double)it.C, (double)it.Clm) : (double)it.C;
+        const int64_t Cmax = (int64_t)ceil(Cmx), M = it.M;
+        if (t < 2) s_rng[t] = t == 0 ? lower_key_j(keys, it.n, i0 - Cmax - M - 1)
+                                     : upper_key_j(keys, it.n, i1 - 1 + Cmax + M + 1);
+        __syncthreads();
+        const int jA = __builtin_amdgcn_readfirstlane(s_rng[0]);
+        const int jF = __builtin_amdgcn_readfirstlane(s_rng[1]);
+        __syncthreads();
+        const float4 *g0 = a.g0 + it.rec_off, *g1 = a.g1 + it.rec_off, *g2 = a.g2 + it.rec_off;
+        const float4 *g3 = a.g3 + it.rec_off, *g4 = a.g4 + it.rec_off, *g5 = a.g5 + it.rec_off;
+        const float C2 = (float)((double)it.C * it.C), C2lm = (float)((double)it.Clm * it.Clm);
+        const float iC2 = (float)(1.0 / ((double)it.C * it.C));
+        const float xfar = (float)it.x_far;
+        const float ispi = 0.56418958354775628f;
+        float s0[JP], c0[JP], sT[JP], cT[JP], sP[JP], cP[JP], sV[JP], cV[JP];
+        float d0[JP], e0[JP], dT_[JP], eT[JP], dP[JP], eP[JP];
+#pragma unroll
+        for (int p = 0; p < JP; ++p) {
+            s0[p] = c0[p] = sT[p] = cT[p] = sP[p] = cP[p] = sV[p] = cV[p] = 0.0f;
+            d0[p] = e0[p] = dT_[p] = eT[p] = dP[p] = eP[p] = 0.0f;
+        }
+        const int tl = (int)(i0 + t);
+        for (int j = jA; j < jF; ++j) {
+            const float4 q0 = g0[j], q1 = g1[j], q2 = g2[j], q3 = g3[j];
+            const float base = (float)(tl - __float_as_int(q0.x));
+            const bool dl = it.lm && g4[j].z != 0.0f;
+#pragma unroll 1
+            for (int p = 0; p < JP; ++p) {
+                const float du = (base + (float)(256 * p)) - q0.y;
+                const float d2 = du * du;
+                const float x = du * q0.z, y = q0.w;
+                float H, Hx, Hy;
+                if (fabsf(x) > xfar) {
+                    const float dd = fmaf(x, x, y * y), r = irs_rcpf(dd), r2 = r * r;
+                    H = y * ispi * r;
+                    Hx = -2.0f * x * y * ispi * r2;
+                    Hy = ispi * (x * x - y * y) * r2;
+                } else {
+                    float K, Lw;
+                    w32(x, y, K, Lw);
+                    H = K;
+                    Hx = -2.0f * x * K + 2.0f * y * Lw;
+                    Hy = 2.0f * (x * Lw + y * K) - 2.0f * ispi;
+                }
+                if (d2 <= C2) {
+                    const float pedfac = fmaf(-d2, iC2, 2.0f);
+                    const float hm = H - pedfac * q3.x;
+                    const float dHt = Hx * (du * q1.z) + Hy * q1.w;
+                    const float dHp = Hx * (-q2.z * q0.z) + Hy * q2.x;
+                    const float dHv = Hx * (-q2.w * q0.z) + Hy * q2.y;
+                    const float dpfp = 2.0f * du * q2.z * iC2, dpfv = 2.0f * du * q2.w * iC2;
+                    kadd(s0[p], c0[p], q1.x * hm);
+                    kadd(sT[p], cT[p], q1.y * hm + q1.x * (dHt - pedfac * q3.y));
+                    kadd(sP[p], cP[p], q1.x * (dHp - (dpfp * q3.x + pedfac * q3.z)));
+                    kadd(sV[p], cV[p], q1.x * (dHv - (dpfv * q3.x + pedfac * q3.w)));
+                }
+                if (dl && d2 <= C2lm) {
+                    const float4 q4 = g4[j], q5 = g5[j];
+                    const float xd = du * q4.x, yd = q4.y;
+                    float Lv, Lx, Ly;
+                    if (fabsf(xd) > xfar) {
+                        const float dd = fmaf(xd, xd, yd * yd), r = irs_rcpf(dd), r2 = r * r;
+                        Lv = xd * ispi * r;
+                        Lx = ispi * (yd * yd - xd * xd) * r2;
+                        Ly = -2.0f * xd * yd * ispi * r2;
+                    } else {
+                        float K, Lw;
+                        w32(xd, yd, K, Lw);
+                        Lv = Lw;
+                        Lx = -2.0f * (xd * Lw + yd * K) + 2.0f * ispi;
+                        Ly = -2.0f * xd * K + 2.0f * yd * Lw;
+                    }
+                    kadd(d0[p], e0[p], q4.z * Lv);
+                    kadd(dT_[p], eT[p], q4.w * Lv + q4.z * (Lx * (du * q5.x) + Ly * q5.y));
+                    kadd(dP[p], eP[p], q5.z * Lv + q4.z * (Lx * (-q2.z * q4.x) + Ly * q5.w));
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < JP; ++p) {
+            double sg = (double)s0[p] - (double)c0[p], gT = (double)sT[p] - (double)cT[p];
+            double gP = (double)sP[p] - (double)cP[p], gV = (double)sV[p] - (double)cV[p];
+            if (sg <= 0.0) { sg = 0.0; gT = 0.0; gP = 0.0; gV = 0.0; }
+            if (it.lm) {
+                sg += (double)d0[p] - (double)e0[p];
+                gT += (double)dT_[p] - (double)eT[p];
+                gP += (double)dP[p] - (double)eP[p];
+                if (sg <= 0.0) { sg = 0.0; gT = 0.0; gP = 0.0; }
+            }
+            tau[p] += (float)sg;
+            dtlo[p] += (float)(gT * it.wT_lo);
+            dthi[p] += (float)(gT * it.wT_hi);
+            if (it.retr >= 0) {
+#pragma unroll
+                for (int r = 0; r < IRS_JAC_MAXR; ++r) {
+                    if (r != it.retr) continue;
+                    vlo[r][p] += (float)(sg * it.gv_lo + gP * it.gp_lo + gT * it.gt_lo + gV * it.gs_lo);
+                    vhi[r][p] += (float)(sg * it.gv_hi + gP * it.gp_hi + gT * it.gt_hi + gV * it.gs_hi);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < JP; ++p) {
+        const int64_t i = i0 + t + 256 * p;
+        if (i >= a.n_hi) continue;
+        const int64_t o = (int64_t)layer * a.n_hi + i;
+        a.tau[o] = tau[p];
+        a.dtlo[o] = dtlo[p];
+        a.dthi[o] = dthi[p];
+#pragma unroll
+        for (int r = 0; r < IRS_JAC_MAXR; ++r) {
+            if (r >= a.n_retr) break;
+            a.vlo[((int64_t)r * a.n_layers + layer) * a.n_hi + i] = vlo[r][p];
+            a.vhi[((int64_t)r * a.n_layers + layer) * a.n_hi + i] = vhi[r][p];
+        }
+    }
+}
+
+hipError_t launch_grad_linesum(const JacSumArgs &a, hipStream_t s) {
+    const unsigned tiles = (unsigned)((a.n_hi + 256 * JP - 1) / (256 * JP));
+    hipLaunchKernelGGL(grad_linesum_kernel, dim3(tiles, a.n_layers), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// RTE Jacobian (FP64), one thread per grid point.  Transmittances come from the
+// FP64 optical-depth sum from the top (T_k = exp(-sum_{j>=k} a_j)).
+__device__ __forceinline__ double cimf(double s, double e) { return s < 0.06 ? s / 6.0 : 1.0 - 2.0 * (e / (e - 1.0) + 1.0 / s); }
+__device__ __forceinline__ double cimd(double s, double e) { return s < 0.06 ? 1.0 / 6.0 : 2.0 / (s * s) - 2.0 * e / ((e - 1.0) * (e - 1.0)); }
+__device__ __forceinline__ double litf(double s, double e) { return s < 1e-4 ? s * (0.5 - s * (1.0 / 3.0 - s * 0.125)) : (1.0 - e) / s - e; }
+__device__ __forceinline__ double litd(double s, double e) { return s < 1e-4 ? 0.5 - s * (2.0 / 3.0 - s * 0.375) : (e * (s + 1.0) - 1.0) / (s * s) + e; }
+__device__ __forceinline__ double dbdt64(double nu, double T) {
+    const double x = IRS_C2_PL * nu / T, ex = exp(x);
+    return IRS_C1_PL * IRS_C2_PL * nu * nu * nu * nu * ex / (T * T * (ex - 1.0) * (ex - 1.0));
+}
+
+__global__ void __launch_bounds__(256) rte_jac_kernel(const RteJacArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int L = a.n_layers;
+    const int64_t n = a.n;
+    const double nu = (a.an + (double)i * a.sn) / a.den;
+    const double invmu = 1.0 / a.mu, eps = a.eps, refl = 1.0 - eps;
+    double Stot = 0.0;
+    for (int k = 0; k < L; ++k) Stot += (double)a.tau[(int64_t)k * n + i] * invmu;
+    const double t1 = exp(-Stot);
+    const double bs = planck64(nu, a.T_sfc);
+    double I = eps * bs * t1;
+    double deps = bs * t1;
+    const double dtsfc = eps * dbdt64(nu, a.T_sfc) * t1;
+    double below = eps * bs * t1;
+    double Aabove = Stot;                          // OD from level m to TOA
+    for (int m = 0; m < L; ++m) {
+        const double am = (double)a.tau[(int64_t)m * n + i] * invmu;
+        const double w = exp(-am), u = 1.0 - w;
+        Aabove -= am;
+        const double ttop = exp(-fmax(Aabove, 0.0));
+        double em, dem;
+        if (a.source_function == 1) {
+            const double bavg = planck64(nu, a.T_ave[m]), btop = planck64(nu, a.T_lev[m + 1]);
+            const double f = cimf(am, w), fp = cimd(am, w);
+            const double core = bavg + (btop - bavg) * f;
+            em = u * core;
+            dem = w * core + u * (btop - bavg) * fp;
+            const double cavg = ttop * u * (1.0 - f) * dbdt64(nu, a.T_ave[m]);
+            a.dTlev[(int64_t)m * n + i] += cavg * (1.0 - a.frac[m]);
+            a.dTlev[(int64_t)(m + 1) * n + i] += cavg * a.frac[m] + ttop * u * f * dbdt64(nu, a.T_lev[m + 1]);
+        } else {
+            const double bbot = planck64(nu, a.T_lev[m]), btop = planck64(nu, a.T_lev[m + 1]);
+            const double fl = litf(am, w), flp = litd(am, w);
+            em = btop * u + (bbot - btop) * fl;
+            dem = btop * w + (bbot - btop) * flp;
+            a.dTlev[(int64_t)m * n + i] += ttop * fl * dbdt64(nu, a.T_lev[m]);
+            a.dTlev[(int64_t)(m + 1) * n + i] += ttop * (u - fl) * dbdt64(nu, a.T_lev[m + 1]);
+        }
+        a.dtau[(int64_t)m * n + i] = invmu * (-below + ttop * dem);
+        I += ttop * em;
+        below += ttop * em;
+    }
+    if (eps < 1.0) {
+        double g = 1.0, down = 0.0;
+        for (int k = 0; k < L; ++k) {
+            const double ak = (double)a.tau[(int64_t)k * n + i] * invmu;
+            const double w = exp(-ak), u = 1.0 - w;
+            double em;
+            if (a.source_function == 1) {
+                const double bavg = planck64(nu, a.T_ave[k]), bbot = planck64(nu, a.T_lev[k]);
+                const double f = cimf(ak, w);
+                em = u * (bavg + (bbot - bavg) * f);
+                const double cavg = refl * t1 * g * u * (1.0 - f) * dbdt64(nu, a.T_ave[k]);
+                a.dTlev[(int64_t)k * n + i] += cavg * (1.0 - a.frac[k]) + refl * t1 * g * u * f * dbdt64(nu, a.T_lev[k]);
+                a.dTlev[(int64_t)(k + 1) * n + i] += cavg * a.frac[k];
+            } else {
+                const double bbot = planck64(nu, a.T_lev[k]), btop = planck64(nu, a.T_lev[k + 1]);
+                const double fl = litf(ak, w);
+                em = bbot * u + (btop - bbot) * fl;
+                a.dTlev[(int64_t)k * n + i] += refl * t1 * g * (u - fl) * dbdt64(nu, a.T_lev[k]);
+                a.dTlev[(int64_t)(k + 1) * n + i] += refl * t1 * g * fl * dbdt64(nu, a.T_lev[k + 1]);
+            }
+            down += g * em;
+            g *= w;
+        }
+        I += refl * down * t1;
+        deps += -down * t1;
+        // backward sweep: g_m recomputed from the OD below level m
+        double above = 0.0, Abelow = Stot;      // Abelow: OD from surface to level m+1 after update
+        double odtop = 0.0;                      // OD from level m+1 to TOA
+        for (int m = L - 1; m >= 0; --m) {
+            const double am = (double)a.tau[(int64_t)m * n + i] * invmu;
+            Abelow = Stot - odtop - am;          // OD from surface to level m
+            const double gm = exp(-fmax(Abelow, 0.0));
+            const double w = exp(-am), u = 1.0 - w;
+            double em, dem;
+            if (a.source_function == 1) {
+                const double bavg = planck64(nu, a.T_ave[m]), bbot = planck64(nu, a.T_lev[m]);
+                const double f = cimf(am, w), fp = cimd(am, w);
+                const double core = bavg + (bbot - bavg) * f;
+                em = u * core;
+                dem = w * core + u * (bbot - bavg) * fp;
+            } else {
+                const double bbot = planck64(nu, a.T_lev[m]), btop = planck64(nu, a.T_lev[m + 1]);
+                const double fl = litf(am, w), flp = litd(am, w);
+                em = bbot * u + (btop - bbot) * fl;
+                dem = bbot * w + (btop - bbot) * flp;
+            }
+            a.dtau[(int64_t)m * n + i] += refl * invmu * t1 * (-above + gm * dem - down);
+            above += gm * em;
+            odtop += am;
+        }
+    }
+    a.I[i] = I;
+    a.dtsfc[i] = dtsfc;
+    a.deps[i] = deps;
+}
+
+hipError_t launch_rte_jac(const RteJacArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(rte_jac_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Column assembly: one thread per grid point, one column per blockIdx.y.
+__global__ void __launch_bounds__(256) jac_columns_kernel(const JacColArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int c = blockIdx.y;
+    const JacColDesc d = a.cols[c];
+    const int64_t n = a.n;
+    const int L = a.n_layers;
+    double v = 0.0;
+    if (d.kind == 0) {            // temperature level j
+        const int j = d.level;
+        v = a.dTlev[(int64_t)j * n + i];
+        if (j < L) v += a.dtau[(int64_t)j * n + i] * (double)a.dtlo[(int64_t)j * n + i];
+        if (j >= 1) v += a.dtau[(int64_t)(j - 1) * n + i] * (double)a.dthi[(int64_t)(j - 1) * n + i];
+    } else if (d.kind == 1) {     // VMR level(s) of retrieved species r (weight per level)
+        for (int j = d.level; j <= d.level_end; ++j) {
+            const double wj = a.level_w[(int64_t)d.r * (L + 1) + j];
+            if (wj == 0.0) continue;
+            double x = 0.0;
+            if (j < L) x += a.dtau[(int64_t)j * n + i] * (double)a.vlo[((int64_t)d.r * L + j) * n + i];
+            if (j >= 1) x += a.dtau[(int64_t)(j - 1) * n + i] * (double)a.vhi[((int64_t)d.r * L + j - 1) * n + i];
+            v += wj * x;
+        }
+    } else if (d.kind == 2) {
+        v = a.dtsfc[i];
+    } else {
+        v = a.deps[i];
+    }
+    a.out[(int64_t)c * n + i] = v;
+}
+
+hipError_t launch_jac_columns(const JacColArgs &a, int n_cols, hipStream_t s) {
+    hipLaunchKernelGGL(jac_columns_kernel, dim3((unsigned)((a.n + 255) / 256), n_cols), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Multi-column channel ILS: block per (channel, column), FP64 reduction, times
+// the per-channel observable factor (dBT/dR or 1).
+__global__ void __launch_bounds__(256) jac_ils_kernel(const JacIlsArgs a) {
+    __shared__ double red[4];
+    const int64_t ch = blockIdx.x;
+    const int c = blockIdx.y;
+    const double *col = a.cols + (int64_t)c * a.n;
+    const int64_t ic = a.ch_idx[ch];
+    const double w = a.ch_w[ch];
+    double out = 0.0;
+    for (int side = 0; side < (w != 0.0 ? 2 : 1); ++side) {
+        const int64_t i = ic + side;
+        double acc = 0.0;
+        const int64_t nt = 2 * a.half + 1;
+        for (int64_t k = threadIdx.x; k < nt; k += 256) {
+            const int64_t jj = i + k - a.half;
+            if (jj >= 0 && jj < a.n) acc += col[jj] * a.taps[k];
+        }
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        const double r = (red[0] + red[1] + red[2] + red[3]) * a.dnu;
+        __syncthreads();
+        out += side == 0 ? r * (w != 0.0 ? 1.0 - w : 1.0) : r * w;
+    }
+    if (threadIdx.x == 0) a.K[(int64_t)c * a.n_ch + ch] = out * a.factor[ch];
+}
+
+hipError_t launch_jac_ils(const JacIlsArgs &a, int n_cols, hipStream_t s) {
+    hipLaunchKernelGGL(jac_ils_kernel, dim3((unsigned)a.n_ch, n_cols), dim3(256), 0, s, a);
+    return hipGetLastError();
+}

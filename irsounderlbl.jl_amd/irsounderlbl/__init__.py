@@ -5,22 +5,24 @@ operators call hand-written gfx950 HIP kernels through the C ABI declared in
 include/irs.h (library: csrc/libirs.so next to this package).  There is no CPU
 fallback: if the library or a GPU is missing, calls raise `IRSError`.
 """
-from .types import (AtmosphericProfile, CrISInstrument, GasSpecies, HITRANLinelist,
-                    IASIInstrument, IASINGInstrument, MTGIRSInstrument, Sounder,
-                    ViewingGeometry, WavenumberGrid, filter_linelist, iasi_scan_angles,
-                    nadir_geometry, scan_angle_to_local_zenith, sounder_grid, wavenumber_grid)
+from .types import (AtmosphericProfile, ColumnScale, CrISInstrument, FullProfile, GasSpecies,
+                    HITRANLinelist, IASIInstrument, IASINGInstrument, Jacobian, MTGIRSInstrument,
+                    PartialColumns, Sounder, StateVectorSpec, ViewingGeometry, WavenumberGrid,
+                    filter_linelist, iasi_scan_angles, nadir_geometry, pack_state,
+                    scan_angle_to_local_zenith, sounder_grid, wavenumber_grid)
 
-__all__ = ["AtmosphericProfile", "CrISInstrument", "GasSpecies", "HITRANLinelist",
-           "IASIInstrument", "IASINGInstrument", "MTGIRSInstrument", "Sounder",
+__all__ = ["AtmosphericProfile", "ColumnScale", "CrISInstrument", "FullProfile", "GasSpecies",
+           "HITRANLinelist", "IASIInstrument", "IASINGInstrument", "Jacobian", "MTGIRSInstrument",
+           "PartialColumns", "Sounder", "StateVectorSpec", "pack_state",
            "ViewingGeometry", "WavenumberGrid", "filter_linelist", "iasi_scan_angles",
            "nadir_geometry", "scan_angle_to_local_zenith", "sounder_grid", "wavenumber_grid"]
 
-try:  # the GPU operators need the built library; importing the types never does
-    from .api import (IRSError, Context, apply_ils, brightness_temperature,  # noqa: F401
-                      compute_voigt_cross_sections, compute_voigt_lm_cross_sections,
-                      forward_model, ils_kernel, planck_radiance, schwarzschild_rte)
-    __all__ += ["IRSError", "Context", "apply_ils", "brightness_temperature",
-                "compute_voigt_cross_sections", "compute_voigt_lm_cross_sections",
-                "forward_model", "ils_kernel", "planck_radiance", "schwarzschild_rte"]
-except ImportError:  # pragma: no cover - api.py not yet present during bootstrap
-    pass
+from .api import (IRSError, Context, analytic_jacobian, apply_ils_channels,  # noqa: E402
+                  brightness_temperature, compute_voigt_cross_sections,
+                  compute_voigt_lm_cross_sections, forward_model, ils_kernel, planck_radiance,
+                  schwarzschild_rte)
+
+__all__ += ["IRSError", "Context", "analytic_jacobian", "apply_ils_channels",
+            "brightness_temperature", "compute_voigt_cross_sections",
+            "compute_voigt_lm_cross_sections", "forward_model", "ils_kernel", "planck_radiance",
+            "schwarzschild_rte"]

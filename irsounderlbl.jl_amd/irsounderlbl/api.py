@@ -7,6 +7,7 @@ Names, argument meaning, defaults and error behaviour follow the reference:
   schwarzschild_rte               src/Solver/schwarzschild.jl:255-334
   ils_kernel / apply_ils          src/Sensor/ils.jl:80-153
   planck_radiance / brightness_temperature   src/Solver/planck.jl:11-57
+  analytic_jacobian               src/Estimation/vmr_jacobian.jl:152-505
 Every numeric call runs in the HIP library; there is no CPU fallback.  A
 missing library or GPU raises IRSError.
 """
@@ -19,8 +20,9 @@ from typing import Dict, Optional
 
 import numpy as np
 
-from .types import (AtmosphericProfile, GasSpecies, HITRANLinelist, Sounder, ViewingGeometry,
-                    WavenumberGrid, IASIInstrument, nadir_geometry, sounder_grid)
+from .types import (AtmosphericProfile, GasSpecies, HITRANLinelist, Jacobian, Sounder,
+                    StateVectorSpec, ViewingGeometry, WavenumberGrid, IASIInstrument,
+                    nadir_geometry, sounder_grid)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "csrc", "libirs.so"))
@@ -64,6 +66,12 @@ class IrsOptions(C.Structure):
                 ("source_function", i32), ("reserved0", i32)]
 
 
+class IrsStateSpec(C.Structure):
+    _fields_ = [("include_temperature", i32), ("n_retrieved", i32), ("retrieved", P),
+                ("column_scale", P), ("include_tsfc", i32), ("include_emissivity", i32),
+                ("log_vmr", i32), ("vmr_coupling", i32), ("observable", i32)]
+
+
 class IrsStats(C.Structure):
     _fields_ = [("ms_total", f64), ("ms_prep", f64), ("ms_linesum", f64), ("ms_rte", f64),
                 ("ms_ils", f64), ("n_hi", i64), ("n_layers", i64), ("linesum_launches", i64),
@@ -74,7 +82,8 @@ class IrsStats(C.Structure):
 EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_last_error",
            "irs_default_options", "irs_last_stats", "irs_linelist_upload", "irs_linelist_free",
            "irs_xsec_layer", "irs_rte", "irs_ils_half", "irs_ils_kernel", "irs_ils_channels",
-           "irs_brightness_temperature", "irs_forward_dims", "irs_forward")
+           "irs_brightness_temperature", "irs_forward_dims", "irs_forward", "irs_state_size",
+           "irs_jacobian")
 
 _lib = None
 
@@ -110,6 +119,9 @@ def load_library(path: str = LIB_PATH):
                                    C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]
     L.irs_forward.argtypes = [P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
                               C.POINTER(IrsOptions), P, P]
+    L.irs_state_size.argtypes = [C.POINTER(IrsStateSpec), i32]
+    L.irs_jacobian.argtypes = [P, C.POINTER(IrsProfile), C.POINTER(IrsSounder), C.POINTER(IrsOptions),
+                               C.POINTER(IrsStateSpec), P, P, P]
     for name in EXPORTS:
         if name not in ("irs_destroy", "irs_default_options", "irs_last_error"):
             getattr(L, name).restype = getattr(L, name).restype or i32
@@ -395,6 +407,95 @@ def forward_model(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     if batch:
         return grid, R, BT
     return grid, R[0], BT[0]
+
+
+def analytic_jacobian(prof: AtmosphericProfile, linelists: Dict[GasSpecies, HITRANLinelist],
+                      spec: StateVectorSpec, *, sounder: Optional[Sounder] = None,
+                      iasi: Optional[Sounder] = None, geom: Optional[ViewingGeometry] = None,
+                      T_sfc=None, eps_sfc: float = 1.0, observable: str = "bt",
+                      internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
+                      cutoff: float = 25.0, lm_cutoff: Optional[float] = None,
+                      apply_continuum: bool = False, with_ils: bool = True, apodization="gaussian",
+                      line_mixing=None, T_method="logp_at_pcg", source_function="cim",
+                      vmr_coupling: bool = True, x_far: float = 122.0,
+                      ctx: Optional[Context] = None) -> Jacobian:
+    """analytic_jacobian(prof, linelists, spec; ...) -> Jacobian
+    (src/Estimation/vmr_jacobian.jl:152-505).
+
+    K = dy/dx about the state packed from prof (+ T_sfc, eps_sfc), y = BT
+    (observable="bt") or radiance.  Full line set (no DPTMIN), as the
+    reference runs it.  Full-profile and column-scale blocks are assembled on
+    the device; a PartialColumns block is the device's per-level log-VMR
+    columns times its basis (the ILS and the channel map are linear, so this
+    equals projecting before the convolution)."""
+    if observable not in ("bt", "radiance"):
+        raise ValueError(f"observable must be :bt or :radiance, got :{observable}")
+    if apply_continuum:
+        raise NotImplementedError("continuum absorption is not implemented in the GPU engine; "
+                                  "pass apply_continuum=False")
+    if prof.n_levels != spec.n_levels:
+        raise ValueError(f"profile has {prof.n_levels} levels, spec expects {spec.n_levels}")
+    inst = iasi if sounder is None else sounder
+    inst = inst if inst is not None else IASIInstrument()
+    geom = geom if geom is not None else nadir_geometry()
+    ctx = ctx or default_context()
+    opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, 0.0, x_far, with_ils,
+                   apodization, line_mixing, T_method, source_function)
+    # device blocks: ColumnScale natively, PartialColumns as full log-VMR columns
+    blocks = spec.vmr_blocks
+    if len(blocks) > 4:
+        raise ValueError("at most 4 retrieved species per call")
+    for b in blocks:
+        if b.species not in prof.vmr:
+            raise ValueError(f"profile is missing retrieved species {b.species.name}")
+    is_cs = [b.basis is not None and b.m == 1 and bool(np.all(b.basis == 1.0)) for b in blocks]
+    retrieved = np.array([int(b.species) for b in blocks] or [0], np.int32)
+    colscale = np.array([1 if c else 0 for c in is_cs] or [0], np.int32)
+    batch = _Batch(ctx, [prof], linelists, [T_sfc], [eps_sfc], [geom.mu])
+    L = ctx.lib
+    n_l = prof.n_levels
+
+    ss = IrsStateSpec(int(spec.include_temperature), len(blocks), _p(retrieved), _p(colscale),
+                      int(spec.include_tsfc), int(spec.include_emissivity), int(spec.log_vmr),
+                      int(bool(vmr_coupling)), 0 if observable == "bt" else 1)
+    n_dev = L.irs_state_size(C.byref(ss), n_l)
+    grid = sounder_grid(inst)
+    Kdev = np.zeros((n_dev, grid.n))
+    y0 = np.zeros(grid.n)
+    xd = np.zeros(n_dev)
+    ctx.check(L.irs_jacobian(ctx.h, batch.arr, C.byref(_sounder(inst)), C.byref(opt), C.byref(ss),
+                             _p(Kdev), _p(y0), _p(xd)))
+    Kd = Kdev.T
+    K = np.zeros((grid.n, spec.n))
+    x0 = np.zeros(spec.n)
+    c = 0
+    if spec.include_temperature:
+        K[:, :n_l] = Kd[:, :n_l]
+        x0[:n_l] = xd[:n_l]
+        c = n_l
+    for b, cs in zip(blocks, is_cs):
+        r = slice(b.range.start, b.range.stop)
+        if b.basis is None:
+            K[:, r] = Kd[:, c:c + n_l]
+            x0[r] = xd[c:c + n_l]
+            c += n_l
+        elif cs:
+            K[:, r] = Kd[:, c:c + 1]
+            c += 1
+        else:
+            cols = Kd[:, c:c + n_l]
+            if not spec.log_vmr:      # reduced blocks are always multiplicative-log
+                cols = cols * prof.vmr[b.species][None, :]
+            K[:, r] = cols @ b.basis
+            c += n_l
+    if spec.include_tsfc:
+        K[:, spec.tsfc_index] = Kd[:, c]
+        x0[spec.tsfc_index] = xd[c]
+        c += 1
+    if spec.include_emissivity:
+        K[:, spec.emis_index] = Kd[:, c]
+        x0[spec.emis_index] = xd[c]
+    return Jacobian(K, spec, grid.nu, x0, y0, observable)
 
 
 def forward_dims(sounder: Sounder, **kw):

@@ -268,3 +268,159 @@ def wavenumber_grid(nu_min: float, nu_max: float, dnu: float) -> WavenumberGrid:
 
 def sounder_grid(s: Sounder) -> WavenumberGrid:
     return WavenumberGrid(s.nu_min, s.nu_max, s.dnu)
+
+
+# ---------------------------------------------------------------------------
+# Retrieval state layout (src/Estimation/state_vector.jl:45-118, 253-300)
+
+@dataclass(frozen=True)
+class FullProfile:
+    """One (log-)VMR per level (state_vector.jl:48-50)."""
+    species: GasSpecies
+
+
+@dataclass(frozen=True)
+class ColumnScale:
+    """One multiplicative log-scale of the whole profile (state_vector.jl:60-62)."""
+    species: GasSpecies
+
+
+@dataclass(frozen=True, eq=False)
+class PartialColumns:
+    """Log-VMR on a basis [n_levels x m] (state_vector.jl:87-104)."""
+    species: GasSpecies
+    basis: np.ndarray
+
+    def __post_init__(self):
+        b = np.asarray(self.basis, np.float64)
+        if b.ndim != 2 or b.shape[1] < 1:
+            raise ValueError("PartialColumns basis must be a non-empty matrix")
+        object.__setattr__(self, "basis", b)
+
+
+def _normalize_param(entry):
+    if isinstance(entry, (FullProfile, ColumnScale, PartialColumns)):
+        return entry
+    return FullProfile(GasSpecies(entry))
+
+
+@dataclass
+class VMRBlock:
+    species: GasSpecies
+    basis: Optional[np.ndarray]      # None = identity (full profile)
+    m: int
+    range: range
+
+
+class StateVectorSpec:
+    """StateVectorSpec(n_levels, species; include_temperature, include_tsfc,
+    include_emissivity, log_vmr) (state_vector.jl:253-300): the state is
+    [T levels][one block per retrieved gas][T_sfc][emissivity], 0-based ranges."""
+
+    def __init__(self, n_levels: int, species=(), *, include_temperature: bool = True,
+                 include_tsfc: bool = True, include_emissivity: bool = True, log_vmr: bool = True):
+        if n_levels <= 0:
+            raise ValueError("n_levels must be positive")
+        self.n_levels = int(n_levels)
+        self.include_temperature = bool(include_temperature)
+        self.include_tsfc = bool(include_tsfc)
+        self.include_emissivity = bool(include_emissivity)
+        self.log_vmr = bool(log_vmr)
+        idx = 0
+        self.temp_range = range(0, 0)
+        if include_temperature:
+            self.temp_range = range(idx, idx + n_levels)
+            idx += n_levels
+        self.vmr_blocks = []
+        for entry in species:
+            p = _normalize_param(entry)
+            if isinstance(p, FullProfile):
+                B, m = None, n_levels
+            elif isinstance(p, ColumnScale):
+                B, m = np.ones((n_levels, 1)), 1
+            else:
+                if p.basis.shape[0] != n_levels:
+                    raise ValueError(f"PartialColumns basis has {p.basis.shape[0]} levels, spec expects {n_levels}")
+                B, m = p.basis, p.basis.shape[1]
+            self.vmr_blocks.append(VMRBlock(p.species, B, m, range(idx, idx + m)))
+            idx += m
+        if len({b.species for b in self.vmr_blocks}) != len(self.vmr_blocks):
+            raise ValueError("a species may appear only once in the state vector")
+        self.tsfc_index = -1
+        if include_tsfc:
+            self.tsfc_index = idx
+            idx += 1
+        self.emis_index = -1
+        if include_emissivity:
+            self.emis_index = idx
+            idx += 1
+        self.n = idx
+
+    @property
+    def species(self):
+        return [b.species for b in self.vmr_blocks]
+
+    @property
+    def vmr_ranges(self):
+        return [(b.species, b.range) for b in self.vmr_blocks]
+
+    def __len__(self):
+        return self.n
+
+    def vmr_range(self, s) -> range:
+        for b in self.vmr_blocks:
+            if b.species == GasSpecies(s):
+                return b.range
+        raise KeyError(f"species {GasSpecies(s).name} is not in the state vector")
+
+    def labels(self):
+        """state_labels (state_vector.jl:335-350), 1-based level indices."""
+        out = [f"T[{i + 1}]" for i in self.temp_range]
+        pre = "logVMR" if self.log_vmr else "VMR"
+        for b in self.vmr_blocks:
+            if b.basis is None:
+                out += [f"{pre}_{b.species.name}[{i + 1}]" for i in range(self.n_levels)]
+            elif b.m == 1:
+                out.append(f"logVMRscale_{b.species.name}")
+            else:
+                out += [f"logVMRscale_{b.species.name}[{m + 1}]" for m in range(b.m)]
+        if self.include_tsfc:
+            out.append("T_sfc")
+        if self.include_emissivity:
+            out.append("emissivity")
+        return out
+
+
+VMR_FLOOR = 1e-30   # state_vector.jl:26
+
+
+def pack_state(spec: StateVectorSpec, prof: AtmosphericProfile, T_sfc: Optional[float] = None,
+               eps_sfc: float = 1.0) -> np.ndarray:
+    """pack_state (state_vector.jl:350-370)."""
+    if prof.n_levels != spec.n_levels:
+        raise ValueError(f"profile has {prof.n_levels} levels, spec expects {spec.n_levels}")
+    x = np.zeros(spec.n)
+    if spec.include_temperature:
+        x[spec.temp_range.start:spec.temp_range.stop] = prof.temperature
+    for b in spec.vmr_blocks:
+        if b.species not in prof.vmr:
+            raise ValueError(f"profile is missing retrieved species {b.species.name}")
+        if b.basis is None:
+            v = prof.vmr[b.species]
+            x[b.range.start:b.range.stop] = np.log(np.maximum(v, VMR_FLOOR)) if spec.log_vmr else v
+    if spec.include_tsfc:
+        x[spec.tsfc_index] = prof.temperature[0] if T_sfc is None else T_sfc
+    if spec.include_emissivity:
+        x[spec.emis_index] = eps_sfc
+    return x
+
+
+@dataclass
+class Jacobian:
+    """Jacobian struct (src/Estimation/jacobian_fd.jl:43-50): K is [n_y x n_x]."""
+    K: np.ndarray
+    spec: StateVectorSpec
+    nu: np.ndarray
+    x0: np.ndarray
+    y0: np.ndarray
+    observable: str
