@@ -1,0 +1,261 @@
+"""
+    IRSounderLBLAMD
+
+Julia binding of libirs.so (include/irs.h): `forward_model` and
+`analytic_jacobian` with the reference signatures (src/Sensor/sounder.jl:233-252,
+src/Estimation/vmr_jacobian.jl:152-173), each a single `ccall` per batch into
+hand-written gfx950 kernels.  The reference's own types (`AtmosphericProfile`,
+`HITRANLinelist`, `Sounder`, `ViewingGeometry`, `StateVectorSpec`, `Jacobian`,
+`WavenumberGrid`) are used unchanged; include this file after `using IRSounderLBL`.
+
+Line mixing: VP_Y needs the first-order coefficient Y of every CO2 line at the
+layer temperature; pass `line_mixing = Y::Vector{Float64}` (atm^-1, one per
+line of the CO2 list, in list order).  Continuum is not computed on the device:
+`apply_continuum=true` raises, as do the other unsupported keywords.
+"""
+module IRSounderLBLAMD
+
+using IRSounderLBL: AtmosphericProfile, HITRANLinelist, GasSpecies, Sounder, IASIInstrument,
+                    ViewingGeometry, nadir_geometry, WavenumberGrid, wavenumber_grid,
+                    StateVectorSpec, Jacobian, H2O, CO2
+
+const libirs = get(ENV, "IRS_LIBRARY",
+                   joinpath(@__DIR__, "..", "csrc", "libirs.so"))
+
+# ---- C structs (layout of include/irs.h) ------------------------------------
+struct CLinelist
+    n::Int64
+    mol::Ptr{Int8}; iso::Ptr{Int8}
+    nu::Ptr{Float64}; S::Ptr{Float64}
+    gamma_air::Ptr{Float32}; gamma_self::Ptr{Float32}
+    elower::Ptr{Float64}; n_air::Ptr{Float32}; delta_air::Ptr{Float32}
+    y_lm::Ptr{Float64}
+end
+
+struct CSounder
+    nu_min::Float64; nu_max::Float64; dnu::Float64
+    opd_max::Float64; fwhm_gauss::Float64
+    n_channels::Int64
+end
+
+struct CProfile
+    n_levels::Cint
+    p_hpa::Ptr{Float64}; T::Ptr{Float64}
+    n_species::Cint
+    species::Ptr{Cint}; linelist::Ptr{Int32}; vmr::Ptr{Float64}
+    T_sfc::Float64; eps_sfc::Float64; mu::Float64
+end
+
+mutable struct COptions
+    internal_dnu::Float64; high_res_factor::Cint; cutoff::Float64; lm_cutoff::Float64
+    dptmn::Float64; x_far::Float64; with_ils::Cint; apodization::Cint; line_mixing::Cint
+    T_method::Cint; source_function::Cint; reserved0::Cint
+end
+
+struct CStateSpec
+    include_temperature::Cint; n_retrieved::Cint
+    retrieved::Ptr{Cint}; column_scale::Ptr{Cint}
+    include_tsfc::Cint; include_emissivity::Cint; log_vmr::Cint; vmr_coupling::Cint
+    observable::Cint
+end
+
+const APOD = Dict(:gaussian => 0, :norton_beer_weak => 1, :norton_beer_medium => 2,
+                  :norton_beer_strong => 3)
+
+# ---- context and line-list cache --------------------------------------------
+mutable struct Context
+    h::Ptr{Cvoid}
+    lists::IdDict{Any, Int32}          # HITRANLinelist (+ Y) => device handle
+    keep::Vector{Any}
+end
+
+function check(ctx::Context, rc::Integer)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:irs_last_error, libirs), Cstring, (Ptr{Cvoid},), ctx.h))
+    error(msg)                          # same ErrorException text as the reference check
+end
+
+const _CTX = Ref{Union{Nothing, Context}}(nothing)
+
+function context(device::Integer = parse(Int, get(ENV, "IRS_DEVICE", "0")))
+    _CTX[] === nothing || return _CTX[]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:irs_create, libirs), Cint, (Cint, Ref{Ptr{Cvoid}}), device, h)
+    rc == 0 || error("irs_create failed ($rc): no usable gfx950 device")
+    ctx = Context(h[], IdDict{Any, Int32}(), Any[])
+    finalizer(c -> ccall((:irs_destroy, libirs), Cvoid, (Ptr{Cvoid},), c.h), ctx)
+    _CTX[] = ctx
+end
+
+function upload!(ctx::Context, ll::HITRANLinelist, y::Union{Nothing, Vector{Float64}})
+    key = y === nothing ? ll : (ll, y)
+    haskey(ctx.lists, key) && return ctx.lists[key]
+    L = ll.lines
+    mol = Int8[l.mol_id for l in L];        iso = Int8[l.iso_id for l in L]
+    nu = Float64[l.wavenumber for l in L];  S = Float64[l.intensity for l in L]
+    ga = Float32[l.air_broad for l in L];   gs = Float32[l.self_broad for l in L]
+    el = Float64[l.lower_energy for l in L]; na = Float32[l.temp_depend for l in L]
+    da = Float32[l.pressure_shift for l in L]
+    y === nothing || length(y) == length(L) || error("line_mixing Y must have one entry per CO2 line")
+    h = Ref{Int32}(-1)
+    GC.@preserve mol iso nu S ga gs el na da y begin
+        c = CLinelist(length(L), pointer(mol), pointer(iso), pointer(nu), pointer(S), pointer(ga),
+                      pointer(gs), pointer(el), pointer(na), pointer(da),
+                      y === nothing ? Ptr{Float64}(C_NULL) : pointer(y))
+        check(ctx, ccall((:irs_linelist_upload, libirs), Cint,
+                         (Ptr{Cvoid}, Ref{CLinelist}, Ref{Int32}), ctx.h, c, h))
+    end
+    ctx.lists[key] = h[]
+end
+
+function options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with_ils,
+                 apodization, line_mixing, T_method, source_function, x_far = 122.0)
+    o = COptions(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    ccall((:irs_default_options, libirs), Cvoid, (Ref{COptions},), o)
+    o.internal_dnu = internal_dnu === nothing ? 0.0 : internal_dnu
+    o.high_res_factor = high_res_factor === nothing ? 0 : high_res_factor
+    o.cutoff = cutoff; o.lm_cutoff = lm_cutoff; o.dptmn = dptmn; o.x_far = x_far
+    o.with_ils = with_ils
+    haskey(APOD, apodization) ||
+        error("Unknown apodization :$apodization; expected :gaussian or a Norton-Beer style")
+    o.apodization = APOD[apodization]
+    o.line_mixing = line_mixing === nothing ? 0 : 1
+    T_method in (:logp_at_pcg, :mass_weighted) ||
+        error("T_method must be :logp_at_pcg or :mass_weighted, got :$T_method")
+    o.T_method = T_method === :logp_at_pcg ? 0 : 1
+    source_function in (:toon, :cim) || error("source_function must be :toon or :cim, got :$source_function")
+    o.source_function = source_function === :toon ? 0 : 1
+    return o
+end
+
+csounder(s::Sounder) = CSounder(s.ν_min, s.ν_max, s.Δν, s.opd_max, s.fwhm_gauss, s.n_channels)
+
+function cprofile(ctx, prof::AtmosphericProfile, linelists, line_mixing, T_sfc, ε_sfc, μ, keep)
+    species = collect(keys(prof.vmr))
+    sp = Cint[Int(s) for s in species]
+    hs = Int32[haskey(linelists, s) && !isempty(linelists[s].lines) ?
+               upload!(ctx, linelists[s], (s == CO2 && line_mixing !== nothing) ? line_mixing : nothing) :
+               Int32(-1) for s in species]
+    vmr = reduce(vcat, [Float64.(prof.vmr[s]) for s in species])
+    p = Float64.(prof.pressure); T = Float64.(prof.temperature)
+    append!(keep, (sp, hs, vmr, p, T))
+    return CProfile(length(p), pointer(p), pointer(T), length(sp), pointer(sp), pointer(hs),
+                    pointer(vmr), T_sfc === nothing ? 0.0 : T_sfc, ε_sfc, μ), species
+end
+
+"""
+    forward_model(prof, linelists; sounder, geom, ...) -> (WavenumberGrid, R, BT)
+
+src/Sensor/sounder.jl:233-360 on the GPU.  `prof` may also be a vector of
+profiles (with `geom`, `T_sfc`, `ε_sfc` scalars or vectors): one call, one
+device launch sequence; R and BT are then n_channels x n_profiles.
+"""
+function forward_model(prof, linelists::Dict{GasSpecies, HITRANLinelist};
+                       sounder::Union{Sounder, Nothing} = nothing,
+                       iasi::Sounder = IASIInstrument(), geom = nadir_geometry(),
+                       T_sfc = nothing, ε_sfc = 1.0,
+                       internal_dnu::Union{Float64, Nothing} = 0.001,
+                       high_res_factor::Union{Int, Nothing} = nothing,
+                       cutoff::Float64 = 25.0, lm_cutoff::Float64 = cutoff, dptmn::Float64 = 1e-6,
+                       apply_continuum::Bool = true, continua = (:h2o, :co2, :co2_cia, :n2, :o2),
+                       with_ils::Bool = true, apodization::Symbol = :gaussian,
+                       line_mixing = nothing, T_method::Symbol = :logp_at_pcg,
+                       source_function::Symbol = :cim, backend = nothing)
+    apply_continuum && error("continuum absorption is not computed by libirs; pass apply_continuum=false")
+    snd = sounder === nothing ? iasi : sounder
+    profs = prof isa AbstractVector ? prof : [prof]
+    nb = length(profs)
+    pick(x, q) = x isa AbstractVector ? x[q] : x
+    ctx = context()
+    o = options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with_ils, apodization,
+                line_mixing, T_method, source_function)
+    keep = Any[]
+    cps = Vector{CProfile}(undef, nb)
+    for q in 1:nb
+        cps[q], _ = cprofile(ctx, profs[q], linelists, line_mixing, pick(T_sfc, q), pick(ε_sfc, q),
+                             pick(geom, q).μ, keep)
+    end
+    grid = wavenumber_grid(snd.ν_min, snd.ν_max, snd.Δν)
+    R = zeros(Float64, grid.n, nb); BT = zeros(Float64, grid.n, nb)
+    cs = csounder(snd)
+    GC.@preserve keep cps R BT begin
+        check(ctx, ccall((:irs_forward, libirs), Cint,
+                         (Ptr{Cvoid}, Ptr{CProfile}, Cint, Ref{CSounder}, Ref{COptions},
+                          Ptr{Float64}, Ptr{Float64}),
+                         ctx.h, cps, nb, cs, o, R, BT))
+    end
+    prof isa AbstractVector && return grid, R, BT
+    return grid, R[:, 1], BT[:, 1]
+end
+
+"""
+    analytic_jacobian(prof, linelists, spec; ...) -> Jacobian
+
+src/Estimation/vmr_jacobian.jl:152-505 on the GPU (full line set, no DPTMIN).
+FullProfile and ColumnScale blocks are assembled on the device; a
+PartialColumns block is the per-level log-VMR columns times its basis.
+"""
+function analytic_jacobian(prof::AtmosphericProfile, linelists::Dict{GasSpecies, HITRANLinelist},
+                           spec::StateVectorSpec;
+                           sounder::Union{Sounder, Nothing} = nothing,
+                           iasi::Sounder = IASIInstrument(), geom::ViewingGeometry = nadir_geometry(),
+                           T_sfc::Union{Float64, Nothing} = nothing, ε_sfc::Float64 = 1.0,
+                           observable::Symbol = :bt,
+                           internal_dnu::Union{Float64, Nothing} = 0.001,
+                           high_res_factor::Union{Int, Nothing} = nothing,
+                           cutoff::Float64 = 25.0, lm_cutoff::Float64 = cutoff,
+                           apply_continuum::Bool = true, continua = (:h2o, :co2, :co2_cia, :n2, :o2),
+                           with_ils::Bool = true, apodization::Symbol = :gaussian,
+                           line_mixing = nothing, T_method::Symbol = :logp_at_pcg,
+                           source_function::Symbol = :cim, vmr_coupling::Bool = true,
+                           backend = nothing)::Jacobian
+    observable in (:bt, :radiance) || error("observable must be :bt or :radiance, got :$observable")
+    apply_continuum && error("continuum absorption is not computed by libirs; pass apply_continuum=false")
+    length(prof.temperature) == spec.n_levels ||
+        error("profile has $(length(prof.temperature)) levels, spec expects $(spec.n_levels)")
+    snd = sounder === nothing ? iasi : sounder
+    ctx = context()
+    o = options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn = 0.0, with_ils,
+                apodization, line_mixing, T_method, source_function)
+    keep = Any[]
+    cp, _ = cprofile(ctx, prof, linelists, line_mixing, T_sfc, ε_sfc, geom.μ, keep)
+    blocks = spec.vmr_blocks
+    iscs = [b.basis !== nothing && size(b.basis, 2) == 1 && all(==(1.0), b.basis) for b in blocks]
+    retr = Cint[Int(b.species) for b in blocks]
+    cscl = Cint[c ? 1 : 0 for c in iscs]
+    ss = CStateSpec(spec.include_temperature, length(blocks), pointer(retr), pointer(cscl),
+                    spec.include_tsfc, spec.include_emissivity, spec.log_vmr, vmr_coupling,
+                    observable === :bt ? 0 : 1)
+    nl = spec.n_levels
+    ndev = ccall((:irs_state_size, libirs), Cint, (Ref{CStateSpec}, Cint), ss, nl)
+    grid = wavenumber_grid(snd.ν_min, snd.ν_max, snd.Δν)
+    Kd = zeros(Float64, grid.n, ndev); y0 = zeros(Float64, grid.n); xd = zeros(Float64, ndev)
+    cs = csounder(snd)
+    GC.@preserve keep retr cscl Kd y0 xd begin
+        check(ctx, ccall((:irs_jacobian, libirs), Cint,
+                         (Ptr{Cvoid}, Ref{CProfile}, Ref{CSounder}, Ref{COptions}, Ref{CStateSpec},
+                          Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.h, cp, cs, o, ss, Kd, y0, xd))
+    end
+    K = zeros(Float64, grid.n, spec.n); x0 = zeros(Float64, spec.n)
+    c = 0
+    if spec.include_temperature
+        K[:, spec.temp_range] .= Kd[:, 1:nl]; x0[spec.temp_range] .= xd[1:nl]; c = nl
+    end
+    for (b, iscol) in zip(blocks, iscs)
+        if b.basis === nothing
+            K[:, b.range] .= Kd[:, c+1:c+nl]; x0[b.range] .= xd[c+1:c+nl]; c += nl
+        elseif iscol
+            K[:, b.range] .= Kd[:, c+1:c+1]; c += 1
+        else
+            cols = Kd[:, c+1:c+nl]
+            spec.log_vmr || (cols = cols .* Float64.(prof.vmr[b.species])')
+            K[:, b.range] .= cols * b.basis; c += nl
+        end
+    end
+    spec.include_tsfc && (K[:, spec.tsfc_index] .= Kd[:, c+1]; x0[spec.tsfc_index] = xd[c+1]; c += 1)
+    spec.include_emissivity && (K[:, spec.emis_index] .= Kd[:, c+1]; x0[spec.emis_index] = xd[c+1])
+    return Jacobian(K, spec, collect(grid.ν), x0, y0, observable)
+end
+
+end # module
