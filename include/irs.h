@@ -29,6 +29,7 @@
  *   irs_jacobian           analytic_jacobian        src/Estimation/vmr_jacobian.jl:152-505
  *   irs_state_size         StateVectorSpec length   src/Estimation/state_vector.jl:253-300
  *   irs_linelist_upload    HITRANLinelist(lines)    src/HITRAN/linelist.jl:52-56
+ *   irs_set_schedule / irs_get_schedule   detect_backend / ka_backend  src/Parallel/strategy.jl:21-90
  */
 #ifndef IRS_H
 #define IRS_H
@@ -135,7 +136,24 @@ typedef struct {
     int64_t active_pairs;       /* nominal pairs |nu - nu0| <= cutoff over active lines */
     int64_t dispersive_pairs;   /* nominal VP_Y pairs */
     int64_t core_pairs_bound;   /* pairs in core ranges (evaluated with w(z)) */
+    double  ms_device;          /* device time of the whole call (graph or kernels) */
+    int64_t graph_launches;     /* profile groups run from a captured graph */
+    int32_t tile_points;        /* schedule used by the line sum */
+    int32_t persistent;
+    int32_t autotuned;          /* 1: this call ran the autotuner */
+    int32_t reserved1;
 } irs_stats;
+
+/* Line-sum schedule of irs_forward.  No counterpart in the reference: it
+ * replaces the backend/thread choice of src/Parallel/strategy.jl:21-90. */
+typedef struct {
+    int tile_points;            /* grid points per thread per tile: 4, 8 or 16 */
+    int persistent;             /* 1: persistent work-queue kernel over heaviest-first tiles */
+    int use_graph;              /* 1: capture each profile group's device sequence in a hipGraph
+                                   and replay it on later calls of the same shape */
+    int autotune;               /* 1: time the (tile_points, persistent) candidates on the first
+                                   profile of each new shape and keep the fastest */
+} irs_schedule;
 
 int         irs_version(void);
 /* Fills buf with "name;gcnArch;CUs;clockMHz;memGB".  Returns IRS_ENODEV without a GPU. */
@@ -145,6 +163,9 @@ void        irs_destroy(irs_ctx *ctx);
 const char *irs_last_error(const irs_ctx *ctx);
 void        irs_default_options(irs_fm_options *opt);
 int         irs_last_stats(const irs_ctx *ctx, irs_stats *out);
+
+int         irs_set_schedule(irs_ctx *ctx, const irs_schedule *schedule);
+int         irs_get_schedule(const irs_ctx *ctx, irs_schedule *schedule);
 
 /* Copies a sorted line list to the device; returns a handle. */
 int irs_linelist_upload(irs_ctx *ctx, const irs_linelist *ll, int32_t *handle);

@@ -49,6 +49,32 @@ struct ListHost {
     std::vector<double> nu_host;    // for host-side geometry
 };
 
+// Grow-only pinned host buffer (staging for graph-captured copies).
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Everything the device sequence of one profile group depends on.  Graphs are
+// cached by the byte image of this struct (zero-filled before use).
+struct GroupPlan {
+    const void *items, *rec0, *rec1, *rec2, *keys, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
+    const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out;
+    int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes;
+    int gn, L, ns, P, pblocks, tiles, source_function, with_ils;
+    double nu_lo, dnu, an, sn, den;
+};
+
+struct GraphEntry {
+    GroupPlan plan;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct TapsKey {
     double dnu, opd, fwhm;
     int apod, n_pts;
@@ -70,7 +96,13 @@ struct irs_ctx {
     DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
     irs_stats stats{};
-    int ls_P = 4;
+    irs_schedule sched{4, 0, 0, 0};
+    std::map<std::string, std::pair<int, int>> tuned;     // shape -> (P, persistent)
+    std::map<std::string, DevBuf> orders;                 // shape -> heaviest-first tile order
+    std::map<int, int> capacity;                          // P -> resident persistent workgroups
+    DevBuf d_queue, d_prof;
+    PinBuf h_items, h_prof, h_out;
+    std::vector<GraphEntry> graphs;
 };
 
 namespace {
@@ -331,7 +363,84 @@ int build_keys(irs_ctx *ctx, const std::vector<int32_t> &lists, double nu_lo, do
     return IRS_OK;
 }
 
-// prep + line sum for a set of items laid out [n_prof][n_layers][n_species].
+hipError_t ensure_pinned(PinBuf &b, size_t bytes) {
+    if (bytes <= b.cap) return hipSuccess;
+    b.release();
+    const size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&b.p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return e;
+    }
+    b.cap = want;
+    return hipSuccess;
+}
+
+std::string shape_key(const std::vector<const ItemDev *> &per_species, int64_t n_hi, double nu_lo, double dnu,
+                      int P) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%lld|%.17g|%.17g|%d", (long long)n_hi, nu_lo, dnu, P);
+    std::string k(buf);
+    for (const ItemDev *it : per_species) {
+        std::snprintf(buf, sizeof buf, "|%d:%d:%d:%.9g:%.9g", it->list, it->lm, it->n, (double)it->C,
+                      (double)it->Clm);
+        k += buf;
+    }
+    return k;
+}
+
+// Heaviest-first order of the spectral tiles: the cost of a tile is the number
+// of lines whose window reaches it, weighted 1.6 on line-mixing lists (the
+// dispersive term), from host copies of the line centres.
+int tile_order(irs_ctx *ctx, const std::vector<const ItemDev *> &per_species, int64_t n_hi, double nu_lo,
+               double dnu, int P, const int32_t **order) {
+    const std::string key = shape_key(per_species, n_hi, nu_lo, dnu, P);
+    auto f = ctx->orders.find(key);
+    if (f != ctx->orders.end()) {
+        *order = (const int32_t *)f->second.p;
+        return IRS_OK;
+    }
+    const int64_t W = IRS_LS_THREADS * (int64_t)P;
+    const int tiles = (int)((n_hi + W - 1) / W);
+    std::vector<double> cost(tiles, 0.0);
+    for (const ItemDev *it : per_species) {
+        const ListHost &lh = ctx->lists[it->list];
+        const int64_t C = (int64_t)std::ceil(std::max((double)it->C, it->lm ? (double)it->Clm : 0.0)) + it->M;
+        // histogram of line keys over [-C, n_hi + C)
+        const int64_t span = n_hi + 2 * C + 1;
+        std::vector<int32_t> pre(span + 1, 0);
+        for (double nu : lh.nu_host) {
+            const double k = std::floor((nu - nu_lo) / dnu);
+            if (k < (double)-C || k >= (double)(n_hi + C)) continue;
+            pre[(int64_t)k + C + 1] += 1;
+        }
+        for (int64_t i = 1; i <= span; ++i) pre[i] += pre[i - 1];
+        const double w = it->lm ? 1.6 : 1.0;
+        for (int t = 0; t < tiles; ++t) {
+            const int64_t a = t * W, b = std::min(a + W, n_hi);   // lines with key in [a - C, b + C)
+            cost[t] += w * (double)(pre[b + 2 * C] - pre[a]);
+        }
+    }
+    std::vector<int32_t> ord(tiles);
+    for (int t = 0; t < tiles; ++t) ord[t] = t;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+    DevBuf &b = ctx->orders[key];
+    HIPCHK(ensure(b, sizeof(int32_t) * tiles));
+    HIPCHK(hipMemcpy(b.p, ord.data(), sizeof(int32_t) * tiles, hipMemcpyHostToDevice));
+    *order = (const int32_t *)b.p;
+    return IRS_OK;
+}
+
+int persistent_blocks(irs_ctx *ctx, int P) {
+    auto f = ctx->capacity.find(P);
+    if (f != ctx->capacity.end()) return f->second;
+    const int c = linesum_persistent_capacity(P);
+    ctx->capacity[P] = c;
+    return c;
+}
+
+// prep + line sum for a set of items laid out [n_prof][n_layers][n_species]
+// (eager; used by the layer-level export).
 int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_layers, int n_species,
                 double nu_lo, double dnu, int64_t n_hi, float *out) {
     int64_t rec = 0;
@@ -346,9 +455,11 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_items, sizeof(ItemDev) * std::max<size_t>(items.size(), 1)));
     HIPCHK(ensure(ctx->d_counters, 64));
+    HIPCHK(ensure(ctx->d_queue, 64));
     HIPCHK(hipMemcpyAsync(ctx->d_items.p, items.data(), sizeof(ItemDev) * items.size(), hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_counters.p, 0, 64, ctx->stream));
+    HIPCHK(launch_reset_counters((unsigned *)ctx->d_queue.p, (unsigned long long *)ctx->d_counters.p, 3,
+                                 ctx->stream));
     PrepArgs pa{};
     pa.items = (const ItemDev *)ctx->d_items.p;
     pa.lists = (const LineListDev *)ctx->d_lists.p;
@@ -376,12 +487,16 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     la.n_species = n_species;
     la.nu_lo = nu_lo;
     la.dnu = dnu;
-    const int P = ctx->ls_P;
+    la.n_prof = n_prof;
+    const int P = ctx->sched.tile_points;
     la.tiles = (int)((n_hi + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
     la.tile_order = nullptr;
-    HIPCHK(launch_linesum(la, n_prof, P, ctx->stream));
+    la.queue = (unsigned *)ctx->d_queue.p;
+    HIPCHK(launch_linesum(la, P, ctx->sched.persistent ? persistent_blocks(ctx, P) : 0, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     ctx->stats.linesum_launches += 1;
+    ctx->stats.tile_points = P;
+    ctx->stats.persistent = ctx->sched.persistent;
     return IRS_OK;
 }
 
@@ -563,7 +678,7 @@ int irs_create(int device, irs_ctx **out) {
     }
     if (const char *e = std::getenv("IRS_LS_P")) {
         int v = std::atoi(e);
-        if (v == 4 || v == 8 || v == 16) ctx->ls_P = v;
+        if (v == 4 || v == 8 || v == 16) ctx->sched.tile_points = v;
     }
     *out = ctx;
     return IRS_OK;
@@ -583,6 +698,14 @@ void irs_destroy(irs_ctx *ctx) {
                       &ctx->j_fac};
     for (DevBuf *b : bufs) b->release();
     for (auto &kv : ctx->taps) kv.second.release();
+    for (auto &kv : ctx->orders) kv.second.release();
+    for (auto &g : ctx->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->d_queue.release();
+    ctx->d_prof.release();
+    ctx->h_items.release();
+    ctx->h_prof.release();
+    ctx->h_out.release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -862,6 +985,193 @@ int irs_forward_dims(const irs_sounder *snd, const irs_fm_options *opt, double *
     return IRS_OK;
 }
 
+// Device sequence of one profile group: staging copies, counter reset, prep,
+// line sum, RTE, channel ILS, copy-back.  Issues no allocation and no memset,
+// so it can be captured in a graph.  Events are recorded only when timing.
+static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) {
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    const int L = pl.L;
+    const size_t NL = (size_t)pl.gn;
+#define ENQ(x)                      \
+    do {                            \
+        e = (x);                    \
+        if (e != hipSuccess) return e; \
+    } while (0)
+    ENQ(hipMemcpyAsync((void *)pl.items, pl.h_items, (size_t)pl.items_bytes, hipMemcpyHostToDevice, st));
+    ENQ(hipMemcpyAsync((void *)pl.prof, pl.h_prof, (size_t)pl.prof_bytes, hipMemcpyHostToDevice, st));
+    ENQ(launch_reset_counters((unsigned *)pl.queue, (unsigned long long *)pl.counters, 3, st));
+    PrepArgs pa{};
+    pa.items = (const ItemDev *)pl.items;
+    pa.lists = (const LineListDev *)pl.lists;
+    pa.tips_q = (const double *)pl.tips;
+    pa.n_items = (int)pl.n_items;
+    pa.nu_lo = pl.nu_lo;
+    pa.dnu = pl.dnu;
+    pa.rec0 = (float4 *)pl.rec0;
+    pa.rec1 = (float4 *)pl.rec1;
+    pa.rec2 = (float4 *)pl.rec2;
+    pa.counters = (unsigned long long *)pl.counters;
+    pa.n_hi = pl.n;
+    if (timing) ENQ(hipEventRecord(ctx->ev[0], st));
+    ENQ(launch_prep(pa, (int)pl.max_lines, st));
+    if (timing) ENQ(hipEventRecord(ctx->ev[1], st));
+    LinesumArgs la{};
+    la.items = pa.items;
+    la.rec0 = pa.rec0;
+    la.rec1 = pa.rec1;
+    la.rec2 = pa.rec2;
+    la.keys = (const int32_t *)pl.keys;
+    la.out = (float *)pl.tau;
+    la.n_hi = pl.n;
+    la.n_layers = L;
+    la.n_species = pl.ns;
+    la.nu_lo = pl.nu_lo;
+    la.dnu = pl.dnu;
+    la.tiles = pl.tiles;
+    la.n_prof = pl.gn;
+    la.tile_order = (const int32_t *)pl.order;
+    la.queue = (unsigned *)pl.queue;
+    ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
+    if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
+    const double *prof = (const double *)pl.prof;
+    RteArgs ra{};
+    ra.tau32 = (const float *)pl.tau;
+    ra.n = pl.n;
+    ra.n_layers = L;
+    ra.T_lev = prof;
+    ra.T_ave = prof + NL * (L + 1);
+    ra.scal = prof + NL * (2 * L + 1);
+    ra.source_function = pl.source_function;
+    ra.an = pl.an;
+    ra.sn = pl.sn;
+    ra.den = pl.den;
+    ra.out = (double *)pl.rhi;
+    ra.n_prof = pl.gn;
+    ENQ(launch_rte(ra, st));
+    if (timing) ENQ(hipEventRecord(ctx->ev[3], st));
+    IlsArgs ia{};
+    ia.spec = (const double *)pl.rhi;
+    ia.n = pl.n;
+    ia.taps = (const double *)pl.taps;
+    ia.half = pl.with_ils ? pl.half : 0;
+    ia.dnu = pl.with_ils ? pl.dnu : 1.0;
+    ia.ch_idx = (const int64_t *)pl.chidx;
+    ia.ch_w = (const double *)pl.chw;
+    ia.ch_nu = (const double *)pl.chnu;
+    ia.n_ch = pl.nch;
+    ia.r_out = (double *)pl.rch;
+    ia.bt_out = (double *)pl.bt;
+    ia.n_prof = pl.gn;
+    ENQ(launch_ils_channels(ia, st));
+    if (timing) ENQ(hipEventRecord(ctx->ev[4], st));
+    double *hout = (double *)pl.h_out;
+    ENQ(hipMemcpyAsync(hout, pl.rch, sizeof(double) * pl.nch * NL, hipMemcpyDeviceToHost, st));
+    ENQ(hipMemcpyAsync(hout + pl.nch * NL, pl.bt, sizeof(double) * pl.nch * NL, hipMemcpyDeviceToHost, st));
+#undef ENQ
+    return hipSuccess;
+}
+
+// Runs the group from a cached graph, capturing it on the first use of a plan.
+static int launch_group_graph(irs_ctx *ctx, const GroupPlan &pl) {
+    GraphEntry *hit = nullptr;
+    for (auto &g : ctx->graphs)
+        if (std::memcmp(&g.plan, &pl, sizeof(GroupPlan)) == 0) hit = &g;
+    if (!hit) {
+        if (ctx->graphs.size() >= 4) {
+            (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+            ctx->graphs.erase(ctx->graphs.begin());
+        }
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_group(ctx, pl, false);
+        hipGraph_t graph = nullptr;
+        hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(ctx, IRS_EHIP, std::string("graph capture: ") +
+                                           hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        GraphEntry ge;
+        ge.plan = pl;
+        e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(ctx, IRS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        ctx->graphs.push_back(ge);
+        hit = &ctx->graphs.back();
+    }
+    HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+    HIPCHK(hipGraphLaunch(hit->exec, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+    ctx->stats.graph_launches += 1;
+    return IRS_OK;
+}
+
+// Times each (tile_points, persistent) candidate on the first profile of the
+// group (records already prepared by a prep launch) and keeps the fastest.
+static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector<const ItemDev *> &per_species,
+                   const std::string &key) {
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync((void *)base.items, base.h_items, (size_t)base.items_bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_reset_counters((unsigned *)base.queue, (unsigned long long *)base.counters, 3, st));
+    PrepArgs pa{};
+    pa.items = (const ItemDev *)base.items;
+    pa.lists = (const LineListDev *)base.lists;
+    pa.tips_q = (const double *)base.tips;
+    pa.n_items = (int)(base.L * base.ns);     // first profile only
+    pa.nu_lo = base.nu_lo;
+    pa.dnu = base.dnu;
+    pa.rec0 = (float4 *)base.rec0;
+    pa.rec1 = (float4 *)base.rec1;
+    pa.rec2 = (float4 *)base.rec2;
+    pa.counters = (unsigned long long *)base.counters;
+    pa.n_hi = base.n;
+    HIPCHK(launch_prep(pa, (int)base.max_lines, st));
+    const int cand_P[3] = {4, 8, 16};
+    double best = 1e300;
+    int bestP = ctx->sched.tile_points, bestPers = ctx->sched.persistent;
+    for (int ci = 0; ci < 3; ++ci)
+        for (int pers = 0; pers < 2; ++pers) {
+            const int P = cand_P[ci];
+            const int32_t *ord = nullptr;
+            int rc = tile_order(ctx, per_species, base.n, base.nu_lo, base.dnu, P, &ord);
+            if (rc) return rc;
+            LinesumArgs la{};
+            la.items = pa.items;
+            la.rec0 = pa.rec0;
+            la.rec1 = pa.rec1;
+            la.rec2 = pa.rec2;
+            la.keys = (const int32_t *)base.keys;
+            la.out = (float *)base.tau;
+            la.n_hi = base.n;
+            la.n_layers = base.L;
+            la.n_species = base.ns;
+            la.nu_lo = base.nu_lo;
+            la.dnu = base.dnu;
+            la.tiles = (int)((base.n + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
+            la.n_prof = 1;
+            la.tile_order = ord;
+            la.queue = (unsigned *)base.queue;
+            const int pb = pers ? persistent_blocks(ctx, P) : 0;
+            if (pers && pb <= 0) continue;
+            double t_min = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+                HIPCHK(launch_reset_counters((unsigned *)base.queue, nullptr, 0, st));
+                HIPCHK(hipEventRecord(ctx->ev[5], st));
+                HIPCHK(launch_linesum(la, P, pb, st));
+                HIPCHK(hipEventRecord(ctx->ev[6], st));
+                HIPCHK(hipEventSynchronize(ctx->ev[6]));
+                t_min = std::min(t_min, elapsed_ms(ctx->ev[5], ctx->ev[6]));
+            }
+            if (t_min < best) {
+                best = t_min;
+                bestP = P;
+                bestPers = pers;
+            }
+        }
+    ctx->tuned[key] = {bestP, bestPers};
+    return IRS_OK;
+}
+
 int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
                 const irs_fm_options *opt_in, double *R, double *BT) {
     if (!ctx) return IRS_EINVAL;
@@ -889,20 +1199,21 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
         if (pr.n_levels - 1 != L || pr.n_species != ns)
             return fail(ctx, IRS_EINVAL, "all profiles of a batch need the same levels and species");
         if (!(pr.mu > 0.0 && pr.mu <= 1.0)) return fail(ctx, IRS_EINVAL, "μ must be in (0, 1]");
-        for (int s = 0; s < ns; ++s)
+        for (int s = 0; s < ns; ++s) {
+            if (pr.species[s] != profiles[0].species[s] || pr.linelist[s] != profiles[0].linelist[s])
+                return fail(ctx, IRS_EINVAL, "all profiles of a batch need the same species and line lists");
             if (pr.linelist[s] >= 0) {
                 int rc = check_list(ctx, pr.linelist[s]);
                 if (rc) return rc;
             }
+        }
     }
+    if (L < 1) return fail(ctx, IRS_EINVAL, "need at least two levels");
     int rc = ensure_common(ctx);
     if (rc) return rc;
     std::vector<int32_t> used;
     for (int s = 0; s < ns; ++s)
         if (profiles[0].linelist[s] >= 0) used.push_back(profiles[0].linelist[s]);
-    for (int q = 1; q < n_prof; ++q)
-        for (int s = 0; s < ns; ++s)
-            if (profiles[q].linelist[s] >= 0) used.push_back(profiles[q].linelist[s]);
     std::map<int32_t, int32_t> koff;
     rc = build_keys(ctx, used, nu_lo, dnu, koff);
     if (rc) return rc;
@@ -912,6 +1223,16 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
     if (opt.with_ils) {
         rc = get_taps(ctx, dnu, snd->opd_max, snd->fwhm_gauss, opt.apodization, 2048, &taps, &half);
         if (rc) return rc;
+    } else {
+        // identity "ILS": a single unit tap, kept in the context
+        TapsKey key{0.0, 0.0, 0.0, -1, 1};
+        DevBuf &b = ctx->taps[key];
+        if (!b.p) {
+            const double one = 1.0;
+            HIPCHK(ensure(b, sizeof(double)));
+            HIPCHK(hipMemcpy(b.p, &one, sizeof(double), hipMemcpyHostToDevice));
+        }
+        taps = (double *)b.p;
     }
     std::vector<int64_t> cidx;
     std::vector<double> cw, cnu;
@@ -920,127 +1241,181 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
     HIPCHK(ensure(ctx->d_chidx, sizeof(int64_t) * nch));
     HIPCHK(ensure(ctx->d_chw, sizeof(double) * nch));
     HIPCHK(ensure(ctx->d_chnu, sizeof(double) * nch));
-    HIPCHK(ensure(ctx->d_rch, sizeof(double) * nch * n_prof));
-    HIPCHK(ensure(ctx->d_bt, sizeof(double) * nch * n_prof));
     HIPCHK(hipMemcpyAsync(ctx->d_chidx.p, cidx.data(), sizeof(int64_t) * nch, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_chw.p, cw.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_chnu.p, cnu.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
 
-    // profiles are processed in groups bounded by device memory for records and tau
+    // ---- profile groups: records + tau + R_hi per profile within a memory budget
     int64_t lines_per_prof = 0;
+    int max_lines = 0;
     for (int s = 0; s < ns; ++s)
-        if (profiles[0].linelist[s] >= 0) lines_per_prof += ctx->lists[profiles[0].linelist[s]].n;
-    const double bytes_per_prof = (double)L * (lines_per_prof * 48.0 + n * 4.0);
-    int group = (int)std::max(1.0, std::min((double)n_prof, 24.0e9 / std::max(bytes_per_prof, 1.0)));
-    if (const char *e = std::getenv("IRS_PROFILE_GROUP")) group = std::max(1, std::min(n_prof, std::atoi(e)));
+        if (profiles[0].linelist[s] >= 0) {
+            lines_per_prof += ctx->lists[profiles[0].linelist[s]].n;
+            max_lines = std::max<int>(max_lines, (int)ctx->lists[profiles[0].linelist[s]].n);
+        }
+    const double bytes_per_prof = (double)L * (lines_per_prof * 48.0 + n * 4.0) + n * 8.0;
+    double budget = 32.0e9;
+    if (const char *e = std::getenv("IRS_GROUP_BYTES")) budget = std::max(1.0, std::atof(e));
+    int gmax = (int)std::max(1.0, std::min((double)n_prof, budget / std::max(bytes_per_prof, 1.0)));
+    if (const char *e = std::getenv("IRS_PROFILE_GROUP")) gmax = std::max(1, std::min(n_prof, std::atoi(e)));
+    const int n_groups = (n_prof + gmax - 1) / gmax;
+    const int group = (n_prof + n_groups - 1) / n_groups;
 
-    HIPCHK(ensure(ctx->d_tmp3, sizeof(double) * (L + 1) * n_prof));
-    HIPCHK(ensure(ctx->d_tmp4, sizeof(double) * L * n_prof));
-    HIPCHK(ensure(ctx->d_rhi, sizeof(double) * n));
-    double ms_prep = 0, ms_ls = 0, ms_rte = 0, ms_ils = 0;
-    std::vector<double> tlev_all((size_t)(L + 1) * n_prof), tave_all((size_t)L * n_prof);
+    // ---- schedule (tile points, persistent kernel), autotuned per shape
+    std::vector<ItemDev> proto(ns);
+    std::vector<const ItemDev *> per_species;
+    int P = ctx->sched.tile_points, persistent = ctx->sched.persistent;
+    double ms_prep = 0, ms_ls = 0, ms_rte = 0, ms_ils = 0, ms_graph = 0;
+    bool tuned_now = false;
+
+    const size_t prof_doubles = (size_t)group * (2 * L + 1 + 3);
+    HIPCHK(ensure(ctx->d_items, sizeof(ItemDev) * (size_t)group * L * ns));
+    HIPCHK(ensure(ctx->d_rec0, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
+    HIPCHK(ensure(ctx->d_rec1, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
+    HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
+    HIPCHK(ensure(ctx->d_tau, sizeof(float) * (size_t)n * L * group));
+    HIPCHK(ensure(ctx->d_rhi, sizeof(double) * (size_t)n * group));
+    HIPCHK(ensure(ctx->d_rch, sizeof(double) * nch * group));
+    HIPCHK(ensure(ctx->d_bt, sizeof(double) * nch * group));
+    HIPCHK(ensure(ctx->d_counters, 64));
+    HIPCHK(ensure(ctx->d_queue, 64));
+    HIPCHK(ensure(ctx->d_prof, sizeof(double) * prof_doubles));
+    HIPCHK(ensure_pinned(ctx->h_items, sizeof(ItemDev) * (size_t)group * L * ns));
+    HIPCHK(ensure_pinned(ctx->h_prof, sizeof(double) * prof_doubles));
+    HIPCHK(ensure_pinned(ctx->h_out, sizeof(double) * 2 * nch * group));
+
     for (int g0 = 0; g0 < n_prof; g0 += group) {
         const int gn = std::min(group, n_prof - g0);
-        std::vector<ItemDev> items((size_t)gn * L * ns);
+        ItemDev *items = (ItemDev *)ctx->h_items.p;
+        double *hp = (double *)ctx->h_prof.p;
+        double *h_tlev = hp, *h_tave = hp + (size_t)gn * (L + 1), *h_scal = hp + (size_t)gn * (2 * L + 1);
+        int64_t rec = 0;
         for (int q = 0; q < gn; ++q) {
             const irs_profile &pr = profiles[g0 + q];
             LayerProps lp;
             rc = layer_properties(ctx, &pr, opt.T_method, lp);
             if (rc) return rc;
-            for (int k = 0; k <= L; ++k) tlev_all[(size_t)(g0 + q) * (L + 1) + k] = pr.T[k];
-            for (int k = 0; k < L; ++k) tave_all[(size_t)(g0 + q) * L + k] = lp.T_ave[k];
+            for (int k = 0; k <= L; ++k) h_tlev[(size_t)q * (L + 1) + k] = pr.T[k];
+            for (int k = 0; k < L; ++k) h_tave[(size_t)q * L + k] = lp.T_ave[k];
+            h_scal[3 * q] = pr.T_sfc > 0.0 ? pr.T_sfc : pr.T[0];   // sounder.jl:337
+            h_scal[3 * q + 1] = pr.mu;
+            h_scal[3 * q + 2] = pr.eps_sfc;
             for (int k = 0; k < L; ++k)
                 for (int s = 0; s < ns; ++s) {
                     ItemDev &it = items[((size_t)q * L + k) * ns + s];
                     it = ItemDev{};
                     const int32_t h = pr.linelist[s];
                     const double vmr = lp.vmr_cg[s][k];
-                    if (h < 0 || vmr == 0.0 || ctx->lists[h].n == 0) continue;   // sounder.jl:285-286
-                    const int sp = pr.species[s];
-                    const double p_atm = lp.p_cg[s][k] / 1013.25;
-                    const double T = lp.T_cg[s][k];
-                    const bool lm = opt.line_mixing == IRS_LM_VPY && sp == 2 && ctx->lists[h].has_lm;
-                    const double vs = (sp == 1 && !lm) ? vmr : 0.0;
-                    const double coef = vmr * lp.dp[k] * 2.1209e22;                // sounder.jl:280,290
-                    // DPTMIN is skipped on the line-mixing CO2 path (sounder.jl:295-299)
-                    it = make_item(ctx->lists[h], h, T, p_atm, vs, coef, opt.dptmn, opt.cutoff, opt.lm_cutoff,
-                                   opt.x_far, lm, dnu);
-                    if (sp == 2 && opt.line_mixing == IRS_LM_VPY) it.dptmn = 0.0;
-                    it.key_off = koff[h];
+                    if (h >= 0 && vmr != 0.0 && ctx->lists[h].n > 0) {   // sounder.jl:285-286
+                        const int sp = pr.species[s];
+                        const double p_atm = lp.p_cg[s][k] / 1013.25;
+                        const double T = lp.T_cg[s][k];
+                        const bool lm = opt.line_mixing == IRS_LM_VPY && sp == 2 && ctx->lists[h].has_lm;
+                        const double vs = (sp == 1 && !lm) ? vmr : 0.0;
+                        const double coef = vmr * lp.dp[k] * 2.1209e22;        // sounder.jl:280,290
+                        it = make_item(ctx->lists[h], h, T, p_atm, vs, coef, opt.dptmn, opt.cutoff,
+                                       opt.lm_cutoff, opt.x_far, lm, dnu);
+                        // DPTMIN is skipped on the line-mixing CO2 path (sounder.jl:295-299)
+                        if (sp == 2 && opt.line_mixing == IRS_LM_VPY) it.dptmn = 0.0;
+                        it.key_off = koff[h];
+                    }
+                    it.rec_off = rec;
+                    rec += it.n;
+                    if (q == 0 && k == L / 2 && it.active) proto[s] = it;
                 }
         }
-        HIPCHK(ensure(ctx->d_tau, sizeof(float) * (size_t)n * L * gn));
-        rc = run_linesum(ctx, items, gn, L, ns, nu_lo, dnu, n, (float *)ctx->d_tau.p);
+        if (g0 == 0) {
+            for (int s = 0; s < ns; ++s)
+                if (proto[s].active) per_species.push_back(&proto[s]);
+            if (ctx->sched.autotune) {
+                const std::string key = shape_key(per_species, n, nu_lo, dnu, 0);
+                if (!ctx->tuned.count(key)) {
+                    GroupPlan tp;
+                    std::memset(&tp, 0, sizeof tp);
+                    tp.items = ctx->d_items.p; tp.rec0 = ctx->d_rec0.p; tp.rec1 = ctx->d_rec1.p;
+                    tp.rec2 = ctx->d_rec2.p; tp.keys = ctx->d_keys.p; tp.tau = ctx->d_tau.p;
+                    tp.counters = ctx->d_counters.p; tp.queue = ctx->d_queue.p; tp.lists = ctx->d_lists.p;
+                    tp.tips = ctx->d_tips.p; tp.h_items = ctx->h_items.p;
+                    tp.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)L * ns);
+                    tp.n = n; tp.L = L; tp.ns = ns; tp.max_lines = max_lines; tp.nu_lo = nu_lo; tp.dnu = dnu;
+                    rc = autotune_group(ctx, tp, per_species, key);
+                    if (rc) return rc;
+                    tuned_now = true;
+                }
+                P = ctx->tuned[key].first;
+                persistent = ctx->tuned[key].second;
+            }
+        }
+        const int32_t *order = nullptr;
+        rc = tile_order(ctx, per_species, n, nu_lo, dnu, P, &order);
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync((double *)ctx->d_tmp3.p + (size_t)g0 * (L + 1), tlev_all.data() + (size_t)g0 * (L + 1),
-                              sizeof(double) * (L + 1) * gn, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync((double *)ctx->d_tmp4.p + (size_t)g0 * L, tave_all.data() + (size_t)g0 * L,
-                              sizeof(double) * L * gn, hipMemcpyHostToDevice, ctx->stream));
-        for (int q = 0; q < gn; ++q) {
-            const irs_profile &pr = profiles[g0 + q];
-            RteArgs ra{};
-            ra.tau32 = (const float *)ctx->d_tau.p + (size_t)q * L * n;
-            ra.n = n;
-            ra.n_layers = L;
-            ra.T_lev = (const double *)ctx->d_tmp3.p + (size_t)(g0 + q) * (L + 1);
-            ra.T_ave = (const double *)ctx->d_tmp4.p + (size_t)(g0 + q) * L;
-            ra.T_sfc = pr.T_sfc > 0.0 ? pr.T_sfc : pr.T[0];
-            ra.mu = pr.mu;
-            ra.eps = pr.eps_sfc;
-            ra.source_function = opt.source_function;
-            ra.an = g.an;
-            ra.sn = g.sn;
-            ra.den = g.den;
-            ra.out = (double *)ctx->d_rhi.p;
-            if (q == 0) HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
-            HIPCHK(launch_rte(ra, ctx->stream));
-            if (q == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-            IlsArgs ia{};
-            ia.spec = (const double *)ctx->d_rhi.p;
-            ia.n = n;
-            ia.ch_idx = (const int64_t *)ctx->d_chidx.p;
-            ia.ch_w = (const double *)ctx->d_chw.p;
-            ia.ch_nu = (const double *)ctx->d_chnu.p;
-            ia.n_ch = nch;
-            ia.r_out = (double *)ctx->d_rch.p + (size_t)(g0 + q) * nch;
-            ia.bt_out = (double *)ctx->d_bt.p + (size_t)(g0 + q) * nch;
-            if (opt.with_ils) {
-                ia.taps = taps;
-                ia.half = half;
-                ia.dnu = dnu;
-            } else {
-                ia.taps = nullptr;
-                ia.half = 0;
-                ia.dnu = 1.0;
-            }
-            if (!opt.with_ils) {
-                // identity "ILS": a single unit tap
-                HIPCHK(ensure(ctx->d_tmp0, sizeof(double)));
-                const double one = 1.0;
-                HIPCHK(hipMemcpyAsync(ctx->d_tmp0.p, &one, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-                ia.taps = (const double *)ctx->d_tmp0.p;
-            }
-            HIPCHK(launch_ils_channels(ia, ctx->stream));
-            if (q == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+        GroupPlan pl;
+        std::memset(&pl, 0, sizeof pl);
+        pl.items = ctx->d_items.p; pl.rec0 = ctx->d_rec0.p; pl.rec1 = ctx->d_rec1.p; pl.rec2 = ctx->d_rec2.p;
+        pl.keys = ctx->d_keys.p; pl.tau = ctx->d_tau.p; pl.rhi = ctx->d_rhi.p; pl.counters = ctx->d_counters.p;
+        pl.queue = ctx->d_queue.p; pl.lists = ctx->d_lists.p; pl.tips = ctx->d_tips.p; pl.order = order;
+        pl.prof = ctx->d_prof.p; pl.rch = ctx->d_rch.p; pl.bt = ctx->d_bt.p; pl.chidx = ctx->d_chidx.p;
+        pl.chw = ctx->d_chw.p; pl.chnu = ctx->d_chnu.p; pl.taps = taps; pl.h_items = ctx->h_items.p;
+        pl.h_prof = ctx->h_prof.p; pl.h_out = ctx->h_out.p;
+        pl.n = n; pl.nch = nch; pl.half = half; pl.n_items = (int64_t)gn * L * ns; pl.max_lines = max_lines;
+        pl.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)gn * L * ns);
+        pl.prof_bytes = (int64_t)(sizeof(double) * (size_t)gn * (2 * L + 1 + 3));
+        pl.gn = gn; pl.L = L; pl.ns = ns; pl.P = P;
+        pl.pblocks = persistent ? persistent_blocks(ctx, P) : 0;
+        pl.tiles = (int)((n + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
+        pl.source_function = opt.source_function; pl.with_ils = opt.with_ils;
+        pl.nu_lo = nu_lo; pl.dnu = dnu; pl.an = g.an; pl.sn = g.sn; pl.den = g.den;
+        if (ctx->sched.use_graph) {
+            rc = launch_group_graph(ctx, pl);
+            if (rc) return rc;
+        } else {
+            hipError_t e = enqueue_group(ctx, pl, true);
+            if (e != hipSuccess) return fail(ctx, IRS_EHIP, std::string("forward: ") + hipGetErrorString(e));
         }
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        ms_prep += elapsed_ms(ctx->ev[0], ctx->ev[1]);
-        ms_ls += elapsed_ms(ctx->ev[1], ctx->ev[2]);
-        ms_rte += elapsed_ms(ctx->ev[3], ctx->ev[4]) * gn;
-        ms_ils += elapsed_ms(ctx->ev[4], ctx->ev[5]) * gn;
+        if (ctx->sched.use_graph) {
+            ms_graph += elapsed_ms(ctx->ev[5], ctx->ev[6]);
+        } else {
+            ms_prep += elapsed_ms(ctx->ev[0], ctx->ev[1]);
+            ms_ls += elapsed_ms(ctx->ev[1], ctx->ev[2]);
+            ms_rte += elapsed_ms(ctx->ev[2], ctx->ev[3]);
+            ms_ils += elapsed_ms(ctx->ev[3], ctx->ev[4]);
+        }
+        const double *hout = (const double *)ctx->h_out.p;
+        std::memcpy(R + (size_t)g0 * nch, hout, sizeof(double) * nch * gn);
+        std::memcpy(BT + (size_t)g0 * nch, hout + (size_t)nch * gn, sizeof(double) * nch * gn);
         read_counters(ctx);
+        ctx->stats.linesum_launches += 1;
     }
-    HIPCHK(hipMemcpyAsync(R, ctx->d_rch.p, sizeof(double) * nch * n_prof, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(BT, ctx->d_bt.p, sizeof(double) * nch * n_prof, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->stats.ms_prep = ms_prep;
     ctx->stats.ms_linesum = ms_ls;
     ctx->stats.ms_rte = ms_rte;
     ctx->stats.ms_ils = ms_ils;
+    ctx->stats.ms_device = ctx->sched.use_graph ? ms_graph : ms_prep + ms_ls + ms_rte + ms_ils;
     ctx->stats.n_hi = n;
     ctx->stats.n_layers = L;
+    ctx->stats.tile_points = P;
+    ctx->stats.persistent = persistent;
+    ctx->stats.autotuned = tuned_now ? 1 : 0;
     ctx->stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return IRS_OK;
+}
+
+int irs_set_schedule(irs_ctx *ctx, const irs_schedule *sc) {
+    if (!ctx || !sc) return IRS_EINVAL;
+    if (sc->tile_points != 4 && sc->tile_points != 8 && sc->tile_points != 16)
+        return fail(ctx, IRS_EINVAL, "tile_points must be 4, 8 or 16");
+    ctx->sched = *sc;
+    ctx->sched.persistent = sc->persistent ? 1 : 0;
+    ctx->sched.use_graph = sc->use_graph ? 1 : 0;
+    ctx->sched.autotune = sc->autotune ? 1 : 0;
+    return IRS_OK;
+}
+
+int irs_get_schedule(const irs_ctx *ctx, irs_schedule *sc) {
+    if (!ctx || !sc) return IRS_EINVAL;
+    *sc = ctx->sched;
     return IRS_OK;
 }
 

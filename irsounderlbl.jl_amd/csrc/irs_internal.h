@@ -55,20 +55,23 @@ struct LinesumArgs {
     int64_t n_hi;
     int n_layers, n_species;
     double nu_lo, dnu;
-    int tiles;
-    const int32_t *tile_order;     // optional permutation of blockIdx.x (heavy tiles first)
+    int tiles, n_prof;
+    const int32_t *tile_order;     // optional tile permutation (heavy tiles first)
+    unsigned *queue;               // work counter of the persistent kernel (zero at launch)
 };
 
 struct RteArgs {
-    const float *tau32;            // [n_layers][n] (one of tau32/tau64)
+    const float *tau32;            // [n_prof][n_layers][n] (one of tau32/tau64)
     const double *tau64;
     int64_t n;
     int n_layers;
-    const double *T_lev, *T_ave;   // device, [L+1], [L]
-    double T_sfc, mu, eps;
+    const double *T_lev, *T_ave;   // device, [n_prof][L+1], [n_prof][L]
+    double T_sfc, mu, eps;         // used when scal is null
+    const double *scal;            // optional [n_prof][3] = {T_sfc, mu, eps}
     int source_function;
     double an, sn, den;            // grid node i = (an + i*sn)/den
-    double *out;
+    double *out;                   // [n_prof][n]
+    int n_prof;                    // blockIdx.y = profile (0 is taken as 1)
 };
 
 struct IlsArgs {
@@ -81,12 +84,15 @@ struct IlsArgs {
     const double *ch_w;            // interpolation weight of the right node (0: exact)
     const double *ch_nu;
     int64_t n_ch;
-    double *r_out, *bt_out;
+    double *r_out, *bt_out;        // [n_prof][n_ch]
+    int n_prof;                    // blockIdx.y = profile; spec is [n_prof][n] (0 is taken as 1)
 };
 
 // launchers (k_*.hip)
 hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s);
-hipError_t launch_linesum(const LinesumArgs &a, int n_prof, int P, hipStream_t s);
+hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hipStream_t s);
+hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s);
+int linesum_persistent_capacity(int P);
 hipError_t launch_rte(const RteArgs &a, hipStream_t s);
 hipError_t launch_ils_taps(const double *A, const double *L, int n_pts, double dL, double dnu,
                            int64_t half, double *taps, hipStream_t s);

@@ -359,14 +359,12 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     }
 }
 
+// One work unit: the tile `tile` of (profile, layer), all species, written to
+// out[profile][layer][i0:i1).
 template <int P>
-__global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumArgs a) {
-    __shared__ int s_rng[8];
-    __shared__ double s_red[12];
-    __shared__ float4 s_rec[2 * LS_CHUNK];
+__device__ __forceinline__ void linesum_unit(const LinesumArgs &a, int tile, int layer, int prof, int *s_rng,
+                                             double *s_red, float4 *s_rec) {
     const int t = threadIdx.x;
-    const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
-    const int layer = blockIdx.y, prof = blockIdx.z;
     const int64_t i0 = (int64_t)tile * (IRS_LS_THREADS * P);
     const int64_t i1 = min(i0 + (int64_t)(IRS_LS_THREADS * P), a.n_hi);
     float tau[P];
@@ -387,10 +385,81 @@ __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumAr
     }
 }
 
-hipError_t launch_linesum(const LinesumArgs &a, int n_prof, int P, hipStream_t s) {
-    dim3 grid(a.tiles, a.n_layers, n_prof);
-    if (P == 16) hipLaunchKernelGGL(linesum_kernel<16>, grid, dim3(IRS_LS_THREADS), 0, s, a);
-    else if (P == 4) hipLaunchKernelGGL(linesum_kernel<4>, grid, dim3(IRS_LS_THREADS), 0, s, a);
-    else hipLaunchKernelGGL(linesum_kernel<8>, grid, dim3(IRS_LS_THREADS), 0, s, a);
+// One workgroup per unit: blockIdx = (tile rank, layer, profile).
+template <int P>
+__global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumArgs a) {
+    __shared__ int s_rng[8];
+    __shared__ double s_red[12];
+    __shared__ float4 s_rec[2 * LS_CHUNK];
+    const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
+    linesum_unit<P>(a, tile, blockIdx.y, blockIdx.z, s_rng, s_red, s_rec);
+}
+
+// Persistent form: a grid sized to the resident capacity of the device pulls
+// units from a queue counter in order of decreasing tile cost (tile_order,
+// heaviest tiles first, every layer and profile of a tile together).  The
+// unit index is fetched by one lane and broadcast through LDS, so the loop
+// exit is block-uniform and every wave of every block reaches it once the
+// queue is past the last unit.  The counter must be zero at launch.
+template <int P>
+__global__ void __launch_bounds__(IRS_LS_THREADS) linesum_persistent_kernel(const LinesumArgs a) {
+    __shared__ int s_rng[8];
+    __shared__ double s_red[12];
+    __shared__ float4 s_rec[2 * LS_CHUNK];
+    __shared__ unsigned s_unit;
+    const unsigned per_tile = (unsigned)a.n_layers * (unsigned)a.n_prof;
+    const unsigned total = (unsigned)a.tiles * per_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const unsigned u = s_unit;
+        __syncthreads();
+        if (u >= total) break;
+        const unsigned rank = u / per_tile, rem = u - rank * per_tile;
+        const int tile = a.tile_order ? a.tile_order[rank] : (int)rank;
+        const int prof = (int)(rem / (unsigned)a.n_layers), layer = (int)(rem % (unsigned)a.n_layers);
+        linesum_unit<P>(a, tile, layer, prof, s_rng, s_red, s_rec);
+    }
+}
+
+// Zeroes the queue counter and the prep work counters (a kernel rather than a
+// memset node, so the sequence can be captured in a graph).
+__global__ void reset_counters_kernel(unsigned *queue, unsigned long long *counters, int n_counters) {
+    const int t = threadIdx.x;
+    if (t == 0 && queue) *queue = 0u;
+    if (t < n_counters && counters) counters[t] = 0ull;
+}
+
+hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, queue, counters, n_counters);
     return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_ls_p(const LinesumArgs &a, int persistent_blocks, hipStream_t s) {
+    if (persistent_blocks > 0)
+        hipLaunchKernelGGL(linesum_persistent_kernel<P>, dim3(persistent_blocks), dim3(IRS_LS_THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL(linesum_kernel<P>, dim3(a.tiles, a.n_layers, a.n_prof), dim3(IRS_LS_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hipStream_t s) {
+    if (persistent_blocks > 0 && !a.queue) return hipErrorInvalidValue;
+    if (P == 16) return launch_ls_p<16>(a, persistent_blocks, s);
+    if (P == 8) return launch_ls_p<8>(a, persistent_blocks, s);
+    return launch_ls_p<4>(a, persistent_blocks, s);
+}
+
+// Resident workgroups of the persistent kernel on the current device.
+int linesum_persistent_capacity(int P) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e;
+    if (P == 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, linesum_persistent_kernel<16>, IRS_LS_THREADS, 0);
+    else if (P == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, linesum_persistent_kernel<8>, IRS_LS_THREADS, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, linesum_persistent_kernel<4>, IRS_LS_THREADS, 0);
+    if (e != hipSuccess) return 0;
+    return (per_cu > 1 ? per_cu : 1) * cus;
 }

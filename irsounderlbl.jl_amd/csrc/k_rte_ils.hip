@@ -25,55 +25,60 @@ __device__ __forceinline__ double lit_corr(double s, double e) {   // schwarzsch
 }
 
 template <typename TT>
-__global__ void __launch_bounds__(256) rte_kernel(const RteArgs a, const TT *__restrict__ tau) {
+__global__ void __launch_bounds__(256) rte_kernel(const RteArgs a, const TT *__restrict__ tau_all) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
-    const double nu = (a.an + (double)i * a.sn) / a.den;
+    const int q = blockIdx.y;
     const int L = a.n_layers;
-    const double inv_mu = 1.0 / a.mu;
+    const TT *__restrict__ tau = tau_all + (int64_t)q * L * a.n;
+    const double *T_lev = a.T_lev + (int64_t)q * (L + 1);
+    const double *T_ave = a.T_ave ? a.T_ave + (int64_t)q * L : nullptr;
+    const double T_sfc = a.scal ? a.scal[3 * q] : a.T_sfc;
+    const double mu = a.scal ? a.scal[3 * q + 1] : a.mu;
+    const double eps = a.scal ? a.scal[3 * q + 2] : a.eps;
+    const double nu = (a.an + (double)i * a.sn) / a.den;
     double tr_top = 1.0;
     double acc = 0.0;
     for (int k = L - 1; k >= 0; --k) {
-        const double s = (double)tau[(int64_t)k * a.n + i] / a.mu;
+        const double s = (double)tau[(int64_t)k * a.n + i] / mu;
         const double e = exp(-s);
         const double tr_bot = tr_top * e;
         const double dT = tr_top - tr_bot;
-        const double bt = planck64(nu, a.T_lev[k + 1]);
+        const double bt = planck64(nu, T_lev[k + 1]);
         if (a.source_function == 1) {
-            const double ba = planck64(nu, a.T_ave[k]);
+            const double ba = planck64(nu, T_ave[k]);
             acc += ba * dT + (bt - ba) * (dT * cim_corr(s, e));
         } else {
-            const double bb = planck64(nu, a.T_lev[k]);
+            const double bb = planck64(nu, T_lev[k]);
             acc += bt * dT + (bb - bt) * (tr_top * lit_corr(s, e));
         }
         tr_top = tr_bot;
     }
-    double I = a.eps * planck64(nu, a.T_sfc) * tr_top + acc;
-    if (a.eps < 1.0) {
+    double I = eps * planck64(nu, T_sfc) * tr_top + acc;
+    if (eps < 1.0) {
         double down = 0.0, g = 1.0;
         for (int k = 0; k < L; ++k) {
-            const double s = (double)tau[(int64_t)k * a.n + i] / a.mu;
+            const double s = (double)tau[(int64_t)k * a.n + i] / mu;
             const double e = exp(-s);
             double em;
             if (a.source_function == 1) {
-                const double ba = planck64(nu, a.T_ave[k]), bb = planck64(nu, a.T_lev[k]);
+                const double ba = planck64(nu, T_ave[k]), bb = planck64(nu, T_lev[k]);
                 em = (1.0 - e) * (ba + (bb - ba) * cim_corr(s, e));
             } else {
-                const double bb = planck64(nu, a.T_lev[k]), bt = planck64(nu, a.T_lev[k + 1]);
+                const double bb = planck64(nu, T_lev[k]), bt = planck64(nu, T_lev[k + 1]);
                 em = bb * (1.0 - e) + (bt - bb) * lit_corr(s, e);
             }
             down += g * em;
             g *= e;
         }
-        I += (1.0 - a.eps) * down * tr_top;
+        I += (1.0 - eps) * down * tr_top;
     }
-    (void)inv_mu;
-    a.out[i] = I;
+    a.out[(int64_t)q * a.n + i] = I;
 }
 
 hipError_t launch_rte(const RteArgs &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    dim3 grid((unsigned)((a.n + 255) / 256));
+    dim3 grid((unsigned)((a.n + 255) / 256), (unsigned)(a.n_prof > 0 ? a.n_prof : 1));
     if (a.tau32) hipLaunchKernelGGL(rte_kernel<float>, grid, dim3(256), 0, s, a, a.tau32);
     else hipLaunchKernelGGL(rte_kernel<double>, grid, dim3(256), 0, s, a, a.tau64);
     return hipGetLastError();
@@ -109,12 +114,13 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
     return r;
 }
 
-__device__ __forceinline__ double ils_at(const IlsArgs &a, int64_t i, double *red) {
+__device__ __forceinline__ double ils_at(const IlsArgs &a, const double *__restrict__ spec, int64_t i,
+                                         double *red) {
     double acc = 0.0;
     const int64_t nt = 2 * a.half + 1;
     for (int64_t k = threadIdx.x; k < nt; k += 256) {
         const int64_t j = i + k - a.half;
-        if (j >= 0 && j < a.n) acc += a.spec[j] * a.taps[k];
+        if (j >= 0 && j < a.n) acc += spec[j] * a.taps[k];
     }
     return block_sum_d(acc, red) * a.dnu;
 }
@@ -122,24 +128,28 @@ __device__ __forceinline__ double ils_at(const IlsArgs &a, int64_t i, double *re
 __global__ void __launch_bounds__(256) ils_channels_kernel(const IlsArgs a) {
     __shared__ double red[4];
     const int64_t c = blockIdx.x;
+    const int q = blockIdx.y;
+    const double *spec = a.spec + (int64_t)q * a.n;
     const int64_t i = a.ch_idx[c];
     const double w = a.ch_w[c];
-    double r = ils_at(a, i, red);
+    double r = ils_at(a, spec, i, red);
     if (w != 0.0) {
-        const double r2 = ils_at(a, i + 1, red);
+        const double r2 = ils_at(a, spec, i + 1, red);
         r = r * (1.0 - w) + r2 * w;
     }
     if (threadIdx.x == 0) {
-        a.r_out[c] = r;
+        const int64_t o = (int64_t)q * a.n_ch + c;
+        a.r_out[o] = r;
         if (a.bt_out) {
             const double nu = a.ch_nu[c];
-            a.bt_out[c] = r <= 0.0 ? 0.0 : IRS_C2_PL * nu / log1p(IRS_C1_PL * nu * nu * nu / r);
+            a.bt_out[o] = r <= 0.0 ? 0.0 : IRS_C2_PL * nu / log1p(IRS_C1_PL * nu * nu * nu / r);
         }
     }
 }
 
 hipError_t launch_ils_channels(const IlsArgs &a, hipStream_t s) {
     if (a.n_ch <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ils_channels_kernel, dim3((unsigned)a.n_ch), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ils_channels_kernel, dim3((unsigned)a.n_ch, (unsigned)(a.n_prof > 0 ? a.n_prof : 1)),
+                       dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -76,14 +76,19 @@ class IrsStats(C.Structure):
     _fields_ = [("ms_total", f64), ("ms_prep", f64), ("ms_linesum", f64), ("ms_rte", f64),
                 ("ms_ils", f64), ("n_hi", i64), ("n_layers", i64), ("linesum_launches", i64),
                 ("active_lines", i64), ("active_pairs", i64), ("dispersive_pairs", i64),
-                ("core_pairs_bound", i64)]
+                ("core_pairs_bound", i64), ("ms_device", f64), ("graph_launches", i64),
+                ("tile_points", i32), ("persistent", i32), ("autotuned", i32), ("reserved1", i32)]
+
+
+class IrsSchedule(C.Structure):
+    _fields_ = [("tile_points", i32), ("persistent", i32), ("use_graph", i32), ("autotune", i32)]
 
 
 EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_last_error",
            "irs_default_options", "irs_last_stats", "irs_linelist_upload", "irs_linelist_free",
            "irs_xsec_layer", "irs_rte", "irs_ils_half", "irs_ils_kernel", "irs_ils_channels",
            "irs_brightness_temperature", "irs_forward_dims", "irs_forward", "irs_state_size",
-           "irs_jacobian")
+           "irs_jacobian", "irs_set_schedule", "irs_get_schedule")
 
 _lib = None
 
@@ -120,6 +125,8 @@ def load_library(path: str = LIB_PATH):
     L.irs_forward.argtypes = [P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
                               C.POINTER(IrsOptions), P, P]
     L.irs_state_size.argtypes = [C.POINTER(IrsStateSpec), i32]
+    L.irs_set_schedule.argtypes = [P, C.POINTER(IrsSchedule)]
+    L.irs_get_schedule.argtypes = [P, C.POINTER(IrsSchedule)]
     L.irs_jacobian.argtypes = [P, C.POINTER(IrsProfile), C.POINTER(IrsSounder), C.POINTER(IrsOptions),
                                C.POINTER(IrsStateSpec), P, P, P]
     for name in EXPORTS:
@@ -181,6 +188,19 @@ class Context:
         s = IrsStats()
         self.check(self.lib.irs_last_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in IrsStats._fields_}
+
+    def schedule(self) -> dict:
+        sc = IrsSchedule()
+        self.check(self.lib.irs_get_schedule(self.h, C.byref(sc)))
+        return {k: getattr(sc, k) for k, _ in IrsSchedule._fields_}
+
+    def set_schedule(self, tile_points: int = 4, persistent: bool = False, use_graph: bool = False,
+                     autotune: bool = False):
+        """Line-sum schedule of forward_model: points per thread (4/8/16), the
+        persistent heaviest-first work-queue kernel, hipGraph capture and
+        replay of each profile group, autotuning on the first call of a shape."""
+        sc = IrsSchedule(int(tile_points), int(bool(persistent)), int(bool(use_graph)), int(bool(autotune)))
+        self.check(self.lib.irs_set_schedule(self.h, C.byref(sc)))
 
     def upload(self, ll: HITRANLinelist) -> int:
         key = id(ll)
