@@ -26,6 +26,7 @@
  *                                                   src/Sensor/ils.jl:131-153,
  *                                                   src/Sensor/sounder.jl:372-377
  *   irs_brightness_temperature  brightness_temperature  src/Solver/planck.jl:43-57
+ *   irs_layer_properties   layer_properties         src/Atmosphere/layers.jl:10-178
  *   irs_jacobian           analytic_jacobian        src/Estimation/vmr_jacobian.jl:152-505
  *   irs_state_size         StateVectorSpec length   src/Estimation/state_vector.jl:253-300
  *   irs_linelist_upload    HITRANLinelist(lines)    src/HITRAN/linelist.jl:52-56
@@ -167,6 +168,27 @@ int         irs_last_stats(const irs_ctx *ctx, irs_stats *out);
 int         irs_set_schedule(irs_ctx *ctx, const irs_schedule *schedule);
 int         irs_get_schedule(const irs_ctx *ctx, irs_schedule *schedule);
 
+/* Per-layer channel optical depths of a batch of profiles along each profile's
+ * viewing path (mu): the emulator's training target (no counterpart in the
+ * reference).  With T_k = ILS (x) exp(-sum_{j>=k} tau_j / mu) at the channel node
+ * (level-to-space) and t_k = ILS (x) exp(-tau_k / mu) (layer alone):
+ *   tau_ch[k] = ln(T_{k+1} / T_k)  where T_{k+1} >= floor,  else  -ln t_k,
+ * clamped to [0, 50].  tau_ch is [n_prof][n_layers][n_channels] (float); the
+ * channels must be internal grid nodes.  cim_g (optional, same shape) is the
+ * channel weight of the CIM source correction, ILS (x) [(T_{k+1} - T_k) f_cim(tau_k/mu)]
+ * / T^ch_{k+1} (layer-alone form under an opaque column), clamped to [0, 1]. */
+int irs_channel_od(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                   const irs_fm_options *opt, double floor, float *tau_ch, float *cim_g);
+/* The emulator's target (no counterpart in the reference): ILS-convolved
+ * level-to-space transmittances along each profile's path T_lev
+ * [n_prof][n_layers+1][n_channels] (index n_layers = top, 1) and the CIM sums
+ * cim_G [n_prof][n_layers][n_channels] = ILS (x) [(T_{k+1} - T_k) f_cim(tau_k/mu)],
+ * so that  R_ch = eps B(T_sfc) T_0 + sum_k B(T_ave,k)(T_{k+1} - T_k)
+ *                 + (B(T_{k+1}) - B(T_ave,k)) G_k  (Planck at the channel centre). */
+int irs_channel_transmittance(irs_ctx *ctx, const irs_profile *profiles, int n_prof,
+                              const irs_sounder *snd, const irs_fm_options *opt, float *T_lev,
+                              float *cim_G);
+
 /* Copies a sorted line list to the device; returns a handle. */
 int irs_linelist_upload(irs_ctx *ctx, const irs_linelist *ll, int32_t *handle);
 int irs_linelist_free(irs_ctx *ctx, int32_t handle);
@@ -198,6 +220,11 @@ int irs_ils_channels(irs_ctx *ctx, const double *spectrum, double nu0, double dn
 
 int irs_brightness_temperature(const double *nu, const double *radiance, int64_t n, double *bt);
 
+/* layer_properties (src/Atmosphere/layers.jl:10-178) of one profile, host only:
+ * p_mid, dp, T_ave (TBAR) [n_layers]; vmr_cg, p_cg, T_cg [n_species][n_layers]. */
+int irs_layer_properties(const irs_profile *profile, int T_method, double *p_mid, double *dp,
+                         double *T_ave, double *vmr_cg, double *p_cg, double *T_cg);
+
 /* Geometry of the internal grid for (snd, opt). */
 int irs_forward_dims(const irs_sounder *snd, const irs_fm_options *opt, double *nu_lo,
                      double *dnu_hi, int64_t *n_hi, int64_t *n_channels);
@@ -224,6 +251,55 @@ int irs_state_size(const irs_state_spec *spec, int n_levels);
 int irs_jacobian(irs_ctx *ctx, const irs_profile *profile, const irs_sounder *snd,
                  const irs_fm_options *opt, const irs_state_spec *spec, double *K, double *y0,
                  double *x0);
+
+/* ---- Emulator (no counterpart in the reference) -------------------------
+ * A device-resident MLP  n_in -> 3 x n_hidden (GELU) -> n_out, trained and
+ * evaluated with hand-written BF16 MFMA kernels (FP32 accumulation, FP32
+ * master weights, Adam).  Parameters are packed W1[n_in_pad][H], b1[H],
+ * W2[H][H], b2, W3[H][H], b3, W4[H][n_out], b4 (rows of W1 beyond n_in are 0).
+ * Batches are multiples of 64 rows; n_hidden and n_out multiples of 64. */
+typedef struct irs_mlp irs_mlp;
+int         irs_mlp_create(int device, int n_in, int n_hidden, int n_out, int max_batch, uint64_t seed,
+                           irs_mlp **out);
+void        irs_mlp_destroy(irs_mlp *mlp);
+const char *irs_mlp_last_error(const irs_mlp *mlp);
+int64_t     irs_mlp_n_params(const irs_mlp *mlp);
+int         irs_mlp_input_pad(const irs_mlp *mlp);
+int         irs_mlp_get_params(irs_mlp *mlp, float *params);
+int         irs_mlp_set_params(irs_mlp *mlp, const float *params);      /* resets the Adam state */
+/* y[n][n_out] = MLP(x[n][n_in]) */
+int         irs_mlp_forward(irs_mlp *mlp, const float *x, int64_t n, float *y);
+/* loss = mean((MLP(x) - t)^2) over B x n_out, grad = d loss / d params (packed) */
+int         irs_mlp_loss_grad(irs_mlp *mlp, const float *x, const float *t, int B, float *grad, float *loss);
+/* training set kept on the device: X[n][n_in], T[n][n_out] */
+int         irs_mlp_set_data(irs_mlp *mlp, const float *X, const float *T, int64_t n);
+/* steps of Adam (b1 0.9, b2 0.999, eps 1e-8) on shuffled minibatches; losses[steps] (may be NULL) */
+int         irs_mlp_train(irs_mlp *mlp, int steps, int batch, float lr, uint64_t seed, float *losses);
+/* J[n][n_out][n_in] = dy/dx (forward-mode tangents) */
+int         irs_mlp_input_jacobian(irs_mlp *mlp, const float *x, int64_t n, float *J);
+
+/* Emulated forward model: two targets per layer and channel decoded as
+ * x_k(c) = t_mean[k][c] + sum_m (y_m * coef_scale[m]) basis[m][c] (first target
+ * on columns 0..n_ch-1, second on n_ch..2 n_ch-1) with
+ * y = MLP((features - x_mean) / x_std):
+ *   kind = 2: the layer channel transmittance t_k = exp(-tau_ch_k) and the CIM
+ *             weight g_k of irs_channel_od (T_k = t_k T_{k+1}, G_k = g_k T_{k+1});
+ *   kind = 3: the level transmittance T_k and the CIM sum G_k of irs_channel_transmittance;
+ * then the channel RTE
+ *   R = eps B(T_sfc) T_0 + sum_k B(T_ave,k)(T_{k+1} - T_k) + (B(T_{k+1}) - B(T_ave,k)) G_k
+ * (CIM source only; for eps < 1 the reflected term is approximate).  The
+ * n_tpc argument of irs_emu_create is this kind. */
+typedef struct irs_emu irs_emu;
+int         irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_tpc,
+                           const float *x_mean, const float *x_std, const float *coef_scale,
+                           const float *basis, const float *t_mean, const double *ch_nu, irs_emu **out);
+void        irs_emu_destroy(irs_emu *emu);
+const char *irs_emu_last_error(const irs_emu *emu);
+/* features[n_prof][L][n_in], T_lev[n_prof][L+1], T_ave[n_prof][L], scal[n_prof][3] =
+ * {T_sfc, mu, eps}; R, BT [n_prof][n_channels]; t_out[n_prof][L][n_channels] (decoded T_k) optional */
+int         irs_emu_forward(irs_emu *emu, const float *features, const double *T_lev, const double *T_ave,
+                            const double *scal, int n_prof, int source_function, double *R, double *BT,
+                            float *t_out);
 
 /* forward_model over a batch of profiles: R, BT are [n_prof][n_channels]. */
 int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,

@@ -68,6 +68,10 @@ struct GroupPlan {
     int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes;
     int gn, L, ns, P, pblocks, tiles, source_function, with_ils;
     double nu_lo, dnu, an, sn, den;
+    int mode;                      // 0 radiance + BT, 1 channel optical depths (2: + CIM weights),
+                                   // 3 level channel transmittances + CIM sums
+    const void *chod, *chlev, *chg;
+    double chod_floor;
 };
 
 struct GraphEntry {
@@ -100,7 +104,7 @@ struct irs_ctx {
     std::map<std::string, std::pair<int, int>> tuned;     // shape -> (P, persistent)
     std::map<std::string, DevBuf> orders;                 // shape -> heaviest-first tile order
     std::map<int, int> capacity;                          // P -> resident persistent workgroups
-    DevBuf d_queue, d_prof;
+    DevBuf d_queue, d_prof, d_chod, d_chlev, d_chg;
     PinBuf h_items, h_prof, h_out;
     std::vector<GraphEntry> graphs;
 };
@@ -703,6 +707,9 @@ void irs_destroy(irs_ctx *ctx) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     ctx->d_queue.release();
     ctx->d_prof.release();
+    ctx->d_chod.release();
+    ctx->d_chlev.release();
+    ctx->d_chg.release();
     ctx->h_items.release();
     ctx->h_prof.release();
     ctx->h_out.release();
@@ -1034,6 +1041,44 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.queue = (unsigned *)pl.queue;
     ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
     if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
+    if (pl.mode >= 1) {
+        ChodArgs ca{};
+        ca.tau = (const float *)pl.tau;
+        ca.n = pl.n;
+        ca.n_layers = L;
+        ca.taps = (const double *)pl.taps;
+        ca.half = pl.with_ils ? pl.half : 0;
+        ca.dnu = pl.with_ils ? pl.dnu : 1.0;
+        ca.ch_idx = (const int64_t *)pl.chidx;
+        ca.n_ch = pl.nch;
+        ca.floor = pl.chod_floor;
+        ca.od_max = 50.0f;
+        ca.scal = (const double *)pl.prof + NL * (2 * L + 1);
+        ca.out = (float *)pl.chod;
+        if (pl.mode >= 2) {
+            ca.lev_out = (float *)pl.chlev;
+            ca.lev_in = ca.lev_out;
+            ca.g_out = (float *)pl.chg;
+            ca.raw_G = pl.mode == 3;
+        }
+        ENQ(launch_chod(ca, pl.gn, st));
+        if (timing) {
+            ENQ(hipEventRecord(ctx->ev[3], st));
+            ENQ(hipEventRecord(ctx->ev[4], st));
+        }
+        if (pl.mode == 3) {
+            ENQ(hipMemcpyAsync((void *)pl.h_out, pl.chlev, sizeof(float) * pl.nch * (L + 1) * NL,
+                               hipMemcpyDeviceToHost, st));
+            ENQ(hipMemcpyAsync((float *)pl.h_out + pl.nch * (L + 1) * NL, pl.chg, sizeof(float) * pl.nch * L * NL,
+                               hipMemcpyDeviceToHost, st));
+            return hipSuccess;
+        }
+        ENQ(hipMemcpyAsync((void *)pl.h_out, pl.chod, sizeof(float) * pl.nch * L * NL, hipMemcpyDeviceToHost, st));
+        if (pl.mode == 2)
+            ENQ(hipMemcpyAsync((float *)pl.h_out + pl.nch * L * NL, pl.chg, sizeof(float) * pl.nch * L * NL,
+                               hipMemcpyDeviceToHost, st));
+        return hipSuccess;
+    }
     const double *prof = (const double *)pl.prof;
     RteArgs ra{};
     ra.tau32 = (const float *)pl.tau;
@@ -1172,11 +1217,14 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
     return IRS_OK;
 }
 
-int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
-                const irs_fm_options *opt_in, double *R, double *BT) {
+static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                       const irs_fm_options *opt_in, int mode, double chod_floor, double *R, double *BT,
+                       float *tau_ch, float *cim_g) {
     if (!ctx) return IRS_EINVAL;
     (void)hipSetDevice(ctx->device);
-    if (!profiles || n_prof <= 0 || !snd || !R || !BT) return fail(ctx, IRS_EINVAL, "null argument");
+    if (!profiles || n_prof <= 0 || !snd || (mode == 0 && (!R || !BT)) || (mode >= 1 && !tau_ch) ||
+        (mode >= 2 && !cim_g))
+        return fail(ctx, IRS_EINVAL, "null argument");
     irs_fm_options opt;
     if (opt_in) opt = *opt_in; else irs_default_options(&opt);
     if (!(snd->dnu > 0.0)) return fail(ctx, IRS_EINVAL, "sounder dnu must be > 0");
@@ -1238,6 +1286,10 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
     std::vector<double> cw, cnu;
     channel_positions(g, gc, cidx, cw, cnu);
     const int64_t nch = gc.n;
+    if (mode >= 1)
+        for (int64_t c = 0; c < nch; ++c)
+            if (cw[c] != 0.0)
+                return fail(ctx, IRS_EINVAL, "channel optical depths need channels on internal grid nodes");
     HIPCHK(ensure(ctx->d_chidx, sizeof(int64_t) * nch));
     HIPCHK(ensure(ctx->d_chw, sizeof(double) * nch));
     HIPCHK(ensure(ctx->d_chnu, sizeof(double) * nch));
@@ -1282,7 +1334,13 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
     HIPCHK(ensure(ctx->d_prof, sizeof(double) * prof_doubles));
     HIPCHK(ensure_pinned(ctx->h_items, sizeof(ItemDev) * (size_t)group * L * ns));
     HIPCHK(ensure_pinned(ctx->h_prof, sizeof(double) * prof_doubles));
-    HIPCHK(ensure_pinned(ctx->h_out, sizeof(double) * 2 * nch * group));
+    HIPCHK(ensure_pinned(ctx->h_out, std::max(sizeof(double) * 2 * nch * group,
+                                              sizeof(float) * (2 * L + 1) * nch * group)));
+    if (mode >= 1) HIPCHK(ensure(ctx->d_chod, sizeof(float) * nch * L * group));
+    if (mode >= 2) {
+        HIPCHK(ensure(ctx->d_chlev, sizeof(float) * nch * (L + 1) * group));
+        HIPCHK(ensure(ctx->d_chg, sizeof(float) * nch * L * group));
+    }
 
     for (int g0 = 0; g0 < n_prof; g0 += group) {
         const int gn = std::min(group, n_prof - g0);
@@ -1365,6 +1423,11 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
         pl.tiles = (int)((n + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
         pl.source_function = opt.source_function; pl.with_ils = opt.with_ils;
         pl.nu_lo = nu_lo; pl.dnu = dnu; pl.an = g.an; pl.sn = g.sn; pl.den = g.den;
+        pl.mode = mode;
+        pl.chod = mode >= 1 ? ctx->d_chod.p : nullptr;
+        pl.chlev = mode >= 2 ? ctx->d_chlev.p : nullptr;
+        pl.chg = mode >= 2 ? ctx->d_chg.p : nullptr;
+        pl.chod_floor = chod_floor;
         if (ctx->sched.use_graph) {
             rc = launch_group_graph(ctx, pl);
             if (rc) return rc;
@@ -1381,9 +1444,20 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
             ms_rte += elapsed_ms(ctx->ev[2], ctx->ev[3]);
             ms_ils += elapsed_ms(ctx->ev[3], ctx->ev[4]);
         }
-        const double *hout = (const double *)ctx->h_out.p;
-        std::memcpy(R + (size_t)g0 * nch, hout, sizeof(double) * nch * gn);
-        std::memcpy(BT + (size_t)g0 * nch, hout + (size_t)nch * gn, sizeof(double) * nch * gn);
+        if (mode == 3) {
+            std::memcpy(tau_ch + (size_t)g0 * (L + 1) * nch, ctx->h_out.p, sizeof(float) * nch * (L + 1) * gn);
+            std::memcpy(cim_g + (size_t)g0 * L * nch, (const float *)ctx->h_out.p + (size_t)nch * (L + 1) * gn,
+                        sizeof(float) * nch * L * gn);
+        } else if (mode >= 1) {
+            std::memcpy(tau_ch + (size_t)g0 * L * nch, ctx->h_out.p, sizeof(float) * nch * L * gn);
+            if (mode == 2)
+                std::memcpy(cim_g + (size_t)g0 * L * nch, (const float *)ctx->h_out.p + (size_t)nch * L * gn,
+                            sizeof(float) * nch * L * gn);
+        } else {
+            const double *hout = (const double *)ctx->h_out.p;
+            std::memcpy(R + (size_t)g0 * nch, hout, sizeof(double) * nch * gn);
+            std::memcpy(BT + (size_t)g0 * nch, hout + (size_t)nch * gn, sizeof(double) * nch * gn);
+        }
         read_counters(ctx);
         ctx->stats.linesum_launches += 1;
     }
@@ -1400,6 +1474,45 @@ int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs
     ctx->stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return IRS_OK;
+}
+
+int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                const irs_fm_options *opt, double *R, double *BT) {
+    return run_forward(ctx, profiles, n_prof, snd, opt, 0, 0.0, R, BT, nullptr, nullptr);
+}
+
+int irs_channel_od(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                   const irs_fm_options *opt, double floor, float *tau_ch, float *cim_g) {
+    if (ctx && !(floor > 0.0 && floor < 1.0)) return fail(ctx, IRS_EINVAL, "floor must be in (0, 1)");
+    return run_forward(ctx, profiles, n_prof, snd, opt, cim_g ? 2 : 1, floor, nullptr, nullptr, tau_ch, cim_g);
+}
+
+int irs_layer_properties(const irs_profile *pr, int T_method, double *p_mid, double *dp, double *T_ave,
+                         double *vmr_cg, double *p_cg, double *T_cg) {
+    if (!pr || !p_mid || !dp || !T_ave || !vmr_cg || !p_cg || !T_cg || !pr->p_hpa || !pr->T ||
+        (pr->n_species > 0 && !pr->vmr))
+        return IRS_EINVAL;
+    LayerProps lp;
+    int rc = layer_properties(nullptr, pr, T_method, lp);
+    if (rc) return rc;
+    const int L = pr->n_levels - 1;
+    for (int k = 0; k < L; ++k) {
+        p_mid[k] = lp.p_mid[k];
+        dp[k] = lp.dp[k];
+        T_ave[k] = lp.T_ave[k];
+    }
+    for (int s = 0; s < pr->n_species; ++s)
+        for (int k = 0; k < L; ++k) {
+            vmr_cg[(size_t)s * L + k] = lp.vmr_cg[s][k];
+            p_cg[(size_t)s * L + k] = lp.p_cg[s][k];
+            T_cg[(size_t)s * L + k] = lp.T_cg[s][k];
+        }
+    return IRS_OK;
+}
+
+int irs_channel_transmittance(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                              const irs_fm_options *opt, float *T_lev, float *cim_G) {
+    return run_forward(ctx, profiles, n_prof, snd, opt, 3, 1e-4, nullptr, nullptr, T_lev, cim_G);
 }
 
 int irs_set_schedule(irs_ctx *ctx, const irs_schedule *sc) {

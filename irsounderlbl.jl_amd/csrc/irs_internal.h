@@ -88,11 +88,33 @@ struct IlsArgs {
     int n_prof;                    // blockIdx.y = profile; spec is [n_prof][n] (0 is taken as 1)
 };
 
+// Channel optical depths (k_chod.hip)
+struct ChodArgs {
+    const float *tau;              // [n_prof][n_layers][n]
+    int64_t n;
+    int n_layers;
+    const double *taps;            // [2*half+1], area-normalised with dnu
+    int64_t half;
+    double dnu;
+    const int64_t *ch_idx;         // exact internal node of each channel
+    int64_t n_ch;
+    double floor;                  // level-to-space channel transmittance below which a layer
+                                   // takes its own (layer-alone) channel optical depth
+    float od_max;
+    const double *scal;            // [n_prof][3] = {T_sfc, mu, eps}: the path is tau / mu
+    float *out;                    // [n_prof][n_layers][n_ch]
+    float *lev_out;                // optional [n_prof][n_layers+1][n_ch] level-to-space channel transmittance
+    const float *lev_in;           // = lev_out, read by the CIM pass
+    float *g_out;                  // optional [n_prof][n_layers][n_ch] CIM channel weight
+    int raw_G;                     // 1: store G_k unnormalised
+};
+
 // launchers (k_*.hip)
 hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s);
 hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hipStream_t s);
 hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s);
 int linesum_persistent_capacity(int P);
+hipError_t launch_chod(const ChodArgs &a, int n_prof, hipStream_t s);
 hipError_t launch_rte(const RteArgs &a, hipStream_t s);
 hipError_t launch_ils_taps(const double *A, const double *L, int n_pts, double dL, double dnu,
                            int64_t half, double *taps, hipStream_t s);

@@ -1,0 +1,54 @@
+// Internal declarations of the emulator MLP and emulator kernels (k_mlp.hip, k_emu.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GEMM_GELU  1    // C32 <- Z, C16 <- bf16(gelu(Z))
+#define GEMM_DGELU 2    // v *= gelu'(Zin[m][n]) before storing
+
+struct GemmArgs {
+    const uint16_t *A, *B;       // bf16
+    int M, N, K, lda, ldb, ldc;
+    float alpha;
+    const float *bias;           // [N] or null
+    const float *Zin;            // [M][ldc] for GEMM_DGELU
+    float *C32;                  // optional f32 output
+    uint16_t *C16;               // optional bf16 output
+    int flags;
+};
+
+hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
+hipError_t launch_f32_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);
+hipError_t launch_gather_batch(const float *X, const float *T, const int32_t *idx, int B, int n_in, int ld,
+                               int n_out, uint16_t *xb, float *tb, hipStream_t s);
+hipError_t launch_mse_grad(const float *Y, const float *T, int64_t n, float scale, uint16_t *dY, float *partial,
+                           hipStream_t s);
+hipError_t launch_colsum_bf16(const uint16_t *dZ, int M, int N, float *db, hipStream_t s);
+hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *w16, int64_t n, float lr, float b1,
+                       float b2, float eps, int step, hipStream_t s);
+hipError_t launch_gelu_tangent(const float *Z, const float *dZ, int64_t n, uint16_t *dH, hipStream_t s);
+hipError_t launch_sum_partials(const float *partial, int n, float scale, float *out, hipStream_t s);
+hipError_t launch_broadcast_row(const float *row, int B, int N, float *out, hipStream_t s);
+
+// Emulated forward model (k_emu.hip): decode of the level channel
+// transmittances and CIM sums from PCA coefficients fused with the channel RTE.
+struct EmuRteArgs {
+    const float *coef;           // [n_prof][L][n_pc]   (MLP output)
+    const float *coef_scale;     // [n_pc]: coefficient = MLP output * coef_scale
+    const float *basis;          // [n_pc][2 n_ch]: two targets per layer and channel
+    const float *t_mean;         // [L][2 n_ch]
+    int target_kind;             // 0: (t_k, g_k) ratios, 1: (T_k, G_k) levels
+    const double *ch_nu;         // [n_ch]
+    const double *T_lev, *T_ave; // [n_prof][L+1], [n_prof][L]
+    const double *scal;          // [n_prof][3] = {T_sfc, mu, eps}
+    int n_prof, L, n_pc, n_ch, source_function;
+    double *R, *BT;              // [n_prof][n_ch]
+    float *t_out;                // optional [n_prof][L][n_ch] decoded level transmittances T_k
+    // Jacobian outputs (optional): dBT/dt_k [n_prof][L][n_ch] and dBT/dT_lev, dBT/dT_ave source terms
+    double *dbt_dt;              // [n_prof][L][n_ch]
+    double *dbt_dtlev;           // [n_prof][L+1][n_ch]
+    double *dbt_dtave;           // [n_prof][L][n_ch]
+};
+
+hipError_t launch_emu_rte(const EmuRteArgs &a, hipStream_t s);

@@ -1,0 +1,590 @@
+// C ABI of the emulator: the device-resident MLP (BF16 MFMA training and
+// inference) and the emulated forward model (PCA decode + channel RTE).
+// Neither has a counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/irs.h"
+#include "irs_mlp.h"
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+}  // namespace
+
+struct irs_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int n_in = 0, n_in_pad = 64, H = 512, n_out = 0, max_batch = 0;
+    int dims[5] = {};
+    int64_t off_W[4] = {}, off_b[4] = {}, n_params = 0;
+    float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
+    uint16_t *P16 = nullptr;
+    uint16_t *X16 = nullptr, *H16[3] = {}, *dY16 = nullptr, *dZ16[3] = {};
+    float *Z[3] = {}, *Y = nullptr, *T = nullptr, *partial = nullptr, *loss = nullptr, *tang = nullptr;
+    float *Xin = nullptr;            // staging for host inputs [max_batch][n_in]
+    float *Xall = nullptr, *Tall = nullptr;
+    int64_t n_data = 0;
+    int32_t *idx = nullptr;
+    int64_t idx_cap = 0;
+    int step = 0;
+};
+
+struct irs_emu {
+    irs_mlp *mlp = nullptr;
+    int L = 0, n_pc = 0, n_ch = 0, n_in = 0, n_tpc = 1;
+    std::vector<float> x_mean, x_std;
+    float *coef_scale = nullptr;     // [n_pc]
+    float *basis = nullptr;          // [n_pc][n_ch]
+    float *t_mean = nullptr;         // [L][n_ch]
+    double *ch_nu = nullptr;         // [n_ch]
+    float *coef = nullptr;           // [cap rows][n_pc]
+    double *prof = nullptr;          // T_lev, T_ave, scal
+    double *R = nullptr, *BT = nullptr, *jac = nullptr;
+    int64_t cap_rows = 0, cap_prof = 0, cap_jac = 0;
+    std::string err;
+};
+
+namespace {
+
+#define MCHK(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            m->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return IRS_EHIP;                                                                \
+        }                                                                                   \
+    } while (0)
+
+int mfail(irs_mlp *m, int code, const char *msg) {
+    m->err = msg;
+    return code;
+}
+
+// Forward pass of B rows already staged in X16; keeps Z (f32) and H (bf16) of
+// the hidden layers; output Y (f32).
+int mlp_forward_staged(irs_mlp *m, int B) {
+    const uint16_t *in = m->X16;
+    int ld_in = m->n_in_pad;
+    for (int l = 0; l < 4; ++l) {
+        GemmArgs g{};
+        g.A = in;
+        g.B = m->P16 + m->off_W[l];
+        g.M = B;
+        g.N = m->dims[l + 1];
+        g.K = m->dims[l];
+        g.lda = ld_in;
+        g.ldb = m->dims[l + 1];
+        g.ldc = m->dims[l + 1];
+        g.alpha = 1.0f;
+        g.bias = m->P + m->off_b[l];
+        if (l < 3) {
+            g.C32 = m->Z[l];
+            g.C16 = m->H16[l];
+            g.flags = GEMM_GELU;
+            in = m->H16[l];
+            ld_in = m->dims[l + 1];
+        } else {
+            g.C32 = m->Y;
+        }
+        MCHK(launch_gemm_bf16(g, false, false, m->stream));
+    }
+    return IRS_OK;
+}
+
+// Backward of the MSE loss for B rows (targets in T); gradient into G.
+int mlp_backward(irs_mlp *m, int B) {
+    const int nout = m->n_out;
+    const int nblk = (int)(((int64_t)B * nout + 255) / 256);
+    MCHK(launch_mse_grad(m->Y, m->T, (int64_t)B * nout, 2.0f / ((float)B * nout), m->dY16, m->partial,
+                         m->stream));
+    MCHK(launch_sum_partials(m->partial, nblk, 1.0f / ((float)B * nout), m->loss, m->stream));
+    const uint16_t *dcur = m->dY16;            // gradient wrt the output of layer l (bf16)
+    for (int l = 3; l >= 0; --l) {
+        const uint16_t *act = l == 0 ? m->X16 : m->H16[l - 1];
+        const int din = m->dims[l], dout = m->dims[l + 1];
+        // dW_l = act^T . dcur      [din x dout]
+        GemmArgs gw{};
+        gw.A = act;
+        gw.B = dcur;
+        gw.M = din;
+        gw.N = dout;
+        gw.K = B;
+        gw.lda = din;
+        gw.ldb = dout;
+        gw.ldc = dout;
+        gw.alpha = 1.0f;
+        gw.C32 = m->G + m->off_W[l];
+        MCHK(launch_gemm_bf16(gw, true, false, m->stream));
+        MCHK(launch_colsum_bf16(dcur, B, dout, m->G + m->off_b[l], m->stream));
+        if (l > 0) {
+            // dZ_{l-1} = (dcur . W_l^T) * gelu'(Z_{l-1})     [B x din]
+            GemmArgs gx{};
+            gx.A = dcur;
+            gx.B = m->P16 + m->off_W[l];
+            gx.M = B;
+            gx.N = din;
+            gx.K = dout;
+            gx.lda = dout;
+            gx.ldb = dout;
+            gx.ldc = din;
+            gx.alpha = 1.0f;
+            gx.Zin = m->Z[l - 1];
+            gx.C16 = m->dZ16[l - 1];
+            gx.flags = GEMM_DGELU;
+            MCHK(launch_gemm_bf16(gx, false, true, m->stream));
+            dcur = m->dZ16[l - 1];
+        }
+    }
+    return IRS_OK;
+}
+
+int stage_inputs(irs_mlp *m, const float *x, int64_t rows, int B) {
+    // host rows -> Xin (f32) -> X16 (bf16, padded); rows beyond `rows` are zero
+    MCHK(hipMemsetAsync(m->Xin, 0, sizeof(float) * (size_t)B * m->n_in, m->stream));
+    MCHK(hipMemcpyAsync(m->Xin, x, sizeof(float) * (size_t)rows * m->n_in, hipMemcpyHostToDevice, m->stream));
+    MCHK(launch_gather_batch(m->Xin, nullptr, nullptr, B, m->n_in, m->n_in_pad, 0, m->X16, nullptr, m->stream));
+    return IRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int irs_mlp_create(int device, int n_in, int n_hidden, int n_out, int max_batch, uint64_t seed, irs_mlp **out) {
+    if (!out) return IRS_EINVAL;
+    *out = nullptr;
+    if (n_in < 1 || n_in > 64 || n_hidden < 64 || n_hidden % 64 || n_out < 64 || n_out % 64 || max_batch < 64 ||
+        max_batch % 64)
+        return IRS_EINVAL;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd) return IRS_ENODEV;
+    if (hipSetDevice(device) != hipSuccess) return IRS_ENODEV;
+    irs_mlp *m = new irs_mlp();
+    m->device = device;
+    m->n_in = n_in;
+    m->H = n_hidden;
+    m->n_out = n_out;
+    m->max_batch = max_batch;
+    const int d[5] = {m->n_in_pad, n_hidden, n_hidden, n_hidden, n_out};
+    std::memcpy(m->dims, d, sizeof d);
+    int64_t o = 0;
+    for (int l = 0; l < 4; ++l) {
+        m->off_W[l] = o;
+        o += (int64_t)d[l] * d[l + 1];
+        m->off_b[l] = o;
+        o += d[l + 1];
+    }
+    m->n_params = o;
+    bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && dalloc(&m->P, o) == hipSuccess && dalloc(&m->G, o) == hipSuccess && dalloc(&m->Mo, o) == hipSuccess &&
+         dalloc(&m->Vo, o) == hipSuccess && dalloc(&m->P16, o) == hipSuccess;
+    const size_t B = max_batch;
+    ok = ok && dalloc(&m->X16, B * m->n_in_pad) == hipSuccess && dalloc(&m->Xin, B * n_in) == hipSuccess &&
+         dalloc(&m->dY16, B * n_out) == hipSuccess && dalloc(&m->Y, B * n_out) == hipSuccess &&
+         dalloc(&m->T, B * n_out) == hipSuccess && dalloc(&m->partial, (B * n_out + 255) / 256) == hipSuccess &&
+         dalloc(&m->loss, 1) == hipSuccess && dalloc(&m->tang, B * n_hidden) == hipSuccess;
+    for (int l = 0; l < 3 && ok; ++l)
+        ok = dalloc(&m->H16[l], B * n_hidden) == hipSuccess && dalloc(&m->dZ16[l], B * n_hidden) == hipSuccess &&
+             dalloc(&m->Z[l], B * n_hidden) == hipSuccess;
+    if (!ok) {
+        irs_mlp_destroy(m);
+        return IRS_ENOMEM;
+    }
+    // He-uniform weights (GELU), zero biases; padded input rows of W1 stay zero
+    std::vector<float> p(o, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < 4; ++l) {
+        const int fan_in = l == 0 ? n_in : d[l];
+        const float lim = std::sqrt(6.0f / (float)fan_in) * (l == 3 ? 0.5f : 1.0f);
+        std::uniform_real_distribution<float> U(-lim, lim);
+        for (int r = 0; r < d[l]; ++r)
+            for (int c = 0; c < d[l + 1]; ++c)
+                p[m->off_W[l] + (int64_t)r * d[l + 1] + c] = (l == 0 && r >= n_in) ? 0.0f : U(rng);
+    }
+    if (irs_mlp_set_params(m, p.data()) != IRS_OK) {
+        irs_mlp_destroy(m);
+        return IRS_EHIP;
+    }
+    *out = m;
+    return IRS_OK;
+}
+
+void irs_mlp_destroy(irs_mlp *m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    void *ptrs[] = {m->P, m->G, m->Mo, m->Vo, m->P16, m->X16, m->dY16, m->Y, m->T, m->partial, m->loss, m->tang,
+                    m->Xin, m->Xall, m->Tall, m->idx, m->H16[0], m->H16[1], m->H16[2], m->dZ16[0], m->dZ16[1],
+                    m->dZ16[2], m->Z[0], m->Z[1], m->Z[2]};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char *irs_mlp_last_error(const irs_mlp *m) { return m ? m->err.c_str() : "null mlp"; }
+
+int64_t irs_mlp_n_params(const irs_mlp *m) { return m ? m->n_params : -1; }
+
+int irs_mlp_input_pad(const irs_mlp *m) { return m ? m->n_in_pad : -1; }
+
+int irs_mlp_get_params(irs_mlp *m, float *p) {
+    if (!m || !p) return IRS_EINVAL;
+    (void)hipSetDevice(m->device);
+    MCHK(hipMemcpyAsync(p, m->P, sizeof(float) * m->n_params, hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    return IRS_OK;
+}
+
+int irs_mlp_set_params(irs_mlp *m, const float *p) {
+    if (!m || !p) return IRS_EINVAL;
+    (void)hipSetDevice(m->device);
+    MCHK(hipMemcpyAsync(m->P, p, sizeof(float) * m->n_params, hipMemcpyHostToDevice, m->stream));
+    MCHK(hipMemsetAsync(m->Mo, 0, sizeof(float) * m->n_params, m->stream));
+    MCHK(hipMemsetAsync(m->Vo, 0, sizeof(float) * m->n_params, m->stream));
+    MCHK(launch_f32_to_bf16(m->P, m->P16, m->n_params, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    m->step = 0;
+    return IRS_OK;
+}
+
+int irs_mlp_forward(irs_mlp *m, const float *x, int64_t n, float *y) {
+    if (!m || !x || !y || n < 0) return IRS_EINVAL;
+    (void)hipSetDevice(m->device);
+    for (int64_t r0 = 0; r0 < n; r0 += m->max_batch) {
+        const int64_t rows = std::min<int64_t>(m->max_batch, n - r0);
+        const int B = (int)((rows + 63) / 64 * 64);
+        int rc = stage_inputs(m, x + r0 * m->n_in, rows, B);
+        if (rc) return rc;
+        rc = mlp_forward_staged(m, B);
+        if (rc) return rc;
+        MCHK(hipMemcpyAsync(y + r0 * m->n_out, m->Y, sizeof(float) * rows * m->n_out, hipMemcpyDeviceToHost,
+                            m->stream));
+    }
+    MCHK(hipStreamSynchronize(m->stream));
+    return IRS_OK;
+}
+
+int irs_mlp_loss_grad(irs_mlp *m, const float *x, const float *t, int B, float *grad, float *loss) {
+    if (!m || !x || !t || !grad || !loss) return IRS_EINVAL;
+    if (B < 64 || B % 64 || B > m->max_batch) return mfail(m, IRS_EINVAL, "batch must be a multiple of 64 <= max_batch");
+    (void)hipSetDevice(m->device);
+    int rc = stage_inputs(m, x, B, B);
+    if (rc) return rc;
+    MCHK(hipMemcpyAsync(m->T, t, sizeof(float) * (size_t)B * m->n_out, hipMemcpyHostToDevice, m->stream));
+    rc = mlp_forward_staged(m, B);
+    if (rc) return rc;
+    rc = mlp_backward(m, B);
+    if (rc) return rc;
+    MCHK(hipMemcpyAsync(grad, m->G, sizeof(float) * m->n_params, hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    return IRS_OK;
+}
+
+int irs_mlp_set_data(irs_mlp *m, const float *X, const float *T, int64_t n) {
+    if (!m || !X || !T || n < 1) return IRS_EINVAL;
+    (void)hipSetDevice(m->device);
+    if (m->Xall) (void)hipFree(m->Xall);
+    if (m->Tall) (void)hipFree(m->Tall);
+    m->Xall = m->Tall = nullptr;
+    m->n_data = 0;
+    MCHK(dalloc(&m->Xall, (size_t)n * m->n_in));
+    MCHK(dalloc(&m->Tall, (size_t)n * m->n_out));
+    MCHK(hipMemcpyAsync(m->Xall, X, sizeof(float) * (size_t)n * m->n_in, hipMemcpyHostToDevice, m->stream));
+    MCHK(hipMemcpyAsync(m->Tall, T, sizeof(float) * (size_t)n * m->n_out, hipMemcpyHostToDevice, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    m->n_data = n;
+    return IRS_OK;
+}
+
+int irs_mlp_train(irs_mlp *m, int steps, int batch, float lr, uint64_t seed, float *losses) {
+    if (!m || steps < 0) return IRS_EINVAL;
+    if (batch < 64 || batch % 64 || batch > m->max_batch) return mfail(m, IRS_EINVAL, "batch must be a multiple of 64 <= max_batch");
+    if (m->n_data < 1) return mfail(m, IRS_EINVAL, "no training data (irs_mlp_set_data)");
+    (void)hipSetDevice(m->device);
+    // minibatch indices: shuffled epochs, drawn on the host once per call
+    const int64_t need = (int64_t)steps * batch;
+    if (need > m->idx_cap) {
+        if (m->idx) (void)hipFree(m->idx);
+        m->idx = nullptr;
+        m->idx_cap = 0;
+        MCHK(dalloc(&m->idx, (size_t)need));
+        m->idx_cap = need;
+    }
+    std::vector<int32_t> idx((size_t)need);
+    std::vector<int32_t> perm((size_t)m->n_data);
+    for (int64_t i = 0; i < m->n_data; ++i) perm[i] = (int32_t)i;
+    std::mt19937_64 rng(seed);
+    int64_t pos = m->n_data;
+    for (int64_t i = 0; i < need; ++i) {
+        if (pos == m->n_data) {
+            std::shuffle(perm.begin(), perm.end(), rng);
+            pos = 0;
+        }
+        idx[i] = perm[pos++];
+    }
+    MCHK(hipMemcpyAsync(m->idx, idx.data(), sizeof(int32_t) * need, hipMemcpyHostToDevice, m->stream));
+    std::vector<float> lh(steps, 0.0f);
+    float *dloss = nullptr;
+    MCHK(dalloc(&dloss, (size_t)std::max(steps, 1)));
+    int rc = IRS_OK;
+    for (int s = 0; s < steps && rc == IRS_OK; ++s) {
+        hipError_t e = launch_gather_batch(m->Xall, m->Tall, m->idx + (int64_t)s * batch, batch, m->n_in,
+                                           m->n_in_pad, m->n_out, m->X16, m->T, m->stream);
+        if (e != hipSuccess) { rc = mfail(m, IRS_EHIP, hipGetErrorString(e)); break; }
+        rc = mlp_forward_staged(m, batch);
+        if (rc) break;
+        rc = mlp_backward(m, batch);
+        if (rc) break;
+        m->step += 1;
+        e = launch_adam(m->P, m->G, m->Mo, m->Vo, m->P16, m->n_params, lr, 0.9f, 0.999f, 1e-8f, m->step, m->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dloss + s, m->loss, sizeof(float), hipMemcpyDeviceToDevice, m->stream);
+        if (e != hipSuccess) { rc = mfail(m, IRS_EHIP, hipGetErrorString(e)); break; }
+    }
+    if (rc == IRS_OK && steps > 0) {
+        hipError_t e = hipMemcpyAsync(lh.data(), dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        if (e != hipSuccess) rc = mfail(m, IRS_EHIP, hipGetErrorString(e));
+    } else {
+        (void)hipStreamSynchronize(m->stream);
+    }
+    (void)hipFree(dloss);
+    if (rc == IRS_OK && losses)
+        for (int s = 0; s < steps; ++s) losses[s] = lh[s];
+    return rc;
+}
+
+// dy/dx for every row: J[r][o][i], forward-mode tangents through the network
+// (bf16 tangents, f32 accumulation).
+int irs_mlp_input_jacobian(irs_mlp *m, const float *x, int64_t n, float *J) {
+    if (!m || !x || !J || n < 0) return IRS_EINVAL;
+    (void)hipSetDevice(m->device);
+    std::vector<float> w1((size_t)m->n_in * m->H);
+    MCHK(hipMemcpyAsync(w1.data(), m->P + m->off_W[0], sizeof(float) * w1.size(), hipMemcpyDeviceToHost, m->stream));
+    MCHK(hipStreamSynchronize(m->stream));
+    float *row = nullptr;
+    MCHK(dalloc(&row, (size_t)m->H));
+    std::vector<float> dy((size_t)m->max_batch * m->n_out);
+    int rc = IRS_OK;
+    for (int64_t r0 = 0; r0 < n && rc == IRS_OK; r0 += m->max_batch) {
+        const int64_t rows = std::min<int64_t>(m->max_batch, n - r0);
+        const int B = (int)((rows + 63) / 64 * 64);
+        rc = stage_inputs(m, x + r0 * m->n_in, rows, B);
+        if (rc) break;
+        rc = mlp_forward_staged(m, B);
+        if (rc) break;
+        for (int i = 0; i < m->n_in && rc == IRS_OK; ++i) {
+            hipError_t e = hipMemcpyAsync(row, w1.data() + (size_t)i * m->H, sizeof(float) * m->H,
+                                          hipMemcpyHostToDevice, m->stream);
+            if (e == hipSuccess) e = launch_broadcast_row(row, B, m->H, m->tang, m->stream);
+            if (e == hipSuccess) e = launch_gelu_tangent(m->Z[0], m->tang, (int64_t)B * m->H, m->dZ16[0], m->stream);
+            for (int l = 1; l < 4 && e == hipSuccess; ++l) {
+                GemmArgs g{};
+                g.A = m->dZ16[l - 1];
+                g.B = m->P16 + m->off_W[l];
+                g.M = B;
+                g.N = m->dims[l + 1];
+                g.K = m->dims[l];
+                g.lda = m->dims[l];
+                g.ldb = m->dims[l + 1];
+                g.ldc = m->dims[l + 1];
+                g.alpha = 1.0f;
+                g.C32 = l < 3 ? m->tang : m->Y;
+                e = launch_gemm_bf16(g, false, false, m->stream);
+                if (e == hipSuccess && l < 3)
+                    e = launch_gelu_tangent(m->Z[l], m->tang, (int64_t)B * m->H, m->dZ16[l], m->stream);
+            }
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dy.data(), m->Y, sizeof(float) * rows * m->n_out, hipMemcpyDeviceToHost, m->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+            if (e != hipSuccess) { rc = mfail(m, IRS_EHIP, hipGetErrorString(e)); break; }
+            for (int64_t r = 0; r < rows; ++r)
+                for (int o = 0; o < m->n_out; ++o)
+                    J[((r0 + r) * m->n_out + o) * m->n_in + i] = dy[(size_t)r * m->n_out + o];
+        }
+    }
+    (void)hipFree(row);
+    return rc;
+}
+
+// ---- emulator ----------------------------------------------------------------
+int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_tpc, const float *x_mean,
+                   const float *x_std, const float *coef_scale, const float *basis, const float *t_mean,
+                   const double *ch_nu, irs_emu **out) {
+    if (!out) return IRS_EINVAL;
+    *out = nullptr;
+    if (!mlp || !x_mean || !x_std || !coef_scale || !basis || !t_mean || !ch_nu || n_layers < 1 || n_pc < 1 ||
+        n_channels < 1 || n_pc != mlp->n_out || (n_tpc != 2 && n_tpc != 3))
+        return IRS_EINVAL;
+    (void)hipSetDevice(mlp->device);
+    irs_emu *e = new irs_emu();
+    e->mlp = mlp;
+    e->L = n_layers;
+    e->n_pc = n_pc;
+    e->n_ch = n_channels;
+    e->n_in = mlp->n_in;
+    e->n_tpc = n_tpc;
+    const size_t W = (size_t)2 * n_channels;
+    e->x_mean.assign(x_mean, x_mean + mlp->n_in);
+    e->x_std.assign(x_std, x_std + mlp->n_in);
+    bool ok = dalloc(&e->coef_scale, n_pc) == hipSuccess && dalloc(&e->basis, (size_t)n_pc * W) == hipSuccess &&
+              dalloc(&e->t_mean, (size_t)n_layers * W) == hipSuccess &&
+              dalloc(&e->ch_nu, n_channels) == hipSuccess;
+    ok = ok && hipMemcpy(e->coef_scale, coef_scale, sizeof(float) * n_pc, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->basis, basis, sizeof(float) * (size_t)n_pc * W, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->t_mean, t_mean, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->ch_nu, ch_nu, sizeof(double) * n_channels, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        irs_emu_destroy(e);
+        return IRS_ENOMEM;
+    }
+    *out = e;
+    return IRS_OK;
+}
+
+void irs_emu_destroy(irs_emu *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->mlp->device);
+    (void)hipStreamSynchronize(e->mlp->stream);
+    void *ptrs[] = {e->coef_scale, e->basis, e->t_mean, e->ch_nu, e->coef, e->prof, e->R, e->BT, e->jac};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete e;
+}
+
+const char *irs_emu_last_error(const irs_emu *e) { return e ? e->err.c_str() : "null emulator"; }
+
+}  // extern "C"
+
+namespace {
+
+#define ECHK(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            e->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return IRS_EHIP;                                                                \
+        }                                                                                   \
+    } while (0)
+
+template <typename T>
+hipError_t grow(T **p, int64_t &cap, int64_t need) {
+    if (need <= cap) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    hipError_t r = dalloc(p, (size_t)need);
+    if (r == hipSuccess) cap = need;
+    return r;
+}
+
+// MLP on standardized features -> coef [rows][n_pc] on the device (physical units)
+int emu_coefficients(irs_emu *e, const float *features, int64_t rows) {
+    irs_mlp *m = e->mlp;
+    int64_t cap = e->cap_rows;
+    ECHK(grow(&e->coef, cap, rows * e->n_pc));
+    e->cap_rows = cap;
+    std::vector<float> xs((size_t)rows * e->n_in);
+    for (int64_t r = 0; r < rows; ++r)
+        for (int i = 0; i < e->n_in; ++i)
+            xs[r * e->n_in + i] = (features[r * e->n_in + i] - e->x_mean[i]) / e->x_std[i];
+    for (int64_t r0 = 0; r0 < rows; r0 += m->max_batch) {
+        const int64_t nr = std::min<int64_t>(m->max_batch, rows - r0);
+        const int B = (int)((nr + 63) / 64 * 64);
+        int rc = stage_inputs(m, xs.data() + r0 * e->n_in, nr, B);
+        if (rc) {
+            e->err = m->err;
+            return rc;
+        }
+        rc = mlp_forward_staged(m, B);
+        if (rc) {
+            e->err = m->err;
+            return rc;
+        }
+        ECHK(hipMemcpyAsync(e->coef + r0 * e->n_pc, m->Y, sizeof(float) * nr * e->n_pc, hipMemcpyDeviceToDevice,
+                            m->stream));
+    }
+    // synchronous here: xs is pageable host memory freed on return
+    ECHK(hipStreamSynchronize(m->stream));
+    return IRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// features [n_prof][L][n_in] (raw), T_lev [n_prof][L+1], T_ave [n_prof][L],
+// scal [n_prof][3] = {T_sfc, mu, eps}; R, BT [n_prof][n_ch].  Optional
+// t_out [n_prof][L][n_ch] (decoded layer channel transmittances at nadir).
+int irs_emu_forward(irs_emu *e, const float *features, const double *T_lev, const double *T_ave,
+                    const double *scal, int n_prof, int source_function, double *R, double *BT, float *t_out) {
+    if (!e || !features || !T_lev || !T_ave || !scal || !R || !BT || n_prof < 1) return IRS_EINVAL;
+    if (source_function != IRS_SRC_CIM) {
+        e->err = "the emulator reproduces the CIM source function only";
+        return IRS_EINVAL;
+    }
+    irs_mlp *m = e->mlp;
+    (void)hipSetDevice(m->device);
+    const int L = e->L;
+    int rc = emu_coefficients(e, features, (int64_t)n_prof * L);
+    if (rc) return rc;
+    const int64_t pd = (int64_t)n_prof * (2 * L + 1 + 3);
+    int64_t cp = e->cap_prof;
+    ECHK(grow(&e->prof, cp, pd));
+    e->cap_prof = cp;
+    double *dp = e->prof;
+    ECHK(hipMemcpyAsync(dp, T_lev, sizeof(double) * n_prof * (L + 1), hipMemcpyHostToDevice, m->stream));
+    ECHK(hipMemcpyAsync(dp + (size_t)n_prof * (L + 1), T_ave, sizeof(double) * n_prof * L, hipMemcpyHostToDevice,
+                        m->stream));
+    ECHK(hipMemcpyAsync(dp + (size_t)n_prof * (2 * L + 1), scal, sizeof(double) * n_prof * 3, hipMemcpyHostToDevice,
+                        m->stream));
+    if (e->R) (void)hipFree(e->R);
+    if (e->BT) (void)hipFree(e->BT);
+    e->R = e->BT = nullptr;
+    ECHK(dalloc(&e->R, (size_t)n_prof * e->n_ch));
+    ECHK(dalloc(&e->BT, (size_t)n_prof * e->n_ch));
+    float *tdev = nullptr;
+    if (t_out) ECHK(dalloc(&tdev, (size_t)n_prof * L * e->n_ch));
+    EmuRteArgs a{};
+    a.coef = e->coef;
+    a.basis = e->basis;
+    a.t_mean = e->t_mean;
+    a.ch_nu = e->ch_nu;
+    a.T_lev = dp;
+    a.T_ave = dp + (size_t)n_prof * (L + 1);
+    a.scal = dp + (size_t)n_prof * (2 * L + 1);
+    a.n_prof = n_prof;
+    a.L = L;
+    a.n_pc = e->n_pc;
+    a.n_ch = e->n_ch;
+    a.source_function = source_function;
+    a.R = e->R;
+    a.BT = e->BT;
+    a.t_out = tdev;
+    a.coef_scale = e->coef_scale;
+    a.target_kind = e->n_tpc == 3 ? 1 : 0;
+    ECHK(launch_emu_rte(a, m->stream));
+    ECHK(hipMemcpyAsync(R, e->R, sizeof(double) * n_prof * e->n_ch, hipMemcpyDeviceToHost, m->stream));
+    ECHK(hipMemcpyAsync(BT, e->BT, sizeof(double) * n_prof * e->n_ch, hipMemcpyDeviceToHost, m->stream));
+    if (t_out)
+        ECHK(hipMemcpyAsync(t_out, tdev, sizeof(float) * n_prof * L * e->n_ch, hipMemcpyDeviceToHost, m->stream));
+    ECHK(hipStreamSynchronize(m->stream));
+    if (tdev) (void)hipFree(tdev);
+    return IRS_OK;
+}
+
+}  // extern "C"

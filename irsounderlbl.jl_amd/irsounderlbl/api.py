@@ -88,7 +88,12 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_default_options", "irs_last_stats", "irs_linelist_upload", "irs_linelist_free",
            "irs_xsec_layer", "irs_rte", "irs_ils_half", "irs_ils_kernel", "irs_ils_channels",
            "irs_brightness_temperature", "irs_forward_dims", "irs_forward", "irs_state_size",
-           "irs_jacobian", "irs_set_schedule", "irs_get_schedule")
+           "irs_jacobian", "irs_set_schedule", "irs_get_schedule", "irs_channel_od",
+           "irs_layer_properties", "irs_mlp_create", "irs_mlp_destroy", "irs_mlp_last_error",
+           "irs_mlp_n_params", "irs_mlp_input_pad", "irs_mlp_get_params", "irs_mlp_set_params",
+           "irs_mlp_forward", "irs_mlp_loss_grad", "irs_mlp_set_data", "irs_mlp_train",
+           "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
+           "irs_emu_forward", "irs_channel_transmittance")
 
 _lib = None
 
@@ -126,11 +131,38 @@ def load_library(path: str = LIB_PATH):
                               C.POINTER(IrsOptions), P, P]
     L.irs_state_size.argtypes = [C.POINTER(IrsStateSpec), i32]
     L.irs_set_schedule.argtypes = [P, C.POINTER(IrsSchedule)]
+    L.irs_channel_od.argtypes = [P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder), C.POINTER(IrsOptions),
+                                 f64, P, P]
     L.irs_get_schedule.argtypes = [P, C.POINTER(IrsSchedule)]
     L.irs_jacobian.argtypes = [P, C.POINTER(IrsProfile), C.POINTER(IrsSounder), C.POINTER(IrsOptions),
                                C.POINTER(IrsStateSpec), P, P, P]
+    L.irs_channel_transmittance.argtypes = [P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
+                                            C.POINTER(IrsOptions), P, P]
+    L.irs_layer_properties.argtypes = [C.POINTER(IrsProfile), i32, P, P, P, P, P, P]
+    L.irs_mlp_create.argtypes = [i32, i32, i32, i32, i32, C.c_uint64, C.POINTER(P)]
+    L.irs_mlp_destroy.argtypes = [P]
+    L.irs_mlp_destroy.restype = None
+    L.irs_mlp_last_error.argtypes = [P]
+    L.irs_mlp_last_error.restype = C.c_char_p
+    L.irs_mlp_n_params.argtypes = [P]
+    L.irs_mlp_n_params.restype = i64
+    L.irs_mlp_input_pad.argtypes = [P]
+    L.irs_mlp_get_params.argtypes = [P, P]
+    L.irs_mlp_set_params.argtypes = [P, P]
+    L.irs_mlp_forward.argtypes = [P, P, i64, P]
+    L.irs_mlp_loss_grad.argtypes = [P, P, P, i32, P, P]
+    L.irs_mlp_set_data.argtypes = [P, P, P, i64]
+    L.irs_mlp_train.argtypes = [P, i32, i32, C.c_float, C.c_uint64, P]
+    L.irs_mlp_input_jacobian.argtypes = [P, P, i64, P]
+    L.irs_emu_create.argtypes = [P, i32, i32, i32, i32, P, P, P, P, P, P, C.POINTER(P)]
+    L.irs_emu_destroy.argtypes = [P]
+    L.irs_emu_destroy.restype = None
+    L.irs_emu_last_error.argtypes = [P]
+    L.irs_emu_last_error.restype = C.c_char_p
+    L.irs_emu_forward.argtypes = [P, P, P, P, P, i32, i32, P, P, P]
     for name in EXPORTS:
-        if name not in ("irs_destroy", "irs_default_options", "irs_last_error"):
+        if name not in ("irs_destroy", "irs_default_options", "irs_last_error", "irs_mlp_destroy",
+                        "irs_mlp_last_error", "irs_mlp_n_params", "irs_emu_destroy", "irs_emu_last_error"):
             getattr(L, name).restype = getattr(L, name).restype or i32
     _lib = L
     return L
@@ -516,6 +548,93 @@ def analytic_jacobian(prof: AtmosphericProfile, linelists: Dict[GasSpecies, HITR
         K[:, spec.emis_index] = Kd[:, c]
         x0[spec.emis_index] = xd[c]
     return Jacobian(K, spec, grid.nu, x0, y0, observable)
+
+
+def channel_optical_depths(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
+                           sounder: Optional[Sounder] = None, geom=None, floor: float = 1e-4,
+                           internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
+                           cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
+                           with_ils: bool = True, apodization="gaussian", line_mixing=None,
+                           T_method="logp_at_pcg", x_far: float = 122.0, cim: bool = False,
+                           ctx: Optional[Context] = None):
+    """Per-layer channel optical depths along each profile's viewing path
+    (geom, default nadir), [n_prof, n_layers, n_channels] float32
+    (irs_channel_od): tau_k = ln(T_{k+1}/T_k) of the ILS-convolved
+    level-to-space transmittances T_k, or the layer-alone channel optical depth
+    where T_{k+1} < floor.  The emulator's training target.  With cim=True also
+    returns the CIM channel weights g (same shape): (tau, g)."""
+    inst = sounder if sounder is not None else IASIInstrument()
+    batch = isinstance(prof, (list, tuple))
+    profs = list(prof) if batch else [prof]
+    ctx = ctx or default_context()
+    opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
+                   apodization, line_mixing, T_method, "cim")
+    nb = len(profs)
+    geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
+    mus = [1.0 if g is None else g.mu for g in geoms]
+    b = _Batch(ctx, profs, linelists, [None] * nb, [1.0] * nb, mus)
+    grid = sounder_grid(inst)
+    out = np.zeros((nb, profs[0].n_levels - 1, grid.n), np.float32)
+    g = np.zeros_like(out) if cim else None
+    ctx.check(ctx.lib.irs_channel_od(ctx.h, b.arr, nb, C.byref(_sounder(inst)), C.byref(opt),
+                                     float(floor), _p(out), _p(g)))
+    if cim:
+        return (out, g) if batch else (out[0], g[0])
+    return out if batch else out[0]
+
+
+def layer_properties(prof: AtmosphericProfile, T_method: str = "logp_at_pcg") -> dict:
+    """layer_properties(prof; T_method) (src/Atmosphere/layers.jl:10-178): p_mid, dp,
+    T_ave (TBAR) per layer and vmr_cg, p_cg, T_cg per species (host C code)."""
+    tm = T_method.lstrip(":")
+    if tm not in T_METHOD:
+        raise ValueError(f"T_method must be :logp_at_pcg or :mass_weighted, got :{tm}")
+    species = list(prof.vmr.keys())
+    sp = np.array([int(s) for s in species] or [0], np.int32)
+    hs = np.full(max(len(species), 1), -1, np.int32)
+    vmr = np.ascontiguousarray(np.stack([prof.vmr[s] for s in species]) if species else np.zeros((1, 1)))
+    p, T = prof.pressure, prof.temperature
+    pr = IrsProfile(len(p), _p(p), _p(T), len(species), _p(sp), _p(hs), _p(vmr), 0.0, 1.0, 1.0)
+    L = prof.n_levels - 1
+    out = {k: np.zeros(L) for k in ("p_mid", "dp", "T_ave")}
+    vc, pc, tc = (np.zeros((max(len(species), 1), L)) for _ in range(3))
+    rc = load_library().irs_layer_properties(C.byref(pr), T_METHOD[tm], _p(out["p_mid"]), _p(out["dp"]),
+                                             _p(out["T_ave"]), _p(vc), _p(pc), _p(tc))
+    if rc != IRS_OK:
+        raise ValueError("bad profile for layer_properties")
+    out["vmr_cg"] = {s: vc[i] for i, s in enumerate(species)}
+    out["p_cg"] = {s: pc[i] for i, s in enumerate(species)}
+    out["T_cg"] = {s: tc[i] for i, s in enumerate(species)}
+    return out
+
+
+def channel_transmittance(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
+                          sounder: Optional[Sounder] = None, geom=None,
+                          internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
+                          cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
+                          with_ils: bool = True, apodization="gaussian", line_mixing=None,
+                          T_method="logp_at_pcg", x_far: float = 122.0, ctx: Optional[Context] = None):
+    """ILS-convolved level-to-space transmittances along each profile's path,
+    T_lev [n_prof, n_layers+1, n_channels], and the CIM sums
+    G [n_prof, n_layers, n_channels] (irs_channel_transmittance): the
+    emulator's training target."""
+    inst = sounder if sounder is not None else IASIInstrument()
+    batch = isinstance(prof, (list, tuple))
+    profs = list(prof) if batch else [prof]
+    ctx = ctx or default_context()
+    opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
+                   apodization, line_mixing, T_method, "cim")
+    nb = len(profs)
+    geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
+    mus = [1.0 if g is None else g.mu for g in geoms]
+    b = _Batch(ctx, profs, linelists, [None] * nb, [1.0] * nb, mus)
+    grid = sounder_grid(inst)
+    L = profs[0].n_levels - 1
+    Tl = np.zeros((nb, L + 1, grid.n), np.float32)
+    G = np.zeros((nb, L, grid.n), np.float32)
+    ctx.check(ctx.lib.irs_channel_transmittance(ctx.h, b.arr, nb, C.byref(_sounder(inst)), C.byref(opt),
+                                                _p(Tl), _p(G)))
+    return (Tl, G) if batch else (Tl[0], G[0])
 
 
 def forward_dims(sounder: Sounder, **kw):

@@ -1,0 +1,197 @@
+"""Emulator path: channel optical depths (training target), the BF16 MFMA
+MLP, PCA compression, emulated forward model with LBL spot check and
+fallback, emulator Jacobians."""
+import numpy as np
+import pytest
+
+import refcpu as o
+from helpers import channel_od_ref
+from irsounderlbl import synthetic as syn
+from irsounderlbl.api import channel_optical_depths, channel_transmittance
+from irsounderlbl.types import GasSpecies, Sounder, ViewingGeometry, filter_linelist
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mu", [1.0, 0.7])
+def test_channel_od_matches_oracle(ctx, mu):
+    """GPU channel optical depths and CIM weights vs the FP64 restatement on
+    the oracle's tau (12 cm^-1 of channels in the 15 um band, VP_Y, 49
+    layers, nadir and slant): 1e-4 relative + 2e-5 absolute, away from the
+    floor switch."""
+    lls = syn.linelists_config1()
+    sub = {k: filter_linelist(v, 700.0 - 42.0, 712.0 + 42.0) for k, v in lls.items()}
+    snd = Sounder.make(700.0, 712.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    prof = syn.base_profile(True)
+    floor = 1e-4
+    geom = ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu)
+    got, g = channel_optical_depths(prof, sub, sounder=snd, geom=geom, floor=floor, line_mixing="vpy",
+                                    cim=True, ctx=ctx)
+    _, _, info, tau = o.forward(prof, sub, snd, want_tau=True, line_mixing=1)
+    _, kern = o.ils_kernel(info.dnu, snd.opd_max, snd.fwhm_gauss)
+    ch = 700.0 + 0.25 * np.arange(got.shape[1])
+    ref, Tch, gref = channel_od_ref(tau, info.nu_lo, info.dnu, ch, kern, floor, mu=mu)
+    assert got.shape == ref.shape == g.shape == (49, 49)
+    near = np.abs(Tch[1:] - floor) < 1e-2 * floor          # branch switch at the floor
+    err = np.abs(got - ref)
+    ok = err <= 2e-5 + 1e-4 * np.abs(ref)
+    assert np.all(ok | near), float(np.max(np.where(near, 0, err)))
+    gerr = np.abs(g - gref)
+    assert np.all((gerr <= 2e-5 + 1e-4 * np.abs(gref)) | near), float(np.max(np.where(near, 0, gerr)))
+    # level transmittances and raw CIM sums (the emulator's target)
+    Tl, G = channel_transmittance(prof, sub, sounder=snd, geom=geom, line_mixing="vpy", ctx=ctx)
+    assert np.max(np.abs(Tl - Tch)) < 2e-6 + 1e-4 * np.abs(Tch).max()
+    Gr = channel_od_ref.last_G
+    assert np.max(np.abs(G - Gr)) < 1e-6 + 1e-4 * np.abs(Gr).max()
+    # telescoping: sum_k tau_ch_k = -ln T_0 where the whole column transmits
+    full = Tch[0] >= floor
+    if np.any(full):
+        s = got[:, full].astype(np.float64).sum(axis=0)
+        assert np.allclose(s, -np.log(Tch[0, full]), rtol=1e-4, atol=1e-4)
+
+
+def test_channel_od_transparent_and_flat(ctx):
+    """No lines: every channel optical depth is 0.  Batched profiles give the
+    same bits as single ones."""
+    snd = Sounder.make(700.0, 703.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    prof = syn.base_profile(False)
+    empty = {GasSpecies.CO2: filter_linelist(syn.linelist_config0(), 10.0, 11.0)}
+    got = channel_optical_depths(prof, empty, sounder=snd, ctx=ctx)
+    assert np.all(got == 0.0)
+    ll = {GasSpecies.CO2: filter_linelist(syn.linelist_config0(), 660.0, 745.0)}
+    pp = [x[0] for x in syn.perturbed_profiles(3, seed=7, with_o3=False)]
+    b = channel_optical_depths(pp, ll, sounder=snd, ctx=ctx)
+    for q in range(3):
+        assert np.array_equal(b[q], channel_optical_depths(pp[q], ll, sounder=snd, ctx=ctx))
+
+
+# ---- BF16 MFMA MLP vs the FP64 reference (oracle/mlp_ref.py) -----------------
+from mlp_ref import MLPRef  # noqa: E402
+from irsounderlbl.emulator import (MLP, ChannelODEmulator, SpotChecker, forward_emulated,  # noqa: E402
+                                   layer_features)
+from irsounderlbl.api import forward_model  # noqa: E402
+
+
+def _mlp(seed=11, max_batch=512):
+    m = MLP(5, 512, 256, max_batch, seed=seed)
+    ref = MLPRef(m.get_params(), 5, m.n_in_pad, 512, 256)
+    return m, ref
+
+
+def test_mlp_forward_matches_reference():
+    m, ref = _mlp()
+    X = np.random.default_rng(1).normal(size=(200, 5)).astype(np.float32)   # not a multiple of 64
+    y = m.forward(X)
+    yr = ref.forward(X.astype(np.float64), emulate_bf16=True)
+    y64 = ref.forward(X.astype(np.float64))
+    scale = np.abs(yr).max()
+    assert np.max(np.abs(y - yr)) < 2e-3 * scale
+    assert np.max(np.abs(y - y64)) < 3e-2 * scale          # bf16 storage vs exact FP64
+    m.close()
+
+
+def test_mlp_loss_gradient_matches_reference():
+    m, ref = _mlp(12)
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(128, 5)).astype(np.float32)
+    T = rng.normal(size=(128, 256)).astype(np.float32)
+    loss, g = m.loss_grad(X, T)
+    lr, gr = ref.loss_grad(X.astype(np.float64), T.astype(np.float64), emulate_bf16=True)
+    _, g64 = ref.loss_grad(X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - lr) < 1e-4 * lr
+    dims = ref.dims
+    o = 0
+    for l in range(4):
+        for size in (dims[l] * dims[l + 1], dims[l + 1]):
+            a, b, c = g[o:o + size], gr[o:o + size], g64[o:o + size]
+            nb = np.linalg.norm(b)
+            assert np.linalg.norm(a - b) < 1e-2 * nb + 1e-12, (l, size)
+            assert np.linalg.norm(a - c) < 5e-2 * np.linalg.norm(c) + 1e-12, (l, size)
+            o += size
+    m.close()
+
+
+def test_mlp_input_jacobian_matches_reference():
+    m, ref = _mlp(13)
+    X = np.random.default_rng(3).normal(size=(64, 5)).astype(np.float32)
+    J = m.input_jacobian(X)
+    Jr = ref.input_jacobian(X.astype(np.float64))
+    assert np.linalg.norm(J - Jr) < 3e-2 * np.linalg.norm(Jr)
+    m.close()
+
+
+def test_mlp_training_reduces_loss():
+    """Adam on a smooth 5 -> 256 regression (targets from a fixed random
+    FP64 teacher network): the loss falls by more than 10x."""
+    rng = np.random.default_rng(4)
+    teacher = MLPRef(rng.normal(0, 0.1, MLPRef(np.zeros(10**7), 5, 64, 64, 256).n_params), 5, 64, 64, 256)
+    X = rng.normal(size=(8192, 5))
+    T = teacher.forward(X)
+    T /= T.std()
+    m = MLP(5, 512, 256, 1024, seed=5)
+    m.set_data(X.astype(np.float32), T.astype(np.float32))
+    losses = m.train(400, 512, 1e-3, seed=1)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.1 * losses[:5].mean()
+    # the trained network evaluates to what its loss says
+    y = m.forward(X[:512].astype(np.float32))
+    assert np.mean((y - T[:512]) ** 2) < 2 * losses[-20:].mean() + 1e-4
+    m.close()
+
+
+# ---- emulator end to end on a 12 cm^-1 window --------------------------------
+@pytest.fixture(scope="module")
+def small_emu(ctx):
+    lls = syn.linelists_config1()
+    sub = {k: filter_linelist(v, 700.0 - 42.0, 712.0 + 42.0) for k, v in lls.items()}
+    snd = Sounder.make(700.0, 712.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    pp = syn.perturbed_profiles(72, seed=31, sigma_T=4.0, sigma_h2o=0.4)
+    train, hold = pp[:64], pp[64:]
+    emu, rep = ChannelODEmulator.train(train, sub, snd, n_pc=64, steps=1500, batch=1024, lr=1e-3, ctx=ctx)
+    return sub, snd, emu, rep, hold
+
+
+def test_emulator_trains_and_beats_mean_predictor(ctx, small_emu):
+    """Hold-out BT error of the trained emulator vs GPU LBL is smaller than
+    that of the same emulator with the network switched off (layer-mean
+    transmittances): what the trained network gains."""
+    sub, snd, emu, rep, hold = small_emu
+    assert rep.pca_rms_t < 1e-3 and np.all(np.isfinite(rep.losses))
+    assert rep.losses[-50:].mean() < 0.2 * rep.losses[:10].mean()
+    profs = [p[0] for p in hold]
+    geoms = [ViewingGeometry(float(np.degrees(np.arccos(p[2]))), 0.0, p[2]) for p in hold]
+    tsf = [p[1] for p in hold]
+    _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    _, bt_emu = emu.forward(profs, geom=geoms, T_sfc=tsf)
+    err_trained = np.abs(bt_emu - bt_lbl).max()
+    saved = emu.mlp.get_params()
+    emu.mlp.set_params(np.zeros_like(saved))              # network off: t = training mean
+    _, bt_mean = emu.forward(profs, geom=geoms, T_sfc=tsf)
+    emu.mlp.set_params(saved)
+    err_mean = np.abs(bt_mean - bt_lbl).max()
+    print(f"hold-out max |dBT|: trained {err_trained:.3f} K, mean predictor {err_mean:.3f} K")
+    assert err_trained < 0.5 * err_mean
+    assert err_trained < 2.0
+
+
+def test_emulator_spot_check_and_fallback(ctx, small_emu):
+    sub, snd, emu, rep, hold = small_emu
+    profs = [p[0] for p in hold]
+    geoms = [ViewingGeometry(float(np.degrees(np.arccos(p[2]))), 0.0, p[2]) for p in hold]
+    tsf = [p[1] for p in hold]
+    spot = SpotChecker(sub, snd, blocks=((701.0, 8), (709.0, 8)))
+    _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    # spot channels computed on sub-grids equal the full LBL at those channels
+    bt_spot = spot(profs, geoms, tsf, [1.0] * len(profs), ctx, line_mixing="vpy")
+    assert np.max(np.abs(bt_spot - bt_lbl[:, spot.channels])) < 1e-4
+    r_all = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=0.0, spot=spot, ctx=ctx)
+    assert r_all.fell_back.all() and np.array_equal(r_all.BT, bt_lbl)
+    r_none = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=1e9, spot=spot, ctx=ctx)
+    assert not r_none.fell_back.any()
+    _, bt_emu = emu.forward(profs, geom=geoms, T_sfc=tsf)
+    assert np.array_equal(r_none.BT, bt_emu)
+    thr = float(np.median(r_none.spot_err))
+    r = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=thr, spot=spot, ctx=ctx)
+    assert np.array_equal(r.fell_back, r_none.spot_err > thr)
+    assert np.array_equal(r.BT[r.fell_back], bt_lbl[r.fell_back])
+    assert np.all(r.spot_err[~r.fell_back] <= thr)

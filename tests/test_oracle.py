@@ -336,3 +336,14 @@ def test_forward_small_semitransparent():
     R0, BT0, _ = o.forward(prof, {GasSpecies.CO2: sub}, snd, internal_dnu=0.002, dptmn=0.0)
     assert np.all(np.isfinite(BT)) and BT.min() > 200 and BT.max() < 290
     assert np.max(np.abs(BT - BT0)) < 6e-4
+
+
+def test_channel_od_helper_cim_closed_forms():
+    """The FP64 channel-OD restatement's f_cim (tests/helpers.py) at the
+    reference's closed-form values (runtests.jl:387-466)."""
+    from helpers import _fcim
+    assert abs(float(_fcim(0.03)) - 0.005) < 1e-15
+    assert abs(float(_fcim(1.0)) - 0.16395341373865302) < 1e-12
+    s = np.array([0.0599999, 0.0600001])
+    assert abs(_fcim(s)[1] - _fcim(s)[0]) < 1e-5           # branch continuity
+    assert abs(float(_fcim(200.0)) - (1.0 - 2.0 / 200.0)) < 1e-12
