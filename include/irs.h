@@ -281,7 +281,7 @@ int         irs_mlp_input_jacobian(irs_mlp *mlp, const float *x, int64_t n, floa
 /* Emulated forward model: two targets per layer and channel decoded as
  * x_k(c) = t_mean[k][c] + sum_m (y_m * coef_scale[m]) basis[m][c] (first target
  * on columns 0..n_ch-1, second on n_ch..2 n_ch-1) with
- * y = MLP((features - x_mean) / x_std):
+ * y = MLP((features - x_mean) / x_std), x_mean and x_std [n_layers][n_in] (per layer):
  *   kind = 2: the layer channel transmittance t_k = exp(-tau_ch_k) and the CIM
  *             weight g_k of irs_channel_od (T_k = t_k T_{k+1}, G_k = g_k T_{k+1});
  *   kind = 3: the level transmittance T_k and the CIM sum G_k of irs_channel_transmittance;
@@ -295,11 +295,20 @@ int         irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels,
                            const float *basis, const float *t_mean, const double *ch_nu, irs_emu **out);
 void        irs_emu_destroy(irs_emu *emu);
 const char *irs_emu_last_error(const irs_emu *emu);
-/* features[n_prof][L][n_in], T_lev[n_prof][L+1], T_ave[n_prof][L], scal[n_prof][3] =
+/* features[n_prof][L][n_in] (raw), T_lev[n_prof][L+1], T_ave[n_prof][L], scal[n_prof][3] =
  * {T_sfc, mu, eps}; R, BT [n_prof][n_channels]; t_out[n_prof][L][n_channels] (decoded T_k) optional */
 int         irs_emu_forward(irs_emu *emu, const float *features, const double *T_lev, const double *T_ave,
                             const double *scal, int n_prof, int source_function, double *R, double *BT,
                             float *t_out);
+
+/* Emulator Jacobians (chain rule through the MLP input Jacobian, the decode and
+ * the adjoint of the channel RTE; kind-2 emulators, eps = 1 exact):
+ * BT [n_prof][n_ch]; dfeat [n_prof][L][n_in][n_ch] = dBT/d(raw feature);
+ * dTave [n_prof][L][n_ch], dTlev [n_prof][L+1][n_ch], dTsfc [n_prof][n_ch]: the
+ * Planck terms of dBT/dT. */
+int         irs_emu_jacobian(irs_emu *emu, const float *features, const double *T_lev,
+                             const double *T_ave, const double *scal, int n_prof, double *BT,
+                             float *dfeat, float *dTave, float *dTlev, float *dTsfc);
 
 /* forward_model over a batch of profiles: R, BT are [n_prof][n_channels]. */
 int irs_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,

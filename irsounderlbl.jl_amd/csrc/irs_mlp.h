@@ -52,3 +52,16 @@ struct EmuRteArgs {
 };
 
 hipError_t launch_emu_rte(const EmuRteArgs &a, hipStream_t s);
+
+// Emulator Jacobian outputs (all multiplied by dBT/dR)
+struct EmuAdjArgs {
+    double *BT;                  // [n_prof][n_ch]
+    float *dt, *dg;              // dBT/dt_k, dBT/dg_k          [n_prof][L][n_ch]
+    float *dTave;                // dBT/dT_ave,k (Planck)       [n_prof][L][n_ch]
+    float *dTlev;                // dBT/dT_lev,j (Planck)       [n_prof][L+1][n_ch]
+    float *dTsfc;                // dBT/dT_sfc                  [n_prof][n_ch]
+    const float *Jmlp;           // dy/dx of the MLP            [n_prof][L][n_pc][n_in]
+    float *dfeat;                // dBT/dfeature                [n_prof][L][n_in][n_ch]
+};
+
+hipError_t launch_emu_adjoint(const EmuRteArgs &a, const EmuAdjArgs &j, int n_in, hipStream_t s);

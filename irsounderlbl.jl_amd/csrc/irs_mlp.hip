@@ -439,8 +439,8 @@ int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_t
     e->n_in = mlp->n_in;
     e->n_tpc = n_tpc;
     const size_t W = (size_t)2 * n_channels;
-    e->x_mean.assign(x_mean, x_mean + mlp->n_in);
-    e->x_std.assign(x_std, x_std + mlp->n_in);
+    e->x_mean.assign(x_mean, x_mean + (size_t)n_layers * mlp->n_in);
+    e->x_std.assign(x_std, x_std + (size_t)n_layers * mlp->n_in);
     bool ok = dalloc(&e->coef_scale, n_pc) == hipSuccess && dalloc(&e->basis, (size_t)n_pc * W) == hipSuccess &&
               dalloc(&e->t_mean, (size_t)n_layers * W) == hipSuccess &&
               dalloc(&e->ch_nu, n_channels) == hipSuccess;
@@ -501,7 +501,8 @@ int emu_coefficients(irs_emu *e, const float *features, int64_t rows) {
     std::vector<float> xs((size_t)rows * e->n_in);
     for (int64_t r = 0; r < rows; ++r)
         for (int i = 0; i < e->n_in; ++i)
-            xs[r * e->n_in + i] = (features[r * e->n_in + i] - e->x_mean[i]) / e->x_std[i];
+            xs[r * e->n_in + i] = (features[r * e->n_in + i] - e->x_mean[(r % e->L) * e->n_in + i]) /
+                                  e->x_std[(r % e->L) * e->n_in + i];
     for (int64_t r0 = 0; r0 < rows; r0 += m->max_batch) {
         const int64_t nr = std::min<int64_t>(m->max_batch, rows - r0);
         const int B = (int)((nr + 63) / 64 * 64);
@@ -584,6 +585,116 @@ int irs_emu_forward(irs_emu *e, const float *features, const double *T_lev, cons
         ECHK(hipMemcpyAsync(t_out, tdev, sizeof(float) * n_prof * L * e->n_ch, hipMemcpyDeviceToHost, m->stream));
     ECHK(hipStreamSynchronize(m->stream));
     if (tdev) (void)hipFree(tdev);
+    return IRS_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Emulator Jacobians for n_prof scenes: BT [n_prof][n_ch]; dBT/dfeature
+// [n_prof][L][n_in][n_ch] (raw features, through the MLP input Jacobian and the
+// decode); the Planck terms dBT/dT_ave [n_prof][L][n_ch], dBT/dT_lev
+// [n_prof][L+1][n_ch], dBT/dT_sfc [n_prof][n_ch].
+int irs_emu_jacobian(irs_emu *e, const float *features, const double *T_lev, const double *T_ave,
+                     const double *scal, int n_prof, double *BT, float *dfeat, float *dTave, float *dTlev,
+                     float *dTsfc) {
+    if (!e || !features || !T_lev || !T_ave || !scal || n_prof < 1 || !BT || !dfeat || !dTave || !dTlev || !dTsfc)
+        return IRS_EINVAL;
+    if (e->n_tpc != 2) {
+        e->err = "emulator Jacobians are implemented for the (t_k, g_k) targets";
+        return IRS_EINVAL;
+    }
+    irs_mlp *m = e->mlp;
+    (void)hipSetDevice(m->device);
+    const int L = e->L, nin = e->n_in, npc = e->n_pc, nch = e->n_ch;
+    const int64_t rows = (int64_t)n_prof * L;
+    int rc = emu_coefficients(e, features, rows);
+    if (rc) return rc;
+    // MLP input Jacobian on standardized inputs, then per raw feature (1/std)
+    std::vector<float> xs((size_t)rows * nin);
+    for (int64_t r = 0; r < rows; ++r)
+        for (int i = 0; i < nin; ++i)
+            xs[r * nin + i] = (features[r * nin + i] - e->x_mean[(r % L) * nin + i]) / e->x_std[(r % L) * nin + i];
+    std::vector<float> J((size_t)rows * npc * nin);
+    rc = irs_mlp_input_jacobian(m, xs.data(), rows, J.data());
+    if (rc) {
+        e->err = m->err;
+        return rc;
+    }
+    for (int64_t r = 0; r < rows; ++r)
+        for (int o = 0; o < npc; ++o)
+            for (int i = 0; i < nin; ++i) J[((size_t)r * npc + o) * nin + i] /= e->x_std[(r % L) * nin + i];
+    // device buffers
+    std::vector<void *> bufs;
+    auto get = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return p;
+    };
+    double *dprof = (double *)get(sizeof(double) * n_prof * (2 * L + 1 + 3));
+    float *dJ = (float *)get(sizeof(float) * J.size());
+    EmuAdjArgs j{};
+    j.BT = (double *)get(sizeof(double) * n_prof * nch);
+    j.dt = (float *)get(sizeof(float) * rows * nch);
+    j.dg = (float *)get(sizeof(float) * rows * nch);
+    j.dTave = (float *)get(sizeof(float) * rows * nch);
+    j.dTlev = (float *)get(sizeof(float) * (rows + n_prof) * nch);
+    j.dTsfc = (float *)get(sizeof(float) * n_prof * nch);
+    j.dfeat = (float *)get(sizeof(float) * rows * nin * nch);
+    j.Jmlp = dJ;
+    bool ok = true;
+    for (void *p : bufs) ok = ok && p;
+    hipError_t er = hipSuccess;
+    if (ok) {
+        er = hipMemcpyAsync(dprof, T_lev, sizeof(double) * n_prof * (L + 1), hipMemcpyHostToDevice, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dprof + (size_t)n_prof * (L + 1), T_ave, sizeof(double) * n_prof * L,
+                                hipMemcpyHostToDevice, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dprof + (size_t)n_prof * (2 * L + 1), scal, sizeof(double) * n_prof * 3,
+                                hipMemcpyHostToDevice, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dJ, J.data(), sizeof(float) * J.size(), hipMemcpyHostToDevice, m->stream);
+        EmuRteArgs a{};
+        a.coef = e->coef;
+        a.coef_scale = e->coef_scale;
+        a.basis = e->basis;
+        a.t_mean = e->t_mean;
+        a.ch_nu = e->ch_nu;
+        a.T_lev = dprof;
+        a.T_ave = dprof + (size_t)n_prof * (L + 1);
+        a.scal = dprof + (size_t)n_prof * (2 * L + 1);
+        a.n_prof = n_prof;
+        a.L = L;
+        a.n_pc = npc;
+        a.n_ch = nch;
+        a.source_function = IRS_SRC_CIM;
+        if (er == hipSuccess) er = launch_emu_adjoint(a, j, nin, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(BT, j.BT, sizeof(double) * n_prof * nch, hipMemcpyDeviceToHost, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dfeat, j.dfeat, sizeof(float) * rows * nin * nch, hipMemcpyDeviceToHost, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dTave, j.dTave, sizeof(float) * rows * nch, hipMemcpyDeviceToHost, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dTlev, j.dTlev, sizeof(float) * (rows + n_prof) * nch, hipMemcpyDeviceToHost,
+                                m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(dTsfc, j.dTsfc, sizeof(float) * n_prof * nch, hipMemcpyDeviceToHost, m->stream);
+        if (er == hipSuccess) er = hipStreamSynchronize(m->stream);
+    }
+    (void)hipStreamSynchronize(m->stream);
+    for (void *p : bufs) (void)hipFree(p);
+    if (!ok) {
+        e->err = "device allocation failed";
+        return IRS_ENOMEM;
+    }
+    if (er != hipSuccess) {
+        e->err = std::string("emulator Jacobian: ") + hipGetErrorString(er);
+        return IRS_EHIP;
+    }
     return IRS_OK;
 }
 

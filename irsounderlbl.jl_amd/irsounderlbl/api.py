@@ -93,7 +93,7 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_mlp_n_params", "irs_mlp_input_pad", "irs_mlp_get_params", "irs_mlp_set_params",
            "irs_mlp_forward", "irs_mlp_loss_grad", "irs_mlp_set_data", "irs_mlp_train",
            "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
-           "irs_emu_forward", "irs_channel_transmittance")
+           "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian")
 
 _lib = None
 
@@ -160,6 +160,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_emu_last_error.argtypes = [P]
     L.irs_emu_last_error.restype = C.c_char_p
     L.irs_emu_forward.argtypes = [P, P, P, P, P, i32, i32, P, P, P]
+    L.irs_emu_jacobian.argtypes = [P, P, P, P, P, i32, P, P, P, P, P]
     for name in EXPORTS:
         if name not in ("irs_destroy", "irs_default_options", "irs_last_error", "irs_mlp_destroy",
                         "irs_mlp_last_error", "irs_mlp_n_params", "irs_emu_destroy", "irs_emu_last_error"):

@@ -244,13 +244,19 @@ class ChannelODEmulator:
         err = dev - rec
         pca_rms, pca_max = float(np.sqrt(np.mean(err * err))), float(np.abs(err).max())
         t2 = time.perf_counter()
-        feats = np.concatenate([layer_features(p, m) for p, m in zip(profs, mus)]).astype(np.float64)
+        feats = np.stack([layer_features(p, m) for p, m in zip(profs, mus)]).astype(np.float64)   # [N, L, F]
+        # per-layer standardisation keeps the bf16 inputs fine-grained (feature 0,
+        # ln p, is the same grid for every scene: global statistics keep the layer identity)
         x_mean = feats.mean(axis=0)
-        x_std = np.maximum(feats.std(axis=0), 1e-6)
+        x_std = feats.std(axis=0)
+        g_mean, g_std = feats.reshape(-1, feats.shape[2]).mean(axis=0), feats.reshape(-1, feats.shape[2]).std(axis=0)
+        x_mean[:, 0], x_std[:, 0] = g_mean[0], max(g_std[0], 1e-6)
+        x_std = np.where(x_std > 1e-3 * np.maximum(g_std, 1e-12)[None], x_std, np.maximum(g_std, 1e-6)[None])
+        feats_n = ((feats - x_mean[None]) / x_std[None]).reshape(N * L, -1)
         scale = float(max(coef[:, 0].std(), 1e-12))
         coef_scale = np.full(n_pc, scale, np.float32)
         mlp = MLP(N_FEATURES, 512, n_pc, max(batch, 64), seed=seed)
-        mlp.set_data(((feats - x_mean) / x_std).astype(np.float32), (coef / scale).astype(np.float32))
+        mlp.set_data(feats_n.astype(np.float32), (coef / scale).astype(np.float32))
         # Adam at lr for half the steps, then lr/3 and lr/10 for a quarter each
         n1, n2 = steps // 2, steps // 4
         losses = np.concatenate([mlp.train(n1, batch, lr, seed), mlp.train(n2, batch, lr / 3.0, seed + 1),
@@ -276,6 +282,17 @@ class ChannelODEmulator:
         mlp.set_params(z["params"])
         return ChannelODEmulator(mlp, z["x_mean"], z["x_std"], z["coef_scale"], z["basis"], z["t_mean"],
                                  z["ch_nu"], n_tpc=n_tpc)
+
+    # ---- Jacobians ----------------------------------------------------------
+    def jacobian(self, profs, *, geom=None, T_sfc=None, eps_sfc=1.0) -> List["EmulatorJacobian"]:
+        """Emulator Jacobians per scene (levels: T and ln q_H2O; T_sfc); the
+        reflected term (eps < 1) is held fixed."""
+        nb = len(profs)
+        geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
+        geoms = [g if g is not None else ViewingGeometry(0.0, 0.0, 1.0) for g in geoms]
+        tsf = T_sfc if isinstance(T_sfc, (list, tuple, np.ndarray)) else [T_sfc] * nb
+        eps = eps_sfc if isinstance(eps_sfc, (list, tuple, np.ndarray)) else [eps_sfc] * nb
+        return _emu_jacobian(self, list(profs), geoms, list(tsf), list(eps))
 
     # ---- inference ----------------------------------------------------------
     def _inputs(self, profs, T_sfcs, eps, mus):
@@ -311,6 +328,77 @@ class ChannelODEmulator:
         if rc != IRS_OK:
             raise IRSError(rc, self.lib.irs_emu_last_error(self.h).decode(errors="replace"))
         return (R, BT, tout) if want_t else (R, BT)
+
+
+def _feature_level_jacobian(prof, mu, T_method, hT=1e-2, hq=1e-3):
+    """d features / d (level T) and d features / d (level ln q_H2O), and
+    d T_ave / d (level T), by central differences of the host layer code
+    (smooth, O(levels x layers) work)."""
+    from .types import AtmosphericProfile
+    nl = prof.n_levels
+    def feats(T=None, vmr=None):
+        p = AtmosphericProfile(prof.pressure, prof.temperature if T is None else T, prof.altitude,
+                               prof.vmr if vmr is None else vmr)
+        return layer_features(p, mu, T_method), layer_properties(p, T_method)["T_ave"]
+    X0, _ = feats()
+    L, nf = X0.shape
+    dXdT = np.zeros((nl, L, nf))
+    dAdT = np.zeros((nl, L))
+    dXdq = np.zeros((nl, L, nf))
+    for j in range(nl):
+        Tp, Tm = prof.temperature.copy(), prof.temperature.copy()
+        Tp[j] += hT
+        Tm[j] -= hT
+        (xp, ap), (xm, am) = feats(T=Tp), feats(T=Tm)
+        dXdT[j] = (xp - xm) / (2 * hT)
+        dAdT[j] = (ap - am) / (2 * hT)
+        if GasSpecies.H2O in prof.vmr:
+            vp = dict(prof.vmr)
+            vm = dict(prof.vmr)
+            q = prof.vmr[GasSpecies.H2O]
+            qp, qm = q.copy(), q.copy()
+            qp[j] *= np.exp(hq)
+            qm[j] *= np.exp(-hq)
+            vp[GasSpecies.H2O], vm[GasSpecies.H2O] = qp, qm
+            dXdq[j] = (feats(vmr=vp)[0] - feats(vmr=vm)[0]) / (2 * hq)
+    return dXdT, dAdT, dXdq
+
+
+@dataclass
+class EmulatorJacobian:
+    """Per scene: BT [n_ch], K_T [n_ch, n_levels] (dBT/dT_level), K_lnq_H2O
+    [n_ch, n_levels] (dBT/d ln q_H2O level), K_Tsfc [n_ch], and the layer
+    form dBT/dfeature [L, n_features, n_ch]."""
+    BT: np.ndarray
+    K_T: np.ndarray
+    K_lnq_H2O: np.ndarray
+    K_Tsfc: np.ndarray
+    dfeat: np.ndarray = field(repr=False, default=None)
+
+
+def _emu_jacobian(emu, profs, geoms, tsf, eps):
+    nb = len(profs)
+    mus = [g.mu for g in geoms]
+    X, tlev, tave, scal = emu._inputs(profs, tsf, eps, mus)
+    L, nin, nch = emu.L, emu.mlp.n_in, emu.n_ch
+    BT = np.zeros((nb, nch))
+    dfeat = np.zeros((nb, L, nin, nch), np.float32)
+    dTave = np.zeros((nb, L, nch), np.float32)
+    dTlev = np.zeros((nb, L + 1, nch), np.float32)
+    dTsfc = np.zeros((nb, nch), np.float32)
+    rc = emu.lib.irs_emu_jacobian(emu.h, _p(X), _p(tlev), _p(tave), _p(scal), nb, _p(BT), _p(dfeat), _p(dTave),
+                                  _p(dTlev), _p(dTsfc))
+    if rc != IRS_OK:
+        raise IRSError(rc, emu.lib.irs_emu_last_error(emu.h).decode(errors="replace"))
+    out = []
+    for q in range(nb):
+        dXdT, dAdT, dXdq = _feature_level_jacobian(profs[q], mus[q], emu.T_method)
+        F = dfeat[q].astype(np.float64)
+        KT = np.einsum("kic,jki->cj", F, dXdT) + np.einsum("kc,jk->cj", dTave[q].astype(np.float64), dAdT)
+        KT += dTlev[q].T.astype(np.float64)
+        Kq = np.einsum("kic,jki->cj", F, dXdq)
+        out.append(EmulatorJacobian(BT[q], KT, Kq, dTsfc[q].astype(np.float64), dfeat[q]))
+    return out
 
 
 # ---- spot check + fallback -----------------------------------------------------

@@ -195,3 +195,50 @@ def test_emulator_spot_check_and_fallback(ctx, small_emu):
     assert np.array_equal(r.fell_back, r_none.spot_err > thr)
     assert np.array_equal(r.BT[r.fell_back], bt_lbl[r.fell_back])
     assert np.all(r.spot_err[~r.fell_back] <= thr)
+
+
+def test_emulator_jacobian_matches_finite_differences(ctx, small_emu):
+    """Chain-rule emulator Jacobians (MLP input Jacobian x decode x RTE
+    adjoint x level->feature map) vs central differences of the emulator's
+    own forward model, for level T, level ln q_H2O and T_sfc; then compared
+    with the FP64 oracle's analytic Jacobian of the LBL model (reported)."""
+    from irsounderlbl.types import AtmosphericProfile
+    sub, snd, emu, rep, hold = small_emu
+    prof, T_sfc, mu = hold[0]
+    geom = ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu)
+    jac = emu.jacobian([prof], geom=[geom], T_sfc=[T_sfc])[0]
+    assert jac.K_T.shape == (49, 50) and jac.K_lnq_H2O.shape == (49, 50)
+    def bt(p, ts=T_sfc):
+        return emu.forward([p], geom=[geom], T_sfc=[ts])[1][0]
+    scaleT = np.abs(jac.K_T).max()
+    for j in (2, 10, 20, 30):
+        h = 1.0
+        Tp, Tm = prof.temperature.copy(), prof.temperature.copy()
+        Tp[j] += h
+        Tm[j] -= h
+        fd = (bt(AtmosphericProfile(prof.pressure, Tp, prof.altitude, prof.vmr)) -
+              bt(AtmosphericProfile(prof.pressure, Tm, prof.altitude, prof.vmr))) / (2 * h)
+        assert np.max(np.abs(fd - jac.K_T[:, j])) < 5e-2 * scaleT, j
+    scaleq = np.abs(jac.K_lnq_H2O).max()
+    for j in (0, 5, 12):
+        h = 0.1
+        vp, vm = dict(prof.vmr), dict(prof.vmr)
+        q = prof.vmr[GasSpecies.H2O]
+        vp[GasSpecies.H2O] = q * np.where(np.arange(len(q)) == j, np.exp(h), 1.0)
+        vm[GasSpecies.H2O] = q * np.where(np.arange(len(q)) == j, np.exp(-h), 1.0)
+        fd = (bt(AtmosphericProfile(prof.pressure, prof.temperature, prof.altitude, vp)) -
+              bt(AtmosphericProfile(prof.pressure, prof.temperature, prof.altitude, vm))) / (2 * h)
+        assert np.max(np.abs(fd - jac.K_lnq_H2O[:, j])) < 5e-2 * max(scaleq, 1e-6), j
+    fd = (bt(prof, T_sfc + 0.5) - bt(prof, T_sfc - 0.5)) / 1.0
+    assert np.max(np.abs(fd - jac.K_Tsfc)) < 1e-2 * max(np.abs(fd).max(), 1e-9)
+    # vs the line-by-line analytic Jacobian (irs_jacobian, itself held to the
+    # FP64 oracle in test_gpu_jacobian.py), temperature block, reported
+    from irsounderlbl.api import analytic_jacobian
+    from irsounderlbl.types import StateVectorSpec
+    ref = analytic_jacobian(prof, sub, StateVectorSpec(50, [GasSpecies.H2O], include_emissivity=False),
+                            sounder=snd, geom=geom, T_sfc=T_sfc, line_mixing="vpy", vmr_coupling=False, ctx=ctx)
+    KT_ref = ref.K[:, :50]
+    rel = np.linalg.norm(jac.K_T - KT_ref) / np.linalg.norm(KT_ref)
+    relq = np.linalg.norm(jac.K_lnq_H2O - ref.K[:, 50:100]) / np.linalg.norm(ref.K[:, 50:100])
+    print(f"emulator vs LBL analytic Jacobian: relative Frobenius error T {rel:.3f}, ln q_H2O {relq:.3f}")
+    assert rel < 0.5
