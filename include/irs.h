@@ -143,6 +143,10 @@ typedef struct {
     int32_t persistent;
     int32_t autotuned;          /* 1: this call ran the autotuner */
     int32_t reserved1;
+    int64_t fine_pairs;         /* line-sum work executed: pairs on the internal grid */
+    int64_t coarse_pairs;       /* pairs on the coarse far-wing nodes */
+    int64_t fine_pairs_lm;      /* of which on line-mixing (VP_Y) items */
+    int64_t coarse_pairs_lm;
 } irs_stats;
 
 /* Line-sum schedule of irs_forward.  No counterpart in the reference: it
@@ -154,6 +158,9 @@ typedef struct {
                                    and replay it on later calls of the same shape */
     int autotune;               /* 1: time the (tile_points, persistent) candidates on the first
                                    profile of each new shape and keep the fastest */
+    double far_wing_cm;         /* > 0: far-wing sums of lines at least this far (cm^-1) from a
+                                   tile are evaluated on a coarse grid (<= 0.032 cm^-1) and
+                                   interpolated (cubic); 0: every pair on the internal grid */
 } irs_schedule;
 
 int         irs_version(void);

@@ -64,9 +64,9 @@ struct PinBuf {
 // cached by the byte image of this struct (zero-filled before use).
 struct GroupPlan {
     const void *items, *rec0, *rec1, *rec2, *keys, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
-    const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out;
-    int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes;
-    int gn, L, ns, P, pblocks, tiles, source_function, with_ils;
+    const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out, *pre;
+    int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes, n_rec;
+    int gn, L, ns, P, pblocks, tiles, source_function, with_ils, far_d, far_hc;
     double nu_lo, dnu, an, sn, den;
     int mode;                      // 0 radiance + BT, 1 channel optical depths (2: + CIM weights),
                                    // 3 level channel transmittances + CIM sums
@@ -100,11 +100,11 @@ struct irs_ctx {
     DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
     irs_stats stats{};
-    irs_schedule sched{4, 0, 0, 0};
+    irs_schedule sched{4, 0, 0, 0, 1.5};
     std::map<std::string, std::pair<int, int>> tuned;     // shape -> (P, persistent)
     std::map<std::string, DevBuf> orders;                 // shape -> heaviest-first tile order
     std::map<int, int> capacity;                          // P -> resident persistent workgroups
-    DevBuf d_queue, d_prof, d_chod, d_chlev, d_chg;
+    DevBuf d_queue, d_prof, d_chod, d_chlev, d_chg, d_pre;
     PinBuf h_items, h_prof, h_out;
     std::vector<GraphEntry> graphs;
 };
@@ -435,6 +435,19 @@ int tile_order(irs_ctx *ctx, const std::vector<const ItemDev *> &per_species, in
     return IRS_OK;
 }
 
+// Coarse far-wing parameters for a grid step and tile: node spacing HC (a power
+// of two dividing the tile, HC dnu <= 0.032 cm^-1) and distance D (grid steps).
+void far_params(const irs_ctx *ctx, double dnu, int P, int &far_d, int &far_hc) {
+    far_d = far_hc = 0;
+    const double fw = ctx->sched.far_wing_cm;
+    if (!(fw > 0.0)) return;
+    int hc = 1;
+    while (hc * 2 <= IRS_LS_THREADS * P / 8 && (hc * 2) * dnu <= 0.032 * (1.0 + 1e-9)) hc *= 2;
+    if (hc < 2) return;
+    far_hc = hc;
+    far_d = (int)std::ceil(fw / dnu);
+}
+
 int persistent_blocks(irs_ctx *ctx, int P) {
     auto f = ctx->capacity.find(P);
     if (f != ctx->capacity.end()) return f->second;
@@ -460,9 +473,10 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     HIPCHK(ensure(ctx->d_items, sizeof(ItemDev) * std::max<size_t>(items.size(), 1)));
     HIPCHK(ensure(ctx->d_counters, 64));
     HIPCHK(ensure(ctx->d_queue, 64));
+    HIPCHK(ensure(ctx->d_pre, sizeof(double) * 3 * std::max<int64_t>(rec, 1)));
     HIPCHK(hipMemcpyAsync(ctx->d_items.p, items.data(), sizeof(ItemDev) * items.size(), hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(launch_reset_counters((unsigned *)ctx->d_queue.p, (unsigned long long *)ctx->d_counters.p, 3,
+    HIPCHK(launch_reset_counters((unsigned *)ctx->d_queue.p, (unsigned long long *)ctx->d_counters.p, 8,
                                  ctx->stream));
     PrepArgs pa{};
     pa.items = (const ItemDev *)ctx->d_items.p;
@@ -476,6 +490,8 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     pa.rec2 = (float4 *)ctx->d_rec2.p;
     pa.counters = (unsigned long long *)ctx->d_counters.p;
     pa.n_hi = n_hi;
+    pa.pre = (double *)ctx->d_pre.p;
+    pa.n_rec = std::max<int64_t>(rec, 1);
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_prep(pa, max_lines, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -496,6 +512,10 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     la.tiles = (int)((n_hi + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
     la.tile_order = nullptr;
     la.queue = (unsigned *)ctx->d_queue.p;
+    far_params(ctx, dnu, P, la.far_d, la.far_hc);
+    la.work = (unsigned long long *)ctx->d_counters.p + 4;
+    la.pre = pa.pre;
+    la.n_rec = pa.n_rec;
     HIPCHK(launch_linesum(la, P, ctx->sched.persistent ? persistent_blocks(ctx, P) : 0, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     ctx->stats.linesum_launches += 1;
@@ -505,11 +525,15 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
 }
 
 void read_counters(irs_ctx *ctx) {
-    unsigned long long c[3] = {0, 0, 0};
+    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (hipMemcpy(c, ctx->d_counters.p, sizeof(c), hipMemcpyDeviceToHost) == hipSuccess) {
         ctx->stats.active_lines += (int64_t)c[0];
         ctx->stats.active_pairs += (int64_t)c[1];
         ctx->stats.dispersive_pairs += (int64_t)c[2];
+        ctx->stats.fine_pairs += (int64_t)c[4];
+        ctx->stats.coarse_pairs += (int64_t)c[5];
+        ctx->stats.fine_pairs_lm += (int64_t)c[6];
+        ctx->stats.coarse_pairs_lm += (int64_t)c[7];
     }
 }
 
@@ -708,6 +732,7 @@ void irs_destroy(irs_ctx *ctx) {
     ctx->d_queue.release();
     ctx->d_prof.release();
     ctx->d_chod.release();
+    ctx->d_pre.release();
     ctx->d_chlev.release();
     ctx->d_chg.release();
     ctx->h_items.release();
@@ -1007,7 +1032,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     } while (0)
     ENQ(hipMemcpyAsync((void *)pl.items, pl.h_items, (size_t)pl.items_bytes, hipMemcpyHostToDevice, st));
     ENQ(hipMemcpyAsync((void *)pl.prof, pl.h_prof, (size_t)pl.prof_bytes, hipMemcpyHostToDevice, st));
-    ENQ(launch_reset_counters((unsigned *)pl.queue, (unsigned long long *)pl.counters, 3, st));
+    ENQ(launch_reset_counters((unsigned *)pl.queue, (unsigned long long *)pl.counters, 8, st));
     PrepArgs pa{};
     pa.items = (const ItemDev *)pl.items;
     pa.lists = (const LineListDev *)pl.lists;
@@ -1020,6 +1045,8 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     pa.rec2 = (float4 *)pl.rec2;
     pa.counters = (unsigned long long *)pl.counters;
     pa.n_hi = pl.n;
+    pa.pre = (double *)pl.pre;
+    pa.n_rec = pl.n_rec;
     if (timing) ENQ(hipEventRecord(ctx->ev[0], st));
     ENQ(launch_prep(pa, (int)pl.max_lines, st));
     if (timing) ENQ(hipEventRecord(ctx->ev[1], st));
@@ -1039,6 +1066,11 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.n_prof = pl.gn;
     la.tile_order = (const int32_t *)pl.order;
     la.queue = (unsigned *)pl.queue;
+    la.far_d = pl.far_d;
+    la.far_hc = pl.far_hc;
+    la.work = (unsigned long long *)pl.counters + 4;
+    la.pre = pa.pre;
+    la.n_rec = pa.n_rec;
     ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
     if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
     if (pl.mode >= 1) {
@@ -1170,6 +1202,8 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
     pa.rec2 = (float4 *)base.rec2;
     pa.counters = (unsigned long long *)base.counters;
     pa.n_hi = base.n;
+    pa.pre = (double *)base.pre;
+    pa.n_rec = base.n_rec;
     HIPCHK(launch_prep(pa, (int)base.max_lines, st));
     const int cand_P[3] = {4, 8, 16};
     double best = 1e300;
@@ -1196,6 +1230,9 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
             la.n_prof = 1;
             la.tile_order = ord;
             la.queue = (unsigned *)base.queue;
+            far_params(ctx, base.dnu, P, la.far_d, la.far_hc);
+            la.pre = pa.pre;
+            la.n_rec = pa.n_rec;
             const int pb = pers ? persistent_blocks(ctx, P) : 0;
             if (pers && pb <= 0) continue;
             double t_min = 1e300;
@@ -1305,7 +1342,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
             lines_per_prof += ctx->lists[profiles[0].linelist[s]].n;
             max_lines = std::max<int>(max_lines, (int)ctx->lists[profiles[0].linelist[s]].n);
         }
-    const double bytes_per_prof = (double)L * (lines_per_prof * 48.0 + n * 4.0) + n * 8.0;
+    const double bytes_per_prof = (double)L * (lines_per_prof * 72.0 + n * 4.0) + n * 8.0;
     double budget = 32.0e9;
     if (const char *e = std::getenv("IRS_GROUP_BYTES")) budget = std::max(1.0, std::atof(e));
     int gmax = (int)std::max(1.0, std::min((double)n_prof, budget / std::max(bytes_per_prof, 1.0)));
@@ -1325,6 +1362,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     HIPCHK(ensure(ctx->d_rec0, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     HIPCHK(ensure(ctx->d_rec1, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
+    const int64_t n_rec_cap = std::max<int64_t>((int64_t)group * L * lines_per_prof, 1);
+    HIPCHK(ensure(ctx->d_pre, sizeof(double) * 3 * n_rec_cap));
     HIPCHK(ensure(ctx->d_tau, sizeof(float) * (size_t)n * L * group));
     HIPCHK(ensure(ctx->d_rhi, sizeof(double) * (size_t)n * group));
     HIPCHK(ensure(ctx->d_rch, sizeof(double) * nch * group));
@@ -1393,7 +1432,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                     tp.items = ctx->d_items.p; tp.rec0 = ctx->d_rec0.p; tp.rec1 = ctx->d_rec1.p;
                     tp.rec2 = ctx->d_rec2.p; tp.keys = ctx->d_keys.p; tp.tau = ctx->d_tau.p;
                     tp.counters = ctx->d_counters.p; tp.queue = ctx->d_queue.p; tp.lists = ctx->d_lists.p;
-                    tp.tips = ctx->d_tips.p; tp.h_items = ctx->h_items.p;
+                    tp.tips = ctx->d_tips.p; tp.h_items = ctx->h_items.p; tp.pre = ctx->d_pre.p;
+                    tp.n_rec = n_rec_cap;
                     tp.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)L * ns);
                     tp.n = n; tp.L = L; tp.ns = ns; tp.max_lines = max_lines; tp.nu_lo = nu_lo; tp.dnu = dnu;
                     rc = autotune_group(ctx, tp, per_species, key);
@@ -1414,6 +1454,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         pl.queue = ctx->d_queue.p; pl.lists = ctx->d_lists.p; pl.tips = ctx->d_tips.p; pl.order = order;
         pl.prof = ctx->d_prof.p; pl.rch = ctx->d_rch.p; pl.bt = ctx->d_bt.p; pl.chidx = ctx->d_chidx.p;
         pl.chw = ctx->d_chw.p; pl.chnu = ctx->d_chnu.p; pl.taps = taps; pl.h_items = ctx->h_items.p;
+        pl.pre = ctx->d_pre.p; pl.n_rec = n_rec_cap;
         pl.h_prof = ctx->h_prof.p; pl.h_out = ctx->h_out.p;
         pl.n = n; pl.nch = nch; pl.half = half; pl.n_items = (int64_t)gn * L * ns; pl.max_lines = max_lines;
         pl.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)gn * L * ns);
@@ -1422,6 +1463,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         pl.pblocks = persistent ? persistent_blocks(ctx, P) : 0;
         pl.tiles = (int)((n + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
         pl.source_function = opt.source_function; pl.with_ils = opt.with_ils;
+        far_params(ctx, dnu, P, pl.far_d, pl.far_hc);
         pl.nu_lo = nu_lo; pl.dnu = dnu; pl.an = g.an; pl.sn = g.sn; pl.den = g.den;
         pl.mode = mode;
         pl.chod = mode >= 1 ? ctx->d_chod.p : nullptr;
@@ -1523,6 +1565,8 @@ int irs_set_schedule(irs_ctx *ctx, const irs_schedule *sc) {
     ctx->sched.persistent = sc->persistent ? 1 : 0;
     ctx->sched.use_graph = sc->use_graph ? 1 : 0;
     ctx->sched.autotune = sc->autotune ? 1 : 0;
+    ctx->sched.far_wing_cm = sc->far_wing_cm > 0.0 ? sc->far_wing_cm : 0.0;
+    ctx->tuned.clear();
     return IRS_OK;
 }
 

@@ -45,6 +45,8 @@ struct PrepArgs {
     int32_t *keys;                 // per list, floor((nu - nu_lo)/dnu)
     unsigned long long *counters;  // [0] active lines, [1] voigt pairs, [2] dispersive pairs
     int64_t n_hi;
+    double *pre;                   // optional [3][n_rec]: per-item inclusive prefix sums of the
+    int64_t n_rec;                 // pedestal amplitude P, P c and P c^2 (c: centre, grid steps)
 };
 
 struct LinesumArgs {
@@ -58,6 +60,11 @@ struct LinesumArgs {
     int tiles, n_prof;
     const int32_t *tile_order;     // optional tile permutation (heavy tiles first)
     unsigned *queue;               // work counter of the persistent kernel (zero at launch)
+    int far_d, far_hc;             // coarse far-wing split: distance and node spacing in grid steps (0: off)
+    const double *pre;             // [3][n_rec] pedestal prefix sums (PrepArgs::pre)
+    int64_t n_rec;
+    unsigned long long *work;      // optional: [0] fine pairs evaluated, [1] coarse-node pairs,
+                                   // [2], [3] the same for line-mixing (VP_Y) items
 };
 
 struct RteArgs {

@@ -270,10 +270,109 @@ __device__ __forceinline__ double block_sum(double v, double *red, int t) {
     return red[0] + red[1] + red[2] + red[3];
 }
 
+// FAR: lines at least D grid steps from every coarse node of the tile's
+// stencil span [Sl, Sh] (nodes Sl + n*HC, n = 0..NC-1) and whose windows cover
+// the span.  Their far-wing sum (A/(du^2+g^2), and B du/(du^2+g^2) for VP_Y
+// lines) is a smooth function of the grid position there: it is evaluated at
+// the NC coarse nodes only and added to every point of the tile by four-point
+// (cubic) Lagrange interpolation.  For a Lorentz wing at distance >= D the
+// interpolation error is below 2.8 (HC/D)^4 of the line's contribution (7e-7 at
+// D = 1.5 cm^-1, HC = 0.032 cm^-1); the pedestal of these lines stays in the
+// per-tile moments.  The far lines stream through LDS in chunks of LS_CHUNK
+// ({centre index relative to node 0, fraction, g^2, A} and B); with S = 256 / NC
+// slices, thread t < S*NC sums the chunk's lines k = s, s+S, ... (s = t / NC) for
+// node n = t % NC (du exact to one FP32 rounding), and the S partial sums per
+// node are combined in LDS.
+#define FAR_MAXNC 132
+#define FAR_LDS (2 * IRS_LS_THREADS + 2 * FAR_MAXNC)
+
+template <int P, bool LM>
+__device__ __forceinline__ void far_coarse(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p, int jl0,
+                                           int jl1, int jr0, int jr1, int t, int64_t i0, int64_t Sl, int HC,
+                                           int NC, float *s_far, float4 *s_rec, float (&acc)[P], float (&cmp)[P],
+                                           float (&accd)[P], float (&cmpd)[P]) {
+    const int S = IRS_LS_THREADS / NC;                 // >= 1 since NC <= 132
+    const int n = t % NC, sl = t / NC;
+    const bool act = t < S * NC;
+    const int nL = jl1 - jl0, ntot = nL + (jr1 - jr0);
+    const int xrel = n * HC;                           // node offset from Sl
+    float a = 0.0f, c = 0.0f, ad = 0.0f, cd = 0.0f;
+    for (int base = 0; base < ntot; base += LS_CHUNK) {
+        const int cnt = min(LS_CHUNK, ntot - base);
+        // stage the chunk: {centre index - Sl (int bits), fraction, g^2, A} and B
+#pragma unroll
+        for (int u = 0; u < LS_PER_THREAD; ++u) {
+            const int k = t + u * IRS_LS_THREADS;
+            if (k < cnt) {
+                const int v = base + k;
+                const int j = v < nL ? jl0 + v : jr0 + (v - nL);
+                const float4 q = r0p[j];
+                s_rec[k] = make_float4(__int_as_float(__float_as_int(q.x) - (int)Sl), q.y, q.z, q.w);
+                if (LM) s_rec[LS_CHUNK + k].x = r1p[j].x;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            float part = 0.0f, partd = 0.0f;
+            int m = 0;
+            for (int k = sl; k < cnt; k += S) {
+                const float4 q = s_rec[k];
+                const float du = (float)(xrel - __float_as_int(q.x)) - q.y;
+                const float r = irs_rcpf(fmaf(du, du, q.z));
+                part = fmaf(q.w, r, part);
+                if (LM) partd = fmaf(s_rec[LS_CHUNK + k].x * du, r, partd);
+                if (++m == 32) {
+                    kahan_add(a, c, part);
+                    if (LM) kahan_add(ad, cd, partd);
+                    part = partd = 0.0f;
+                    m = 0;
+                }
+            }
+            kahan_add(a, c, part);
+            if (LM) kahan_add(ad, cd, partd);
+        }
+        __syncthreads();
+    }
+    if (act) {
+        s_far[t] = a - c;
+        s_far[IRS_LS_THREADS + t] = ad - cd;
+    }
+    __syncthreads();
+    float *F = s_far + 2 * IRS_LS_THREADS, *Fd = F + FAR_MAXNC;
+    if (t < NC) {
+        double v = 0.0, vd = 0.0;
+        for (int q = 0; q < S; ++q) {
+            v += (double)s_far[q * NC + t];
+            vd += (double)s_far[IRS_LS_THREADS + q * NC + t];
+        }
+        F[t] = (float)v;
+        Fd[t] = (float)vd;
+    }
+    __syncthreads();
+    const float ihc = 1.0f / (float)HC;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int off = (int)(i0 - Sl) + t + 256 * p;          // grid offset from node 0
+        const int n0 = off / HC;                               // node left of the point, >= 1
+        const float u = (float)(off - n0 * HC) * ihc;          // in [0, 1)
+        const float wm = -u * (u - 1.0f) * (u - 2.0f) * (1.0f / 6.0f);
+        const float w0 = (u + 1.0f) * (u - 1.0f) * (u - 2.0f) * 0.5f;
+        const float w1 = -(u + 1.0f) * u * (u - 2.0f) * 0.5f;
+        const float w2 = (u + 1.0f) * u * (u - 1.0f) * (1.0f / 6.0f);
+        const float v = wm * F[n0 - 1] + w0 * F[n0] + w1 * F[n0 + 1] + w2 * F[n0 + 2];
+        kahan_add(acc[p], cmp[p], v);
+        if (LM) {
+            const float vd = wm * Fd[n0 - 1] + w0 * Fd[n0] + w1 * Fd[n0 + 1] + w2 * Fd[n0 + 2];
+            kahan_add(accd[p], cmpd[p], vd);
+        }
+    }
+    __syncthreads();
+}
+
 template <int P, bool LM>
 __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev &it, int t, int64_t i0,
                                              int64_t i1, float (&tau)[P], int *s_rng, double *s_red,
-                                             float4 *s_rec) {
+                                             float4 *s_rec, float *s_far) {
     const int32_t *keys = a.keys + it.key_off;
     const int n = it.n;
     const double dnu = a.dnu;
@@ -291,7 +390,17 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     const int64_t vE = i0 + Cmin - M - 2;
     const int64_t vF = i1 - 1 + Cmax + M + 1;
     const bool nondeg = vA <= vB && vB <= vC && vC <= vD && vD <= vE && vE <= vF;
-    if (t < 6) {
+    // coarse far-wing split (see far_coarse); HC divides the tile width
+    const int HC = a.far_hc;
+    const int NC = HC > 0 ? (IRS_LS_THREADS * P) / HC + 3 : 0;
+    const int64_t Sl = i0 - HC, Sh = Sl + (int64_t)(NC - 1) * HC, Dd = a.far_d;
+    const int64_t vB2 = Sh - Cmin + M + 1;
+    const int64_t vG = Sl - Dd - M - 1;
+    const int64_t vH = Sh + Dd + M;
+    const int64_t vE2 = Sl + Cmin - M - 2;
+    const bool far = nondeg && HC > 0 && NC <= FAR_MAXNC && vA <= vB2 && vB2 <= vG && vG < vC && vD < vH &&
+                     vH <= vE2 && vE2 <= vF;
+    if (t < 10) {
         int r;
         switch (t) {
             case 0: r = lower_key(keys, n, vA); break;
@@ -299,13 +408,22 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
             case 2: r = lower_key(keys, n, vC); break;
             case 3: r = upper_key(keys, n, vD); break;
             case 4: r = upper_key(keys, n, vE); break;
-            default: r = upper_key(keys, n, vF); break;
+            case 5: r = upper_key(keys, n, vF); break;
+            case 6: r = far ? lower_key(keys, n, vB2) : 0; break;
+            case 7: r = far ? upper_key(keys, n, vG) : 0; break;
+            case 8: r = far ? lower_key(keys, n, vH) : 0; break;
+            default: r = far ? upper_key(keys, n, vE2) : 0; break;
         }
         s_rng[t] = r;
     }
     __syncthreads();
-    const int jA = uni(s_rng[0]), jB = uni(s_rng[1]), jC = uni(s_rng[2]);
-    const int jD = uni(s_rng[3]), jE = uni(s_rng[4]), jF = uni(s_rng[5]);
+    const int jA = uni(s_rng[0]), jC = uni(s_rng[2]);
+    const int jD = uni(s_rng[3]), jF = uni(s_rng[5]);
+    // interior ranges: [jB, jE) without the far split, [jB2, jE2) with it
+    const int jB = far ? uni(s_rng[6]) : uni(s_rng[1]);
+    const int jE = far ? uni(s_rng[9]) : uni(s_rng[4]);
+    const int jG = far ? uni(s_rng[7]) : jB;
+    const int jH = far ? uni(s_rng[8]) : jE;
     __syncthreads();
 
     const float4 *r0p = a.rec0 + it.rec_off;
@@ -325,25 +443,42 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
 
     double q0 = 0.0, q1 = 0.0, q2 = 0.0;
     if (nondeg) {
-        // pedestal moments of the interior (FAST + CORE) lines about the tile centre, FP64
-        for (int j = jB + t; j < jE; j += IRS_LS_THREADS) {
-            const float4 r0 = r0p[j];
-            const double pj = (double)r1p[j].y;
-            const double w = (double)((int64_t)__float_as_int(r0.x) - imid) + (double)r0.y;
-            q0 += pj;
-            q1 += pj * w;
-            q2 += pj * w * w;
+        // pedestal moments of the interior (FAR + FAST + CORE) lines about the
+        // tile centre, from the FP64 prefix sums of P, P c, P c^2
+        if (jE > jB) {
+            const double *P0 = a.pre + it.rec_off, *P1 = P0 + a.n_rec, *P2 = P1 + a.n_rec;
+            const double S0 = P0[jE - 1] - (jB > 0 ? P0[jB - 1] : 0.0);
+            const double S1 = P1[jE - 1] - (jB > 0 ? P1[jB - 1] : 0.0);
+            const double S2 = P2[jE - 1] - (jB > 0 ? P2[jB - 1] : 0.0);
+            const double m = (double)imid;
+            q0 = S0;
+            q1 = S1 - m * S0;
+            q2 = S2 - 2.0 * m * S1 + m * m * S0;
         }
-        q0 = block_sum(q0, s_red, t);
-        q1 = block_sum(q1, s_red + 4, t);
-        q2 = block_sum(q2, s_red + 8, t);
-        fast_range<P, LM>(r0p, r1p, jB, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
+        if (far && (jG > jB || jE > jH))
+            far_coarse<P, LM>(r0p, r1p, jB, jG, jH, jE, t, i0, Sl, HC, NC, s_far, s_rec, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jG, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         core_range<P, LM>(r0p, r1p, r2p, jC, jD, tl, xfar, acc, cmp, accd, cmpd);
-        fast_range<P, LM>(r0p, r1p, jD, jE, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jD, jH, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         edge_range<P, LM>(r0p, r1p, jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         edge_range<P, LM>(r0p, r1p, jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        if (a.work && t == 0) {
+            const unsigned long long fine = (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC) + (jB - jA) +
+                                                                 (jF - jE)) * (unsigned long long)(i1 - i0);
+            const unsigned long long coarse = far ? (unsigned long long)((jG - jB) + (jE - jH)) * NC : 0ull;
+            atomicAdd(&a.work[0], fine);
+            atomicAdd(&a.work[1], coarse);
+            if (LM) {
+                atomicAdd(&a.work[2], fine);
+                atomicAdd(&a.work[3], coarse);
+            }
+        }
     } else {
         general_range<P, LM>(r0p, r1p, r2p, jA, jF, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        if (a.work && t == 0) {
+            atomicAdd(&a.work[0], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
+            if (LM) atomicAdd(&a.work[2], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
+        }
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -363,7 +498,7 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
 // out[profile][layer][i0:i1).
 template <int P>
 __device__ __forceinline__ void linesum_unit(const LinesumArgs &a, int tile, int layer, int prof, int *s_rng,
-                                             double *s_red, float4 *s_rec) {
+                                             double *s_red, float4 *s_rec, float *s_far) {
     const int t = threadIdx.x;
     const int64_t i0 = (int64_t)tile * (IRS_LS_THREADS * P);
     const int64_t i1 = min(i0 + (int64_t)(IRS_LS_THREADS * P), a.n_hi);
@@ -374,8 +509,8 @@ __device__ __forceinline__ void linesum_unit(const LinesumArgs &a, int tile, int
     for (int s = 0; s < a.n_species; ++s) {
         const ItemDev it = items[s];
         if (!it.active || it.n == 0) continue;
-        if (it.lm) species_tile<P, true>(a, it, t, i0, i1, tau, s_rng, s_red, s_rec);
-        else species_tile<P, false>(a, it, t, i0, i1, tau, s_rng, s_red, s_rec);
+        if (it.lm) species_tile<P, true>(a, it, t, i0, i1, tau, s_rng, s_red, s_rec, s_far);
+        else species_tile<P, false>(a, it, t, i0, i1, tau, s_rng, s_red, s_rec, s_far);
     }
     float *out = a.out + ((int64_t)prof * a.n_layers + layer) * a.n_hi;
 #pragma unroll
@@ -388,11 +523,12 @@ __device__ __forceinline__ void linesum_unit(const LinesumArgs &a, int tile, int
 // One workgroup per unit: blockIdx = (tile rank, layer, profile).
 template <int P>
 __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumArgs a) {
-    __shared__ int s_rng[8];
+    __shared__ int s_rng[12];
     __shared__ double s_red[12];
     __shared__ float4 s_rec[2 * LS_CHUNK];
+    __shared__ float s_far[FAR_LDS];
     const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
-    linesum_unit<P>(a, tile, blockIdx.y, blockIdx.z, s_rng, s_red, s_rec);
+    linesum_unit<P>(a, tile, blockIdx.y, blockIdx.z, s_rng, s_red, s_rec, s_far);
 }
 
 // Persistent form: a grid sized to the resident capacity of the device pulls
@@ -403,9 +539,10 @@ __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumAr
 // queue is past the last unit.  The counter must be zero at launch.
 template <int P>
 __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_persistent_kernel(const LinesumArgs a) {
-    __shared__ int s_rng[8];
+    __shared__ int s_rng[12];
     __shared__ double s_red[12];
     __shared__ float4 s_rec[2 * LS_CHUNK];
+    __shared__ float s_far[FAR_LDS];
     __shared__ unsigned s_unit;
     const unsigned per_tile = (unsigned)a.n_layers * (unsigned)a.n_prof;
     const unsigned total = (unsigned)a.tiles * per_tile;
@@ -418,7 +555,7 @@ __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_persistent_kernel(cons
         const unsigned rank = u / per_tile, rem = u - rank * per_tile;
         const int tile = a.tile_order ? a.tile_order[rank] : (int)rank;
         const int prof = (int)(rem / (unsigned)a.n_layers), layer = (int)(rem % (unsigned)a.n_layers);
-        linesum_unit<P>(a, tile, layer, prof, s_rng, s_red, s_rec);
+        linesum_unit<P>(a, tile, layer, prof, s_rng, s_red, s_rec, s_far);
     }
 }
 

@@ -185,10 +185,51 @@ __global__ void __launch_bounds__(256) keys_kernel(const double *nu, int64_t n, 
     }
 }
 
+// Per item, inclusive prefix sums over its lines (list order) of the pedestal
+// amplitude P_j, P_j c_j and P_j c_j^2, c_j the centre in grid steps (FP64), so
+// that a tile obtains the three pedestal moments of any line range by two
+// lookups.  One workgroup per item, 256-line chunks, wave-shuffle scan.
+__global__ void __launch_bounds__(256) pedestal_scan_kernel(const PrepArgs a) {
+    __shared__ double wsum[3][4];
+    const ItemDev it = a.items[blockIdx.x];
+    if (!it.active || it.n == 0) return;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double *p0 = a.pre + it.rec_off, *p1 = p0 + a.n_rec, *p2 = p1 + a.n_rec;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    for (int64_t j0 = 0; j0 < it.n; j0 += 256) {
+        const int64_t j = j0 + t;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (j < it.n) {
+            const float4 r0 = a.rec0[it.rec_off + j];
+            const double P = (double)a.rec1[it.rec_off + j].y;
+            const double c = (double)__float_as_int(r0.x) + (double)r0.y;
+            v0 = P;
+            v1 = P * c;
+            v2 = P * c * c;
+        }
+        for (int o = 1; o < 64; o <<= 1) {
+            const double u0 = __shfl_up(v0, o, 64), u1 = __shfl_up(v1, o, 64), u2 = __shfl_up(v2, o, 64);
+            if (lane >= o) { v0 += u0; v1 += u1; v2 += u2; }
+        }
+        if (lane == 63) { wsum[0][wv] = v0; wsum[1][wv] = v1; wsum[2][wv] = v2; }
+        __syncthreads();
+        double b0 = c0, b1 = c1, b2 = c2;
+        for (int w = 0; w < wv; ++w) { b0 += wsum[0][w]; b1 += wsum[1][w]; b2 += wsum[2][w]; }
+        if (j < it.n) { p0[j] = b0 + v0; p1[j] = b1 + v1; p2[j] = b2 + v2; }
+        c0 += wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
+        c1 += wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
+        c2 += wsum[2][0] + wsum[2][1] + wsum[2][2] + wsum[2][3];
+        __syncthreads();
+    }
+}
+
 hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s) {
     if (a.n_items <= 0 || max_lines <= 0) return hipSuccess;
     dim3 grid((max_lines + 255) / 256, a.n_items);
     hipLaunchKernelGGL(prep_kernel, grid, dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.pre) return e;
+    hipLaunchKernelGGL(pedestal_scan_kernel, dim3(a.n_items), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -77,11 +77,14 @@ class IrsStats(C.Structure):
                 ("ms_ils", f64), ("n_hi", i64), ("n_layers", i64), ("linesum_launches", i64),
                 ("active_lines", i64), ("active_pairs", i64), ("dispersive_pairs", i64),
                 ("core_pairs_bound", i64), ("ms_device", f64), ("graph_launches", i64),
-                ("tile_points", i32), ("persistent", i32), ("autotuned", i32), ("reserved1", i32)]
+                ("tile_points", i32), ("persistent", i32), ("autotuned", i32), ("reserved1", i32),
+                ("fine_pairs", i64), ("coarse_pairs", i64), ("fine_pairs_lm", i64),
+                ("coarse_pairs_lm", i64)]
 
 
 class IrsSchedule(C.Structure):
-    _fields_ = [("tile_points", i32), ("persistent", i32), ("use_graph", i32), ("autotune", i32)]
+    _fields_ = [("tile_points", i32), ("persistent", i32), ("use_graph", i32), ("autotune", i32),
+                ("far_wing_cm", f64)]
 
 
 EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_last_error",
@@ -228,11 +231,14 @@ class Context:
         return {k: getattr(sc, k) for k, _ in IrsSchedule._fields_}
 
     def set_schedule(self, tile_points: int = 4, persistent: bool = False, use_graph: bool = False,
-                     autotune: bool = False):
+                     autotune: bool = False, far_wing_cm: float = 1.5):
         """Line-sum schedule of forward_model: points per thread (4/8/16), the
         persistent heaviest-first work-queue kernel, hipGraph capture and
-        replay of each profile group, autotuning on the first call of a shape."""
-        sc = IrsSchedule(int(tile_points), int(bool(persistent)), int(bool(use_graph)), int(bool(autotune)))
+        replay of each profile group, autotuning on the first call of a shape,
+        and the distance beyond which far wings are summed on a coarse grid
+        and interpolated (0: every pair on the internal grid)."""
+        sc = IrsSchedule(int(tile_points), int(bool(persistent)), int(bool(use_graph)), int(bool(autotune)),
+                         float(far_wing_cm))
         self.check(self.lib.irs_set_schedule(self.h, C.byref(sc)))
 
     def upload(self, ll: HITRANLinelist) -> int:

@@ -102,3 +102,37 @@ def test_empty_and_single_line(ctx):
     assert rel_err_masked(got, ref)[0] < TOL
     far = HITRANLinelist(1, 1, [900.0], [1e-20], [0.07], [0.3], [100.0], [0.7], [0.0])
     assert np.all(compute_voigt_cross_sections(g, far, 250.0, 0.5, ctx=ctx) == 0.0)
+
+
+@pytest.mark.parametrize("lm", [False, True])
+def test_coarse_far_wing_matches_exact_sum(ctx, lm):
+    """The coarse-grid far-wing sum (cubic interpolation beyond 1.5 cm^-1)
+    against the every-pair sum on a dense 15 um band layer, and both against
+    the FP64 oracle (1e-4 where sigma > 1e-6 peak)."""
+    import refcpu as o
+    from irsounderlbl import synthetic as syn
+    from irsounderlbl.api import compute_voigt_cross_sections, compute_voigt_lm_cross_sections
+    from irsounderlbl.types import GasSpecies, filter_linelist, wavenumber_grid
+    co2 = filter_linelist(syn.linelists_config1()[GasSpecies.CO2], 640.0, 760.0)
+    g = wavenumber_grid(690.0, 700.0, 0.001)
+    sub = filter_linelist(co2, 664.0, 726.0)
+    T, p = 240.0, 0.3
+    def run():
+        if lm:
+            return compute_voigt_lm_cross_sections(g, sub, T, p, ctx=ctx)
+        return compute_voigt_cross_sections(g, sub, T, p, ctx=ctx)
+    try:
+        ctx.set_schedule(4, far_wing_cm=0.0)
+        exact = run()
+        ctx.set_schedule(4, far_wing_cm=1.5)
+        far = run()
+        st = ctx.stats()
+    finally:
+        ctx.set_schedule(4)
+    assert st["coarse_pairs"] > 0 and st["fine_pairs"] > 0
+    m = exact > 1e-6 * exact.max()
+    assert np.max(np.abs(far[m] - exact[m]) / exact[m]) < 5e-6
+    ref = (o.voigt_lm_xsec(sub, g.nu_min, g.dnu, g.n, T, p) if lm
+           else o.voigt_xsec(sub, g.nu_min, g.dnu, g.n, T, p, 0.0))
+    mr = ref > 1e-6 * ref.max()
+    assert np.max(np.abs(far[mr] - ref[mr]) / ref[mr]) < 1e-4
