@@ -12,8 +12,16 @@ A second figure, "batched", times one scan line of 120 perturbed profiles
 U(0, 48.3 deg); DPTMIN live, so every profile has its own active line sets)
 in one forward_model call with the autotuned schedule and hipGraph replay.
 
+A third figure, "emulator", trains the channel-transmittance emulator on GPU
+LBL spectra of --emu-train perturbed profiles (T GP(0, 4 K), H2O x
+lognormal(0.6), O3 x lognormal(0.2), CO2 U(390, 430) ppm, scan angles
+U(0, 48.3 deg)), then runs --emu-eval new profiles of the same distribution
+through emulator + 64-channel LBL spot check + full-LBL fallback at 0.05 K,
+with the BT error against the GPU LBL (itself within 1e-9 K of the FP64 oracle).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-accuracy]
-                    [--no-batch] [--batch N] [--no-exact]
+                    [--no-batch] [--batch N] [--no-exact] [--no-emulator]
+                    [--emu-train N] [--emu-eval N]
 """
 import argparse
 import json
@@ -112,6 +120,55 @@ def batched(ctx, lls, snd, n_prof):
             "bt_finite": bool(np.all(np.isfinite(BT)))}
 
 
+def emulator_figure(ctx, lls, snd, n_train, n_eval):
+    from irsounderlbl import synthetic as syn
+    from irsounderlbl.api import forward_model
+    from irsounderlbl.emulator import ChannelODEmulator, SpotChecker, forward_emulated
+    from irsounderlbl.types import ViewingGeometry
+    kw = dict(sigma_T=4.0, sigma_h2o=0.6, sigma_o3=0.2, co2_range=(390e-6, 430e-6))
+    train = syn.perturbed_profiles(n_train, seed=3003, **kw)
+    log(f"bench: emulator training set ({n_train} profiles, GPU LBL) + PCA + MLP")
+    t0 = time.perf_counter()
+    emu, rep = ChannelODEmulator.train(train, lls, snd, n_pc=256, steps=6000, batch=4096, lr=1e-3, ctx=ctx)
+    t_train = time.perf_counter() - t0
+    ev = syn.perturbed_profiles(n_eval, seed=4004, **kw)
+    profs = [p[0] for p in ev]
+    geoms = [ViewingGeometry(float(np.degrees(np.arccos(p[2]))), 0.0, p[2]) for p in ev]
+    tsf = [p[1] for p in ev]
+    log(f"bench: emulator evaluation ({n_eval} profiles)")
+    spot = SpotChecker(lls, snd)
+    forward_emulated(profs[:8], lls, emu, sounder=snd, geom=geoms[:8], T_sfc=tsf[:8], spot=spot, ctx=ctx)
+    t0 = time.perf_counter()
+    _, bt_emu_only = emu.forward(profs, geom=geoms, T_sfc=tsf)
+    t_emu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = forward_emulated(profs, lls, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=0.05, spot=spot, ctx=ctx)
+    t_full = time.perf_counter() - t0
+    log("bench: emulator reference LBL")
+    _, _, bt_lbl = forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    d_emu = np.abs(bt_emu_only - bt_lbl)
+    d_fin = np.abs(res.BT - bt_lbl)
+    t0 = time.perf_counter()
+    jac = emu.jacobian(profs[:4], geom=geoms[:4], T_sfc=tsf[:4])
+    t_jac = (time.perf_counter() - t0) / 4
+    emu.close()
+    emu.mlp.close()
+    return {"workload": f"train {n_train} / evaluate {n_eval} perturbed profiles, full IASI 8461 channels, "
+                        "256 PCs, MLP 9 -> 3x512 GELU -> 256 (BF16 MFMA), spot check 4 x 16 channels",
+            "training_seconds": round(t_train, 1), "lbl_dataset_seconds": round(rep.seconds_lbl, 1),
+            "pca_seconds": round(rep.seconds_pca, 1), "mlp_train_seconds": round(rep.seconds_train, 1),
+            "final_loss": float(np.mean(rep.losses[-50:])),
+            "emulator_only_spectra_per_s": round(n_eval / t_emu, 1),
+            "emulator_plus_fallback_spectra_per_s": round(n_eval / t_full, 3),
+            "fallback_rate": float(np.mean(res.fell_back)),
+            "seconds": {k: round(v, 3) for k, v in res.seconds.items()},
+            "bt_err_emulator_only_K": {"max": float(d_emu.max()), "rms": float(np.sqrt(np.mean(d_emu ** 2))),
+                                       "p99_per_scene_max": float(np.percentile(d_emu.max(axis=1), 99))},
+            "bt_err_after_fallback_K": {"max": float(d_fin.max()), "rms": float(np.sqrt(np.mean(d_fin ** 2)))},
+            "jacobian_seconds_per_profile": round(t_jac, 3),
+            "jacobian_shape": list(jac[0].K_T.shape)}
+
+
 def exact_sum(ctx, prof, lls, snd):
     """One step with the coarse far-wing sum switched off (every pair on the
     internal grid, the reference's brute force) for comparison."""
@@ -147,6 +204,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch", type=int, default=120)
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--no-emulator", action="store_true")
+    ap.add_argument("--emu-train", type=int, default=512)
+    ap.add_argument("--emu-eval", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this engine shards profiles across GPUs with no "
@@ -218,6 +278,8 @@ def main():
                           "rte": round(stats["ms_rte"], 3), "ils": round(stats["ms_ils"], 3)}}
     if not args.no_batch:
         out["batched"] = batched(ctx, lls, snd, args.batch)
+    if not args.no_emulator:
+        out["emulator"] = emulator_figure(ctx, lls, snd, args.emu_train, args.emu_eval)
     if not args.no_accuracy:
         log("bench: accuracy sample vs FP64 oracle")
         out["accuracy_vs_fp64"] = accuracy_sample(ctx, prof, lls)
