@@ -286,7 +286,8 @@ int         irs_mlp_train(irs_mlp *mlp, int steps, int batch, float lr, uint64_t
 int         irs_mlp_input_jacobian(irs_mlp *mlp, const float *x, int64_t n, float *J);
 
 /* Emulated forward model: two targets per layer and channel decoded as
- * x_k(c) = t_mean[k][c] + sum_m (y_m * coef_scale[m]) basis[m][c] (first target
+ * x_k(c) = t_mean[k][c] + inv_w[k][c] sum_m (y_m * coef_scale[m]) basis[m][c] (inv_w
+ * [n_layers][2 n_channels], NULL: 1; first target
  * on columns 0..n_ch-1, second on n_ch..2 n_ch-1) with
  * y = MLP((features - x_mean) / x_std), x_mean and x_std [n_layers][n_in] (per layer):
  *   kind = 2: the layer channel transmittance t_k = exp(-tau_ch_k) and the CIM
@@ -299,7 +300,8 @@ int         irs_mlp_input_jacobian(irs_mlp *mlp, const float *x, int64_t n, floa
 typedef struct irs_emu irs_emu;
 int         irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_tpc,
                            const float *x_mean, const float *x_std, const float *coef_scale,
-                           const float *basis, const float *t_mean, const double *ch_nu, irs_emu **out);
+                           const float *basis, const float *t_mean, const float *inv_w,
+                           const double *ch_nu, irs_emu **out);
 void        irs_emu_destroy(irs_emu *emu);
 const char *irs_emu_last_error(const irs_emu *emu);
 /* features[n_prof][L][n_in] (raw), T_lev[n_prof][L+1], T_ave[n_prof][L], scal[n_prof][3] =

@@ -38,6 +38,7 @@ struct EmuRteArgs {
     const float *coef_scale;     // [n_pc]: coefficient = MLP output * coef_scale
     const float *basis;          // [n_pc][2 n_ch]: two targets per layer and channel
     const float *t_mean;         // [L][2 n_ch]
+    const float *inv_w;          // [L][2 n_ch] decode weights (null: 1)
     int target_kind;             // 0: (t_k, g_k) ratios, 1: (T_k, G_k) levels
     const double *ch_nu;         // [n_ch]
     const double *T_lev, *T_ave; // [n_prof][L+1], [n_prof][L]

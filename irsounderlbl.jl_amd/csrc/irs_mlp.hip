@@ -49,6 +49,7 @@ struct irs_emu {
     float *coef_scale = nullptr;     // [n_pc]
     float *basis = nullptr;          // [n_pc][n_ch]
     float *t_mean = nullptr;         // [L][n_ch]
+    float *inv_w = nullptr;          // [L][2 n_ch] or null
     double *ch_nu = nullptr;         // [n_ch]
     float *coef = nullptr;           // [cap rows][n_pc]
     double *prof = nullptr;          // T_lev, T_ave, scal
@@ -424,7 +425,7 @@ int irs_mlp_input_jacobian(irs_mlp *m, const float *x, int64_t n, float *J) {
 // ---- emulator ----------------------------------------------------------------
 int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_tpc, const float *x_mean,
                    const float *x_std, const float *coef_scale, const float *basis, const float *t_mean,
-                   const double *ch_nu, irs_emu **out) {
+                   const float *inv_w, const double *ch_nu, irs_emu **out) {
     if (!out) return IRS_EINVAL;
     *out = nullptr;
     if (!mlp || !x_mean || !x_std || !coef_scale || !basis || !t_mean || !ch_nu || n_layers < 1 || n_pc < 1 ||
@@ -443,11 +444,13 @@ int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_t
     e->x_std.assign(x_std, x_std + (size_t)n_layers * mlp->n_in);
     bool ok = dalloc(&e->coef_scale, n_pc) == hipSuccess && dalloc(&e->basis, (size_t)n_pc * W) == hipSuccess &&
               dalloc(&e->t_mean, (size_t)n_layers * W) == hipSuccess &&
-              dalloc(&e->ch_nu, n_channels) == hipSuccess;
+              dalloc(&e->ch_nu, n_channels) == hipSuccess &&
+              (!inv_w || dalloc(&e->inv_w, (size_t)n_layers * W) == hipSuccess);
     ok = ok && hipMemcpy(e->coef_scale, coef_scale, sizeof(float) * n_pc, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(e->basis, basis, sizeof(float) * (size_t)n_pc * W, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(e->t_mean, t_mean, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(e->ch_nu, ch_nu, sizeof(double) * n_channels, hipMemcpyHostToDevice) == hipSuccess;
+         hipMemcpy(e->ch_nu, ch_nu, sizeof(double) * n_channels, hipMemcpyHostToDevice) == hipSuccess &&
+         (!inv_w || hipMemcpy(e->inv_w, inv_w, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess);
     if (!ok) {
         irs_emu_destroy(e);
         return IRS_ENOMEM;
@@ -460,7 +463,7 @@ void irs_emu_destroy(irs_emu *e) {
     if (!e) return;
     (void)hipSetDevice(e->mlp->device);
     (void)hipStreamSynchronize(e->mlp->stream);
-    void *ptrs[] = {e->coef_scale, e->basis, e->t_mean, e->ch_nu, e->coef, e->prof, e->R, e->BT, e->jac};
+    void *ptrs[] = {e->coef_scale, e->basis, e->t_mean, e->ch_nu, e->coef, e->prof, e->R, e->BT, e->jac, e->inv_w};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete e;
@@ -564,6 +567,7 @@ int irs_emu_forward(irs_emu *e, const float *features, const double *T_lev, cons
     a.coef = e->coef;
     a.basis = e->basis;
     a.t_mean = e->t_mean;
+    a.inv_w = e->inv_w;
     a.ch_nu = e->ch_nu;
     a.T_lev = dp;
     a.T_ave = dp + (size_t)n_prof * (L + 1);
@@ -662,6 +666,7 @@ int irs_emu_jacobian(irs_emu *e, const float *features, const double *T_lev, con
         a.coef_scale = e->coef_scale;
         a.basis = e->basis;
         a.t_mean = e->t_mean;
+        a.inv_w = e->inv_w;
         a.ch_nu = e->ch_nu;
         a.T_lev = dprof;
         a.T_ave = dprof + (size_t)n_prof * (L + 1);

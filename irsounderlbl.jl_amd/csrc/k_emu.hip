@@ -35,10 +35,7 @@ __global__ void __launch_bounds__(256) emu_rte_kernel(const EmuRteArgs a) {
     const int64_t W = 2 * (int64_t)a.n_ch;
     float T[MAXL], G[MAXL];
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k) {
-        T[k] = k < L ? a.t_mean[(int64_t)k * W + c] : 1.0f;
-        G[k] = k < L ? a.t_mean[(int64_t)k * W + a.n_ch + c] : 0.0f;
-    }
+    for (int k = 0; k < MAXL; ++k) T[k] = G[k] = 0.0f;
     for (int m = 0; m < npc; ++m) {
         const float v = a.basis[(int64_t)m * W + c];
         const float vg = a.basis[(int64_t)m * W + a.n_ch + c];
@@ -48,6 +45,17 @@ __global__ void __launch_bounds__(256) emu_rte_kernel(const EmuRteArgs a) {
                 T[k] = fmaf(s_coef[k * npc + m], v, T[k]);
                 G[k] = fmaf(s_coef[k * npc + m], vg, G[k]);
             }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+        if (k < L) {
+            const int64_t o = (int64_t)k * W + c;
+            const float wt = a.inv_w ? a.inv_w[o] : 1.0f, wg = a.inv_w ? a.inv_w[o + a.n_ch] : 1.0f;
+            T[k] = fmaf(T[k], wt, a.t_mean[o]);
+            G[k] = fmaf(G[k], wg, a.t_mean[o + a.n_ch]);
+        } else {
+            T[k] = 1.0f;
+        }
     }
     const double nu = a.ch_nu[c];
     const double *T_lev = a.T_lev + (int64_t)q * (L + 1), *T_ave = a.T_ave + (int64_t)q * L;
@@ -121,10 +129,7 @@ __global__ void __launch_bounds__(256) emu_adjoint_kernel(const EmuRteArgs a, Em
     const int64_t W = 2 * (int64_t)a.n_ch;
     float T[MAXL], G[MAXL];
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k) {
-        T[k] = k < L ? a.t_mean[(int64_t)k * W + c] : 1.0f;
-        G[k] = k < L ? a.t_mean[(int64_t)k * W + a.n_ch + c] : 0.0f;
-    }
+    for (int k = 0; k < MAXL; ++k) T[k] = G[k] = 0.0f;
     for (int m = 0; m < npc; ++m) {
         const float v = a.basis[(int64_t)m * W + c];
         const float vg = a.basis[(int64_t)m * W + a.n_ch + c];
@@ -134,6 +139,17 @@ __global__ void __launch_bounds__(256) emu_adjoint_kernel(const EmuRteArgs a, Em
                 T[k] = fmaf(s_coef[k * npc + m], v, T[k]);
                 G[k] = fmaf(s_coef[k * npc + m], vg, G[k]);
             }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+        if (k < L) {
+            const int64_t o = (int64_t)k * W + c;
+            const float wt = a.inv_w ? a.inv_w[o] : 1.0f, wg = a.inv_w ? a.inv_w[o + a.n_ch] : 1.0f;
+            T[k] = fmaf(T[k], wt, a.t_mean[o]);
+            G[k] = fmaf(G[k], wg, a.t_mean[o + a.n_ch]);
+        } else {
+            T[k] = 1.0f;
+        }
     }
     const double nu = a.ch_nu[c];
     const double *T_lev = a.T_lev + (int64_t)q * (L + 1), *T_ave = a.T_ave + (int64_t)q * L;
@@ -187,7 +203,8 @@ __global__ void __launch_bounds__(256) emu_adjoint_kernel(const EmuRteArgs a, Em
     j.dTlev[((int64_t)q * (L + 1)) * a.n_ch + c] = 0.0f;
 }
 
-// dBT/dfeature: F[q][k][i][c] = sum_m Jm[q][k][m][i] (dt[q][k][c] V_m(c) + dg[q][k][c] Vg_m(c)) * scale_m.
+// dBT/dfeature: F[q][k][i][c] = sum_m Jm[q][k][m][i] (dt[q][k][c] V_m(c) iw_t + dg[q][k][c] Vg_m(c) iw_g) * scale_m
+// (iw: the decode weights of the layer and channel).
 // One workgroup per (channel block, layer, profile); J of the layer in LDS.
 __global__ void __launch_bounds__(256) emu_feature_jac_kernel(const EmuRteArgs a, EmuAdjArgs j, int n_in) {
     extern __shared__ float s_j[];             // [n_pc][n_in]
@@ -199,7 +216,8 @@ __global__ void __launch_bounds__(256) emu_feature_jac_kernel(const EmuRteArgs a
     if (c >= a.n_ch) return;
     const int64_t W = 2 * (int64_t)a.n_ch;
     const int64_t ok = ((int64_t)q * L + k) * a.n_ch + c;
-    const float at = j.dt[ok], ag = j.dg[ok];
+    const float at = j.dt[ok] * (a.inv_w ? a.inv_w[(int64_t)k * W + c] : 1.0f);
+    const float ag = j.dg[ok] * (a.inv_w ? a.inv_w[(int64_t)k * W + a.n_ch + c] : 1.0f);
     float acc[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;

@@ -157,7 +157,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_mlp_set_data.argtypes = [P, P, P, i64]
     L.irs_mlp_train.argtypes = [P, i32, i32, C.c_float, C.c_uint64, P]
     L.irs_mlp_input_jacobian.argtypes = [P, P, i64, P]
-    L.irs_emu_create.argtypes = [P, i32, i32, i32, i32, P, P, P, P, P, P, C.POINTER(P)]
+    L.irs_emu_create.argtypes = [P, i32, i32, i32, i32, P, P, P, P, P, P, P, C.POINTER(P)]
     L.irs_emu_destroy.argtypes = [P]
     L.irs_emu_destroy.restype = None
     L.irs_emu_last_error.argtypes = [P]

@@ -168,7 +168,7 @@ class TrainReport:
     seconds_lbl: float
     seconds_pca: float
     seconds_train: float
-    pca_rms_t: float
+    pca_rms_t: float          # PCA truncation error of the relevance-weighted targets
     pca_max_t: float
     losses: np.ndarray = field(repr=False, default=None)
 
@@ -177,7 +177,7 @@ class ChannelODEmulator:
     """Trained emulator: MLP + PCA basis + normalisation, on the device."""
 
     def __init__(self, mlp: MLP, x_mean, x_std, coef_scale, basis, t_mean, ch_nu, T_method="logp_at_pcg",
-                 n_tpc: int = 2):
+                 n_tpc: int = 2, inv_w=None):
         self.mlp = mlp
         self.lib = mlp.lib
         self.x_mean = np.ascontiguousarray(x_mean, np.float32)
@@ -186,6 +186,7 @@ class ChannelODEmulator:
         self.basis = np.ascontiguousarray(basis, np.float32)
         self.t_mean = np.ascontiguousarray(t_mean, np.float32)
         self.ch_nu = np.ascontiguousarray(ch_nu, np.float64)
+        self.inv_w = None if inv_w is None else np.ascontiguousarray(inv_w, np.float32)
         self.n_tpc = int(n_tpc)            # target kind: 2 (t_k, g_k) ratios, 3 (T_k, G_k) levels
         self.n_pc = self.basis.shape[0]
         self.n_ch = self.basis.shape[1] // 2
@@ -195,8 +196,8 @@ class ChannelODEmulator:
         self.T_method = T_method
         h = C.c_void_p()
         rc = self.lib.irs_emu_create(mlp.h, self.L, self.n_pc, self.n_ch, self.n_tpc, _p(self.x_mean), _p(self.x_std),
-                                     _p(self.coef_scale), _p(self.basis), _p(self.t_mean), _p(self.ch_nu),
-                                     C.byref(h))
+                                     _p(self.coef_scale), _p(self.basis), _p(self.t_mean), _p(self.inv_w),
+                                     _p(self.ch_nu), C.byref(h))
         if rc != IRS_OK:
             raise IRSError(rc, f"irs_emu_create failed ({rc})")
         self.h = h
@@ -216,7 +217,8 @@ class ChannelODEmulator:
     @staticmethod
     def train(profiles: Sequence, linelists: Dict[GasSpecies, HITRANLinelist], sounder: Sounder, *,
               n_pc: int = 256, steps: int = 3000, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
-              floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None, targets=None, **kw):
+              floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None, targets=None,
+              relevance_floor: Optional[float] = None, **kw):
         """Train on scenes `profiles` (AtmosphericProfile or (profile, T_sfc, mu)
         tuples); the targets along each scene's path come from the GPU LBL
         engine unless `targets` = (tau_ch [N, L, n_ch], g [N, L, n_ch]) is given."""
@@ -231,18 +233,38 @@ class ChannelODEmulator:
         tau_ch, g = targets
         t1 = time.perf_counter()
         N, L, nch = g.shape
+        # optional relevance weights (relevance_floor not None): a layer's channel
+        # values reach the top of the atmosphere through the transmittance above it,
+        # W[k, c] = max over scenes of T_{k+1}(c) (floor relevance_floor), so a
+        # weighted-space error e changes the radiance of a training-like scene by at
+        # most ~e; the CIM weights enter the radiance through a Planck difference
+        # across half a layer, ~10x less: weight 0.1 W.  Scenes more transparent than
+        # every training scene see errors amplified by up to 1/W (measured: the
+        # unweighted targets give the smaller maximum error on held-out scenes).
+        if relevance_floor is None:
+            W = np.ones((L, nch), np.float32)
+        else:
+            above = np.zeros((L, nch))
+            for q in range(N):
+                cum = np.cumsum(tau_ch[q, ::-1].astype(np.float64), axis=0)[::-1]     # sum_{j>=k}
+                Ta = np.exp(-np.vstack([cum[1:], np.zeros((1, nch))]))               # T_{k+1}
+                np.maximum(above, Ta, out=above)
+            W = np.maximum(above, relevance_floor).astype(np.float32)
+        Wv = np.concatenate([W, 0.1 * W], axis=1)                                  # [L, 2 nch]
         t = np.concatenate([np.exp(-tau_ch.astype(np.float32)), g.astype(np.float32)], axis=2)
         del targets, tau_ch, g
         t_mean = t.mean(axis=0)
         t -= t_mean[None]
+        t *= Wv[None]
         dev = t.reshape(N * L, 2 * nch)
         V, _ = _randomized_pca(dev, n_pc, seed=seed)
         Vf = np.zeros((n_pc, 2 * nch), np.float32)           # fewer columns than n_pc: zero rows
         Vf[:len(V)] = V
         coef = dev @ Vf.T                                           # [N L, n_pc]
         rec = coef @ Vf
-        err = dev - rec
+        err = dev - rec                                             # relevance-weighted units
         pca_rms, pca_max = float(np.sqrt(np.mean(err * err))), float(np.abs(err).max())
+        del err, rec
         t2 = time.perf_counter()
         feats = np.stack([layer_features(p, m) for p, m in zip(profs, mus)]).astype(np.float64)   # [N, L, F]
         # per-layer standardisation keeps the bf16 inputs fine-grained (feature 0,
@@ -263,7 +285,8 @@ class ChannelODEmulator:
                                  mlp.train(steps - n1 - n2, batch, lr / 10.0, seed + 2)])
         t3 = time.perf_counter()
         sg = sounder_grid(sounder)
-        emu = ChannelODEmulator(mlp, x_mean, x_std, coef_scale, Vf, t_mean, sg.nu, n_tpc=2)
+        # the decode undoes the weights: target_k = mean_k + (sum_m a_{k,m} V_m) / W_k
+        emu = ChannelODEmulator(mlp, x_mean, x_std, coef_scale, Vf, t_mean, sg.nu, n_tpc=2, inv_w=1.0 / Wv)
         rep = TrainReport(N, N * L, t1 - t0, t2 - t1, t3 - t2, pca_rms, pca_max, losses)
         return emu, rep
 
@@ -272,6 +295,7 @@ class ChannelODEmulator:
         np.savez_compressed(path, params=self.mlp.get_params(), x_mean=self.x_mean, x_std=self.x_std,
                             coef_scale=self.coef_scale, basis=self.basis.astype(np.float32),
                             t_mean=self.t_mean, ch_nu=self.ch_nu,
+                            inv_w=self.inv_w if self.inv_w is not None else np.ones_like(self.t_mean),
                             dims=np.array([self.mlp.n_in, self.mlp.n_hidden, self.mlp.n_out, self.n_tpc]))
 
     @staticmethod
@@ -281,7 +305,7 @@ class ChannelODEmulator:
         mlp = MLP(n_in, H, n_out, max_batch)
         mlp.set_params(z["params"])
         return ChannelODEmulator(mlp, z["x_mean"], z["x_std"], z["coef_scale"], z["basis"], z["t_mean"],
-                                 z["ch_nu"], n_tpc=n_tpc)
+                                 z["ch_nu"], n_tpc=n_tpc, inv_w=z["inv_w"])
 
     # ---- Jacobians ----------------------------------------------------------
     def jacobian(self, profs, *, geom=None, T_sfc=None, eps_sfc=1.0) -> List["EmulatorJacobian"]:
