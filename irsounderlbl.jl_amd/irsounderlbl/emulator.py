@@ -233,14 +233,14 @@ class ChannelODEmulator:
         tau_ch, g = targets
         t1 = time.perf_counter()
         N, L, nch = g.shape
-        # optional relevance weights (relevance_floor not None): a layer's channel
+        # target weights W (PCA and loss run on W x deviation, the decode divides by
+        # W): the CIM weights enter the radiance through a Planck difference across
+        # half a layer, ~10x less than the transmittances, so they carry 0.1 W.
+        # Optional relevance weights (relevance_floor not None): a layer's channel
         # values reach the top of the atmosphere through the transmittance above it,
-        # W[k, c] = max over scenes of T_{k+1}(c) (floor relevance_floor), so a
-        # weighted-space error e changes the radiance of a training-like scene by at
-        # most ~e; the CIM weights enter the radiance through a Planck difference
-        # across half a layer, ~10x less: weight 0.1 W.  Scenes more transparent than
-        # every training scene see errors amplified by up to 1/W (measured: the
-        # unweighted targets give the smaller maximum error on held-out scenes).
+        # W[k, c] = max over scenes of T_{k+1}(c) (floor relevance_floor); scenes more
+        # transparent than every training scene then see errors amplified by up to
+        # 1/W (measured: without them the maximum held-out error is smaller).
         if relevance_floor is None:
             W = np.ones((L, nch), np.float32)
         else:
