@@ -63,7 +63,7 @@ struct PinBuf {
 // Everything the device sequence of one profile group depends on.  Graphs are
 // cached by the byte image of this struct (zero-filled before use).
 struct GroupPlan {
-    const void *items, *rec0, *rec1, *rec2, *keys, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
+    const void *items, *rec0, *rec1, *rec2, *rec3, *keys, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
     const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out, *pre;
     int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes, n_rec;
     int gn, L, ns, P, pblocks, tiles, source_function, with_ils, far_d, far_hc;
@@ -95,7 +95,7 @@ struct irs_ctx {
     hipEvent_t ev[8] = {};
     std::string err;
     std::vector<ListHost> lists;
-    DevBuf d_lists, d_tips, d_items, d_rec0, d_rec1, d_rec2, d_keys, d_tau, d_rhi, d_counters;
+    DevBuf d_lists, d_tips, d_items, d_rec0, d_rec1, d_rec2, d_rec3, d_keys, d_tau, d_rhi, d_counters;
     DevBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3, d_tmp4, d_tmp5, d_chidx, d_chw, d_chnu, d_rch, d_bt;
     DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
@@ -470,6 +470,7 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     HIPCHK(ensure(ctx->d_rec0, sizeof(float4) * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_rec1, sizeof(float4) * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>(rec, 1)));
+    HIPCHK(ensure(ctx->d_rec3, sizeof(float4) * 6 * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_items, sizeof(ItemDev) * std::max<size_t>(items.size(), 1)));
     HIPCHK(ensure(ctx->d_counters, 64));
     HIPCHK(ensure(ctx->d_queue, 64));
@@ -488,6 +489,7 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     pa.rec0 = (float4 *)ctx->d_rec0.p;
     pa.rec1 = (float4 *)ctx->d_rec1.p;
     pa.rec2 = (float4 *)ctx->d_rec2.p;
+    pa.rec3 = (float4 *)ctx->d_rec3.p;
     pa.counters = (unsigned long long *)ctx->d_counters.p;
     pa.n_hi = n_hi;
     pa.pre = (double *)ctx->d_pre.p;
@@ -516,6 +518,7 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     la.work = (unsigned long long *)ctx->d_counters.p + 4;
     la.pre = pa.pre;
     la.n_rec = pa.n_rec;
+    la.rec3 = pa.rec3;
     HIPCHK(launch_linesum(la, P, ctx->sched.persistent ? persistent_blocks(ctx, P) : 0, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     ctx->stats.linesum_launches += 1;
@@ -718,7 +721,7 @@ void irs_destroy(irs_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &l : ctx->lists)
         if (l.block) (void)hipFree(l.block);
-    DevBuf *bufs[] = {&ctx->d_lists, &ctx->d_tips, &ctx->d_items, &ctx->d_rec0, &ctx->d_rec1, &ctx->d_rec2,
+    DevBuf *bufs[] = {&ctx->d_lists, &ctx->d_tips, &ctx->d_items, &ctx->d_rec0, &ctx->d_rec1, &ctx->d_rec2, &ctx->d_rec3,
                       &ctx->d_keys, &ctx->d_tau, &ctx->d_rhi, &ctx->d_counters, &ctx->d_tmp0, &ctx->d_tmp1,
                       &ctx->d_tmp2, &ctx->d_tmp3, &ctx->d_tmp4, &ctx->d_tmp5, &ctx->d_chidx, &ctx->d_chw,
                       &ctx->d_chnu, &ctx->d_rch, &ctx->d_bt, &ctx->j_items, &ctx->j_g, &ctx->j_tau,
@@ -1043,6 +1046,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     pa.rec0 = (float4 *)pl.rec0;
     pa.rec1 = (float4 *)pl.rec1;
     pa.rec2 = (float4 *)pl.rec2;
+    pa.rec3 = (float4 *)pl.rec3;
     pa.counters = (unsigned long long *)pl.counters;
     pa.n_hi = pl.n;
     pa.pre = (double *)pl.pre;
@@ -1071,6 +1075,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.work = (unsigned long long *)pl.counters + 4;
     la.pre = pa.pre;
     la.n_rec = pa.n_rec;
+    la.rec3 = pa.rec3;
     ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
     if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
     if (pl.mode >= 1) {
@@ -1200,6 +1205,7 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
     pa.rec0 = (float4 *)base.rec0;
     pa.rec1 = (float4 *)base.rec1;
     pa.rec2 = (float4 *)base.rec2;
+    pa.rec3 = (float4 *)base.rec3;
     pa.counters = (unsigned long long *)base.counters;
     pa.n_hi = base.n;
     pa.pre = (double *)base.pre;
@@ -1233,6 +1239,7 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
             far_params(ctx, base.dnu, P, la.far_d, la.far_hc);
             la.pre = pa.pre;
             la.n_rec = pa.n_rec;
+            la.rec3 = pa.rec3;
             const int pb = pers ? persistent_blocks(ctx, P) : 0;
             if (pers && pb <= 0) continue;
             double t_min = 1e300;
@@ -1342,7 +1349,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
             lines_per_prof += ctx->lists[profiles[0].linelist[s]].n;
             max_lines = std::max<int>(max_lines, (int)ctx->lists[profiles[0].linelist[s]].n);
         }
-    const double bytes_per_prof = (double)L * (lines_per_prof * 72.0 + n * 4.0) + n * 8.0;
+    const double bytes_per_prof = (double)L * (lines_per_prof * 168.0 + n * 4.0) + n * 8.0;
     double budget = 32.0e9;
     if (const char *e = std::getenv("IRS_GROUP_BYTES")) budget = std::max(1.0, std::atof(e));
     int gmax = (int)std::max(1.0, std::min((double)n_prof, budget / std::max(bytes_per_prof, 1.0)));
@@ -1362,6 +1369,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     HIPCHK(ensure(ctx->d_rec0, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     HIPCHK(ensure(ctx->d_rec1, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
+    HIPCHK(ensure(ctx->d_rec3, sizeof(float4) * 6 * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     const int64_t n_rec_cap = std::max<int64_t>((int64_t)group * L * lines_per_prof, 1);
     HIPCHK(ensure(ctx->d_pre, sizeof(double) * 3 * n_rec_cap));
     HIPCHK(ensure(ctx->d_tau, sizeof(float) * (size_t)n * L * group));
@@ -1430,7 +1438,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                     GroupPlan tp;
                     std::memset(&tp, 0, sizeof tp);
                     tp.items = ctx->d_items.p; tp.rec0 = ctx->d_rec0.p; tp.rec1 = ctx->d_rec1.p;
-                    tp.rec2 = ctx->d_rec2.p; tp.keys = ctx->d_keys.p; tp.tau = ctx->d_tau.p;
+                    tp.rec2 = ctx->d_rec2.p; tp.rec3 = ctx->d_rec3.p; tp.keys = ctx->d_keys.p; tp.tau = ctx->d_tau.p;
                     tp.counters = ctx->d_counters.p; tp.queue = ctx->d_queue.p; tp.lists = ctx->d_lists.p;
                     tp.tips = ctx->d_tips.p; tp.h_items = ctx->h_items.p; tp.pre = ctx->d_pre.p;
                     tp.n_rec = n_rec_cap;
@@ -1449,7 +1457,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         if (rc) return rc;
         GroupPlan pl;
         std::memset(&pl, 0, sizeof pl);
-        pl.items = ctx->d_items.p; pl.rec0 = ctx->d_rec0.p; pl.rec1 = ctx->d_rec1.p; pl.rec2 = ctx->d_rec2.p;
+        pl.items = ctx->d_items.p; pl.rec0 = ctx->d_rec0.p; pl.rec1 = ctx->d_rec1.p; pl.rec2 = ctx->d_rec2.p; pl.rec3 = ctx->d_rec3.p;
         pl.keys = ctx->d_keys.p; pl.tau = ctx->d_tau.p; pl.rhi = ctx->d_rhi.p; pl.counters = ctx->d_counters.p;
         pl.queue = ctx->d_queue.p; pl.lists = ctx->d_lists.p; pl.tips = ctx->d_tips.p; pl.order = order;
         pl.prof = ctx->d_prof.p; pl.rch = ctx->d_rch.p; pl.bt = ctx->d_bt.p; pl.chidx = ctx->d_chidx.p;

@@ -11,6 +11,10 @@
 // Line-sum tile: 256 threads x P points, thread t owns points i0 + t + 256*p.
 #define IRS_LS_THREADS 256
 
+// |z| from which the line sum evaluates w(z) by its asymptotic series (the
+// mid-core records written by the prep kernel); exact FP32 w(z) below it.
+#define IRS_Z1 8.0
+
 // One uploaded line list on the device (structure of arrays, sorted by nu).
 struct LineListDev {
     int64_t n;
@@ -47,6 +51,7 @@ struct PrepArgs {
     int64_t n_hi;
     double *pre;                   // optional [3][n_rec]: per-item inclusive prefix sums of the
     int64_t n_rec;                 // pedestal amplitude P, P c and P c^2 (c: centre, grid steps)
+    float4 *rec3;                  // optional [n_rec][6]: mid-core records (Voigt, then dispersive)
 };
 
 struct LinesumArgs {
@@ -63,6 +68,7 @@ struct LinesumArgs {
     int far_d, far_hc;             // coarse far-wing split: distance and node spacing in grid steps (0: off)
     const double *pre;             // [3][n_rec] pedestal prefix sums (PrepArgs::pre)
     int64_t n_rec;
+    const float4 *rec3;            // [n_rec][6] mid-core records (PrepArgs::rec3)
     unsigned long long *work;      // optional: [0] fine pairs evaluated, [1] coarse-node pairs,
                                    // [2], [3] the same for line-mixing (VP_Y) items
 };

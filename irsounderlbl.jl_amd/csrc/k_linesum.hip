@@ -17,7 +17,8 @@
 // sorted unshifted-centre keys into
 //   EDGE  : window boundary crosses the tile   -> per-pair window test + pedestal
 //   FAST  : far wing for every point of tile   -> A/(du^2+g^2): 3 VALU + 1 rcp
-//   CORE  : may have |x| <= x_far in the tile  -> far wing, or exact FP32 w(z)
+//   CORE  : may have |x| <= x_far in the tile  -> far wing, asymptotic series of
+//           w(z) for |z| >= 8, or exact FP32 w(z), chosen per 64-point slice
 // The interior (FAST + CORE) lines' pedestal, a quadratic in the grid offset,
 // is summed per tile in FP64 (three moments) instead of per pair.  Partial
 // sums run over 16- or 32-line chunks and are folded into Kahan-compensated
@@ -53,6 +54,18 @@ __device__ __forceinline__ int upper_key(const int32_t *k, int n, int64_t v) {
 }
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Line records read with a wave-uniform index go through the constant address
+// space, so that they are fetched by scalar loads into SGPRs (the records are
+// written by the prep kernel and only read here).
+#define IRS_CONST __attribute__((address_space(4)))
+typedef float irs_f4 __attribute__((ext_vector_type(4)));
+typedef const IRS_CONST irs_f4 *crec;
+__device__ __forceinline__ crec as_crec(const float4 *p) { return (crec)p; }
+__device__ __forceinline__ float4 ld(crec p, int64_t j) {
+    const irs_f4 v = p[j];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 // FAST: far wing, window covers the whole tile, pedestal handled per tile.
 // Line records stream through LDS in chunks of LS_CHUNK lines: each thread
@@ -136,44 +149,95 @@ __device__ __forceinline__ void fast_range(const float4 *__restrict__ r0p, const
     __syncthreads();
 }
 
-// CORE: interior lines whose Doppler core (|x| <= x_far) may fall in the tile.
-// The far-wing value is used unless a pair is in the core, where w(z) is
-// evaluated exactly; the branch is taken only by waves that hold a core pair.
-// The pedestal of these lines is part of the per-tile moments.
+// CORE: interior lines whose Voigt core (|x| <= x_far) may reach the tile.
+// For each line and each 64-point slice of a wave, the slice's distance from
+// the line centre (a wave-uniform integer, compared with the line's KO and KI
+// from its mid-core record) selects the form evaluated for the slice:
+//   far   : every point has |x| > x_far            -> Lorentz wing A/(du^2 + g^2)
+//   mid   : no point has |z| < IRS_Z1              -> asymptotic series of w(z) where
+//                                                     |x| <= x_far, Lorentz wing beyond
+//   inner : some point may have |z| < IRS_Z1       -> as mid, and exact FP32 w(z) at the
+//                                                     points with |z| < IRS_Z1
+// The series (see midcore_coeffs in the prep kernel) is a degree-8 polynomial
+// in v = 1/(x^2 + y^2) after one reciprocal, within 3.3e-7 (Re w) and 2.6e-7
+// (Im w) relative of w(z) for |z| >= 8; the exact branch is taken only by waves
+// holding a point with |z| < 8.  The pedestal of these lines is part of the
+// per-tile moments.
+__device__ __forceinline__ float series_core(float x, const float4 k0, const float4 k1, float k2) {
+    // 1 + k2 v + k3 v^2 + ... + k9 v^8 at v = 1/(x^2 + Y), Horner in FP32
+    const float v = irs_rcpf(fmaf(x, x, k1.w));
+    float q = fmaf(k1.z, v, k1.y);
+    q = fmaf(q, v, k1.x);
+    q = fmaf(q, v, k0.w);
+    q = fmaf(q, v, k0.z);
+    q = fmaf(q, v, k0.y);
+    q = fmaf(q, v, k0.x);
+    q = fmaf(q, v, k2);
+    return fmaf(q, v, 1.0f) * v;
+}
+
+// Voigt term of one CORE pair outside the far form: Sn Re w(x + iy).
+__device__ __forceinline__ float core_voigt(float du, const float4 q1, float sn, crec c, float xfar,
+                                            bool inner) {
+    const float4 k0 = ld(c, 0), k1 = ld(c, 1), k2 = ld(c, 2);
+    const float x = du * q1.z;
+    const bool in_core = fabsf(x) <= xfar;
+    float h = in_core ? k2.x * series_core(x, k0, k1, 1.5f) : k2.x * irs_rcpf(fmaf(x, x, k1.w));
+    if (inner && in_core && fmaf(x, x, k1.w) < (float)(IRS_Z1 * IRS_Z1)) {
+        float wr, wi;
+        w32(x, q1.w, wr, wi);
+        h = sn * wr;
+    }
+    return h;
+}
+
+// Dispersive term of one CORE pair outside the far form: YSn Im w(x_d + i y_d).
+__device__ __forceinline__ float core_disp(float du, const float4 q2, crec c, float xfar, bool inner) {
+    const float4 k0 = ld(c, 3), k1 = ld(c, 4), k2 = ld(c, 5);
+    const float x = du * q2.y;
+    const bool in_core = fabsf(x) <= xfar;
+    float d = in_core ? (k2.x * x) * series_core(x, k0, k1, 0.5f) : (k2.x * x) * irs_rcpf(fmaf(x, x, k1.w));
+    if (inner && in_core && fmaf(x, x, k1.w) < (float)(IRS_Z1 * IRS_Z1)) {
+        float wr, wi;
+        w32(x, q2.z, wr, wi);
+        d = q2.w * wi;
+    }
+    return d;
+}
+
 template <int P, bool LM>
-__device__ __forceinline__ void core_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
-                                           const float4 *__restrict__ r2p, int jlo, int jhi, int tl,
-                                           float xfar, float (&acc)[P], float (&cmp)[P], float (&accd)[P],
-                                           float (&cmpd)[P]) {
+__device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3p,
+                                           int jlo, int jhi, int tl, float xfar, float (&acc)[P], float (&cmp)[P],
+                                           float (&accd)[P], float (&cmpd)[P]) {
+    const int tw = uni(tl & ~63);                      // first point of this wave's slices
     for (int j0 = jlo; j0 < jhi; j0 += 16) {
         const int je = min(j0 + 16, jhi);
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
         for (int j = j0; j < je; ++j) {
-            const float4 q0 = r0p[j];
-            const float4 q1 = r1p[j];
-            const float4 q2 = r2p[j];
-            const float base = (float)(tl - __float_as_int(q0.x));
-#pragma unroll 1
+            const float4 q0 = ld(r0p, j), q1 = ld(r1p, j), q2 = ld(r2p, j), c2 = ld(r3p + 6 * (int64_t)j, 2);
+            const float4 c5 = LM ? ld(r3p + 6 * (int64_t)j, 5) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const crec c = r3p + 6 * (int64_t)j;
+            const int cidx = __float_as_int(q0.x);
+            const int KO = __float_as_int(c2.y), KI = __float_as_int(c2.z);
+            const int KOd = LM ? __float_as_int(c5.y) : -1, KId = LM ? __float_as_int(c5.z) : -1;
+            const float base = (float)(tl - cidx);
+            const int dw = tw - cidx;
+#pragma unroll
             for (int p = 0; p < P; ++p) {
+                // lower bound of |du| over the slice: du in (d0 - 1, d0 + 63]
+                const int d0 = dw + 256 * p;
+                const int mind = d0 >= 1 ? d0 - 1 : (d0 + 63 < 0 ? -(d0 + 63) : 0);
                 const float du = (base + (float)(256 * p)) - q0.y;
-                const float r = irs_rcpf(fmaf(du, du, q0.z));
-                float h = q0.w * r;
-                if (fabsf(du * q1.z) <= xfar) {
-                    float wr, wi;
-                    w32(du * q1.z, q1.w, wr, wi);
-                    h = q2.x * wr;
-                }
-                part[p] += h;
+                const bool farV = mind > KO, farD = !LM || mind > KOd;
+                float r = 0.0f;
+                if (farV || farD) r = irs_rcpf(fmaf(du, du, q0.z));
+                if (farV) part[p] = fmaf(q0.w, r, part[p]);
+                else part[p] += core_voigt(du, q1, q2.x, c, xfar, mind <= KI);
                 if (LM) {
-                    float dd = q1.x * du * r;
-                    if (fabsf(du * q2.y) <= xfar) {
-                        float wr, wi;
-                        w32(du * q2.y, q2.z, wr, wi);
-                        dd = q2.w * wi;
-                    }
-                    partd[p] += dd;
+                    if (farD) partd[p] = fmaf(q1.x * du, r, partd[p]);
+                    else partd[p] += core_disp(du, q2, c, xfar, mind <= KId);
                 }
             }
         }
@@ -187,7 +251,7 @@ __device__ __forceinline__ void core_range(const float4 *__restrict__ r0p, const
 
 // EDGE: far wing with per-pair window test and pedestal.
 template <int P, bool LM>
-__device__ __forceinline__ void edge_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+__device__ __forceinline__ void edge_range(crec r0p, crec r1p,
                                            int jlo, int jhi, int tl, float C2, float C2lm, float iC2,
                                            float (&acc)[P], float (&cmp)[P], float (&accd)[P],
                                            float (&cmpd)[P]) {
@@ -197,8 +261,7 @@ __device__ __forceinline__ void edge_range(const float4 *__restrict__ r0p, const
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
         for (int j = j0; j < je; ++j) {
-            const float4 q0 = r0p[j];
-            const float4 q1 = r1p[j];
+            const float4 q0 = ld(r0p, j), q1 = ld(r1p, j);
             const float base = (float)(tl - __float_as_int(q0.x));
 #pragma unroll
             for (int p = 0; p < P; ++p) {
@@ -221,14 +284,13 @@ __device__ __forceinline__ void edge_range(const float4 *__restrict__ r0p, const
 // Degenerate small-cutoff case (window narrower than tile + core): exact
 // branch, window test and pedestal for every pair.
 template <int P, bool LM>
-__device__ __forceinline__ void general_range(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
-                                              const float4 *__restrict__ r2p, int jlo, int jhi, int tl,
+__device__ __forceinline__ void general_range(crec r0p, crec r1p, crec r2p, int jlo, int jhi, int tl,
                                               float xfar, float C2, float C2lm, float iC2, float (&acc)[P],
                                               float (&cmp)[P], float (&accd)[P], float (&cmpd)[P]) {
     for (int j = jlo; j < jhi; ++j) {
-        const float4 q0 = r0p[j];
-        const float4 q1 = r1p[j];
-        const float4 q2 = r2p[j];
+        const float4 q0 = ld(r0p, j);
+        const float4 q1 = ld(r1p, j);
+        const float4 q2 = ld(r2p, j);
         const float base = (float)(tl - __float_as_int(q0.x));
         const bool dline = LM && (q1.x != 0.0f || q2.w != 0.0f);
 #pragma unroll
@@ -458,10 +520,10 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         if (far && (jG > jB || jE > jH))
             far_coarse<P, LM>(r0p, r1p, jB, jG, jH, jE, t, i0, Sl, HC, NC, s_far, s_rec, acc, cmp, accd, cmpd);
         fast_range<P, LM>(r0p, r1p, jG, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
-        core_range<P, LM>(r0p, r1p, r2p, jC, jD, tl, xfar, acc, cmp, accd, cmpd);
+        core_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), as_crec(a.rec3 + 6 * it.rec_off), jC, jD, tl, xfar, acc, cmp, accd, cmpd);
         fast_range<P, LM>(r0p, r1p, jD, jH, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
-        edge_range<P, LM>(r0p, r1p, jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
-        edge_range<P, LM>(r0p, r1p, jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         if (a.work && t == 0) {
             const unsigned long long fine = (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC) + (jB - jA) +
                                                                  (jF - jE)) * (unsigned long long)(i1 - i0);
@@ -474,7 +536,7 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
             }
         }
     } else {
-        general_range<P, LM>(r0p, r1p, r2p, jA, jF, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        general_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), jA, jF, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         if (a.work && t == 0) {
             atomicAdd(&a.work[0], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
             if (LM) atomicAdd(&a.work[2], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
@@ -521,8 +583,13 @@ __device__ __forceinline__ void linesum_unit(const LinesumArgs &a, int tile, int
 }
 
 // One workgroup per unit: blockIdx = (tile rank, layer, profile).
+// Occupancy: the line sum is bound by the latency of its scalar record loads
+// and LDS reads rather than by VALU issue, so the register budget is capped for
+// 32 / P resident workgroups per CU (P = 4: eight waves per SIMD, 64 VGPRs; the
+// LDS footprint of a workgroup, ~19 KB, still allows eight per CU).
+#define IRS_LS_MINB(P) (32 / (P))
 template <int P>
-__global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumArgs a) {
+__global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_kernel(const LinesumArgs a) {
     __shared__ int s_rng[12];
     __shared__ double s_red[12];
     __shared__ float4 s_rec[2 * LS_CHUNK];
@@ -538,7 +605,7 @@ __global__ void __launch_bounds__(IRS_LS_THREADS) linesum_kernel(const LinesumAr
 // exit is block-uniform and every wave of every block reaches it once the
 // queue is past the last unit.  The counter must be zero at launch.
 template <int P>
-__global__ void __launch_bounds__(IRS_LS_THREADS) linesum_persistent_kernel(const LinesumArgs a) {
+__global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_persistent_kernel(const LinesumArgs a) {
     __shared__ int s_rng[12];
     __shared__ double s_red[12];
     __shared__ float4 s_rec[2 * LS_CHUNK];
@@ -583,6 +650,7 @@ static hipError_t launch_ls_p(const LinesumArgs &a, int persistent_blocks, hipSt
 
 hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hipStream_t s) {
     if (persistent_blocks > 0 && !a.queue) return hipErrorInvalidValue;
+    if (!a.rec3 || !a.pre) return hipErrorInvalidValue;
     if (P == 16) return launch_ls_p<16>(a, persistent_blocks, s);
     if (P == 8) return launch_ls_p<8>(a, persistent_blocks, s);
     return launch_ls_p<4>(a, persistent_blocks, s);

@@ -6,7 +6,8 @@
 // weights of _append_dispersive_lines! (src/CrossSections/line_mixing.jl:1193-1220)
 // and the line-shape formulas of src/HITRAN/broadening.jl:40-160 and
 // src/HITRAN/partition.jl:10-41.  Results are packed into three float4
-// records per line, in grid-step units, for the FP32 line-sum kernel:
+// records per line, in grid-step units, for the FP32 line-sum kernel (plus the
+// optional mid-core records of midcore_coeffs):
 //   rec0 = {centre index (int bits), centre fraction, g^2, A}
 //   rec1 = {B, P, fu, y}
 //   rec2 = {Sn, fu_d, y_d, YSn}
@@ -60,6 +61,52 @@ __device__ __forceinline__ double lm_mass(int mol, int iso) {   // line_mixing.j
         }
     }
     return 44.0;
+}
+
+// Mid-core record of one line (three float4, see the CORE range of the line
+// sum): coefficients of the asymptotic series of w(x + iy) for |z| >= IRS_Z1,
+// regrouped in powers of v = 1/(x^2 + y^2) (Y = y^2):
+//   Re w = (y/sqrt(pi)) v (1 + 3/2 v + (15/4 - 2Y) v^2 + (105/8 - 15Y) v^3
+//          + (945/16 - 105Y + 12Y^2) v^4 + (210Y^2 - 1575Y/2) v^5 + (2835Y^2 - 120Y^3) v^6
+//          - 3780Y^3 v^7 + 1680Y^4 v^8)
+//   Im w = (x/sqrt(pi)) v (1 + 1/2 v + (3/4 - 2Y) v^2 + (15/8 - 9Y) v^3
+//          + (105/16 - 45Y + 12Y^2) v^4 + (150Y^2 - 525Y/2) v^5 + (1575Y^2 - 120Y^3) v^6
+//          - 2940Y^3 v^7 + 1680Y^4 v^8)
+// (w ~ (i/sqrt(pi)) sum_{n<=4} (2n-1)!!/(2^n z^{2n+1}); Im z^{2n+1} / y and
+// Re z^{2n+1} / x expanded with x^2 = 1/v - Y).  Layout:
+//   c[0] = {k3, k4, k5, k6}, c[1] = {k7, k8, k9, Y}, c[2] = {amp, KO, KI, 0}
+// with amp the factor in front of v (...): Sn y/sqrt(pi) for Re w, YSn/sqrt(pi)
+// for the dispersive record (whose series is further multiplied by x), and
+// the integer distances (grid steps) KO: |du| > KO implies |x| > x_far, KI:
+// |du| > KI implies |z| >= IRS_Z1 (KI = -1 when y >= IRS_Z1).  x_far < 0 marks
+// a line with no core (KO = KI = -1).
+__device__ void midcore_coeffs(double y, double s, double amp, double x_far, bool disp, float4 *c) {
+    const double Y = y * y;
+    double k[10];
+    if (!disp) {
+        k[3] = 3.75 - 2.0 * Y;
+        k[4] = 13.125 - 15.0 * Y;
+        k[5] = 59.0625 + Y * (12.0 * Y - 105.0);
+        k[6] = Y * (210.0 * Y - 787.5);
+        k[7] = Y * Y * (2835.0 - 120.0 * Y);
+    } else {
+        k[3] = 0.75 - 2.0 * Y;
+        k[4] = 1.875 - 9.0 * Y;
+        k[5] = 6.5625 + Y * (12.0 * Y - 45.0);
+        k[6] = Y * (150.0 * Y - 262.5);
+        k[7] = Y * Y * (1575.0 - 120.0 * Y);
+    }
+    k[8] = (disp ? -2940.0 : -3780.0) * Y * Y * Y;
+    k[9] = 1680.0 * Y * Y * Y * Y;
+    const double lim = 1.0e9;
+    int KO = -1, KI = -1;
+    if (x_far >= 0.0) {
+        KO = (int)fmin(ceil(x_far / s) + 1.0, lim);
+        if (Y < IRS_Z1 * IRS_Z1) KI = (int)fmin(ceil(sqrt(IRS_Z1 * IRS_Z1 - Y) / s) + 1.0, lim);
+    }
+    c[0] = make_float4((float)k[3], (float)k[4], (float)k[5], (float)k[6]);
+    c[1] = make_float4((float)k[7], (float)k[8], (float)k[9], (float)Y);
+    c[2] = make_float4((float)amp, __int_as_float(KO), __int_as_float(KI), 0.0f);
 }
 
 __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
@@ -125,6 +172,7 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
         r1.x = 0.0f; r2.y = 1.0f; r2.z = 1.0f; r2.w = 0.0f;
         // VP_Y dispersive weights with the S-file conventions (line_mixing.jl:1193-1220)
         bool disp = false;
+        double yd = 0.0, sd = 1.0, ysn = 0.0;
         if (it.lm && L.ylm && L.ylm[j] != 0.0) {
             const double Y = L.ylm[j];
             const double md = lm_mass(mol, iso);
@@ -140,12 +188,24 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
                 r2.z = (float)(gld * fd);
                 r2.w = (float)(coef * Y * p * st * fd * IRS_INV_SQRT_PI);
                 disp = true;
+                yd = gld * fd;
+                sd = a.dnu * fd;
+                ysn = coef * Y * p * st * fd * IRS_INV_SQRT_PI;
             }
         }
         const int64_t o = it.rec_off + j;
         a.rec0[o] = r0;
         a.rec1[o] = r1;
         a.rec2[o] = r2;
+        if (a.rec3) {
+            float4 *c = a.rec3 + 6 * o;
+            if (keep) midcore_coeffs(y, a.dnu * f, coef * snorm * y * IRS_INV_SQRT_PI, it.x_far, false, c);
+            else midcore_coeffs(0.0, 1.0, 0.0, -1.0, false, c);
+            if (it.lm) {
+                if (disp) midcore_coeffs(yd, sd, ysn * IRS_INV_SQRT_PI, it.x_far, true, c + 3);
+                else midcore_coeffs(0.0, 1.0, 0.0, -1.0, true, c + 3);
+            }
+        }
         if (keep) {
             my_active = 1;
             const double lo = ceil((c - it.cutoff - a.nu_lo) / a.dnu);
