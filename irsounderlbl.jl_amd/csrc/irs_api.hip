@@ -472,12 +472,12 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_rec3, sizeof(float4) * 6 * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_items, sizeof(ItemDev) * std::max<size_t>(items.size(), 1)));
-    HIPCHK(ensure(ctx->d_counters, 64));
+    HIPCHK(ensure(ctx->d_counters, sizeof(unsigned long long) * IRS_NCNT));
     HIPCHK(ensure(ctx->d_queue, 64));
     HIPCHK(ensure(ctx->d_pre, sizeof(double) * 3 * std::max<int64_t>(rec, 1)));
     HIPCHK(hipMemcpyAsync(ctx->d_items.p, items.data(), sizeof(ItemDev) * items.size(), hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(launch_reset_counters((unsigned *)ctx->d_queue.p, (unsigned long long *)ctx->d_counters.p, 8,
+    HIPCHK(launch_reset_counters((unsigned *)ctx->d_queue.p, (unsigned long long *)ctx->d_counters.p, IRS_NCNT,
                                  ctx->stream));
     PrepArgs pa{};
     pa.items = (const ItemDev *)ctx->d_items.p;
@@ -528,8 +528,9 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
 }
 
 void read_counters(irs_ctx *ctx) {
-    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (hipMemcpy(c, ctx->d_counters.p, sizeof(c), hipMemcpyDeviceToHost) == hipSuccess) {
+    unsigned long long cs[IRS_NCNT], c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMemcpy(cs, ctx->d_counters.p, sizeof(cs), hipMemcpyDeviceToHost) == hipSuccess) {
+        for (int i = 0; i < IRS_NCNT; ++i) c[i % 8] += cs[i];
         ctx->stats.active_lines += (int64_t)c[0];
         ctx->stats.active_pairs += (int64_t)c[1];
         ctx->stats.dispersive_pairs += (int64_t)c[2];
@@ -1035,7 +1036,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     } while (0)
     ENQ(hipMemcpyAsync((void *)pl.items, pl.h_items, (size_t)pl.items_bytes, hipMemcpyHostToDevice, st));
     ENQ(hipMemcpyAsync((void *)pl.prof, pl.h_prof, (size_t)pl.prof_bytes, hipMemcpyHostToDevice, st));
-    ENQ(launch_reset_counters((unsigned *)pl.queue, (unsigned long long *)pl.counters, 8, st));
+    ENQ(launch_reset_counters((unsigned *)pl.queue, (unsigned long long *)pl.counters, IRS_NCNT, st));
     PrepArgs pa{};
     pa.items = (const ItemDev *)pl.items;
     pa.lists = (const LineListDev *)pl.lists;
@@ -1194,7 +1195,7 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
                    const std::string &key) {
     hipStream_t st = ctx->stream;
     HIPCHK(hipMemcpyAsync((void *)base.items, base.h_items, (size_t)base.items_bytes, hipMemcpyHostToDevice, st));
-    HIPCHK(launch_reset_counters((unsigned *)base.queue, (unsigned long long *)base.counters, 3, st));
+    HIPCHK(launch_reset_counters((unsigned *)base.queue, (unsigned long long *)base.counters, IRS_NCNT, st));
     PrepArgs pa{};
     pa.items = (const ItemDev *)base.items;
     pa.lists = (const LineListDev *)base.lists;
@@ -1376,7 +1377,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     HIPCHK(ensure(ctx->d_rhi, sizeof(double) * (size_t)n * group));
     HIPCHK(ensure(ctx->d_rch, sizeof(double) * nch * group));
     HIPCHK(ensure(ctx->d_bt, sizeof(double) * nch * group));
-    HIPCHK(ensure(ctx->d_counters, 64));
+    HIPCHK(ensure(ctx->d_counters, sizeof(unsigned long long) * IRS_NCNT));
     HIPCHK(ensure(ctx->d_queue, 64));
     HIPCHK(ensure(ctx->d_prof, sizeof(double) * prof_doubles));
     HIPCHK(ensure_pinned(ctx->h_items, sizeof(ItemDev) * (size_t)group * L * ns));

@@ -8,6 +8,14 @@
 #define IRS_INV_SQRT_PI 0.56418958354775628695
 #define IRS_PI          3.14159265358979323846
 
+// Work counters: IRS_CNT_SLOTS copies of 8 counters (the host sums the copies);
+// each workgroup adds into the copy of its slot, so that concurrent atomics are
+// spread over many addresses instead of serialising on one.
+//   [0] active lines, [1] Voigt pairs, [2] dispersive pairs (prep)
+//   [4] fine pairs, [5] coarse-node pairs, [6], [7] the same for VP_Y items (line sum)
+#define IRS_CNT_SLOTS 16
+#define IRS_NCNT (8 * IRS_CNT_SLOTS)
+
 // Line-sum tile: 256 threads x P points, thread t owns points i0 + t + 256*p.
 #define IRS_LS_THREADS 256
 
@@ -47,7 +55,7 @@ struct PrepArgs {
     double nu_lo, dnu;
     float4 *rec0, *rec1, *rec2;
     int32_t *keys;                 // per list, floor((nu - nu_lo)/dnu)
-    unsigned long long *counters;  // [0] active lines, [1] voigt pairs, [2] dispersive pairs
+    unsigned long long *counters;  // [IRS_CNT_SLOTS][8] work counters (slots 0..2 of each copy)
     int64_t n_hi;
     double *pre;                   // optional [3][n_rec]: per-item inclusive prefix sums of the
     int64_t n_rec;                 // pedestal amplitude P, P c and P c^2 (c: centre, grid steps)
@@ -69,8 +77,8 @@ struct LinesumArgs {
     const double *pre;             // [3][n_rec] pedestal prefix sums (PrepArgs::pre)
     int64_t n_rec;
     const float4 *rec3;            // [n_rec][6] mid-core records (PrepArgs::rec3)
-    unsigned long long *work;      // optional: [0] fine pairs evaluated, [1] coarse-node pairs,
-                                   // [2], [3] the same for line-mixing (VP_Y) items
+    unsigned long long *work;      // optional: counters + 4; per copy [0] fine pairs evaluated,
+                                   // [1] coarse-node pairs, [2], [3] the same for VP_Y items
 };
 
 struct RteArgs {

@@ -525,21 +525,23 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         if (a.work && t == 0) {
+            unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
             const unsigned long long fine = (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC) + (jB - jA) +
                                                                  (jF - jE)) * (unsigned long long)(i1 - i0);
             const unsigned long long coarse = far ? (unsigned long long)((jG - jB) + (jE - jH)) * NC : 0ull;
-            atomicAdd(&a.work[0], fine);
-            atomicAdd(&a.work[1], coarse);
+            atomicAdd(&w[0], fine);
+            if (coarse) atomicAdd(&w[1], coarse);
             if (LM) {
-                atomicAdd(&a.work[2], fine);
-                atomicAdd(&a.work[3], coarse);
+                atomicAdd(&w[2], fine);
+                if (coarse) atomicAdd(&w[3], coarse);
             }
         }
     } else {
         general_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), jA, jF, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
         if (a.work && t == 0) {
-            atomicAdd(&a.work[0], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
-            if (LM) atomicAdd(&a.work[2], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
+            unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
+            atomicAdd(&w[0], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
+            if (LM) atomicAdd(&w[2], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
         }
     }
 #pragma unroll
@@ -631,7 +633,8 @@ __global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_persis
 __global__ void reset_counters_kernel(unsigned *queue, unsigned long long *counters, int n_counters) {
     const int t = threadIdx.x;
     if (t == 0 && queue) *queue = 0u;
-    if (t < n_counters && counters) counters[t] = 0ull;
+    if (counters)
+        for (int i = t; i < n_counters; i += 64) counters[i] = 0ull;
 }
 
 hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s) {

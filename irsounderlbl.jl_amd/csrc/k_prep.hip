@@ -221,16 +221,25 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
             if (h2 >= l2) my_dpairs = (unsigned long long)(h2 - l2 + 1.0);
         }
     }
-    // block reduction of the work counters
+    // block reduction of the work counters, one atomic per counter and block
+    __shared__ unsigned long long red[3][4];
     for (int o = 32; o > 0; o >>= 1) {
         my_active += __shfl_xor(my_active, o, 64);
         my_pairs += __shfl_xor(my_pairs, o, 64);
         my_dpairs += __shfl_xor(my_dpairs, o, 64);
     }
-    if ((threadIdx.x & 63) == 0 && a.counters && (my_active | my_pairs | my_dpairs)) {
-        atomicAdd(&a.counters[0], my_active);
-        atomicAdd(&a.counters[1], my_pairs);
-        atomicAdd(&a.counters[2], my_dpairs);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wv] = my_active;
+        red[1][wv] = my_pairs;
+        red[2][wv] = my_dpairs;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 && a.counters) {
+        const unsigned long long v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                                     red[threadIdx.x][3];
+        const int slot = (int)((blockIdx.x + blockIdx.y) % IRS_CNT_SLOTS);
+        if (v) atomicAdd(&a.counters[8 * slot + threadIdx.x], v);
     }
 }
 
