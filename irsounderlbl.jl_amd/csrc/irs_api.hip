@@ -335,6 +335,7 @@ ItemDev make_item(const ListHost &lh, int32_t list, double T, double p_atm, doub
     it.dptmn = dptmn;
     it.cutoff = cutoff;
     it.x_far = x_far;
+    it.lm_cutoff = lm_cutoff;
     return it;
 }
 
@@ -1656,7 +1657,7 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
             it.n = base.n; it.key_off = koff[h]; it.list = h; it.lm = base.lm; it.active = 1; it.M = base.M;
             it.C = base.C; it.Clm = base.Clm;
             it.T = T; it.p_atm = p_atm; it.vmr_self = vs; it.coef = base.coef; it.cutoff = opt.cutoff;
-            it.x_far = opt.x_far;
+            it.x_far = opt.x_far; it.lm_cutoff = opt.lm_cutoff;
             const double p1 = P[k], p2 = P[k + 1];
             const double wT = opt.T_method == IRS_TMETHOD_MASS_WEIGHTED ? cg_mass_frac(p1, p2)
                                                                          : std::log(lp.p_cg[s][k] / p1) / std::log(p2 / p1);
@@ -1715,6 +1716,7 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     sa.n_layers = L;
     sa.n_species = ns;
     sa.n_retr = nr;
+    sa.dnu = dnu;
     sa.tau = (float *)ctx->j_tau.p;
     sa.dtlo = (float *)ctx->j_dt.p;
     sa.dthi = sa.dtlo + NL;

@@ -22,6 +22,16 @@
 
 __device__ __forceinline__ float irs_rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// |du| <= C for du = dI - frac (dI the exact integer offset of the grid point
+// from the line's centre index, frac the centre's fraction) and C = Ci + Cf:
+// decided on the exact integer parts and the FP32 fractions, so the window
+// boundary is resolved to ~1e-7 grid steps (an FP32 |du| near a 25000-step
+// cutoff resolves only 2e-3).  Used for the VP_Y dispersive window, whose
+// term does not vanish at the boundary.
+__device__ __forceinline__ bool in_window(int dI, float frac, int Ci, float Cf) {
+    return (float)(dI - Ci) <= frac + Cf && (float)(dI + Ci) >= frac - Cf;
+}
+
 __device__ __forceinline__ void w32_trap(float x, float y, float &re, float &im) {
     const float h = 0.7f, inv_h = 1.0f / 0.7f, tmax = 4.2f;
     const int klo = (int)ceilf((x - tmax) * inv_h - 0.5f);
@@ -67,6 +77,21 @@ __device__ __forceinline__ void w32(float x, float y, float &re, float &im) {
     const float ax = fabsf(x);
     float r, i;
     if (ax + y < 5.5f) w32_trap(ax, y, r, i);
+    else w32_cf(ax, y, r, i);
+    re = r;
+    im = x < 0.0f ? -i : i;
+}
+
+// w(x + iy) for x^2 + y^2 < 64 (the inner core of the line sum): the
+// trapezoid rule for y < 6 and the continued fraction (depth 6/4 by s) for
+// y >= 6.  Within that disc the float32 model of each branch stays within
+// 1.5e-6 (trapezoid, Re w) and 2.6e-7 (fraction) relative of w(z); the branch
+// depends on y only, which is one value per line, so the lanes of a wave that
+// evaluate one line's inner core do not split between the two methods.
+__device__ __forceinline__ void w32_core(float x, float y, float &re, float &im) {
+    const float ax = fabsf(x);
+    float r, i;
+    if (y < 6.0f) w32_trap(ax, y, r, i);
     else w32_cf(ax, y, r, i);
     re = r;
     im = x < 0.0f ? -i : i;

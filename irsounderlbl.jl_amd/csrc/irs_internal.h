@@ -45,6 +45,7 @@ struct ItemDev {
     float   C, Clm;         // cutoff / lm_cutoff in grid steps
     double  kcore_per_nu;   // x_far * max Doppler coefficient * sqrt(T) / (sqrt(ln2) * dnu)
     double  T, p_atm, vmr_self, coef, dptmn, cutoff, x_far;
+    double  lm_cutoff;      // cm^-1 (the VP_Y window; lm_cutoff / dnu = Clm)
 };
 
 struct PrepArgs {

@@ -14,6 +14,7 @@ struct JacItemDev {
     int32_t n, key_off, list, lm, active, M;
     float C, Clm;
     double T, p_atm, vmr_self, coef, cutoff, x_far;
+    double lm_cutoff;             // cm^-1, the VP_Y window (Clm = lm_cutoff / dnu)
     double wT_lo, wT_hi;          // dT_cg/dT_lev at the lower / upper level
     int32_t retr, pad;            // retrieved-species index or -1
     double gv_lo, gv_hi;          // d vmr_cg / d v_level / vmr_cg   (number-density term)
@@ -38,6 +39,7 @@ struct JacSumArgs {
     int64_t n_hi;
     int n_layers, n_species, n_retr;
     float *tau, *dtlo, *dthi, *vlo, *vhi;   // [L][n]; vlo/vhi [r][L][n]
+    double dnu;                   // internal grid step (cm^-1)
 };
 
 struct RteJacArgs {

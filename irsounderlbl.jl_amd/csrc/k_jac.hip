@@ -202,7 +202,10 @@ __global__ void __launch_bounds__(256) grad_linesum_kernel(const JacSumArgs a) {
         __syncthreads();
         const float4 *g0 = a.g0 + it.rec_off, *g1 = a.g1 + it.rec_off, *g2 = a.g2 + it.rec_off;
         const float4 *g3 = a.g3 + it.rec_off, *g4 = a.g4 + it.rec_off, *g5 = a.g5 + it.rec_off;
-        const float C2 = (float)((double)it.C * it.C), C2lm = (float)((double)it.Clm * it.Clm);
+        const float C2 = (float)((double)it.C * it.C);
+        const double Cld = fmin(it.lm_cutoff / a.dnu, 1.0e9);
+        const int CiD = (int)floor(Cld);
+        const float CfD = (float)(Cld - (double)CiD);
         const float iC2 = (float)(1.0 / ((double)it.C * it.C));
         const float xfar = (float)it.x_far;
         const float ispi = 0.56418958354775628f;
@@ -248,7 +251,7 @@ __global__ void __launch_bounds__(256) grad_linesum_kernel(const JacSumArgs a) {
                     kadd(sP[p], cP[p], q1.x * (dHp - (dpfp * q3.x + pedfac * q3.z)));
                     kadd(sV[p], cV[p], q1.x * (dHv - (dpfv * q3.x + pedfac * q3.w)));
                 }
-                if (dl && d2 <= C2lm) {
+                if (dl && in_window(tl - __float_as_int(q0.x) + 256 * p, q0.y, CiD, CfD)) {
                     const float4 q4 = g4[j], q5 = g5[j];
                     const float xd = du * q4.x, yd = q4.y;
                     float Lv, Lx, Ly;

@@ -185,7 +185,7 @@ __device__ __forceinline__ float core_voigt(float du, const float4 q1, float sn,
     float h = in_core ? k2.x * series_core(x, k0, k1, 1.5f) : k2.x * irs_rcpf(fmaf(x, x, k1.w));
     if (inner && in_core && fmaf(x, x, k1.w) < (float)(IRS_Z1 * IRS_Z1)) {
         float wr, wi;
-        w32(x, q1.w, wr, wi);
+        w32_core(x, q1.w, wr, wi);
         h = sn * wr;
     }
     return h;
@@ -199,7 +199,7 @@ __device__ __forceinline__ float core_disp(float du, const float4 q2, crec c, fl
     float d = in_core ? (k2.x * x) * series_core(x, k0, k1, 0.5f) : (k2.x * x) * irs_rcpf(fmaf(x, x, k1.w));
     if (inner && in_core && fmaf(x, x, k1.w) < (float)(IRS_Z1 * IRS_Z1)) {
         float wr, wi;
-        w32(x, q2.z, wr, wi);
+        w32_core(x, q2.z, wr, wi);
         d = q2.w * wi;
     }
     return d;
@@ -215,6 +215,10 @@ __device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
+#ifndef IRS_CORE_UNROLL
+#define IRS_CORE_UNROLL 1
+#endif
+#pragma unroll IRS_CORE_UNROLL
         for (int j = j0; j < je; ++j) {
             const float4 q0 = ld(r0p, j), q1 = ld(r1p, j), q2 = ld(r2p, j), c2 = ld(r3p + 6 * (int64_t)j, 2);
             const float4 c5 = LM ? ld(r3p + 6 * (int64_t)j, 5) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -252,7 +256,7 @@ __device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3
 // EDGE: far wing with per-pair window test and pedestal.
 template <int P, bool LM>
 __device__ __forceinline__ void edge_range(crec r0p, crec r1p,
-                                           int jlo, int jhi, int tl, float C2, float C2lm, float iC2,
+                                           int jlo, int jhi, int tl, float C2, int CiD, float CfD, float iC2,
                                            float (&acc)[P], float (&cmp)[P], float (&accd)[P],
                                            float (&cmpd)[P]) {
     for (int j0 = jlo; j0 < jhi; j0 += 16) {
@@ -260,6 +264,10 @@ __device__ __forceinline__ void edge_range(crec r0p, crec r1p,
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
+#ifndef IRS_EDGE_UNROLL
+#define IRS_EDGE_UNROLL 4
+#endif
+#pragma unroll IRS_EDGE_UNROLL
         for (int j = j0; j < je; ++j) {
             const float4 q0 = ld(r0p, j), q1 = ld(r1p, j);
             const float base = (float)(tl - __float_as_int(q0.x));
@@ -270,7 +278,9 @@ __device__ __forceinline__ void edge_range(crec r0p, crec r1p,
                 const float r = irs_rcpf(d2 + q0.z);
                 const float v = fmaf(q0.w, r, -q1.y * fmaf(-d2, iC2, 2.0f));
                 part[p] += d2 <= C2 ? v : 0.0f;
-                if (LM) partd[p] += d2 <= C2lm ? q1.x * du * r : 0.0f;
+                if (LM)
+                    partd[p] += in_window(tl - __float_as_int(q0.x) + 256 * p, q0.y, CiD, CfD) ? q1.x * du * r
+                                                                                                 : 0.0f;
             }
         }
 #pragma unroll
@@ -281,11 +291,128 @@ __device__ __forceinline__ void edge_range(crec r0p, crec r1p,
     }
 }
 
+// EDGE lines by local expansion.  An EDGE line lies about one cutoff from the
+// tile (the expansion is used when every EDGE line is at least
+// EDGE_EXPAND_MIN grid steps from every slice centre), so on a 64-point slice
+// its far wing and pedestal are, to within 5 (32/d)^4 of its Lorentz value at
+// slice distance d (2e-8 at d = 4000; 2e-11 for a 25 cm^-1 cutoff on a
+// 0.001 cm^-1 grid), the cubic in the offset delta from the slice centre
+//   A/((d+delta)^2 + g^2) - P (2 - (d+delta)^2/C^2)
+//     = sum_k a_k delta^k,  a_0 = A/D - P (2 - d^2/C^2),  a_1 = -2dA/D^2 + 2Pd/C^2,
+//       a_2 = A (3d^2 - g^2)/D^3 + P/C^2,  a_3 = -4dA (d^2 - g^2)/D^4     (D = d^2 + g^2)
+// and the dispersive wing B (d+delta)/((d+delta)^2 + g^2) the cubic
+//       b_0 = Bd/D,  b_1 = B (g^2 - d^2)/D^2,  b_2 = Bd (d^2 - 3g^2)/D^3,
+//       b_3 = -B (d^4 - 6d^2g^2 + g^4)/D^4.
+// Lanes take one line each: per slice of this wave, a line whose window covers
+// the whole slice adds its coefficients to the lane's partial sums, a line
+// whose window misses the slice adds nothing, and a line whose window boundary
+// falls inside the slice is evaluated pair by pair with the per-point test (as
+// in edge_range) in a wave-uniform loop over those lines.  The coefficient
+// sums are reduced across the wave and every point evaluates its slice's cubic.
+#ifndef EDGE_EXPAND_MIN
+#define EDGE_EXPAND_MIN 4000
+#endif
+template <int P, bool LM>
+__device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+                                            int jl0, int jl1, int jr0, int jr1, int t, int tl, float C, float Clm,
+                                            float C2, int CiD, float CfD, float iC2, float (&acc)[P], float (&cmp)[P],
+                                            float (&accd)[P], float (&cmpd)[P]) {
+    const int lane = t & 63;
+    const int tw = uni(tl & ~63);
+    const int nL = jl1 - jl0, ntot = nL + (jr1 - jr0);
+    if (ntot <= 0) return;
+    float a0[P], a1[P], a2[P], a3[P], b0[P], b1[P], b2[P], b3[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) a0[p] = a1[p] = a2[p] = a3[p] = b0[p] = b1[p] = b2[p] = b3[p] = 0.0f;
+    const float inV = C - 31.55f, outV = C + 31.55f, inD = Clm - 31.55f, outD = Clm + 31.55f;
+    for (int g0 = 0; g0 < ntot; g0 += 64) {
+        const int v = g0 + lane;
+        const bool ok = v < ntot;
+        const int j = ok ? (v < nL ? jl0 + v : jr0 + (v - nL)) : (nL > 0 ? jl0 : jr0);
+        const float4 q0 = r0p[j];
+        const float4 q1 = r1p[j];
+        const int cidx = __float_as_int(q0.x);
+        const float A = ok ? q0.w : 0.0f, Pd = ok ? q1.y : 0.0f, B = ok ? q1.x : 0.0f, g2 = q0.z;
+        const float cfr = 31.5f - q0.y;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const float d = (float)(tw + 256 * p - cidx) + cfr;     // slice centre - line centre
+            const float ad = fabsf(d);
+            const bool fullV = ad <= inV, bndV = ok && !fullV && ad < outV;
+            const float d2 = d * d;
+            const float iD = irs_rcpf(d2 + g2);
+            const float iD2 = iD * iD;
+            if (fullV) {
+                const float L = A * iD;
+                a0[p] += L - Pd * fmaf(-d2, iC2, 2.0f);
+                a1[p] += 2.0f * d * (Pd * iC2 - L * iD);
+                a2[p] = fmaf(L * iD2, 3.0f * d2 - g2, fmaf(Pd, iC2, a2[p]));
+                a3[p] = fmaf(-4.0f * d * L * iD2 * iD, d2 - g2, a3[p]);
+            }
+            bool bndD = false;
+            if (LM) {
+                const bool fullD = ad <= inD;
+                bndD = ok && !fullD && ad < outD;
+                if (fullD) {
+                    const float Bi = B * iD;
+                    b0[p] = fmaf(Bi, d, b0[p]);
+                    b1[p] = fmaf(Bi * iD, g2 - d2, b1[p]);
+                    b2[p] = fmaf(Bi * iD2 * d, d2 - 3.0f * g2, b2[p]);
+                    b3[p] = fmaf(-Bi * iD2 * iD, fmaf(d2, d2 - 6.0f * g2, g2 * g2), b3[p]);
+                }
+            }
+            // lines whose window boundary falls in this slice: pair by pair
+            unsigned long long mV = __ballot(bndV), mD = LM ? __ballot(bndD) : 0ull;
+            unsigned long long m = mV | mD;
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                const int jj = uni(__shfl(j, l, 64));
+                const float4 e0 = r0p[jj], e1 = r1p[jj];
+                const float du = ((float)(tl - __float_as_int(e0.x)) + (float)(256 * p)) - e0.y;
+                const float dd2 = du * du;
+                const float r = irs_rcpf(dd2 + e0.z);
+                if ((mV >> l) & 1ull) {
+                    const float w = fmaf(e0.w, r, -e1.y * fmaf(-dd2, iC2, 2.0f));
+                    kahan_add(acc[p], cmp[p], dd2 <= C2 ? w : 0.0f);
+                }
+                if (LM && ((mD >> l) & 1ull)) {
+                    const bool in = in_window(tl - __float_as_int(e0.x) + 256 * p, e0.y, CiD, CfD);
+                    kahan_add(accd[p], cmpd[p], in ? e1.x * du * r : 0.0f);
+                }
+            }
+        }
+    }
+    // wave sums of the coefficients, then the cubic at each point
+    const float dl = (float)lane - 31.5f;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        float c0 = a0[p], c1 = a1[p], c2 = a2[p], c3 = a3[p];
+        for (int o = 32; o > 0; o >>= 1) {
+            c0 += __shfl_xor(c0, o, 64);
+            c1 += __shfl_xor(c1, o, 64);
+            c2 += __shfl_xor(c2, o, 64);
+            c3 += __shfl_xor(c3, o, 64);
+        }
+        kahan_add(acc[p], cmp[p], fmaf(fmaf(fmaf(c3, dl, c2), dl, c1), dl, c0));
+        if (LM) {
+            float e0 = b0[p], e1 = b1[p], e2 = b2[p], e3 = b3[p];
+            for (int o = 32; o > 0; o >>= 1) {
+                e0 += __shfl_xor(e0, o, 64);
+                e1 += __shfl_xor(e1, o, 64);
+                e2 += __shfl_xor(e2, o, 64);
+                e3 += __shfl_xor(e3, o, 64);
+            }
+            kahan_add(accd[p], cmpd[p], fmaf(fmaf(fmaf(e3, dl, e2), dl, e1), dl, e0));
+        }
+    }
+}
+
 // Degenerate small-cutoff case (window narrower than tile + core): exact
 // branch, window test and pedestal for every pair.
 template <int P, bool LM>
 __device__ __forceinline__ void general_range(crec r0p, crec r1p, crec r2p, int jlo, int jhi, int tl,
-                                              float xfar, float C2, float C2lm, float iC2, float (&acc)[P],
+                                              float xfar, float C2, int CiD, float CfD, float iC2, float (&acc)[P],
                                               float (&cmp)[P], float (&accd)[P], float (&cmpd)[P]) {
     for (int j = jlo; j < jhi; ++j) {
         const float4 q0 = ld(r0p, j);
@@ -318,7 +445,7 @@ __device__ __forceinline__ void general_range(crec r0p, crec r1p, crec r2p, int 
                     w32(xd, q2.z, wr, wi);
                     dd = q2.w * wi;
                 }
-                if (d2 <= C2lm) kahan_add(accd[p], cmpd[p], dd);
+                if (in_window(tl - __float_as_int(q0.x) + 256 * p, q0.y, CiD, CfD)) kahan_add(accd[p], cmpd[p], dd);
             }
         }
     }
@@ -341,27 +468,34 @@ __device__ __forceinline__ double block_sum(double v, double *red, int t) {
 // interpolation error is below 2.8 (HC/D)^4 of the line's contribution (7e-7 at
 // D = 1.5 cm^-1, HC = 0.032 cm^-1); the pedestal of these lines stays in the
 // per-tile moments.  The far lines stream through LDS in chunks of LS_CHUNK
-// ({centre index relative to node 0, fraction, g^2, A} and B); with S = 256 / NC
-// slices, thread t < S*NC sums the chunk's lines k = s, s+S, ... (s = t / NC) for
-// node n = t % NC (du exact to one FP32 rounding), and the S partial sums per
-// node are combined in LDS.
+// ({centre index relative to node 0, fraction, g^2, A} and B).  Thread t
+// evaluates the four consecutive nodes 4g .. 4g+3 (g = t % NG, NG = ceil(NC/4))
+// for the chunk's lines k = s, s+S, ... (s = t / NG, S = 256 / NG slices), so
+// one LDS record and one index conversion serve four nodes; the offset to node
+// 4g+m is formed by m FP32 additions of HC (relative error of du <= 4 ulp).  The S partial
+// sums per node are combined in FP64 through the (then idle) staging buffer.
 #define FAR_MAXNC 132
-#define FAR_LDS (2 * IRS_LS_THREADS + 2 * FAR_MAXNC)
+#define FAR_LDS (2 * FAR_MAXNC)
 
 template <int P, bool LM>
 __device__ __forceinline__ void far_coarse(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p, int jl0,
                                            int jl1, int jr0, int jr1, int t, int64_t i0, int64_t Sl, int HC,
                                            int NC, float *s_far, float4 *s_rec, float (&acc)[P], float (&cmp)[P],
                                            float (&accd)[P], float (&cmpd)[P]) {
-    const int S = IRS_LS_THREADS / NC;                 // >= 1 since NC <= 132
-    const int n = t % NC, sl = t / NC;
-    const bool act = t < S * NC;
+    const int NG = (NC + 3) >> 2;                      // node groups, <= 33
+    const int S = IRS_LS_THREADS / NG;                 // >= 7
+    const int g = t % NG, sl = t / NG;
+    const bool act = t < S * NG;
     const int nL = jl1 - jl0, ntot = nL + (jr1 - jr0);
-    const int xrel = n * HC;                           // node offset from Sl
-    float a = 0.0f, c = 0.0f, ad = 0.0f, cd = 0.0f;
+    const float xrel = (float)(4 * g * HC);           // offset of node 4g from Sl
+    const float fhc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)HC)));
+    float a[4], c[4], ad[4], cd[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a[m] = c[m] = ad[m] = cd[m] = 0.0f;
     for (int base = 0; base < ntot; base += LS_CHUNK) {
         const int cnt = min(LS_CHUNK, ntot - base);
-        // stage the chunk: {centre index - Sl (int bits), fraction, g^2, A} and B
+        // stage the chunk: {centre index - Sl, fraction, g^2, A} and B; the index
+        // difference is below 2^24 in magnitude, so it is exact as a float
 #pragma unroll
         for (int u = 0; u < LS_PER_THREAD; ++u) {
             const int k = t + u * IRS_LS_THREADS;
@@ -369,43 +503,61 @@ __device__ __forceinline__ void far_coarse(const float4 *__restrict__ r0p, const
                 const int v = base + k;
                 const int j = v < nL ? jl0 + v : jr0 + (v - nL);
                 const float4 q = r0p[j];
-                s_rec[k] = make_float4(__int_as_float(__float_as_int(q.x) - (int)Sl), q.y, q.z, q.w);
+                s_rec[k] = make_float4((float)(__float_as_int(q.x) - (int)Sl), q.y, q.z, q.w);
                 if (LM) s_rec[LS_CHUNK + k].x = r1p[j].x;
             }
         }
         __syncthreads();
         if (act) {
-            float part = 0.0f, partd = 0.0f;
-            int m = 0;
+            float part[4], partd[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) part[m] = partd[m] = 0.0f;
+            int n = 0;
             for (int k = sl; k < cnt; k += S) {
                 const float4 q = s_rec[k];
-                const float du = (float)(xrel - __float_as_int(q.x)) - q.y;
-                const float r = irs_rcpf(fmaf(du, du, q.z));
-                part = fmaf(q.w, r, part);
-                if (LM) partd = fmaf(s_rec[LS_CHUNK + k].x * du, r, partd);
-                if (++m == 32) {
-                    kahan_add(a, c, part);
-                    if (LM) kahan_add(ad, cd, partd);
-                    part = partd = 0.0f;
-                    m = 0;
+                const float b = LM ? s_rec[LS_CHUNK + k].x : 0.0f;
+                float du = (xrel - q.x) - q.y;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (m) du += fhc;
+                    const float r = irs_rcpf(fmaf(du, du, q.z));
+                    part[m] = fmaf(q.w, r, part[m]);
+                    if (LM) partd[m] = fmaf(b * du, r, partd[m]);
+                }
+                if (++n == 32) {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        kahan_add(a[m], c[m], part[m]);
+                        if (LM) kahan_add(ad[m], cd[m], partd[m]);
+                        part[m] = partd[m] = 0.0f;
+                    }
+                    n = 0;
                 }
             }
-            kahan_add(a, c, part);
-            if (LM) kahan_add(ad, cd, partd);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                kahan_add(a[m], c[m], part[m]);
+                if (LM) kahan_add(ad[m], cd[m], partd[m]);
+            }
         }
         __syncthreads();
     }
+    // per-(slice, node) partial sums in the staging buffer: [S][4 NG] twice (<= 2048 floats)
+    float *red = (float *)s_rec, *redd = red + S * 4 * NG;
     if (act) {
-        s_far[t] = a - c;
-        s_far[IRS_LS_THREADS + t] = ad - cd;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            red[sl * 4 * NG + 4 * g + m] = a[m] - c[m];
+            redd[sl * 4 * NG + 4 * g + m] = ad[m] - cd[m];
+        }
     }
     __syncthreads();
-    float *F = s_far + 2 * IRS_LS_THREADS, *Fd = F + FAR_MAXNC;
+    float *F = s_far, *Fd = s_far + FAR_MAXNC;
     if (t < NC) {
         double v = 0.0, vd = 0.0;
         for (int q = 0; q < S; ++q) {
-            v += (double)s_far[q * NC + t];
-            vd += (double)s_far[IRS_LS_THREADS + q * NC + t];
+            v += (double)red[q * 4 * NG + t];
+            vd += (double)redd[q * 4 * NG + t];
         }
         F[t] = (float)v;
         Fd[t] = (float)vd;
@@ -494,7 +646,10 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     const int64_t imid = i0 + 128 * P;
     const int tl = (int)(i0 + t);
     const float C2 = (float)((double)it.C * (double)it.C);
-    const float C2lm = (float)((double)it.Clm * (double)it.Clm);
+    // the dispersive window Clm = lm_cutoff / dnu as integer part + FP32 fraction (in_window)
+    const double Cld = fmin(it.lm_cutoff / dnu, 1.0e9);
+    const int CiD = (int)floor(Cld);
+    const float CfD = (float)(Cld - (double)CiD);
     const double iC2d = 1.0 / (Cd * Cd);
     const float iC2 = (float)iC2d;
     const float xfar = (float)it.x_far;
@@ -522,8 +677,14 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         fast_range<P, LM>(r0p, r1p, jG, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         core_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), as_crec(a.rec3 + 6 * it.rec_off), jC, jD, tl, xfar, acc, cmp, accd, cmpd);
         fast_range<P, LM>(r0p, r1p, jD, jH, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
-        edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jA, jB, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
-        edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jE, jF, tl, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        // EDGE lines: local expansion when every one of them is far from the tile
+        if (Cmn - (double)(256 * P + HC + M + 34) >= (double)EDGE_EXPAND_MIN) {
+            edge_expand<P, LM>(r0p, r1p, jA, jB, jE, jF, t, tl, it.C, it.Clm, C2, CiD, CfD, iC2, acc, cmp, accd,
+                               cmpd);
+        } else {
+            edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jA, jB, tl, C2, CiD, CfD, iC2, acc, cmp, accd, cmpd);
+            edge_range<P, LM>(as_crec(r0p), as_crec(r1p), jE, jF, tl, C2, CiD, CfD, iC2, acc, cmp, accd, cmpd);
+        }
         if (a.work && t == 0) {
             unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
             const unsigned long long fine = (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC) + (jB - jA) +
@@ -537,7 +698,7 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
             }
         }
     } else {
-        general_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), jA, jF, tl, xfar, C2, C2lm, iC2, acc, cmp, accd, cmpd);
+        general_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), jA, jF, tl, xfar, C2, CiD, CfD, iC2, acc, cmp, accd, cmpd);
         if (a.work && t == 0) {
             unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
             atomicAdd(&w[0], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));

@@ -136,3 +136,39 @@ def test_coarse_far_wing_matches_exact_sum(ctx, lm):
            else o.voigt_xsec(sub, g.nu_min, g.dnu, g.n, T, p, 0.0))
     mr = ref > 1e-6 * ref.max()
     assert np.max(np.abs(far[mr] - ref[mr]) / ref[mr]) < 1e-4
+
+
+@pytest.mark.parametrize("lm", [False, True])
+@pytest.mark.parametrize("cutoff", [25.0, 6.0])
+def test_edge_lines_by_local_expansion_match_oracle(ctx, lm, cutoff):
+    """A grid that only EDGE lines reach: CO2 lines within 1.5 cm^-1 of one
+    cutoff below and above a 2 cm^-1 grid, so that nearly every line's window
+    boundary crosses it.  The line sum evaluates them by the per-slice cubic
+    expansion (boundary slices pair by pair); the FP64 oracle sums every pair.
+    With cutoff 6 cm^-1 the lines are ~5000 grid steps away, where the cubic
+    terms of the expansion matter at the 1e-6 level.  The achievable accuracy
+    on such a grid is set by the FP32 accumulation of wings that the pedestal
+    nearly cancels (the net sum is a few % of the accumulated wings)."""
+    import refcpu as o
+    from irsounderlbl.api import compute_voigt_cross_sections, compute_voigt_lm_cross_sections
+    co2 = syn.linelists_config1()[GasSpecies.CO2]
+    g = wavenumber_grid(700.0, 702.0, 0.001)
+    lo, hi = 701.0 - cutoff, 701.0 + cutoff
+    m = (((co2.wavenumber >= lo - 1.5) & (co2.wavenumber <= lo + 1.5)) |
+         ((co2.wavenumber >= hi - 1.5) & (co2.wavenumber <= hi + 1.5)))
+    sub = co2.subset(m)
+    assert len(sub) > 100
+    for T, p in [(240.0, 0.3), (290.0, 1.0)]:
+        if lm:
+            got = compute_voigt_lm_cross_sections(g, sub, T, p, cutoff=cutoff, lm_cutoff=cutoff, ctx=ctx)
+            ref = o.voigt_lm_xsec(sub, g.nu_min, g.dnu, g.n, T, p, cutoff=cutoff, lm_cutoff=cutoff)
+        else:
+            got = compute_voigt_cross_sections(g, sub, T, p, cutoff=cutoff, ctx=ctx)
+            ref = o.voigt_xsec(sub, g.nu_min, g.dnu, g.n, T, p, 0.0, cutoff=cutoff)
+        peak = np.abs(ref).max()
+        assert peak > 0
+        err = np.abs(got - ref).max() / peak
+        e, _ = rel_err_masked(got, ref, 1e-3)
+        print(f"EDGE-only lm={lm} cutoff={cutoff} T={T} p={p}: max abs err {err:.2e} of peak, rel {e:.2e}")
+        assert err < 2e-5
+        assert e < 1e-4

@@ -74,3 +74,30 @@ def test_series_fp32_horner_is_accurate():
         s = ((q * v + f(1)) * v).astype(f) * f(y / np.sqrt(np.pi))
         w = wofz(x + 1j * y).real
         assert np.max(np.abs(s - w) / w) < 2e-6
+
+
+def test_edge_expansion_coefficients():
+    """The cubic (in the offset delta from a slice centre) that the line sum
+    uses for EDGE lines (k_linesum.hip edge_expand), restated from its comment,
+    against direct evaluation of the far wing + pedestal and of the dispersive
+    wing at slice distances 4000 and 25000 grid steps."""
+    A, B, P, C = 1.3, 0.7, 2.1e-9, 25000.0
+    for d in (4000.0, -4000.0, 25000.0, -24000.0):
+        for g in (0.5, 30.0, 90.0):
+            D = d * d + g * g
+            a = [A / D - P * (2 - d * d / C**2), -2 * d * A / D**2 + 2 * P * d / C**2,
+                 A * (3 * d * d - g * g) / D**3 + P / C**2, -4 * d * A * (d * d - g * g) / D**4]
+            b = [B * d / D, B * (g * g - d * d) / D**2, B * d * (d * d - 3 * g * g) / D**3,
+                 -B * (d**4 - 6 * d * d * g * g + g**4) / D**4]
+            x = np.arange(64) - 31.5
+            f = A / ((d + x) ** 2 + g * g) - P * (2 - (d + x) ** 2 / C**2)
+            h = B * (d + x) / ((d + x) ** 2 + g * g)
+            fa = a[0] + x * (a[1] + x * (a[2] + x * a[3]))
+            hb = b[0] + x * (b[1] + x * (b[2] + x * b[3]))
+            L = A / ((d + x) ** 2 + g * g)
+            assert np.max(np.abs(fa - f) / L) < 5 * (32 / abs(d)) ** 4
+            assert np.max(np.abs(hb - h) / np.abs(h)) < 5 * (32 / abs(d)) ** 4
+            # the cubic term is needed at d = 4000: dropping it costs ~(32/d)^3
+            if abs(d) < 5000:
+                f2 = a[0] + x * (a[1] + x * a[2])
+                assert np.max(np.abs(f2 - f) / L) > 1e-7
