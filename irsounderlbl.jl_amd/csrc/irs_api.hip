@@ -63,9 +63,9 @@ struct PinBuf {
 // Everything the device sequence of one profile group depends on.  Graphs are
 // cached by the byte image of this struct (zero-filled before use).
 struct GroupPlan {
-    const void *items, *rec0, *rec1, *rec2, *rec3, *keys, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
+    const void *items, *rec0, *rec1, *rec2, *rec3, *keys, *ktab, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
     const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out, *pre;
-    int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes, n_rec;
+    int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes, n_rec, ktab_lo, ktab_n;
     int gn, L, ns, P, pblocks, tiles, source_function, with_ils, far_d, far_hc;
     double nu_lo, dnu, an, sn, den;
     int mode;                      // 0 radiance + BT, 1 channel optical depths (2: + CIM weights),
@@ -95,7 +95,8 @@ struct irs_ctx {
     hipEvent_t ev[8] = {};
     std::string err;
     std::vector<ListHost> lists;
-    DevBuf d_lists, d_tips, d_items, d_rec0, d_rec1, d_rec2, d_rec3, d_keys, d_tau, d_rhi, d_counters;
+    DevBuf d_lists, d_tips, d_items, d_rec0, d_rec1, d_rec2, d_rec3, d_keys, d_ktab, d_tau, d_rhi, d_counters;
+    int64_t ktab_lo = 0, ktab_n = 0;   // key tables built by build_keys (0: none)
     DevBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3, d_tmp4, d_tmp5, d_chidx, d_chw, d_chnu, d_rch, d_bt;
     DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
@@ -351,9 +352,14 @@ int ensure_common(irs_ctx *ctx) {
     return IRS_OK;
 }
 
-// Keys for each used list; returns per-list key offsets.
+// Keys for each used list; returns per-list key offsets.  With n_hi > 0 it
+// also builds each list's key table over v in [-OFF, n_hi + OFF], OFF = the
+// larger window in grid steps + 64 (the line sum's window bounds fall in that
+// range unless the pressure shift exceeds 62 grid steps; outside it the line
+// sum searches the keys), and returns the tables' offsets.
 int build_keys(irs_ctx *ctx, const std::vector<int32_t> &lists, double nu_lo, double dnu,
-               std::map<int32_t, int32_t> &key_off) {
+               std::map<int32_t, int32_t> &key_off, int64_t n_hi = 0, double cut_max = 0.0,
+               std::map<int32_t, int64_t> *ktab_off = nullptr) {
     int64_t tot = 0;
     for (int32_t h : lists)
         if (!key_off.count(h)) {
@@ -364,6 +370,21 @@ int build_keys(irs_ctx *ctx, const std::vector<int32_t> &lists, double nu_lo, do
     for (auto &kv : key_off) {
         const ListHost &lh = ctx->lists[kv.first];
         HIPCHK(launch_keys(lh.dev.nu, lh.n, nu_lo, dnu, (int32_t *)ctx->d_keys.p + kv.second, ctx->stream));
+    }
+    ctx->ktab_lo = ctx->ktab_n = 0;
+    if (ktab_off && n_hi > 0 && cut_max / dnu < 1.0e8) {
+        const int64_t off = (int64_t)std::ceil(cut_max / dnu) + 64;
+        const int64_t cnt = n_hi + 2 * off + 1;
+        HIPCHK(ensure(ctx->d_ktab, sizeof(int32_t) * (size_t)cnt * key_off.size()));
+        int64_t o = 0;
+        for (auto &kv : key_off) {
+            (*ktab_off)[kv.first] = o;
+            HIPCHK(launch_keytab((const int32_t *)ctx->d_keys.p + kv.second, (int)ctx->lists[kv.first].n, -off,
+                                 cnt, (int32_t *)ctx->d_ktab.p + o, ctx->stream));
+            o += cnt;
+        }
+        ctx->ktab_lo = -off;
+        ctx->ktab_n = cnt;
     }
     return IRS_OK;
 }
@@ -504,6 +525,9 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     la.rec1 = pa.rec1;
     la.rec2 = pa.rec2;
     la.keys = (const int32_t *)ctx->d_keys.p;
+    la.ktab = ctx->ktab_n > 0 ? (const int32_t *)ctx->d_ktab.p : nullptr;
+    la.ktab_lo = ctx->ktab_lo;
+    la.ktab_n = ctx->ktab_n;
     la.out = out;
     la.n_hi = n_hi;
     la.n_layers = n_layers;
@@ -724,7 +748,7 @@ void irs_destroy(irs_ctx *ctx) {
     for (auto &l : ctx->lists)
         if (l.block) (void)hipFree(l.block);
     DevBuf *bufs[] = {&ctx->d_lists, &ctx->d_tips, &ctx->d_items, &ctx->d_rec0, &ctx->d_rec1, &ctx->d_rec2, &ctx->d_rec3,
-                      &ctx->d_keys, &ctx->d_tau, &ctx->d_rhi, &ctx->d_counters, &ctx->d_tmp0, &ctx->d_tmp1,
+                      &ctx->d_keys, &ctx->d_ktab, &ctx->d_tau, &ctx->d_rhi, &ctx->d_counters, &ctx->d_tmp0, &ctx->d_tmp1,
                       &ctx->d_tmp2, &ctx->d_tmp3, &ctx->d_tmp4, &ctx->d_tmp5, &ctx->d_chidx, &ctx->d_chw,
                       &ctx->d_chnu, &ctx->d_rch, &ctx->d_bt, &ctx->j_items, &ctx->j_g, &ctx->j_tau,
                       &ctx->j_dt, &ctx->j_v, &ctx->j_rte, &ctx->j_cols, &ctx->j_desc, &ctx->j_w, &ctx->j_K,
@@ -879,9 +903,11 @@ int irs_xsec_layer(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t
     rc = ensure_common(ctx);
     if (rc) return rc;
     std::map<int32_t, int32_t> koff;
-    rc = build_keys(ctx, {handle}, nu0, dnu, koff);
+    std::map<int32_t, int64_t> ktoff;
+    rc = build_keys(ctx, {handle}, nu0, dnu, koff, n_nu, std::max(cutoff, lm_cutoff), &ktoff);
     if (rc) return rc;
     items[0].key_off = koff[handle];
+    items[0].ktab_off = ktoff.count(handle) ? ktoff[handle] : 0;
     HIPCHK(ensure(ctx->d_tau, sizeof(float) * n_nu));
     rc = run_linesum(ctx, items, 1, 1, 1, nu0, dnu, n_nu, (float *)ctx->d_tau.p);
     if (rc) return rc;
@@ -1062,6 +1088,9 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.rec1 = pa.rec1;
     la.rec2 = pa.rec2;
     la.keys = (const int32_t *)pl.keys;
+    la.ktab = (const int32_t *)pl.ktab;
+    la.ktab_lo = pl.ktab_lo;
+    la.ktab_n = pl.ktab_n;
     la.out = (float *)pl.tau;
     la.n_hi = pl.n;
     la.n_layers = L;
@@ -1228,6 +1257,9 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
             la.rec1 = pa.rec1;
             la.rec2 = pa.rec2;
             la.keys = (const int32_t *)base.keys;
+            la.ktab = (const int32_t *)base.ktab;
+            la.ktab_lo = base.ktab_lo;
+            la.ktab_n = base.ktab_n;
             la.out = (float *)base.tau;
             la.n_hi = base.n;
             la.n_layers = base.L;
@@ -1309,7 +1341,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     for (int s = 0; s < ns; ++s)
         if (profiles[0].linelist[s] >= 0) used.push_back(profiles[0].linelist[s]);
     std::map<int32_t, int32_t> koff;
-    rc = build_keys(ctx, used, nu_lo, dnu, koff);
+    std::map<int32_t, int64_t> ktoff;
+    rc = build_keys(ctx, used, nu_lo, dnu, koff, n, std::max(opt.cutoff, opt.lm_cutoff), &ktoff);
     if (rc) return rc;
 
     double *taps = nullptr;
@@ -1425,6 +1458,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                         // DPTMIN is skipped on the line-mixing CO2 path (sounder.jl:295-299)
                         if (sp == 2 && opt.line_mixing == IRS_LM_VPY) it.dptmn = 0.0;
                         it.key_off = koff[h];
+                        it.ktab_off = ktoff.count(h) ? ktoff[h] : 0;
                     }
                     it.rec_off = rec;
                     rec += it.n;
@@ -1440,7 +1474,9 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                     GroupPlan tp;
                     std::memset(&tp, 0, sizeof tp);
                     tp.items = ctx->d_items.p; tp.rec0 = ctx->d_rec0.p; tp.rec1 = ctx->d_rec1.p;
-                    tp.rec2 = ctx->d_rec2.p; tp.rec3 = ctx->d_rec3.p; tp.keys = ctx->d_keys.p; tp.tau = ctx->d_tau.p;
+                    tp.rec2 = ctx->d_rec2.p; tp.rec3 = ctx->d_rec3.p; tp.keys = ctx->d_keys.p;
+                    tp.ktab = ctx->ktab_n > 0 ? ctx->d_ktab.p : nullptr; tp.ktab_lo = ctx->ktab_lo;
+                    tp.ktab_n = ctx->ktab_n; tp.tau = ctx->d_tau.p;
                     tp.counters = ctx->d_counters.p; tp.queue = ctx->d_queue.p; tp.lists = ctx->d_lists.p;
                     tp.tips = ctx->d_tips.p; tp.h_items = ctx->h_items.p; tp.pre = ctx->d_pre.p;
                     tp.n_rec = n_rec_cap;
@@ -1460,7 +1496,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         GroupPlan pl;
         std::memset(&pl, 0, sizeof pl);
         pl.items = ctx->d_items.p; pl.rec0 = ctx->d_rec0.p; pl.rec1 = ctx->d_rec1.p; pl.rec2 = ctx->d_rec2.p; pl.rec3 = ctx->d_rec3.p;
-        pl.keys = ctx->d_keys.p; pl.tau = ctx->d_tau.p; pl.rhi = ctx->d_rhi.p; pl.counters = ctx->d_counters.p;
+        pl.keys = ctx->d_keys.p; pl.tau = ctx->d_tau.p;
+        pl.ktab = ctx->ktab_n > 0 ? ctx->d_ktab.p : nullptr; pl.ktab_lo = ctx->ktab_lo; pl.ktab_n = ctx->ktab_n; pl.rhi = ctx->d_rhi.p; pl.counters = ctx->d_counters.p;
         pl.queue = ctx->d_queue.p; pl.lists = ctx->d_lists.p; pl.tips = ctx->d_tips.p; pl.order = order;
         pl.prof = ctx->d_prof.p; pl.rch = ctx->d_rch.p; pl.bt = ctx->d_bt.p; pl.chidx = ctx->d_chidx.p;
         pl.chw = ctx->d_chw.p; pl.chnu = ctx->d_chnu.p; pl.taps = taps; pl.h_items = ctx->h_items.p;

@@ -46,6 +46,7 @@ struct ItemDev {
     double  kcore_per_nu;   // x_far * max Doppler coefficient * sqrt(T) / (sqrt(ln2) * dnu)
     double  T, p_atm, vmr_self, coef, dptmn, cutoff, x_far;
     double  lm_cutoff;      // cm^-1 (the VP_Y window; lm_cutoff / dnu = Clm)
+    int64_t ktab_off;       // offset of the list's key table (LinesumArgs::ktab)
 };
 
 struct PrepArgs {
@@ -80,6 +81,8 @@ struct LinesumArgs {
     const float4 *rec3;            // [n_rec][6] mid-core records (PrepArgs::rec3)
     unsigned long long *work;      // optional: counters + 4; per copy [0] fine pairs evaluated,
                                    // [1] coarse-node pairs, [2], [3] the same for VP_Y items
+    const int32_t *ktab;           // optional key tables: per list, ktab[off + v - ktab_lo] = first line
+    int64_t ktab_lo, ktab_n;       // with key >= v, for v in [ktab_lo, ktab_lo + ktab_n)
 };
 
 struct RteArgs {
@@ -133,6 +136,7 @@ struct ChodArgs {
 
 // launchers (k_*.hip)
 hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s);
+hipError_t launch_keytab(const int32_t *keys, int n, int64_t lo, int64_t count, int32_t *out, hipStream_t s);
 hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hipStream_t s);
 hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s);
 int linesum_persistent_capacity(int P);

@@ -55,6 +55,15 @@ __device__ __forceinline__ int upper_key(const int32_t *k, int n, int64_t v) {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// First line with key >= v (lower bound): one load from the list's key table
+// when v lies in its range, else a binary search of the keys.
+__device__ __forceinline__ int first_ge(const LinesumArgs &a, const ItemDev &it, const int32_t *k, int n,
+                                        int64_t v) {
+    const int64_t r = v - a.ktab_lo;
+    if (a.ktab && r >= 0 && r < a.ktab_n) return a.ktab[it.ktab_off + r];
+    return lower_key(k, n, v);
+}
+
 // Line records read with a wave-uniform index go through the constant address
 // space, so that they are fetched by scalar loads into SGPRs (the records are
 // written by the prep kernel and only read here).
@@ -163,9 +172,8 @@ __device__ __forceinline__ void fast_range(const float4 *__restrict__ r0p, const
 // (Im w) relative of w(z) for |z| >= 8; the exact branch is taken only by waves
 // holding a point with |z| < 8.  The pedestal of these lines is part of the
 // per-tile moments.
-__device__ __forceinline__ float series_core(float x, const float4 k0, const float4 k1, float k2) {
-    // 1 + k2 v + k3 v^2 + ... + k9 v^8 at v = 1/(x^2 + Y), Horner in FP32
-    const float v = irs_rcpf(fmaf(x, x, k1.w));
+// 1 + k2 v + k3 v^2 + ... + k9 v^8 (Horner in FP32); the caller forms v = 1/(x^2 + Y).
+__device__ __forceinline__ float series_core(float v, const float4 k0, const float4 k1, float k2) {
     float q = fmaf(k1.z, v, k1.y);
     q = fmaf(q, v, k1.x);
     q = fmaf(q, v, k0.w);
@@ -173,17 +181,19 @@ __device__ __forceinline__ float series_core(float x, const float4 k0, const flo
     q = fmaf(q, v, k0.y);
     q = fmaf(q, v, k0.x);
     q = fmaf(q, v, k2);
-    return fmaf(q, v, 1.0f) * v;
+    return fmaf(q, v, 1.0f);
 }
 
-// Voigt term of one CORE pair outside the far form: Sn Re w(x + iy).
-__device__ __forceinline__ float core_voigt(float du, const float4 q1, float sn, crec c, float xfar,
-                                            bool inner) {
+// Voigt term of one CORE pair outside the far form: Sn Re w(x + iy) (the series
+// where |x| <= x_far, the Lorentz wing beyond it: both share v).
+__device__ __forceinline__ float core_voigt(float du, const float4 q1, float sn, crec c, float xfar, bool inner) {
     const float4 k0 = ld(c, 0), k1 = ld(c, 1), k2 = ld(c, 2);
     const float x = du * q1.z;
+    const float zz = fmaf(x, x, k1.w);
+    const float v = irs_rcpf(zz);
     const bool in_core = fabsf(x) <= xfar;
-    float h = in_core ? k2.x * series_core(x, k0, k1, 1.5f) : k2.x * irs_rcpf(fmaf(x, x, k1.w));
-    if (inner && in_core && fmaf(x, x, k1.w) < (float)(IRS_Z1 * IRS_Z1)) {
+    float h = k2.x * v * (in_core ? series_core(v, k0, k1, 1.5f) : 1.0f);
+    if (inner && in_core && zz < (float)(IRS_Z1 * IRS_Z1)) {
         float wr, wi;
         w32_core(x, q1.w, wr, wi);
         h = sn * wr;
@@ -195,9 +205,11 @@ __device__ __forceinline__ float core_voigt(float du, const float4 q1, float sn,
 __device__ __forceinline__ float core_disp(float du, const float4 q2, crec c, float xfar, bool inner) {
     const float4 k0 = ld(c, 3), k1 = ld(c, 4), k2 = ld(c, 5);
     const float x = du * q2.y;
+    const float zz = fmaf(x, x, k1.w);
+    const float v = irs_rcpf(zz);
     const bool in_core = fabsf(x) <= xfar;
-    float d = in_core ? (k2.x * x) * series_core(x, k0, k1, 0.5f) : (k2.x * x) * irs_rcpf(fmaf(x, x, k1.w));
-    if (inner && in_core && fmaf(x, x, k1.w) < (float)(IRS_Z1 * IRS_Z1)) {
+    float d = (k2.x * x) * v * (in_core ? series_core(v, k0, k1, 0.5f) : 1.0f);
+    if (inner && in_core && zz < (float)(IRS_Z1 * IRS_Z1)) {
         float wr, wi;
         w32_core(x, q2.z, wr, wi);
         d = q2.w * wi;
@@ -617,16 +629,16 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     if (t < 10) {
         int r;
         switch (t) {
-            case 0: r = lower_key(keys, n, vA); break;
-            case 1: r = lower_key(keys, n, vB); break;
-            case 2: r = lower_key(keys, n, vC); break;
-            case 3: r = upper_key(keys, n, vD); break;
-            case 4: r = upper_key(keys, n, vE); break;
-            case 5: r = upper_key(keys, n, vF); break;
-            case 6: r = far ? lower_key(keys, n, vB2) : 0; break;
-            case 7: r = far ? upper_key(keys, n, vG) : 0; break;
-            case 8: r = far ? lower_key(keys, n, vH) : 0; break;
-            default: r = far ? upper_key(keys, n, vE2) : 0; break;
+            case 0: r = first_ge(a, it, keys, n, vA); break;
+            case 1: r = first_ge(a, it, keys, n, vB); break;
+            case 2: r = first_ge(a, it, keys, n, vC); break;
+            case 3: r = first_ge(a, it, keys, n, vD + 1); break;      // first key > vD
+            case 4: r = first_ge(a, it, keys, n, vE + 1); break;
+            case 5: r = first_ge(a, it, keys, n, vF + 1); break;
+            case 6: r = far ? first_ge(a, it, keys, n, vB2) : 0; break;
+            case 7: r = far ? first_ge(a, it, keys, n, vG + 1) : 0; break;
+            case 8: r = far ? first_ge(a, it, keys, n, vH) : 0; break;
+            default: r = far ? first_ge(a, it, keys, n, vE2 + 1) : 0; break;
         }
         s_rng[t] = r;
     }

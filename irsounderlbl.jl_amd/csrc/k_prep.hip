@@ -302,6 +302,29 @@ hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Key table of one list: out[t] = first line with key >= lo + t (lower bound in
+// the sorted keys), so that the line sum replaces its window searches by one
+// load per bound.
+__global__ void __launch_bounds__(256) keytab_kernel(const int32_t *keys, int n, int64_t lo, int64_t count,
+                                                     int32_t *out) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    const int64_t v = lo + t;
+    int a = 0, b = n;
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if ((int64_t)keys[mid] < v) a = mid + 1; else b = mid;
+    }
+    out[t] = a;
+}
+
+hipError_t launch_keytab(const int32_t *keys, int n, int64_t lo, int64_t count, int32_t *out, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(keytab_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, keys, n, lo, count,
+                       out);
+    return hipGetLastError();
+}
+
 hipError_t launch_keys(const double *nu, int64_t n, double nu_lo, double dnu, int32_t *keys,
                        hipStream_t s) {
     if (n <= 0) return hipSuccess;
