@@ -540,7 +540,7 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     la.tile_order = nullptr;
     la.queue = (unsigned *)ctx->d_queue.p;
     far_params(ctx, dnu, P, la.far_d, la.far_hc);
-    la.work = (unsigned long long *)ctx->d_counters.p + 4;
+    la.work = (unsigned long long *)ctx->d_counters.p + 3;
     la.pre = pa.pre;
     la.n_rec = pa.n_rec;
     la.rec3 = pa.rec3;
@@ -563,6 +563,7 @@ void read_counters(irs_ctx *ctx) {
         ctx->stats.coarse_pairs += (int64_t)c[5];
         ctx->stats.fine_pairs_lm += (int64_t)c[6];
         ctx->stats.coarse_pairs_lm += (int64_t)c[7];
+        ctx->stats.edge_line_slices += (int64_t)c[3];
     }
 }
 
@@ -1103,7 +1104,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.queue = (unsigned *)pl.queue;
     la.far_d = pl.far_d;
     la.far_hc = pl.far_hc;
-    la.work = (unsigned long long *)pl.counters + 4;
+    la.work = (unsigned long long *)pl.counters + 3;
     la.pre = pa.pre;
     la.n_rec = pa.n_rec;
     la.rec3 = pa.rec3;

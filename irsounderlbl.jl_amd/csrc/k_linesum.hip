@@ -690,7 +690,8 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         core_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), as_crec(a.rec3 + 6 * it.rec_off), jC, jD, tl, xfar, acc, cmp, accd, cmpd);
         fast_range<P, LM>(r0p, r1p, jD, jH, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         // EDGE lines: local expansion when every one of them is far from the tile
-        if (Cmn - (double)(256 * P + HC + M + 34) >= (double)EDGE_EXPAND_MIN) {
+        const bool expand = Cmn - (double)(256 * P + HC + M + 34) >= (double)EDGE_EXPAND_MIN;
+        if (expand) {
             edge_expand<P, LM>(r0p, r1p, jA, jB, jE, jF, t, tl, it.C, it.Clm, C2, CiD, CfD, iC2, acc, cmp, accd,
                                cmpd);
         } else {
@@ -699,22 +700,25 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         }
         if (a.work && t == 0) {
             unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
-            const unsigned long long fine = (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC) + (jB - jA) +
-                                                                 (jF - jE)) * (unsigned long long)(i1 - i0);
+            const unsigned long long edge = (unsigned long long)((jB - jA) + (jF - jE));
+            const unsigned long long fine =
+                (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC)) * (unsigned long long)(i1 - i0) +
+                (expand ? 0ull : edge * (unsigned long long)(i1 - i0));
             const unsigned long long coarse = far ? (unsigned long long)((jG - jB) + (jE - jH)) * NC : 0ull;
-            atomicAdd(&w[0], fine);
-            if (coarse) atomicAdd(&w[1], coarse);
+            if (expand && edge) atomicAdd(&w[0], edge * (unsigned long long)((i1 - i0 + 63) / 64));
+            atomicAdd(&w[1], fine);
+            if (coarse) atomicAdd(&w[2], coarse);
             if (LM) {
-                atomicAdd(&w[2], fine);
-                if (coarse) atomicAdd(&w[3], coarse);
+                atomicAdd(&w[3], fine);
+                if (coarse) atomicAdd(&w[4], coarse);
             }
         }
     } else {
         general_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), jA, jF, tl, xfar, C2, CiD, CfD, iC2, acc, cmp, accd, cmpd);
         if (a.work && t == 0) {
             unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
-            atomicAdd(&w[0], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
-            if (LM) atomicAdd(&w[2], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
+            atomicAdd(&w[1], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
+            if (LM) atomicAdd(&w[3], (unsigned long long)(jF - jA) * (unsigned long long)(i1 - i0));
         }
     }
 #pragma unroll

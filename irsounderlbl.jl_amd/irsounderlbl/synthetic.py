@@ -82,10 +82,20 @@ def linelist_config0(seed: int = 1000) -> HITRANLinelist:
     return HITRANLinelist(2, 1, nu, S, ga, gs, el, na, da)
 
 
-def linelists_config1(seed: int = 1001, with_lm: bool = True):
+def linelists_config1(seed: int = 1001, with_lm: bool = True, windows: bool = False):
     """300,000 lines in 620–2785 cm^-1: CO2-like 90k U(640,760) + 90k U(2250,2400),
     H2O-like 100k U(620,2785) (gamma_air U(.02,.10)), O3-like 20k U(980,1080);
-    Y ~ N(0, 1e-2) atm^-1 on the 640–760 CO2-like lines."""
+    Y ~ N(0, 1e-2) atm^-1 on the 640–760 CO2-like lines.
+
+    With these intensities (S = 10^U(-27,-18) for every species) the H2O-like
+    wings alone make the base profile opaque across IASI: every channel's
+    brightness temperature is the 217 K of the upper levels.  That is the
+    performance workload (the work per spectrum does not depend on the values).
+    windows=True maps the H2O-like intensity exponents linearly onto
+    U(-27, -23), with every other field (positions, widths, CO2, O3) drawn
+    identically, so that the spectrum has partially transparent windows
+    (brightness temperatures 217-282 K; 244-281 K between the bands): the
+    variant used for brightness-temperature accuracy checks and the emulator."""
     rng = np.random.default_rng(seed)
     n15 = 90000
     nu, S, ga, gs, na, el, da = _lines(rng, 0, 0, 0, 2, band=[(640.0, 760.0, n15), (2250.0, 2400.0, 90000)])
@@ -94,6 +104,8 @@ def linelists_config1(seed: int = 1001, with_lm: bool = True):
         y[:n15] = rng.normal(0.0, 1e-2, n15)
     co2 = HITRANLinelist(2, 1, nu, S, ga, gs, el, na, da, y_lm=y if with_lm else None)
     nu, S, ga, gs, na, el, da = _lines(rng, 100000, 620.0, 2785.0, 1, g_air=(0.02, 0.10))
+    if windows:
+        S = 10.0 ** (-27.0 + (np.log10(S) + 27.0) * (4.0 / 9.0))
     h2o = HITRANLinelist(1, 1, nu, S, ga, gs, el, na, da)
     nu, S, ga, gs, na, el, da = _lines(rng, 20000, 980.0, 1080.0, 3)
     o3 = HITRANLinelist(3, 1, nu, S, ga, gs, el, na, da)

@@ -63,3 +63,29 @@ def test_forward_full_range_subset_bt(ctx):
         d = np.abs(BT - BTo)
         print(f"subset lm={lm} BT max|d| {d.max():.2e} K")
         assert d.max() < BT_TOL
+
+
+@pytest.mark.parametrize("lo,hi", [(1205.0, 1215.0), (778.0, 784.0), (2420.0, 2426.0)])
+def test_forward_windows_variant_bt(ctx, lo, hi):
+    """Brightness temperatures where the spectrum is partly transparent (the
+    config-1 line lists with the H2O-like intensities of the windows variant:
+    BT 240-275 K here, so every layer's optical depth matters), slant path and
+    a non-black surface, with and without VP_Y (778-784 cm^-1 carries the wings
+    of the line-mixing CO2 lines), vs the FP64 oracle at 0.005 K."""
+    from irsounderlbl.api import forward_model
+    from irsounderlbl.types import ViewingGeometry
+    lls = syn.linelists_config1(windows=True)
+    prof = syn.base_profile(True)
+    snd = Sounder.make(lo, hi, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls.items()}
+    mu, T_sfc, eps = 0.8, 295.0, 0.97
+    geom = ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu)
+    for lm in ((None, "vpy") if lo < 800.0 else ("vpy",)):
+        nu, R, BT = forward_model(prof, sub, sounder=snd, geom=geom, T_sfc=T_sfc, eps_sfc=eps, line_mixing=lm,
+                                  ctx=ctx)
+        Ro, BTo, info = refcpu.forward(prof, sub, snd, line_mixing=1 if lm else 0, mu=mu, T_sfc=T_sfc,
+                                       eps_sfc=eps)
+        assert BTo.min() > 230.0 and BTo.max() - BTo.min() > 5.0        # not saturated, not flat
+        d = np.abs(BT - BTo)
+        print(f"windows {lo}-{hi} lm={lm}: BT {BTo.min():.1f}-{BTo.max():.1f} K, max|d| {d.max():.2e} K")
+        assert d.max() < BT_TOL
