@@ -161,7 +161,10 @@ typedef struct {
                                    profile of each new shape and keep the fastest */
     double far_wing_cm;         /* > 0: far-wing sums of lines at least this far (cm^-1) from a
                                    tile are evaluated on a coarse grid (<= 0.032 cm^-1) and
-                                   interpolated (cubic); 0: every pair on the internal grid */
+                                   interpolated (cubic), and lines far from a 64-point slice
+                                   (interior lines beyond 500 grid steps, cutoff-boundary
+                                   lines) by local expansion in the slice; 0: every pair on
+                                   the internal grid */
 } irs_schedule;
 
 int         irs_version(void);

@@ -420,6 +420,80 @@ __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, cons
     }
 }
 
+// FAST lines at least FAST_EXPAND_R grid steps from the tile, by local
+// expansion: on a 64-point slice at distance d >= FAST_EXPAND_R + 32 from the
+// line centre, A/((d+delta)^2 + g^2) = A sum_k h_k delta^k with
+//   h_0 = 1/D, h_1 = -2d h_0/D, h_k = -(2d h_{k-1} + h_{k-2})/D   (D = d^2 + g^2)
+// (the series of 1/u for the quadratic u(delta)), and the dispersive wing
+// B (d+delta)/u = B sum_k (d h_k + h_{k-1}) delta^k.  Degree 6: the first
+// omitted term is below 8 (32/d)^7 <= 2.3e-8 of the line's value at d = 532.
+// Lanes take one line each and add A h_k (and the dispersive coefficients) for
+// one slice of the wave at a time; a wave reduction and one polynomial per
+// point follow.  The lines' windows cover the tile (they are interior lines) and
+// their pedestal is in the per-tile moments.
+#ifndef FAST_EXPAND_R
+#define FAST_EXPAND_R 500
+#endif
+#define FAST_EXPAND_K 7
+template <int P, bool LM>
+__device__ __forceinline__ void fast_expand(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
+                                            int jl0, int jl1, int jr0, int jr1, int t, int tl, float (&acc)[P],
+                                            float (&cmp)[P], float (&accd)[P], float (&cmpd)[P]) {
+    const int lane = t & 63;
+    const int tw = uni(tl & ~63);
+    const int nL = max(jl1 - jl0, 0), ntot = nL + max(jr1 - jr0, 0);
+    if (ntot <= 0) return;
+    const float dl = (float)lane - 31.5f;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        float a[FAST_EXPAND_K], b[FAST_EXPAND_K];
+#pragma unroll
+        for (int k = 0; k < FAST_EXPAND_K; ++k) a[k] = b[k] = 0.0f;
+        for (int g0 = 0; g0 < ntot; g0 += 64) {
+            const int v = g0 + lane;
+            const bool ok = v < ntot;
+            const int j = ok ? (v < nL ? jl0 + v : jr0 + (v - nL)) : (nL > 0 ? jl0 : jr0);
+            const float4 q0 = r0p[j];
+            const float A = ok ? q0.w : 0.0f;
+            const float B = (LM && ok) ? r1p[j].x : 0.0f;
+            const float d = (float)(tw + 256 * p - __float_as_int(q0.x)) + (31.5f - q0.y);   // slice centre - line
+            const float iD = irs_rcpf(fmaf(d, d, q0.z));
+            const float d2 = -2.0f * d;
+            float hm = 0.0f, h = iD;                                 // h_{k-1}, h_k
+#pragma unroll
+            for (int k = 0; k < FAST_EXPAND_K; ++k) {
+                a[k] = fmaf(A, h, a[k]);
+                if (LM) b[k] = fmaf(B, fmaf(d, h, hm), b[k]);
+                const float hn = fmaf(d2, h, -hm) * iD;
+                hm = h;
+                h = hn;
+            }
+        }
+        float c = a[FAST_EXPAND_K - 1];
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        float val = c;
+#pragma unroll
+        for (int k = FAST_EXPAND_K - 2; k >= 0; --k) {
+            float ck = a[k];
+            for (int o = 32; o > 0; o >>= 1) ck += __shfl_xor(ck, o, 64);
+            val = fmaf(val, dl, ck);
+        }
+        kahan_add(acc[p], cmp[p], val);
+        if (LM) {
+            float e = b[FAST_EXPAND_K - 1];
+            for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+            float vd = e;
+#pragma unroll
+            for (int k = FAST_EXPAND_K - 2; k >= 0; --k) {
+                float ek = b[k];
+                for (int o = 32; o > 0; o >>= 1) ek += __shfl_xor(ek, o, 64);
+                vd = fmaf(vd, dl, ek);
+            }
+            kahan_add(accd[p], cmpd[p], vd);
+        }
+    }
+}
+
 // Degenerate small-cutoff case (window narrower than tile + core): exact
 // branch, window test and pedestal for every pair.
 template <int P, bool LM>
@@ -626,7 +700,11 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     const int64_t vE2 = Sl + Cmin - M - 2;
     const bool far = nondeg && HC > 0 && NC <= FAR_MAXNC && vA <= vB2 && vB2 <= vG && vG < vC && vD < vH &&
                      vH <= vE2 && vE2 <= vF;
-    if (t < 10) {
+    // FAST lines beyond FAST_EXPAND_R of the tile go to fast_expand: keys <= vL
+    // (centre <= i0 - FAST_EXPAND_R - 3) and keys >= vR (centre >= i1 + FAST_EXPAND_R + 2)
+    const int64_t vL = i0 - FAST_EXPAND_R - M - 2;
+    const int64_t vR = i1 - 1 + FAST_EXPAND_R + M + 1;
+    if (t < 12) {
         int r;
         switch (t) {
             case 0: r = first_ge(a, it, keys, n, vA); break;
@@ -638,7 +716,9 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
             case 6: r = far ? first_ge(a, it, keys, n, vB2) : 0; break;
             case 7: r = far ? first_ge(a, it, keys, n, vG + 1) : 0; break;
             case 8: r = far ? first_ge(a, it, keys, n, vH) : 0; break;
-            default: r = far ? first_ge(a, it, keys, n, vE2 + 1) : 0; break;
+            case 9: r = far ? first_ge(a, it, keys, n, vE2 + 1) : 0; break;
+            case 10: r = first_ge(a, it, keys, n, vL + 1); break;
+            default: r = first_ge(a, it, keys, n, vR); break;
         }
         s_rng[t] = r;
     }
@@ -650,6 +730,17 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     const int jE = far ? uni(s_rng[9]) : uni(s_rng[4]);
     const int jG = far ? uni(s_rng[7]) : jB;
     const int jH = far ? uni(s_rng[8]) : jE;
+    // FAST ranges [jG, jC) and [jD, jH): far parts [jG, jLf) and [jRf, jH) by expansion.
+    // far_hc == 0 (far_wing_cm = 0) turns every shortcut off: every pair on the grid.
+    const bool shortcuts = HC > 0;
+    // (only when there are enough of them: the per-slice reductions cost about
+    // as much as 128 lines evaluated pair by pair)
+#ifndef FAST_EXPAND_MINLINES
+#define FAST_EXPAND_MINLINES 128
+#endif
+    int jLf = shortcuts ? min(max(uni(s_rng[10]), jG), jC) : jG;
+    int jRf = shortcuts ? max(min(uni(s_rng[11]), jH), jD) : jH;
+    if ((jLf - jG) + (jH - jRf) < FAST_EXPAND_MINLINES) { jLf = jG; jRf = jH; }
     __syncthreads();
 
     const float4 *r0p = a.rec0 + it.rec_off;
@@ -686,11 +777,13 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         }
         if (far && (jG > jB || jE > jH))
             far_coarse<P, LM>(r0p, r1p, jB, jG, jH, jE, t, i0, Sl, HC, NC, s_far, s_rec, acc, cmp, accd, cmpd);
-        fast_range<P, LM>(r0p, r1p, jG, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
-        core_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), as_crec(a.rec3 + 6 * it.rec_off), jC, jD, tl, xfar, acc, cmp, accd, cmpd);
-        fast_range<P, LM>(r0p, r1p, jD, jH, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
+        fast_expand<P, LM>(r0p, r1p, jG, jLf, jRf, jH, t, tl, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jLf, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
+        core_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), as_crec(a.rec3 + 6 * it.rec_off), jC, jD, tl,
+                          xfar, acc, cmp, accd, cmpd);
+        fast_range<P, LM>(r0p, r1p, jD, jRf, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         // EDGE lines: local expansion when every one of them is far from the tile
-        const bool expand = Cmn - (double)(256 * P + HC + M + 34) >= (double)EDGE_EXPAND_MIN;
+        const bool expand = shortcuts && Cmn - (double)(256 * P + HC + M + 34) >= (double)EDGE_EXPAND_MIN;
         if (expand) {
             edge_expand<P, LM>(r0p, r1p, jA, jB, jE, jF, t, tl, it.C, it.Clm, C2, CiD, CfD, iC2, acc, cmp, accd,
                                cmpd);
