@@ -268,7 +268,7 @@ def main():
     prof_pmc = pmc_from_profiles()
     executed = {"fine_pairs": stats["fine_pairs"], "coarse_far_wing_node_pairs": stats["coarse_pairs"],
                 "fine_pairs_vpy": stats["fine_pairs_lm"], "coarse_node_pairs_vpy": stats["coarse_pairs_lm"],
-                "edge_line_slices_expanded": stats["edge_line_slices"]}
+                "line_slices_expanded": stats["expanded_line_slices"]}
     if prof_pmc.get("valu_insts_per_launch"):
         rate = prof_pmc["valu_insts_per_launch"] / (ls_avg * 1e-3)
         executed["valu_wave_insts_per_launch"] = prof_pmc["valu_insts_per_launch"]

@@ -147,7 +147,8 @@ typedef struct {
     int64_t coarse_pairs;       /* pairs on the coarse far-wing nodes */
     int64_t fine_pairs_lm;      /* of which on line-mixing (VP_Y) items */
     int64_t coarse_pairs_lm;
-    int64_t edge_line_slices;   /* EDGE lines x 64-point slices evaluated by local expansion */
+    int64_t expanded_line_slices; /* lines x 64-point slices evaluated by local expansion
+                                     (cutoff-boundary and far FAST lines) */
 } irs_stats;
 
 /* Line-sum schedule of irs_forward.  No counterpart in the reference: it

@@ -563,7 +563,7 @@ void read_counters(irs_ctx *ctx) {
         ctx->stats.coarse_pairs += (int64_t)c[5];
         ctx->stats.fine_pairs_lm += (int64_t)c[6];
         ctx->stats.coarse_pairs_lm += (int64_t)c[7];
-        ctx->stats.edge_line_slices += (int64_t)c[3];
+        ctx->stats.expanded_line_slices += (int64_t)c[3];
     }
 }
 

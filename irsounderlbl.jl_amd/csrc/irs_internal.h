@@ -12,7 +12,7 @@
 // each workgroup adds into the copy of its slot, so that concurrent atomics are
 // spread over many addresses instead of serialising on one.
 //   [0] active lines, [1] Voigt pairs, [2] dispersive pairs (prep)
-//   [3] EDGE line x 64-point slice expansions, [4] fine pairs, [5] coarse-node pairs,
+//   [3] line x 64-point slice expansions (EDGE and far FAST lines), [4] fine pairs, [5] coarse-node pairs,
 //   [6], [7] the same for VP_Y items (line sum)
 #define IRS_CNT_SLOTS 16
 #define IRS_NCNT (8 * IRS_CNT_SLOTS)
@@ -80,7 +80,7 @@ struct LinesumArgs {
     const double *pre;             // [3][n_rec] pedestal prefix sums (PrepArgs::pre)
     int64_t n_rec;
     const float4 *rec3;            // [n_rec][6] mid-core records (PrepArgs::rec3)
-    unsigned long long *work;      // optional: counters + 3; per copy [0] EDGE line-slice expansions,
+    unsigned long long *work;      // optional: counters + 3; per copy [0] line-slice expansions,
                                    // [1] fine pairs evaluated, [2] coarse-node pairs, [3], [4] the
                                    // same for VP_Y items
     const int32_t *ktab;           // optional key tables: per list, ktab[off + v - ktab_lo] = first line

@@ -227,10 +227,6 @@ __device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
-#ifndef IRS_CORE_UNROLL
-#define IRS_CORE_UNROLL 1
-#endif
-#pragma unroll IRS_CORE_UNROLL
         for (int j = j0; j < je; ++j) {
             const float4 q0 = ld(r0p, j), q1 = ld(r1p, j), q2 = ld(r2p, j), c2 = ld(r3p + 6 * (int64_t)j, 2);
             const float4 c5 = LM ? ld(r3p + 6 * (int64_t)j, 5) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -276,10 +272,6 @@ __device__ __forceinline__ void edge_range(crec r0p, crec r1p,
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
-#ifndef IRS_EDGE_UNROLL
-#define IRS_EDGE_UNROLL 4
-#endif
-#pragma unroll IRS_EDGE_UNROLL
         for (int j = j0; j < je; ++j) {
             const float4 q0 = ld(r0p, j), q1 = ld(r1p, j);
             const float base = (float)(tl - __float_as_int(q0.x));
@@ -321,9 +313,7 @@ __device__ __forceinline__ void edge_range(crec r0p, crec r1p,
 // falls inside the slice is evaluated pair by pair with the per-point test (as
 // in edge_range) in a wave-uniform loop over those lines.  The coefficient
 // sums are reduced across the wave and every point evaluates its slice's cubic.
-#ifndef EDGE_EXPAND_MIN
 #define EDGE_EXPAND_MIN 4000
-#endif
 template <int P, bool LM>
 __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
                                             int jl0, int jl1, int jr0, int jr1, int t, int tl, float C, float Clm,
@@ -431,9 +421,7 @@ __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, cons
 // one slice of the wave at a time; a wave reduction and one polynomial per
 // point follow.  The lines' windows cover the tile (they are interior lines) and
 // their pedestal is in the per-tile moments.
-#ifndef FAST_EXPAND_R
 #define FAST_EXPAND_R 500
-#endif
 #define FAST_EXPAND_K 7
 template <int P, bool LM>
 __device__ __forceinline__ void fast_expand(const float4 *__restrict__ r0p, const float4 *__restrict__ r1p,
@@ -735,9 +723,7 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
     const bool shortcuts = HC > 0;
     // (only when there are enough of them: the per-slice reductions cost about
     // as much as 128 lines evaluated pair by pair)
-#ifndef FAST_EXPAND_MINLINES
 #define FAST_EXPAND_MINLINES 128
-#endif
     int jLf = shortcuts ? min(max(uni(s_rng[10]), jG), jC) : jG;
     int jRf = shortcuts ? max(min(uni(s_rng[11]), jH), jD) : jH;
     if ((jLf - jG) + (jH - jRf) < FAST_EXPAND_MINLINES) { jLf = jG; jRf = jH; }
@@ -794,11 +780,14 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         if (a.work && t == 0) {
             unsigned long long *w = a.work + 8 * (blockIdx.x % IRS_CNT_SLOTS);
             const unsigned long long edge = (unsigned long long)((jB - jA) + (jF - jE));
+            const unsigned long long fexp = (unsigned long long)((jLf - jG) + (jH - jRf));
+            const unsigned long long slices = (unsigned long long)((i1 - i0 + 63) / 64);
             const unsigned long long fine =
-                (unsigned long long)((jC - jG) + (jH - jD) + (jD - jC)) * (unsigned long long)(i1 - i0) +
+                (unsigned long long)((jC - jLf) + (jRf - jD) + (jD - jC)) * (unsigned long long)(i1 - i0) +
                 (expand ? 0ull : edge * (unsigned long long)(i1 - i0));
             const unsigned long long coarse = far ? (unsigned long long)((jG - jB) + (jE - jH)) * NC : 0ull;
-            if (expand && edge) atomicAdd(&w[0], edge * (unsigned long long)((i1 - i0 + 63) / 64));
+            const unsigned long long expd = ((expand ? edge : 0ull) + fexp) * slices;
+            if (expd) atomicAdd(&w[0], expd);
             atomicAdd(&w[1], fine);
             if (coarse) atomicAdd(&w[2], coarse);
             if (LM) {

@@ -79,7 +79,7 @@ class IrsStats(C.Structure):
                 ("core_pairs_bound", i64), ("ms_device", f64), ("graph_launches", i64),
                 ("tile_points", i32), ("persistent", i32), ("autotuned", i32), ("reserved1", i32),
                 ("fine_pairs", i64), ("coarse_pairs", i64), ("fine_pairs_lm", i64),
-                ("coarse_pairs_lm", i64), ("edge_line_slices", i64)]
+                ("coarse_pairs_lm", i64), ("expanded_line_slices", i64)]
 
 
 class IrsSchedule(C.Structure):
