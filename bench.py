@@ -21,8 +21,11 @@ U(0, 48.3 deg)), then runs --emu-eval new profiles of the same distribution
 through emulator + 64-channel LBL spot check + full-LBL fallback at 0.05 K,
 with the BT error against the GPU LBL (itself within 1e-9 K of the FP64 oracle).
 
+"lbl_jacobian" times one full-IASI analytic Jacobian (102 columns) of a
+perturbed profile on the windows variant of the line lists.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-accuracy]
-                    [--no-batch] [--batch N] [--no-exact] [--no-emulator]
+                    [--no-batch] [--batch N] [--no-exact] [--no-emulator] [--no-jacobian]
                     [--emu-train N] [--emu-eval N]
 """
 import argparse
@@ -176,6 +179,31 @@ def emulator_figure(ctx, lls, snd, n_train, n_eval):
             "jacobian_shape": list(jac[0].K_T.shape)}
 
 
+def lbl_jacobian_figure(ctx, lls_w, snd):
+    """Seconds per full-IASI analytic Jacobian (temperature, H2O log-VMR,
+    surface temperature and emissivity columns; VP_Y on) of one perturbed
+    profile, second call timed."""
+    from irsounderlbl import synthetic as syn
+    from irsounderlbl.api import analytic_jacobian
+    from irsounderlbl.types import GasSpecies, StateVectorSpec, ViewingGeometry
+    prof, T_sfc, mu = syn.perturbed_profiles(1, seed=5005)[0]
+    spec = StateVectorSpec(prof.n_levels, [GasSpecies.H2O])
+    kw = dict(sounder=snd, geom=ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu), T_sfc=T_sfc,
+              line_mixing="vpy", ctx=ctx)
+    log("bench: LBL analytic Jacobian")
+    analytic_jacobian(prof, lls_w, spec, **kw)
+    t0 = time.perf_counter()
+    jac = analytic_jacobian(prof, lls_w, spec, **kw)
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    return {"workload": "full IASI 8461 channels, windows variant of the line lists, 1 perturbed profile, "
+                        "columns T(50) + H2O log-VMR(50) + T_sfc + emissivity",
+            "seconds_per_profile": round(dt, 3), "K_shape": list(jac.K.shape),
+            "K_finite": bool(np.all(np.isfinite(jac.K))),
+            "kernels_ms": {"prep": round(st["ms_prep"], 2), "gradient_linesum": round(st["ms_linesum"], 2),
+                           "rte_jacobian": round(st["ms_rte"], 2), "ils": round(st["ms_ils"], 2)}}
+
+
 def exact_sum(ctx, prof, lls, snd):
     """One step with the coarse far-wing sum switched off (every far-wing pair
     on the internal grid; EDGE lines and the mid-core series as in the default
@@ -216,6 +244,7 @@ def main():
     ap.add_argument("--batch", type=int, default=120)
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-emulator", action="store_true")
+    ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--emu-train", type=int, default=512)
     ap.add_argument("--emu-eval", type=int, default=256)
     args = ap.parse_args()
@@ -300,6 +329,8 @@ def main():
                           "rte": round(stats["ms_rte"], 3), "ils": round(stats["ms_ils"], 3)}}
     if not args.no_batch:
         out["batched"] = batched(ctx, lls, snd, args.batch)
+    if not args.no_jacobian:
+        out["lbl_jacobian"] = lbl_jacobian_figure(ctx, lls_w, snd)
     if not args.no_emulator:
         out["emulator"] = emulator_figure(ctx, lls_w, snd, args.emu_train, args.emu_eval)
     if not args.no_accuracy:

@@ -1695,7 +1695,7 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
             it.n = base.n; it.key_off = koff[h]; it.list = h; it.lm = base.lm; it.active = 1; it.M = base.M;
             it.C = base.C; it.Clm = base.Clm;
             it.T = T; it.p_atm = p_atm; it.vmr_self = vs; it.coef = base.coef; it.cutoff = opt.cutoff;
-            it.x_far = opt.x_far; it.lm_cutoff = opt.lm_cutoff;
+            it.x_far = opt.x_far; it.lm_cutoff = opt.lm_cutoff; it.kcore_per_nu = base.kcore_per_nu;
             const double p1 = P[k], p2 = P[k + 1];
             const double wT = opt.T_method == IRS_TMETHOD_MASS_WEIGHTED ? cg_mass_frac(p1, p2)
                                                                          : std::log(lp.p_cg[s][k] / p1) / std::log(p2 / p1);
@@ -1755,6 +1755,8 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     sa.n_species = ns;
     sa.n_retr = nr;
     sa.dnu = dnu;
+    sa.nu_lo = nu_lo;
+    far_params(ctx, dnu, IRS_JAC_TILE_P, sa.far_d, sa.far_hc);
     sa.tau = (float *)ctx->j_tau.p;
     sa.dtlo = (float *)ctx->j_dt.p;
     sa.dthi = sa.dtlo + NL;

@@ -7,6 +7,7 @@
 #include "irs_internal.h"
 
 #define IRS_JAC_MAXR 4   // retrieved species handled in one gradient pass
+#define IRS_JAC_TILE_P 4 // grid points per thread of the gradient line sum (tile = 256 * P)
 
 // Per (layer, species) gradient work item.
 struct JacItemDev {
@@ -15,6 +16,7 @@ struct JacItemDev {
     float C, Clm;
     double T, p_atm, vmr_self, coef, cutoff, x_far;
     double lm_cutoff;             // cm^-1, the VP_Y window (Clm = lm_cutoff / dnu)
+    double kcore_per_nu;          // grid steps per cm^-1 of wavenumber bounding |x| <= x_far
     double wT_lo, wT_hi;          // dT_cg/dT_lev at the lower / upper level
     int32_t retr, pad;            // retrieved-species index or -1
     double gv_lo, gv_hi;          // d vmr_cg / d v_level / vmr_cg   (number-density term)
@@ -40,6 +42,8 @@ struct JacSumArgs {
     int n_layers, n_species, n_retr;
     float *tau, *dtlo, *dthi, *vlo, *vhi;   // [L][n]; vlo/vhi [r][L][n]
     double dnu;                   // internal grid step (cm^-1)
+    double nu_lo;                 // wavenumber of grid point 0
+    int far_d, far_hc;            // coarse far-wing split (grid steps); far_hc = 0 turns it off
 };
 
 struct RteJacArgs {

@@ -150,11 +150,24 @@ hipError_t launch_prep_grad(const JacPrepArgs &a, int max_lines, hipStream_t s) 
 }
 
 // ---------------------------------------------------------------------------
-// Gradient line sum: per (tile, layer) block, every line in the window, every
-// pair with the exact branch selection, window test and pedestal.  Per species
-// the clamped (sigma, dT, dp, dvs) (coef folded) are combined with the item's
-// chain-rule weights into tau, DTlo, DThi and, for retrieved species, Vlo/Vhi.
-#define JP 4
+// Gradient line sum: per (tile, layer) block of 256 * JP grid points, all
+// species.  Per species the seven sums (sigma, dsigma/dT, /dp, /dvs and, for
+// VP_Y lines, the dispersive term with its T and p derivatives) are accumulated
+// per grid point in FP32 with compensation; the clamped totals (coef folded)
+// are combined with the item's chain-rule weights into tau, DTlo, DThi and,
+// for retrieved species, Vlo/Vhi.  The block owns its outputs: they are
+// cleared once and updated per species (read-modify-write by the owning
+// thread), so no per-point state lives across species.
+//
+// Line ranges per species, as in the forward line sum: lines whose window
+// covers the whole coarse span and that lie at least far_d grid steps from it
+// (and beyond the |x| > x_far core distance from every point of the tile)
+// are FAR: their seven sums are evaluated at the coarse nodes and
+// interpolated; every other line in the window is evaluated pair by pair.
+#define JP IRS_JAC_TILE_P
+#define JFAR_MAXNC 128                  // coarse nodes per tile
+#define JFAR_CH 160                     // FAR lines per LDS chunk
+#define JNC 7                           // sums per grid point
 
 __device__ __forceinline__ void kadd(float &s, float &c, float v) {
     const float yv = v - c;
@@ -169,147 +182,357 @@ __device__ __forceinline__ int lower_key_j(const int32_t *k, int n, int64_t v) {
     return lo;
 }
 
-__device__ __forceinline__ int upper_key_j(const int32_t *k, int n, int64_t v) {
-    int lo = 0, hi = n;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int64_t)k[mid] <= v) lo = mid + 1; else hi = mid; }
-    return lo;
+// Gradient records are read with a wave-uniform line index: through the constant
+// address space they arrive by scalar loads into SGPRs.
+typedef float jac_f4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) jac_f4 *jrec;
+__device__ __forceinline__ float4 jld(jrec p, int64_t j) {
+    const jac_f4 v = p[j];
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 
-__global__ void __launch_bounds__(256) grad_linesum_kernel(const JacSumArgs a) {
-    __shared__ int s_rng[2];
+// Far-branch (|x| > x_far) Voigt terms of one line at offset du from its
+// centre, pedestal included (cross_section_jacobian.jl:43-56 and the pedestal
+// of :150-200): adds the line's {sigma, dT, dp, dvs} contributions to v[0..3].
+__device__ __forceinline__ void far_voigt_terms(float du, const float4 q0, const float4 q1, const float4 q2,
+                                                const float4 q3, float iC2, float (&v)[4]) {
+    const float ispi = 0.56418958354775628f;
+    const float x = du * q0.z, y = q0.w;
+    const float r = irs_rcpf(fmaf(x, x, y * y)), r2 = r * r;
+    const float H = y * ispi * r;
+    const float Hx = -2.0f * x * y * ispi * r2;
+    const float Hy = ispi * (x * x - y * y) * r2;
+    const float pedfac = fmaf(-du * du, iC2, 2.0f);
+    const float hm = H - pedfac * q3.x;
+    const float dHt = Hx * (du * q1.z) + Hy * q1.w;
+    const float dHp = Hx * (-q2.z * q0.z) + Hy * q2.x;
+    const float dHv = Hx * (-q2.w * q0.z) + Hy * q2.y;
+    const float dpfp = 2.0f * du * q2.z * iC2, dpfv = 2.0f * du * q2.w * iC2;
+    v[0] = fmaf(q1.x, hm, v[0]);
+    v[1] += q1.y * hm + q1.x * (dHt - pedfac * q3.y);
+    v[2] = fmaf(q1.x, dHp - (dpfp * q3.x + pedfac * q3.z), v[2]);
+    v[3] = fmaf(q1.x, dHv - (dpfv * q3.x + pedfac * q3.w), v[3]);
+}
+
+// Far-branch dispersive (VP_Y) terms: adds {d0, dT, dp} to v[0..2].  A line
+// without line mixing has q4 = {1, 1, 0, 0}, q5 = 0 and adds zero.
+__device__ __forceinline__ void far_disp_terms(float du, const float4 q2, const float4 q4, const float4 q5,
+                                               float (&v)[3]) {
+    const float ispi = 0.56418958354775628f;
+    const float xd = du * q4.x, yd = q4.y;
+    const float r = irs_rcpf(fmaf(xd, xd, yd * yd)), r2 = r * r;
+    const float Lv = xd * ispi * r;
+    const float Lx = ispi * (yd * yd - xd * xd) * r2;
+    const float Ly = -2.0f * xd * yd * ispi * r2;
+    v[0] = fmaf(q4.z, Lv, v[0]);
+    v[1] += q4.w * Lv + q4.z * (Lx * (du * q5.x) + Ly * q5.y);
+    v[2] += q5.z * Lv + q4.z * (Lx * (-q2.z * q4.x) + Ly * q5.w);
+}
+
+// Pair-by-pair sums of the lines [jlo, jhi) with the exact branch selection,
+// window tests and pedestal (cross_section_jacobian.jl:150-240).
+__device__ __forceinline__ void grad_pairs(jrec g0, jrec g1, jrec g2, jrec g3, jrec g4, jrec g5, int jlo, int jhi,
+                                           int tl, bool lm, float C2, float iC2, int CiD, float CfD, float xfar,
+                                           float (&sa)[JNC][JP], float (&ca)[JNC][JP]) {
+    const float ispi = 0.56418958354775628f;
+    for (int j = jlo; j < jhi; ++j) {
+        const float4 q0 = jld(g0, j), q1 = jld(g1, j), q2 = jld(g2, j), q3 = jld(g3, j);
+        const int dI0 = tl - __float_as_int(q0.x);
+        const float base = (float)dI0;
+        const bool dl = lm && jld(g4, j).z != 0.0f;
+#pragma unroll 1
+        for (int p = 0; p < JP; ++p) {
+            const float du = (base + (float)(256 * p)) - q0.y;
+            const float d2 = du * du;
+            const float x = du * q0.z, y = q0.w;
+            float H, Hx, Hy;
+            if (fabsf(x) > xfar) {
+                const float dd = fmaf(x, x, y * y), r = irs_rcpf(dd), r2 = r * r;
+                H = y * ispi * r;
+                Hx = -2.0f * x * y * ispi * r2;
+                Hy = ispi * (x * x - y * y) * r2;
+            } else {
+                float K, Lw;
+                w32(x, y, K, Lw);
+                H = K;
+                Hx = -2.0f * x * K + 2.0f * y * Lw;
+                Hy = 2.0f * (x * Lw + y * K) - 2.0f * ispi;
+            }
+            if (d2 <= C2) {
+                const float pedfac = fmaf(-d2, iC2, 2.0f);
+                const float hm = H - pedfac * q3.x;
+                const float dHt = Hx * (du * q1.z) + Hy * q1.w;
+                const float dHp = Hx * (-q2.z * q0.z) + Hy * q2.x;
+                const float dHv = Hx * (-q2.w * q0.z) + Hy * q2.y;
+                const float dpfp = 2.0f * du * q2.z * iC2, dpfv = 2.0f * du * q2.w * iC2;
+                kadd(sa[0][p], ca[0][p], q1.x * hm);
+                kadd(sa[1][p], ca[1][p], q1.y * hm + q1.x * (dHt - pedfac * q3.y));
+                kadd(sa[2][p], ca[2][p], q1.x * (dHp - (dpfp * q3.x + pedfac * q3.z)));
+                kadd(sa[3][p], ca[3][p], q1.x * (dHv - (dpfv * q3.x + pedfac * q3.w)));
+            }
+            if (dl && in_window(dI0 + 256 * p, q0.y, CiD, CfD)) {
+                const float4 q4 = jld(g4, j), q5 = jld(g5, j);
+                const float xd = du * q4.x, yd = q4.y;
+                float Lv, Lx, Ly;
+                if (fabsf(xd) > xfar) {
+                    const float dd = fmaf(xd, xd, yd * yd), r = irs_rcpf(dd), r2 = r * r;
+                    Lv = xd * ispi * r;
+                    Lx = ispi * (yd * yd - xd * xd) * r2;
+                    Ly = -2.0f * xd * yd * ispi * r2;
+                } else {
+                    float K, Lw;
+                    w32(xd, yd, K, Lw);
+                    Lv = Lw;
+                    Lx = -2.0f * (xd * Lw + yd * K) + 2.0f * ispi;
+                    Ly = -2.0f * xd * K + 2.0f * yd * Lw;
+                }
+                kadd(sa[4][p], ca[4][p], q4.z * Lv);
+                kadd(sa[5][p], ca[5][p], q4.w * Lv + q4.z * (Lx * (du * q5.x) + Ly * q5.y));
+                kadd(sa[6][p], ca[6][p], q5.z * Lv + q4.z * (Lx * (-q2.z * q4.x) + Ly * q5.w));
+            }
+        }
+    }
+}
+
+// FAR lines [jl0, jl1) and [jr0, jr1) at the NC coarse nodes Sl + n*HC.  The
+// records stream through LDS in chunks of JFAR_CH ({centre index - Sl,
+// fraction, dnu f, y} and g1..g3, plus g4, g5 with line mixing).  Thread t
+// evaluates the nodes 2g and 2g+1 (g = t % NG, NG = ceil(NC/2)) for the
+// chunk's lines k = s, s+S, ... (s = t / NG, S = 256 / NG slices); the node
+// offsets are exact integers, so du carries one rounding (the fraction).  Per
+// node the S slice sums of the seven terms are combined in FP64 through the
+// (then idle) staging buffer, and every grid point of the tile adds the
+// four-point (cubic) Lagrange interpolant of them.  The interpolant
+// reproduces the quadratic pedestal exactly, and for a Lorentz wing at
+// distance >= D its error is below 2.8 (HC/D)^4 of the line's contribution
+// (15 (HC/D)^4 for the 1/du^3 derivative terms).
+__device__ __forceinline__ void grad_far_coarse(const JacSumArgs &a, const JacItemDev &it, int jl0, int jl1, int jr0,
+                                                int jr1, int t, int64_t i0, int64_t Sl, int HC, int NC, bool lm,
+                                                float iC2, float4 *s_q, float *s_F, float (&sa)[JNC][JP],
+                                                float (&ca)[JNC][JP]) {
+    const int NG = (NC + 1) >> 1;
+    const int S = 256 / NG;
+    const int g = t % NG, sl = t / NG;
+    const bool act = t < S * NG;
+    const int nL = jl1 - jl0, ntot = nL + (jr1 - jr0);
+    const float xrel = (float)(2 * g * HC);           // offset of node 2g from Sl (exact)
+    const float fhc = (float)HC;
+    float A[JNC][2], Ac[JNC][2];
+#pragma unroll
+    for (int c = 0; c < JNC; ++c) A[c][0] = A[c][1] = Ac[c][0] = Ac[c][1] = 0.0f;
+    const float4 *g0 = a.g0 + it.rec_off, *g1 = a.g1 + it.rec_off, *g2 = a.g2 + it.rec_off;
+    const float4 *g3 = a.g3 + it.rec_off, *g4 = a.g4 + it.rec_off, *g5 = a.g5 + it.rec_off;
+    for (int base = 0; base < ntot; base += JFAR_CH) {
+        const int cnt = min(JFAR_CH, ntot - base);
+        if (t < cnt) {                                   // JFAR_CH <= 256: one record per thread
+            const int v = base + t;
+            const int j = v < nL ? jl0 + v : jr0 + (v - nL);
+            const float4 q = g0[j];
+            s_q[t] = make_float4((float)(__float_as_int(q.x) - (int)Sl), q.y, q.z, q.w);
+            s_q[JFAR_CH + t] = g1[j];
+            s_q[2 * JFAR_CH + t] = g2[j];
+            s_q[3 * JFAR_CH + t] = g3[j];
+            if (lm) {
+                s_q[4 * JFAR_CH + t] = g4[j];
+                s_q[5 * JFAR_CH + t] = g5[j];
+            }
+        }
+        __syncthreads();
+        if (act) {
+            float part[JNC][2];
+#pragma unroll
+            for (int c = 0; c < JNC; ++c) part[c][0] = part[c][1] = 0.0f;
+            int n = 0;
+            for (int k = sl; k < cnt; k += S) {
+                const float4 q0 = s_q[k], q1 = s_q[JFAR_CH + k], q2 = s_q[2 * JFAR_CH + k];
+                const float4 q3 = s_q[3 * JFAR_CH + k];
+                const float dI = xrel - q0.x;            // integer, exact
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const float du = (m ? dI + fhc : dI) - q0.y;
+                    float v[4] = {part[0][m], part[1][m], part[2][m], part[3][m]};
+                    far_voigt_terms(du, q0, q1, q2, q3, iC2, v);
+                    part[0][m] = v[0]; part[1][m] = v[1]; part[2][m] = v[2]; part[3][m] = v[3];
+                }
+                if (lm) {
+                    const float4 q4 = s_q[4 * JFAR_CH + k], q5 = s_q[5 * JFAR_CH + k];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        const float du = (m ? dI + fhc : dI) - q0.y;
+                        float v[3] = {part[4][m], part[5][m], part[6][m]};
+                        far_disp_terms(du, q2, q4, q5, v);
+                        part[4][m] = v[0]; part[5][m] = v[1]; part[6][m] = v[2];
+                    }
+                }
+                if (++n == 32) {
+#pragma unroll
+                    for (int c = 0; c < JNC; ++c) {
+                        kadd(A[c][0], Ac[c][0], part[c][0]);
+                        kadd(A[c][1], Ac[c][1], part[c][1]);
+                        part[c][0] = part[c][1] = 0.0f;
+                    }
+                    n = 0;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < JNC; ++c) {
+                kadd(A[c][0], Ac[c][0], part[c][0]);
+                kadd(A[c][1], Ac[c][1], part[c][1]);
+            }
+        }
+        __syncthreads();
+    }
+    // per-(slice, node, term) partial sums in the staging buffer: S * 2NG * 7 <= 3584 floats
+    float *red = (float *)s_q;
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < JNC; ++c) {
+            red[((sl * 2 * NG) + 2 * g) * JNC + c] = A[c][0] - Ac[c][0];
+            red[((sl * 2 * NG) + 2 * g + 1) * JNC + c] = A[c][1] - Ac[c][1];
+        }
+    }
+    __syncthreads();
+    if (t < NC) {
+#pragma unroll
+        for (int c = 0; c < JNC; ++c) {
+            double v = 0.0;
+            for (int q = 0; q < S; ++q) v += (double)red[((q * 2 * NG) + t) * JNC + c];
+            s_F[c * JFAR_MAXNC + t] = (float)v;
+        }
+    }
+    __syncthreads();
+    const float ihc = 1.0f / (float)HC;
+    const int nterm = lm ? JNC : 4;
+#pragma unroll
+    for (int p = 0; p < JP; ++p) {
+        const int off = (int)(i0 - Sl) + t + 256 * p;          // grid offset from node 0
+        const int n0 = off / HC;                               // node left of the point, >= 1
+        const float u = (float)(off - n0 * HC) * ihc;          // in [0, 1)
+        const float wm = -u * (u - 1.0f) * (u - 2.0f) * (1.0f / 6.0f);
+        const float w0 = (u + 1.0f) * (u - 1.0f) * (u - 2.0f) * 0.5f;
+        const float w1 = -(u + 1.0f) * u * (u - 2.0f) * 0.5f;
+        const float w2 = (u + 1.0f) * u * (u - 1.0f) * (1.0f / 6.0f);
+#pragma unroll
+        for (int c = 0; c < JNC; ++c) {
+            if (c >= nterm) break;
+            const float *F = s_F + c * JFAR_MAXNC + n0;
+            kadd(sa[c][p], ca[c][p], wm * F[-1] + w0 * F[0] + w1 * F[1] + w2 * F[2]);
+        }
+    }
+    __syncthreads();
+}
+
+#ifndef IRS_JAC_MINB
+#define IRS_JAC_MINB 1
+#endif
+__global__ void __launch_bounds__(256, IRS_JAC_MINB) grad_linesum_kernel(const JacSumArgs a) {
+    __shared__ int s_rng[6];
+    __shared__ float4 s_q[6 * JFAR_CH];
+    __shared__ float s_F[JNC * JFAR_MAXNC];
     const int t = threadIdx.x;
     const int layer = blockIdx.y;
     const int64_t i0 = (int64_t)blockIdx.x * (256 * JP);
     const int64_t i1 = min(i0 + (int64_t)(256 * JP), a.n_hi);
-    float tau[JP], dtlo[JP], dthi[JP], vlo[IRS_JAC_MAXR][JP], vhi[IRS_JAC_MAXR][JP];
+    const int tl = (int)(i0 + t);
 #pragma unroll
     for (int p = 0; p < JP; ++p) {
-        tau[p] = dtlo[p] = dthi[p] = 0.0f;
-#pragma unroll
-        for (int r = 0; r < IRS_JAC_MAXR; ++r) vlo[r][p] = vhi[r][p] = 0.0f;
+        const int64_t i = i0 + t + 256 * p;
+        if (i >= a.n_hi) continue;
+        const int64_t o = (int64_t)layer * a.n_hi + i;
+        a.tau[o] = 0.0f;
+        a.dtlo[o] = 0.0f;
+        a.dthi[o] = 0.0f;
+        for (int r = 0; r < a.n_retr; ++r) {
+            a.vlo[((int64_t)r * a.n_layers + layer) * a.n_hi + i] = 0.0f;
+            a.vhi[((int64_t)r * a.n_layers + layer) * a.n_hi + i] = 0.0f;
+        }
     }
+    // coarse far-wing split: nodes Sl + n*HC, n = 0..NC-1 (HC divides the tile width)
+    const int HC = a.far_hc;
+    const int NC = HC > 0 ? (256 * JP) / HC + 3 : 0;
+    const int64_t Sl = i0 - HC, Sh = Sl + (int64_t)(NC - 1) * HC, Dd = a.far_d;
     for (int s = 0; s < a.n_species; ++s) {
         const JacItemDev it = a.items[(int64_t)layer * a.n_species + s];
         if (!it.active || it.n == 0) continue;
+        const bool lm = it.lm != 0;
         const int32_t *keys = a.keys + it.key_off;
-        const double Cmx = it.lm ? fmax((double)it.C, (double)it.Clm) : (double)it.C;
-        const int64_t Cmax = (int64_t)ceil(Cmx), M = it.M;
-        if (t < 2) s_rng[t] = t == 0 ? lower_key_j(keys, it.n, i0 - Cmax - M - 1)
-                                     : upper_key_j(keys, it.n, i1 - 1 + Cmax + M + 1);
+        const double Cmx = lm ? fmax((double)it.C, (double)it.Clm) : (double)it.C;
+        const double Cmn = lm ? fmin((double)it.C, (double)it.Clm) : (double)it.C;
+        const int64_t Cmax = (int64_t)ceil(Cmx), Cmin = (int64_t)floor(Cmn), M = it.M;
+        // key bounds (centre index c in [key - M, key + M]): the window [vA, vF],
+        // the core distance K (|x| <= x_far within K grid steps of the centre)
+        const double numax = a.nu_lo + (double)(i1 + Cmax + M) * a.dnu;
+        const int64_t K = (int64_t)ceil(it.kcore_per_nu * numax) + 2;
+        const int64_t vA = i0 - Cmax - M - 1, vF = i1 - 1 + Cmax + M + 1;
+        const int64_t vC = i0 - K - M - 1, vD = i1 - 1 + K + M + 1;
+        const int64_t vB2 = Sh - Cmin + M + 1, vG = Sl - Dd - M - 1;
+        const int64_t vH = Sh + Dd + M, vE2 = Sl + Cmin - M - 2;
+        const bool far = HC > 0 && NC <= JFAR_MAXNC && vA <= vB2 && vB2 <= vG && vG < vC && vD < vH &&
+                         vH <= vE2 && vE2 <= vF;
+        if (t < 6) {
+            int64_t v;
+            switch (t) {
+                case 0: v = vA; break;
+                case 1: v = vF + 1; break;
+                case 2: v = vB2; break;
+                case 3: v = vG + 1; break;
+                case 4: v = vH; break;
+                default: v = vE2 + 1; break;
+            }
+            s_rng[t] = (t < 2 || far) ? lower_key_j(keys, it.n, v) : 0;
+        }
         __syncthreads();
         const int jA = __builtin_amdgcn_readfirstlane(s_rng[0]);
         const int jF = __builtin_amdgcn_readfirstlane(s_rng[1]);
+        const int jB2 = far ? __builtin_amdgcn_readfirstlane(s_rng[2]) : jF;
+        const int jG = far ? __builtin_amdgcn_readfirstlane(s_rng[3]) : jF;
+        const int jH = far ? __builtin_amdgcn_readfirstlane(s_rng[4]) : jF;
+        const int jE2 = far ? __builtin_amdgcn_readfirstlane(s_rng[5]) : jF;
         __syncthreads();
-        const float4 *g0 = a.g0 + it.rec_off, *g1 = a.g1 + it.rec_off, *g2 = a.g2 + it.rec_off;
-        const float4 *g3 = a.g3 + it.rec_off, *g4 = a.g4 + it.rec_off, *g5 = a.g5 + it.rec_off;
+        const jrec g0 = (jrec)(a.g0 + it.rec_off), g1 = (jrec)(a.g1 + it.rec_off), g2 = (jrec)(a.g2 + it.rec_off);
+        const jrec g3 = (jrec)(a.g3 + it.rec_off), g4 = (jrec)(a.g4 + it.rec_off), g5 = (jrec)(a.g5 + it.rec_off);
         const float C2 = (float)((double)it.C * it.C);
         const double Cld = fmin(it.lm_cutoff / a.dnu, 1.0e9);
         const int CiD = (int)floor(Cld);
         const float CfD = (float)(Cld - (double)CiD);
         const float iC2 = (float)(1.0 / ((double)it.C * it.C));
         const float xfar = (float)it.x_far;
-        const float ispi = 0.56418958354775628f;
-        float s0[JP], c0[JP], sT[JP], cT[JP], sP[JP], cP[JP], sV[JP], cV[JP];
-        float d0[JP], e0[JP], dT_[JP], eT[JP], dP[JP], eP[JP];
+        float sa[JNC][JP], ca[JNC][JP];
 #pragma unroll
-        for (int p = 0; p < JP; ++p) {
-            s0[p] = c0[p] = sT[p] = cT[p] = sP[p] = cP[p] = sV[p] = cV[p] = 0.0f;
-            d0[p] = e0[p] = dT_[p] = eT[p] = dP[p] = eP[p] = 0.0f;
-        }
-        const int tl = (int)(i0 + t);
-        for (int j = jA; j < jF; ++j) {
-            const float4 q0 = g0[j], q1 = g1[j], q2 = g2[j], q3 = g3[j];
-            const float base = (float)(tl - __float_as_int(q0.x));
-            const bool dl = it.lm && g4[j].z != 0.0f;
-#pragma unroll 1
-            for (int p = 0; p < JP; ++p) {
-                const float du = (base + (float)(256 * p)) - q0.y;
-                const float d2 = du * du;
-                const float x = du * q0.z, y = q0.w;
-                float H, Hx, Hy;
-                if (fabsf(x) > xfar) {
-                    const float dd = fmaf(x, x, y * y), r = irs_rcpf(dd), r2 = r * r;
-                    H = y * ispi * r;
-                    Hx = -2.0f * x * y * ispi * r2;
-                    Hy = ispi * (x * x - y * y) * r2;
-                } else {
-                    float K, Lw;
-                    w32(x, y, K, Lw);
-                    H = K;
-                    Hx = -2.0f * x * K + 2.0f * y * Lw;
-                    Hy = 2.0f * (x * Lw + y * K) - 2.0f * ispi;
-                }
-                if (d2 <= C2) {
-                    const float pedfac = fmaf(-d2, iC2, 2.0f);
-                    const float hm = H - pedfac * q3.x;
-                    const float dHt = Hx * (du * q1.z) + Hy * q1.w;
-                    const float dHp = Hx * (-q2.z * q0.z) + Hy * q2.x;
-                    const float dHv = Hx * (-q2.w * q0.z) + Hy * q2.y;
-                    const float dpfp = 2.0f * du * q2.z * iC2, dpfv = 2.0f * du * q2.w * iC2;
-                    kadd(s0[p], c0[p], q1.x * hm);
-                    kadd(sT[p], cT[p], q1.y * hm + q1.x * (dHt - pedfac * q3.y));
-                    kadd(sP[p], cP[p], q1.x * (dHp - (dpfp * q3.x + pedfac * q3.z)));
-                    kadd(sV[p], cV[p], q1.x * (dHv - (dpfv * q3.x + pedfac * q3.w)));
-                }
-                if (dl && in_window(tl - __float_as_int(q0.x) + 256 * p, q0.y, CiD, CfD)) {
-                    const float4 q4 = g4[j], q5 = g5[j];
-                    const float xd = du * q4.x, yd = q4.y;
-                    float Lv, Lx, Ly;
-                    if (fabsf(xd) > xfar) {
-                        const float dd = fmaf(xd, xd, yd * yd), r = irs_rcpf(dd), r2 = r * r;
-                        Lv = xd * ispi * r;
-                        Lx = ispi * (yd * yd - xd * xd) * r2;
-                        Ly = -2.0f * xd * yd * ispi * r2;
-                    } else {
-                        float K, Lw;
-                        w32(xd, yd, K, Lw);
-                        Lv = Lw;
-                        Lx = -2.0f * (xd * Lw + yd * K) + 2.0f * ispi;
-                        Ly = -2.0f * xd * K + 2.0f * yd * Lw;
-                    }
-                    kadd(d0[p], e0[p], q4.z * Lv);
-                    kadd(dT_[p], eT[p], q4.w * Lv + q4.z * (Lx * (du * q5.x) + Ly * q5.y));
-                    kadd(dP[p], eP[p], q5.z * Lv + q4.z * (Lx * (-q2.z * q4.x) + Ly * q5.w));
-                }
-            }
+        for (int c = 0; c < JNC; ++c)
+#pragma unroll
+            for (int p = 0; p < JP; ++p) sa[c][p] = ca[c][p] = 0.0f;
+        if (far) {
+            if (jG > jB2 || jE2 > jH)
+                grad_far_coarse(a, it, jB2, jG, jH, jE2, t, i0, Sl, HC, NC, lm, iC2, s_q, s_F, sa, ca);
+            grad_pairs(g0, g1, g2, g3, g4, g5, jA, jB2, tl, lm, C2, iC2, CiD, CfD, xfar, sa, ca);
+            grad_pairs(g0, g1, g2, g3, g4, g5, jG, jH, tl, lm, C2, iC2, CiD, CfD, xfar, sa, ca);
+            grad_pairs(g0, g1, g2, g3, g4, g5, jE2, jF, tl, lm, C2, iC2, CiD, CfD, xfar, sa, ca);
+        } else {
+            grad_pairs(g0, g1, g2, g3, g4, g5, jA, jF, tl, lm, C2, iC2, CiD, CfD, xfar, sa, ca);
         }
 #pragma unroll
         for (int p = 0; p < JP; ++p) {
-            double sg = (double)s0[p] - (double)c0[p], gT = (double)sT[p] - (double)cT[p];
-            double gP = (double)sP[p] - (double)cP[p], gV = (double)sV[p] - (double)cV[p];
+            const int64_t i = i0 + t + 256 * p;
+            if (i >= a.n_hi) continue;
+            double sg = (double)sa[0][p] - (double)ca[0][p], gT = (double)sa[1][p] - (double)ca[1][p];
+            double gP = (double)sa[2][p] - (double)ca[2][p], gV = (double)sa[3][p] - (double)ca[3][p];
             if (sg <= 0.0) { sg = 0.0; gT = 0.0; gP = 0.0; gV = 0.0; }
-            if (it.lm) {
-                sg += (double)d0[p] - (double)e0[p];
-                gT += (double)dT_[p] - (double)eT[p];
-                gP += (double)dP[p] - (double)eP[p];
+            if (lm) {
+                sg += (double)sa[4][p] - (double)ca[4][p];
+                gT += (double)sa[5][p] - (double)ca[5][p];
+                gP += (double)sa[6][p] - (double)ca[6][p];
                 if (sg <= 0.0) { sg = 0.0; gT = 0.0; gP = 0.0; }
             }
-            tau[p] += (float)sg;
-            dtlo[p] += (float)(gT * it.wT_lo);
-            dthi[p] += (float)(gT * it.wT_hi);
+            const int64_t o = (int64_t)layer * a.n_hi + i;
+            a.tau[o] += (float)sg;
+            a.dtlo[o] += (float)(gT * it.wT_lo);
+            a.dthi[o] += (float)(gT * it.wT_hi);
             if (it.retr >= 0) {
-#pragma unroll
-                for (int r = 0; r < IRS_JAC_MAXR; ++r) {
-                    if (r != it.retr) continue;
-                    vlo[r][p] += (float)(sg * it.gv_lo + gP * it.gp_lo + gT * it.gt_lo + gV * it.gs_lo);
-                    vhi[r][p] += (float)(sg * it.gv_hi + gP * it.gp_hi + gT * it.gt_hi + gV * it.gs_hi);
-                }
+                const int64_t ov = ((int64_t)it.retr * a.n_layers + layer) * a.n_hi + i;
+                a.vlo[ov] += (float)(sg * it.gv_lo + gP * it.gp_lo + gT * it.gt_lo + gV * it.gs_lo);
+                a.vhi[ov] += (float)(sg * it.gv_hi + gP * it.gp_hi + gT * it.gt_hi + gV * it.gs_hi);
             }
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < JP; ++p) {
-        const int64_t i = i0 + t + 256 * p;
-        if (i >= a.n_hi) continue;
-        const int64_t o = (int64_t)layer * a.n_hi + i;
-        a.tau[o] = tau[p];
-        a.dtlo[o] = dtlo[p];
-        a.dthi[o] = dthi[p];
-#pragma unroll
-        for (int r = 0; r < IRS_JAC_MAXR; ++r) {
-            if (r >= a.n_retr) break;
-            a.vlo[((int64_t)r * a.n_layers + layer) * a.n_hi + i] = vlo[r][p];
-            a.vhi[((int64_t)r * a.n_layers + layer) * a.n_hi + i] = vhi[r][p];
         }
     }
 }
