@@ -860,11 +860,15 @@ __global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_kernel
 }
 
 // Persistent form: a grid sized to the resident capacity of the device pulls
-// units from a queue counter in order of decreasing tile cost (tile_order,
-// heaviest tiles first, every layer and profile of a tile together).  The
-// unit index is fetched by one lane and broadcast through LDS, so the loop
-// exit is block-uniform and every wave of every block reaches it once the
-// queue is past the last unit.  The counter must be zero at launch.
+// units from a queue counter profile by profile, and within a profile in order
+// of decreasing tile cost (tile_order, heaviest tiles first, every layer of a
+// tile together).  The resident units then span some tens of tiles of each of
+// one profile's layers, whose neighbouring tiles read the same line records
+// through L2; a tile-major order across profiles would spread them over two
+// tiles of every (profile, layer) item with no such reuse.  The unit index is
+// fetched by one lane and broadcast through LDS, so the loop exit is
+// block-uniform and every wave of every block reaches it once the queue is
+// past the last unit.  The counter must be zero at launch.
 template <int P>
 __global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_persistent_kernel(const LinesumArgs a) {
     __shared__ int s_rng[12];
@@ -872,18 +876,19 @@ __global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_persis
     __shared__ float4 s_rec[2 * LS_CHUNK];
     __shared__ float s_far[FAR_LDS];
     __shared__ unsigned s_unit;
-    const unsigned per_tile = (unsigned)a.n_layers * (unsigned)a.n_prof;
-    const unsigned total = (unsigned)a.tiles * per_tile;
+    const unsigned per_prof = (unsigned)a.tiles * (unsigned)a.n_layers;
+    const unsigned total = per_prof * (unsigned)a.n_prof;
     for (;;) {
         if (threadIdx.x == 0) s_unit = atomicAdd(a.queue, 1u);
         __syncthreads();
         const unsigned u = s_unit;
         __syncthreads();
         if (u >= total) break;
-        const unsigned rank = u / per_tile, rem = u - rank * per_tile;
+        const unsigned prof = u / per_prof, rem = u - prof * per_prof;
+        const unsigned rank = rem / (unsigned)a.n_layers;
         const int tile = a.tile_order ? a.tile_order[rank] : (int)rank;
-        const int prof = (int)(rem / (unsigned)a.n_layers), layer = (int)(rem % (unsigned)a.n_layers);
-        linesum_unit<P>(a, tile, layer, prof, s_rng, s_red, s_rec, s_far);
+        const int layer = (int)(rem - rank * (unsigned)a.n_layers);
+        linesum_unit<P>(a, tile, layer, (int)prof, s_rng, s_red, s_rec, s_far);
     }
 }
 
