@@ -16,6 +16,9 @@
  *
  * Reference interfaces replaced (paths relative to the reference repository):
  *   irs_forward            forward_model            src/Sensor/sounder.jl:233-360
+ *   irs_faddeeva32         faddeeva_voigt / _voigt_H (Re w) and the Im w of the VP_Y term
+ *                                                   src/CrossSections/voigt.jl:158-172,
+ *                                                   src/CrossSections/line_mixing.jl:966-1005
  *   irs_xsec_layer         compute_voigt_cross_sections (+ _reject_weak_lines,
  *                          compute_voigt_lm_cross_sections)
  *                                                   src/CrossSections/voigt.jl:333-462,
@@ -204,6 +207,14 @@ int irs_channel_transmittance(irs_ctx *ctx, const irs_profile *profiles, int n_p
 /* Copies a sorted line list to the device; returns a handle. */
 int irs_linelist_upload(irs_ctx *ctx, const irs_linelist *ll, int32_t *handle);
 int irs_linelist_free(irs_ctx *ctx, int32_t handle);
+
+/* w(x + iy) = exp(-z^2) erfc(-iz), y >= 0, for n host points, evaluated on the
+ * device by the FP32 routines the line sums use: mode 0 follows the CORE range
+ * (inside |z| < 8 the trapezoid rule for y < 6, else the continued fraction),
+ * mode 1 the per-pair ranges and the gradient sum (trapezoid for |x| + y < 5.5).
+ * Re w is the Voigt H function (faddeeva_voigt); Im w enters the VP_Y term. */
+int irs_faddeeva32(irs_ctx *ctx, const double *x, const double *y, int64_t n, int mode, double *w_re,
+                   double *w_im);
 
 /* Cross sections of one species in one layer on nu_i = grid node i of
  * (nu0, dnu): sigma[n_nu] in cm^2/molec.  dptmn > 0 applies DPTMIN with the

@@ -876,6 +876,28 @@ int irs_linelist_free(irs_ctx *ctx, int32_t handle) {
     return IRS_OK;
 }
 
+int irs_faddeeva32(irs_ctx *ctx, const double *x, const double *y, int64_t n, int mode, double *w_re,
+                   double *w_im) {
+    if (!ctx) return IRS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (!x || !y || !w_re || !w_im || n < 0) return fail(ctx, IRS_EINVAL, "bad arguments");
+    if (mode != 0 && mode != 1) return fail(ctx, IRS_EINVAL, "mode must be 0 (core rule) or 1 (pair rule)");
+    for (int64_t i = 0; i < n; ++i)
+        if (!(y[i] >= 0.0) || !std::isfinite(x[i]) || !std::isfinite(y[i]))
+            return fail(ctx, IRS_EINVAL, "w(z) needs finite x and y >= 0");
+    if (n == 0) return IRS_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    HIPCHK(ensure(ctx->d_tmp2, 4 * b));
+    double *d = (double *)ctx->d_tmp2.p;
+    HIPCHK(hipMemcpyAsync(d, x, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d + n, y, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_faddeeva32(d, d + n, n, mode, d + 2 * n, d + 3 * n, ctx->stream));
+    HIPCHK(hipMemcpyAsync(w_re, d + 2 * n, b, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(w_im, d + 3 * n, b, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IRS_OK;
+}
+
 int irs_xsec_layer(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t n_nu, double T, double p_atm,
                    double vmr_self, double coef, double dptmn, double cutoff, double lm_cutoff, double x_far,
                    int line_mixing, double *sigma) {

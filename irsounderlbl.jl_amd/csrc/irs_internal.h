@@ -139,6 +139,8 @@ struct ChodArgs {
 // launchers (k_*.hip)
 hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s);
 hipError_t launch_keytab(const int32_t *keys, int n, int64_t lo, int64_t count, int32_t *out, hipStream_t s);
+hipError_t launch_faddeeva32(const double *x, const double *y, int64_t n, int mode, double *re, double *im,
+                             hipStream_t s);
 hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hipStream_t s);
 hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s);
 int linesum_persistent_capacity(int P);

@@ -332,3 +332,25 @@ hipError_t launch_keys(const double *nu, int64_t n, double nu_lo, double dnu, in
                        dnu, keys);
     return hipGetLastError();
 }
+
+// w(x + iy) with the FP32 routines of the line sums, one thread per point:
+// mode 0 is the rule of the CORE range (w32_core inside |z| < 8, else w32),
+// mode 1 the rule of the per-pair ranges and the gradient sum (w32).
+__global__ void __launch_bounds__(256) faddeeva32_kernel(const double *x, const double *y, int64_t n, int mode,
+                                                         double *re, double *im) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float xf = (float)x[i], yf = (float)y[i];
+    float wr, wi;
+    if (mode == 0 && fmaf(xf, xf, yf * yf) < (float)(IRS_Z1 * IRS_Z1)) w32_core(xf, yf, wr, wi);
+    else w32(xf, yf, wr, wi);
+    re[i] = (double)wr;
+    im[i] = (double)wi;
+}
+
+hipError_t launch_faddeeva32(const double *x, const double *y, int64_t n, int mode, double *re, double *im,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(faddeeva32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n, mode, re, im);
+    return hipGetLastError();
+}

@@ -89,7 +89,7 @@ class IrsSchedule(C.Structure):
 
 EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_last_error",
            "irs_default_options", "irs_last_stats", "irs_linelist_upload", "irs_linelist_free",
-           "irs_xsec_layer", "irs_rte", "irs_ils_half", "irs_ils_kernel", "irs_ils_channels",
+           "irs_xsec_layer", "irs_faddeeva32", "irs_rte", "irs_ils_half", "irs_ils_kernel", "irs_ils_channels",
            "irs_brightness_temperature", "irs_forward_dims", "irs_forward", "irs_state_size",
            "irs_jacobian", "irs_set_schedule", "irs_get_schedule", "irs_channel_od",
            "irs_layer_properties", "irs_mlp_create", "irs_mlp_destroy", "irs_mlp_last_error",
@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_xsec_layer.argtypes = [P, C.c_int32, f64, f64, i64, f64, f64, f64, f64, f64, f64, f64,
                                  f64, i32, P]
     L.irs_rte.argtypes = [P, f64, f64, i64, P, i32, P, f64, f64, f64, i32, P, P]
+    L.irs_faddeeva32.argtypes = [P, P, P, i64, i32, P, P]
     L.irs_ils_half.argtypes = [f64, C.POINTER(i64)]
     L.irs_ils_kernel.argtypes = [P, f64, f64, f64, i32, i32, P]
     L.irs_ils_channels.argtypes = [P, P, f64, f64, i64, C.POINTER(IrsSounder), i32, P]
@@ -276,6 +277,21 @@ def compute_voigt_cross_sections(nu_grid: WavenumberGrid, linelist: HITRANLineli
                                      float(p_atm), float(vmr_self), float(coef), float(dptmn),
                                      float(cutoff), float(cutoff), float(x_far), 0, _p(out)))
     return out
+
+
+def faddeeva32(x, y, *, rule: str = "core", ctx: Optional[Context] = None) -> np.ndarray:
+    """w(x + iy) (complex, y >= 0) from the FP32 device routines of the line
+    sums: rule "core" is the CORE-range rule, "pair" the per-pair and gradient
+    rule.  Re w is faddeeva_voigt (voigt.jl:158-160)."""
+    ctx = ctx or default_context()
+    shape = np.broadcast(np.asarray(x), np.asarray(y)).shape
+    x = np.ascontiguousarray(np.broadcast_to(np.asarray(x, np.float64), shape)).ravel()
+    y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float64), shape)).ravel()
+    if rule not in ("core", "pair"):
+        raise ValueError("rule must be 'core' or 'pair'")
+    re, im = np.zeros(len(x)), np.zeros(len(x))
+    ctx.check(ctx.lib.irs_faddeeva32(ctx.h, _p(x), _p(y), len(x), 0 if rule == "core" else 1, _p(re), _p(im)))
+    return (re + 1j * im).reshape(shape)
 
 
 def compute_voigt_lm_cross_sections(nu_grid: WavenumberGrid, ll_co2: HITRANLinelist, T: float,
