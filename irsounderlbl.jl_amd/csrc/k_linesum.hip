@@ -185,49 +185,91 @@ __device__ __forceinline__ float series_core(float v, const float4 k0, const flo
 }
 
 // Voigt term of one CORE pair outside the far form: Sn Re w(x + iy) (the series
-// where |x| <= x_far, the Lorentz wing beyond it: both share v).
-__device__ __forceinline__ float core_voigt(float du, const float4 q1, float sn, crec c, float xfar, bool inner) {
+// where |x| <= x_far, the Lorentz wing beyond it: both share v).  `pin` is set
+// where the point needs the exact w(z) (|x| <= x_far and |z| < IRS_Z1); the
+// returned value there is the series value, which the caller replaces.
+__device__ __forceinline__ float core_voigt(float du, const float4 q1, crec c, float xfar, bool &pin) {
     const float4 k0 = ld(c, 0), k1 = ld(c, 1), k2 = ld(c, 2);
     const float x = du * q1.z;
     const float zz = fmaf(x, x, k1.w);
     const float v = irs_rcpf(zz);
     const bool in_core = fabsf(x) <= xfar;
-    float h = k2.x * v * (in_core ? series_core(v, k0, k1, 1.5f) : 1.0f);
-    if (inner && in_core && zz < (float)(IRS_Z1 * IRS_Z1)) {
-        float wr, wi;
-        w32_core(x, q1.w, wr, wi);
-        h = sn * wr;
-    }
-    return h;
+    pin = in_core && zz < (float)(IRS_Z1 * IRS_Z1);
+    return k2.x * v * (in_core ? series_core(v, k0, k1, 1.5f) : 1.0f);
 }
 
-// Dispersive term of one CORE pair outside the far form: YSn Im w(x_d + i y_d).
-__device__ __forceinline__ float core_disp(float du, const float4 q2, crec c, float xfar, bool inner) {
+// Dispersive term of one CORE pair outside the far form: YSn Im w(x_d + i y_d),
+// with `pin` as for core_voigt.
+__device__ __forceinline__ float core_disp(float du, const float4 q2, crec c, float xfar, bool &pin) {
     const float4 k0 = ld(c, 3), k1 = ld(c, 4), k2 = ld(c, 5);
     const float x = du * q2.y;
     const float zz = fmaf(x, x, k1.w);
     const float v = irs_rcpf(zz);
     const bool in_core = fabsf(x) <= xfar;
-    float d = (k2.x * x) * v * (in_core ? series_core(v, k0, k1, 0.5f) : 1.0f);
-    if (inner && in_core && zz < (float)(IRS_Z1 * IRS_Z1)) {
-        float wr, wi;
-        w32_core(x, q2.z, wr, wi);
-        d = q2.w * wi;
-    }
-    return d;
+    pin = in_core && zz < (float)(IRS_Z1 * IRS_Z1);
+    return (k2.x * x) * v * (in_core ? series_core(v, k0, k1, 0.5f) : 1.0f);
 }
+
+// Exact inner-core values: Sn Re w (Voigt, type 0) or YSn Im w (dispersive, type 1),
+// one w(z) evaluation site for both.
+__device__ __forceinline__ float inner_value(float du, const float4 q1, const float4 q2, int type) {
+    float wr, wi;
+    w32_core(du * (type ? q2.y : q1.z), type ? q2.z : q1.w, wr, wi);
+    return type ? q2.w * wi : q2.x * wr;
+}
+
+// Queues the inner points m (a contiguous lane range) of slice p as one task of
+// at most 32 lanes, or two when the range is wider.
+__device__ __forceinline__ void queue_inner(unsigned long long m, int j, int p, int type, int lane, int &qn, int &qj,
+                                            int &qi) {
+    const int n = __popcll(m);
+    if (n == 0) return;
+    const int a = __ffsll((long long)m) - 1;
+    const int n0 = min(n, 32);
+    if (lane == qn) { qj = j; qi = p | (a << 4) | (n0 << 10) | (type << 17); }
+    ++qn;
+    if (n > 32) {
+        if (lane == qn) { qj = j; qi = p | ((a + 32) << 4) | ((n - 32) << 10) | (type << 17); }
+        ++qn;
+    }
+}
+
+// Inner-core work is packed: a line's points with |z| < IRS_Z1 span ~30 grid
+// points, so evaluating them where they fall would run w(z) over whole 64-lane
+// slices (1.5 slices per line on average).  Instead the slice's inner points
+// (a contiguous lane range [a, a + n), found by ballot) are zeroed in the main
+// pass and queued as a task (line, slice, a, n, type) in two registers; full
+// queues are flushed two tasks per wave evaluation (one per half-wave), each
+// lane adding its value with an LDS atomic into the wave's private slot array
+// [type][P][64] (in the otherwise idle FAST staging buffer), which the lanes
+// fold into their compensated sums at the end of the range (a slice with more
+// than 32 inner points makes two tasks).  w(z) then has one code site in the
+// kernel instead of one per slice and term.  The deferred slots need
+// P * (LM ? 2 : 1) * 4 * 64 floats of the staging buffer.
+#define CORE_DEFER(P, LM) ((P) * ((LM) ? 2 : 1) * 4 * 64 <= 2 * LS_CHUNK * 4)
 
 template <int P, bool LM>
 __device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3p,
-                                           int jlo, int jhi, int tl, float xfar, float (&acc)[P], float (&cmp)[P],
+                                           int jlo, int jhi, int tl, float xfar, float *s_inner,
+                                           float (&acc)[P], float (&cmp)[P],
                                            float (&accd)[P], float (&cmpd)[P]) {
+    constexpr bool DEFER = CORE_DEFER(P, LM);
+    const int lane = threadIdx.x & 63;
     const int tw = uni(tl & ~63);                      // first point of this wave's slices
-    for (int j0 = jlo; j0 < jhi; j0 += 16) {
-        const int je = min(j0 + 16, jhi);
+    float *iv = s_inner + (threadIdx.x >> 6) * (P * 64 * (LM ? 2 : 1));
+    if (DEFER) {
+        __syncthreads();                               // the staging buffer is free
+#pragma unroll
+        for (int p = 0; p < P * (LM ? 2 : 1); ++p) iv[p * 64 + lane] = 0.0f;
+    }
+    int qn = 0, qj = 0, qi = 0;                        // task queue (lane q holds task q)
+    int j = jlo;
+    while (j < jhi) {
+        const int je = min(j + 16, jhi);
         float part[P], partd[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) { part[p] = 0.0f; partd[p] = 0.0f; }
-        for (int j = j0; j < je; ++j) {
+        for (; j < je && (!DEFER || qn <= 64 - 4 * P); ++j) {
             const float4 q0 = ld(r0p, j), q1 = ld(r1p, j), q2 = ld(r2p, j), c2 = ld(r3p + 6 * (int64_t)j, 2);
             const float4 c5 = LM ? ld(r3p + 6 * (int64_t)j, 5) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const crec c = r3p + 6 * (int64_t)j;
@@ -245,11 +287,37 @@ __device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3
                 const bool farV = mind > KO, farD = !LM || mind > KOd;
                 float r = 0.0f;
                 if (farV || farD) r = irs_rcpf(fmaf(du, du, q0.z));
-                if (farV) part[p] = fmaf(q0.w, r, part[p]);
-                else part[p] += core_voigt(du, q1, q2.x, c, xfar, mind <= KI);
+                if (farV) {
+                    part[p] = fmaf(q0.w, r, part[p]);
+                } else {
+                    bool pin;
+                    float h = core_voigt(du, q1, c, xfar, pin);
+                    if (mind <= KI) {
+                        if (DEFER) {
+                            queue_inner(__ballot(pin), j, p, 0, lane, qn, qj, qi);
+                            if (pin) h = 0.0f;
+                        } else if (pin) {
+                            h = inner_value(du, q1, q2, 0);
+                        }
+                    }
+                    part[p] += h;
+                }
                 if (LM) {
-                    if (farD) partd[p] = fmaf(q1.x * du, r, partd[p]);
-                    else partd[p] += core_disp(du, q2, c, xfar, mind <= KId);
+                    if (farD) {
+                        partd[p] = fmaf(q1.x * du, r, partd[p]);
+                    } else {
+                        bool pin;
+                        float d = core_disp(du, q2, c, xfar, pin);
+                        if (mind <= KId) {
+                            if (DEFER) {
+                                queue_inner(__ballot(pin), j, p, 1, lane, qn, qj, qi);
+                                if (pin) d = 0.0f;
+                            } else if (pin) {
+                                d = inner_value(du, q1, q2, 1);
+                            }
+                        }
+                        partd[p] += d;
+                    }
                 }
             }
         }
@@ -258,6 +326,32 @@ __device__ __forceinline__ void core_range(crec r0p, crec r1p, crec r2p, crec r3
             kahan_add(acc[p], cmp[p], part[p]);
             if (LM) kahan_add(accd[p], cmpd[p], partd[p]);
         }
+        if (DEFER && (qn > 64 - 4 * P || j >= jhi)) {
+            // flush: tasks e (lanes 0-31) and e + 1 (lanes 32-63)
+            const int h = lane >> 5, l = lane & 31;
+            for (int e = 0; e < qn; e += 2) {
+                const int e1 = min(e + 1, qn - 1);
+                const int jt = h ? __builtin_amdgcn_readlane(qj, e1) : __builtin_amdgcn_readlane(qj, e);
+                const int it = h ? __builtin_amdgcn_readlane(qi, e1) : __builtin_amdgcn_readlane(qi, e);
+                const int pt = it & 15, at = (it >> 4) & 63, nt = (it >> 10) & 127, ty = it >> 17;
+                if ((h == 0 || e + 1 < qn) && l < nt) {
+                    const float4 q0 = ld(r0p, jt), q1 = ld(r1p, jt), q2 = ld(r2p, jt);
+                    const int d = at + l;                              // lane of the point in slice pt
+                    const float du = (float)(tw + d + 256 * pt - __float_as_int(q0.x)) - q0.y;
+                    atomicAdd(&iv[(ty * P + pt) * 64 + d], inner_value(du, q1, q2, ty));
+                }
+            }
+            qn = 0;
+        }
+    }
+    if (DEFER) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            kahan_add(acc[p], cmp[p], iv[p * 64 + lane]);
+            if (LM) kahan_add(accd[p], cmpd[p], iv[(P + p) * 64 + lane]);
+        }
+        __syncthreads();                               // before the staging buffer is reused
     }
 }
 
@@ -766,7 +860,7 @@ __device__ __forceinline__ void species_tile(const LinesumArgs &a, const ItemDev
         fast_expand<P, LM>(r0p, r1p, jG, jLf, jRf, jH, t, tl, acc, cmp, accd, cmpd);
         fast_range<P, LM>(r0p, r1p, jLf, jC, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         core_range<P, LM>(as_crec(r0p), as_crec(r1p), as_crec(r2p), as_crec(a.rec3 + 6 * it.rec_off), jC, jD, tl,
-                          xfar, acc, cmp, accd, cmpd);
+                          xfar, (float *)s_rec, acc, cmp, accd, cmpd);
         fast_range<P, LM>(r0p, r1p, jD, jRf, t, (int)i0, s_rec, acc, cmp, accd, cmpd);
         // EDGE lines: local expansion when every one of them is far from the tile
         const bool expand = shortcuts && Cmn - (double)(256 * P + HC + M + 34) >= (double)EDGE_EXPAND_MIN;
