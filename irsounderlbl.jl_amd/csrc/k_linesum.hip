@@ -417,22 +417,22 @@ __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, cons
     const int tw = uni(tl & ~63);
     const int nL = jl1 - jl0, ntot = nL + (jr1 - jr0);
     if (ntot <= 0) return;
-    float a0[P], a1[P], a2[P], a3[P], b0[P], b1[P], b2[P], b3[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) a0[p] = a1[p] = a2[p] = a3[p] = b0[p] = b1[p] = b2[p] = b3[p] = 0.0f;
     const float inV = C - 31.55f, outV = C + 31.55f, inD = Clm - 31.55f, outD = Clm + 31.55f;
-    for (int g0 = 0; g0 < ntot; g0 += 64) {
-        const int v = g0 + lane;
-        const bool ok = v < ntot;
-        const int j = ok ? (v < nL ? jl0 + v : jr0 + (v - nL)) : (nL > 0 ? jl0 : jr0);
-        const float4 q0 = r0p[j];
-        const float4 q1 = r1p[j];
-        const int cidx = __float_as_int(q0.x);
-        const float A = ok ? q0.w : 0.0f, Pd = ok ? q1.y : 0.0f, B = ok ? q1.x : 0.0f, g2 = q0.z;
-        const float cfr = 31.5f - q0.y;
+    const float dl = (float)lane - 31.5f;
+    // one slice at a time, so that only its eight coefficient sums are live
+    // (the lines' two records are re-read per slice from cache)
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const float d = (float)(tw + 256 * p - cidx) + cfr;     // slice centre - line centre
+    for (int p = 0; p < P; ++p) {
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+        for (int g0 = 0; g0 < ntot; g0 += 64) {
+            const int v = g0 + lane;
+            const bool ok = v < ntot;
+            const int j = ok ? (v < nL ? jl0 + v : jr0 + (v - nL)) : (nL > 0 ? jl0 : jr0);
+            const float4 q0 = r0p[j];
+            const float4 q1 = r1p[j];
+            const int cidx = __float_as_int(q0.x);
+            const float A = ok ? q0.w : 0.0f, Pd = ok ? q1.y : 0.0f, B = ok ? q1.x : 0.0f, g2 = q0.z;
+            const float d = (float)(tw + 256 * p - cidx) + (31.5f - q0.y);   // slice centre - line centre
             const float ad = fabsf(d);
             const bool fullV = ad <= inV, bndV = ok && !fullV && ad < outV;
             const float d2 = d * d;
@@ -440,10 +440,10 @@ __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, cons
             const float iD2 = iD * iD;
             if (fullV) {
                 const float L = A * iD;
-                a0[p] += L - Pd * fmaf(-d2, iC2, 2.0f);
-                a1[p] += 2.0f * d * (Pd * iC2 - L * iD);
-                a2[p] = fmaf(L * iD2, 3.0f * d2 - g2, fmaf(Pd, iC2, a2[p]));
-                a3[p] = fmaf(-4.0f * d * L * iD2 * iD, d2 - g2, a3[p]);
+                a0 += L - Pd * fmaf(-d2, iC2, 2.0f);
+                a1 += 2.0f * d * (Pd * iC2 - L * iD);
+                a2 = fmaf(L * iD2, 3.0f * d2 - g2, fmaf(Pd, iC2, a2));
+                a3 = fmaf(-4.0f * d * L * iD2 * iD, d2 - g2, a3);
             }
             bool bndD = false;
             if (LM) {
@@ -451,10 +451,10 @@ __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, cons
                 bndD = ok && !fullD && ad < outD;
                 if (fullD) {
                     const float Bi = B * iD;
-                    b0[p] = fmaf(Bi, d, b0[p]);
-                    b1[p] = fmaf(Bi * iD, g2 - d2, b1[p]);
-                    b2[p] = fmaf(Bi * iD2 * d, d2 - 3.0f * g2, b2[p]);
-                    b3[p] = fmaf(-Bi * iD2 * iD, fmaf(d2, d2 - 6.0f * g2, g2 * g2), b3[p]);
+                    b0 = fmaf(Bi, d, b0);
+                    b1 = fmaf(Bi * iD, g2 - d2, b1);
+                    b2 = fmaf(Bi * iD2 * d, d2 - 3.0f * g2, b2);
+                    b3 = fmaf(-Bi * iD2 * iD, fmaf(d2, d2 - 6.0f * g2, g2 * g2), b3);
                 }
             }
             // lines whose window boundary falls in this slice: pair by pair
@@ -478,28 +478,22 @@ __device__ __forceinline__ void edge_expand(const float4 *__restrict__ r0p, cons
                 }
             }
         }
-    }
-    // wave sums of the coefficients, then the cubic at each point
-    const float dl = (float)lane - 31.5f;
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        float c0 = a0[p], c1 = a1[p], c2 = a2[p], c3 = a3[p];
+        // wave sums of the coefficients, then the cubic at each point
         for (int o = 32; o > 0; o >>= 1) {
-            c0 += __shfl_xor(c0, o, 64);
-            c1 += __shfl_xor(c1, o, 64);
-            c2 += __shfl_xor(c2, o, 64);
-            c3 += __shfl_xor(c3, o, 64);
+            a0 += __shfl_xor(a0, o, 64);
+            a1 += __shfl_xor(a1, o, 64);
+            a2 += __shfl_xor(a2, o, 64);
+            a3 += __shfl_xor(a3, o, 64);
         }
-        kahan_add(acc[p], cmp[p], fmaf(fmaf(fmaf(c3, dl, c2), dl, c1), dl, c0));
+        kahan_add(acc[p], cmp[p], fmaf(fmaf(fmaf(a3, dl, a2), dl, a1), dl, a0));
         if (LM) {
-            float e0 = b0[p], e1 = b1[p], e2 = b2[p], e3 = b3[p];
             for (int o = 32; o > 0; o >>= 1) {
-                e0 += __shfl_xor(e0, o, 64);
-                e1 += __shfl_xor(e1, o, 64);
-                e2 += __shfl_xor(e2, o, 64);
-                e3 += __shfl_xor(e3, o, 64);
+                b0 += __shfl_xor(b0, o, 64);
+                b1 += __shfl_xor(b1, o, 64);
+                b2 += __shfl_xor(b2, o, 64);
+                b3 += __shfl_xor(b3, o, 64);
             }
-            kahan_add(accd[p], cmpd[p], fmaf(fmaf(fmaf(e3, dl, e2), dl, e1), dl, e0));
+            kahan_add(accd[p], cmpd[p], fmaf(fmaf(fmaf(b3, dl, b2), dl, b1), dl, b0));
         }
     }
 }
