@@ -258,6 +258,11 @@ __global__ void __launch_bounds__(256) keys_kernel(const double *nu, int64_t n, 
 // amplitude P_j, P_j c_j and P_j c_j^2, c_j the centre in grid steps (FP64), so
 // that a tile obtains the three pedestal moments of any line range by two
 // lookups.  One workgroup per item, 256-line chunks, wave-shuffle scan.
+// Per-item inclusive prefix sums of P, P c, P c^2 (FP64), one workgroup per
+// item: each thread scans PS_K consecutive lines in registers, the thread
+// totals are scanned across the wave (shuffles) and the block (LDS), so a
+// workgroup advances 256 * PS_K lines per step.
+#define PS_K 8
 __global__ void __launch_bounds__(256) pedestal_scan_kernel(const PrepArgs a) {
     __shared__ double wsum[3][4];
     const ItemDev it = a.items[blockIdx.x];
@@ -265,26 +270,40 @@ __global__ void __launch_bounds__(256) pedestal_scan_kernel(const PrepArgs a) {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double *p0 = a.pre + it.rec_off, *p1 = p0 + a.n_rec, *p2 = p1 + a.n_rec;
     double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-    for (int64_t j0 = 0; j0 < it.n; j0 += 256) {
-        const int64_t j = j0 + t;
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-        if (j < it.n) {
-            const float4 r0 = a.rec0[it.rec_off + j];
-            const double P = (double)a.rec1[it.rec_off + j].y;
-            const double c = (double)__float_as_int(r0.x) + (double)r0.y;
-            v0 = P;
-            v1 = P * c;
-            v2 = P * c * c;
+    for (int64_t j0 = 0; j0 < it.n; j0 += 256 * PS_K) {
+        const int64_t jt = j0 + (int64_t)t * PS_K;
+        double v0[PS_K], v1[PS_K], v2[PS_K];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < PS_K; ++k) {
+            const int64_t j = jt + k;
+            double P = 0.0, c = 0.0;
+            if (j < it.n) {
+                const float4 r0 = a.rec0[it.rec_off + j];
+                P = (double)a.rec1[it.rec_off + j].y;
+                c = (double)__float_as_int(r0.x) + (double)r0.y;
+            }
+            s0 += P;
+            s1 += P * c;
+            s2 += P * c * c;
+            v0[k] = s0;
+            v1[k] = s1;
+            v2[k] = s2;
         }
+        double u0 = s0, u1 = s1, u2 = s2;             // inclusive scan of the thread totals
         for (int o = 1; o < 64; o <<= 1) {
-            const double u0 = __shfl_up(v0, o, 64), u1 = __shfl_up(v1, o, 64), u2 = __shfl_up(v2, o, 64);
-            if (lane >= o) { v0 += u0; v1 += u1; v2 += u2; }
+            const double w0 = __shfl_up(u0, o, 64), w1 = __shfl_up(u1, o, 64), w2 = __shfl_up(u2, o, 64);
+            if (lane >= o) { u0 += w0; u1 += w1; u2 += w2; }
         }
-        if (lane == 63) { wsum[0][wv] = v0; wsum[1][wv] = v1; wsum[2][wv] = v2; }
+        if (lane == 63) { wsum[0][wv] = u0; wsum[1][wv] = u1; wsum[2][wv] = u2; }
         __syncthreads();
-        double b0 = c0, b1 = c1, b2 = c2;
+        double b0 = c0 + (u0 - s0), b1 = c1 + (u1 - s1), b2 = c2 + (u2 - s2);   // offset of this thread
         for (int w = 0; w < wv; ++w) { b0 += wsum[0][w]; b1 += wsum[1][w]; b2 += wsum[2][w]; }
-        if (j < it.n) { p0[j] = b0 + v0; p1[j] = b1 + v1; p2[j] = b2 + v2; }
+#pragma unroll
+        for (int k = 0; k < PS_K; ++k) {
+            const int64_t j = jt + k;
+            if (j < it.n) { p0[j] = b0 + v0[k]; p1[j] = b1 + v1[k]; p2[j] = b2 + v2[k]; }
+        }
         c0 += wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
         c1 += wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
         c2 += wsum[2][0] + wsum[2][1] + wsum[2][2] + wsum[2][3];
