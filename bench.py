@@ -305,7 +305,8 @@ def main():
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": prof_pmc.get("hbm_bytes_per_launch"),
-            "kernel": f"linesum_kernel<{stats['tile_points']}>", "flop_per_launch": nominal,
+            "kernel": ("linesum_persistent_kernel" if stats["persistent"] else "linesum_kernel")
+                      + f"<{stats['tile_points']}>", "flop_per_launch": nominal,
             "launch_ms_avg": round(ls_avg, 3),
             "pairs_reference": {"voigt": stats["active_pairs"], "dispersive": stats["dispersive_pairs"]},
             "executed": executed,

@@ -158,7 +158,8 @@ typedef struct {
  * replaces the backend/thread choice of src/Parallel/strategy.jl:21-90. */
 typedef struct {
     int tile_points;            /* grid points per thread per tile: 4, 8 or 16 */
-    int persistent;             /* 1: persistent work-queue kernel over heaviest-first tiles */
+    int persistent;             /* 1 (default): persistent work-queue kernel over heaviest-first
+                                   tiles; 0: one workgroup per (tile, layer, profile) */
     int use_graph;              /* 1: capture each profile group's device sequence in a hipGraph
                                    and replay it on later calls of the same shape */
     int autotune;               /* 1: time the (tile_points, persistent) candidates on the first

@@ -101,7 +101,7 @@ struct irs_ctx {
     DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
     irs_stats stats{};
-    irs_schedule sched{4, 0, 0, 0, 1.5};
+    irs_schedule sched{4, 1, 0, 0, 1.5};      // persistent heaviest-first queue: 26.7 vs 27.9 ms per full-IASI step
     std::map<std::string, std::pair<int, int>> tuned;     // shape -> (P, persistent)
     std::map<std::string, DevBuf> orders;                 // shape -> heaviest-first tile order
     std::map<int, int> capacity;                          // P -> resident persistent workgroups

@@ -231,7 +231,7 @@ class Context:
         self.check(self.lib.irs_get_schedule(self.h, C.byref(sc)))
         return {k: getattr(sc, k) for k, _ in IrsSchedule._fields_}
 
-    def set_schedule(self, tile_points: int = 4, persistent: bool = False, use_graph: bool = False,
+    def set_schedule(self, tile_points: int = 4, persistent: bool = True, use_graph: bool = False,
                      autotune: bool = False, far_wing_cm: float = 1.5):
         """Line-sum schedule of forward_model: points per thread (4/8/16), the
         persistent heaviest-first work-queue kernel, hipGraph capture and

@@ -48,7 +48,7 @@ def bctx():
 
 
 def test_batch_equals_single_and_oracle(bctx, scene):
-    bctx.set_schedule(4)
+    bctx.set_schedule(4, persistent=False)
     _, Rb, BTb = _run(bctx, scene)
     st = bctx.stats()
     assert st["tile_points"] == 4 and st["persistent"] == 0
@@ -63,7 +63,7 @@ def test_batch_equals_single_and_oracle(bctx, scene):
 
 
 def test_groups_persistent_and_tile_points(bctx, scene, monkeypatch):
-    bctx.set_schedule(8)
+    bctx.set_schedule(8, persistent=False)
     _, R0, BT0 = _run(bctx, scene)
     monkeypatch.setenv("IRS_PROFILE_GROUP", "2")
     _, R1, BT1 = _run(bctx, scene)
