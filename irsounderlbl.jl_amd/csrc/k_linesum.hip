@@ -932,10 +932,11 @@ __device__ __forceinline__ void linesum_unit(const LinesumArgs &a, int tile, int
 }
 
 // One workgroup per unit: blockIdx = (tile rank, layer, profile).
-// Occupancy: the line sum is bound by the latency of its scalar record loads
-// and LDS reads rather than by VALU issue, so the register budget is capped for
-// 32 / P resident workgroups per CU (P = 4: eight waves per SIMD, 64 VGPRs; the
-// LDS footprint of a workgroup, ~19 KB, still allows eight per CU).
+// Occupancy: the register budget is capped for 32 / P resident workgroups per CU
+// (P = 4: eight waves per SIMD, 64 VGPRs, with spills; the LDS footprint of a
+// workgroup, ~17.5 KB, still allows eight per CU).  The extra waves hide the
+// latency of the scalar record loads, LDS reads and spill traffic: on full IASI
+// 8 / 6 / 5 / 4 waves per SIMD measured 26.9 / 28.9 / 31.2 / 35.3 ms.
 #define IRS_LS_MINB(P) (32 / (P))
 template <int P>
 __global__ void __launch_bounds__(IRS_LS_THREADS, IRS_LS_MINB(P)) linesum_kernel(const LinesumArgs a) {
