@@ -120,9 +120,10 @@ def test_jacobian_line_mixing_window(ctx):
 
 
 def test_jacobian_coarse_far_wing_matches_every_pair(ctx):
-    """The coarse far-wing split of the gradient line sum (the seven sums of
-    lines beyond 1.5 cm^-1 at coarse nodes, cubic interpolation) against the
-    every-pair gradient sum (far_wing_cm = 0): 15 um CO2 band with VP_Y and
+    """The shortcuts of the gradient line sum (the seven sums of lines beyond
+    1.5 cm^-1 at coarse nodes, and of the cutoff-edge lines at four nodes per
+    64-point slice, both with cubic interpolation) against the every-pair
+    gradient sum (far_wing_cm = 0): 15 um CO2 band with VP_Y and
     H2O lines in 675-735 cm^-1, channels 700-710 cm^-1, internal grid
     0.001 cm^-1, 49 layers, CO2 + H2O retrieved.  The profile is a perturbed
     one: above 11 km the base profile is isothermal, where the VMR columns
