@@ -328,7 +328,7 @@ __device__ __forceinline__ void grad_pairs(jrec g0, jrec g1, jrec g2, jrec g3, j
 __device__ __forceinline__ void grad_edge_nodes(const JacSumArgs &a, const JacItemDev &it, jrec g0s, jrec g1s,
                                                 jrec g2s, jrec g3s, jrec g4s, jrec g5s, int jl0, int jl1, int jr0,
                                                 int jr1, int tl, bool lm, float C2, float iC2, int CiD, float CfD,
-                                                float xfar, float (&sa)[JNC][JP], float (&ca)[JNC][JP]) {
+                                                float (&sa)[JNC][JP], float (&ca)[JNC][JP]) {
     const int nL = jl1 - jl0, ntot = nL + (jr1 - jr0);
     if (ntot <= 0) return;
     const int lane = threadIdx.x & 63;
@@ -660,8 +660,7 @@ __global__ void __launch_bounds__(256, IRS_JAC_MINB) grad_linesum_kernel(const J
             const bool edge_nodes = (double)Cmin - (double)(256 * JP + 2 * HC + 2 * M + 64) >= (double)JEDGE_MIN &&
                                     K + 64 < JEDGE_MIN;
             if (edge_nodes) {
-                grad_edge_nodes(a, it, g0, g1, g2, g3, g4, g5, jA, jB2, jE2, jF, tl, lm, C2, iC2, CiD, CfD, xfar,
-                                sa, ca);
+                grad_edge_nodes(a, it, g0, g1, g2, g3, g4, g5, jA, jB2, jE2, jF, tl, lm, C2, iC2, CiD, CfD, sa, ca);
             } else {
                 grad_pairs(g0, g1, g2, g3, g4, g5, jA, jB2, tl, lm, C2, iC2, CiD, CfD, xfar, sa, ca);
                 grad_pairs(g0, g1, g2, g3, g4, g5, jE2, jF, tl, lm, C2, iC2, CiD, CfD, xfar, sa, ca);
