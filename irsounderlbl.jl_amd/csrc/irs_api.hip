@@ -1779,6 +1779,13 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     sa.dnu = dnu;
     sa.nu_lo = nu_lo;
     far_params(ctx, dnu, IRS_JAC_TILE_P, sa.far_d, sa.far_hc);
+    // the gradient sum's per-pair lines cost seven sums each: half the far-wing
+    // distance and half the node spacing (the same HC / D, so the same interpolation
+    // bound) move half of them onto coarse nodes (149.5 -> 139.7 ms on full IASI)
+    if (sa.far_hc >= 4 && (256 * IRS_JAC_TILE_P) / (sa.far_hc / 2) + 3 <= IRS_JAC_FAR_MAXNC) {
+        sa.far_hc /= 2;
+        sa.far_d = (sa.far_d + 1) / 2;
+    }
     sa.tau = (float *)ctx->j_tau.p;
     sa.dtlo = (float *)ctx->j_dt.p;
     sa.dthi = sa.dtlo + NL;

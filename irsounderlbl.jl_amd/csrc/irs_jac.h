@@ -8,6 +8,7 @@
 
 #define IRS_JAC_MAXR 4   // retrieved species handled in one gradient pass
 #define IRS_JAC_TILE_P 4 // grid points per thread of the gradient line sum (tile = 256 * P)
+#define IRS_JAC_FAR_MAXNC 128   // coarse far-wing nodes per gradient tile
 
 // Per (layer, species) gradient work item.
 struct JacItemDev {

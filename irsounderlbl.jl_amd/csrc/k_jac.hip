@@ -165,7 +165,7 @@ hipError_t launch_prep_grad(const JacPrepArgs &a, int max_lines, hipStream_t s) 
 // are FAR: their seven sums are evaluated at the coarse nodes and
 // interpolated; every other line in the window is evaluated pair by pair.
 #define JP IRS_JAC_TILE_P
-#define JFAR_MAXNC 128                  // coarse nodes per tile
+#define JFAR_MAXNC IRS_JAC_FAR_MAXNC     // coarse nodes per tile
 #define JFAR_CH 160                     // FAR lines per LDS chunk
 #define JNC 7                           // sums per grid point
 
