@@ -258,4 +258,23 @@ function analytic_jacobian(prof::AtmosphericProfile, linelists::Dict{GasSpecies,
     return Jacobian(K, spec, collect(grid.ν), x0, y0, observable)
 end
 
+"""
+    faddeeva_voigt32(x, y; rule = :core, ctx = context()) -> Vector{Float64}
+
+Re w(x + iy) (the Voigt H function of `faddeeva_voigt`, src/CrossSections/voigt.jl:158-160)
+from the FP32 device routines of the line sums, for arrays `x`, `y >= 0`;
+`rule = :core` is the CORE-range rule, `:pair` the per-pair and gradient rule.
+"""
+function faddeeva_voigt32(x::AbstractVector{<:Real}, y::AbstractVector{<:Real}; rule::Symbol = :core,
+                          ctx::Context = context())
+    length(x) == length(y) || throw(ArgumentError("x and y must have the same length"))
+    rule in (:core, :pair) || throw(ArgumentError("rule must be :core or :pair"))
+    xs, ys = Float64.(collect(x)), Float64.(collect(y))
+    wr, wi = zeros(Float64, length(xs)), zeros(Float64, length(xs))
+    check(ctx, ccall((:irs_faddeeva32, libirs), Cint,
+                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Cint, Ptr{Float64}, Ptr{Float64}),
+                     ctx.h, xs, ys, length(xs), rule === :core ? 0 : 1, wr, wi))
+    return wr
+end
+
 end # module
