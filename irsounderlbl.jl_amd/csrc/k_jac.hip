@@ -163,7 +163,9 @@ hipError_t launch_prep_grad(const JacPrepArgs &a, int max_lines, hipStream_t s) 
 // covers the whole coarse span and that lie at least far_d grid steps from it
 // (and beyond the |x| > x_far core distance from every point of the tile)
 // are FAR: their seven sums are evaluated at the coarse nodes and
-// interpolated; every other line in the window is evaluated pair by pair.
+// interpolated; cutoff-edge lines at least JEDGE_MIN grid steps from every slice
+// are evaluated at four nodes per 64-point slice (grad_edge_nodes); every other
+// line in the window is evaluated pair by pair.
 #define JP IRS_JAC_TILE_P
 #define JFAR_MAXNC IRS_JAC_FAR_MAXNC     // coarse nodes per tile
 #define JFAR_CH 160                     // FAR lines per LDS chunk
