@@ -82,7 +82,7 @@ def accuracy_sample(ctx, prof, lls, lls_w):
     lo, hi = 1205.0, 1215.0
     snd = Sounder.make(lo, hi, 0.25, opd_max=2.0, fwhm_gauss=0.5)
     sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls_w.items()}
-    _, _, bt = forward_model(prof, sub, sounder=snd, line_mixing="vpy", ctx=ctx)
+    _, _, bt = forward_model(prof, sub, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     _, bto, _ = refcpu.forward(prof, sub, snd, line_mixing=1)
     d = np.abs(bt - bto)
     out["bt_1205-1215_windows_variant"] = {"max_abs_K": float(d.max()), "rms_K": float(np.sqrt(np.mean(d * d))),
@@ -111,10 +111,10 @@ def batched(ctx, lls, snd, n_prof):
     tsf = [x[1] for x in pp]
     ctx.set_schedule(4, use_graph=True, autotune=True)
     log(f"bench: batched warm-up (autotune + graph capture)")
-    forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     log(f"bench: batched {n_prof} profiles")
     t0 = time.perf_counter()
-    _, R, BT = forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    _, R, BT = forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     dt = time.perf_counter() - t0
     st = ctx.stats()
     ctx.set_schedule(4)
@@ -144,7 +144,7 @@ def emulator_figure(ctx, lls, snd, n_train, n_eval):
         sc = train[c0:c0 + 256]
         tt, gg = channel_optical_depths([p[0] for p in sc], lls, sounder=snd,
                                         geom=[ViewingGeometry(float(np.degrees(np.arccos(min(p[2], 1.0)))), 0.0, p[2])
-                                              for p in sc], floor=1e-4, cim=True, line_mixing="vpy", ctx=ctx)
+                                              for p in sc], floor=1e-4, cim=True, line_mixing="vpy", ctx=ctx, apply_continuum=False)
         taus.append(tt)
         gs.append(gg)
         log(f"bench: emulator targets {c0 + len(sc)}/{n_train}")
@@ -170,7 +170,7 @@ def emulator_figure(ctx, lls, snd, n_train, n_eval):
     res = forward_emulated(profs, lls, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=0.05, spot=spot, ctx=ctx)
     t_full = time.perf_counter() - t0
     log("bench: emulator reference LBL")
-    _, _, bt_lbl = forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    _, _, bt_lbl = forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     d_emu = np.abs(bt_emu_only - bt_lbl)
     d_fin = np.abs(res.BT - bt_lbl)
     t0 = time.perf_counter()
@@ -208,9 +208,9 @@ def lbl_jacobian_figure(ctx, lls_w, snd):
     kw = dict(sounder=snd, geom=ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu), T_sfc=T_sfc,
               line_mixing="vpy", ctx=ctx)
     log("bench: LBL analytic Jacobian")
-    analytic_jacobian(prof, lls_w, spec, **kw)
+    analytic_jacobian(prof, lls_w, spec, **kw, apply_continuum=False)
     t0 = time.perf_counter()
-    jac = analytic_jacobian(prof, lls_w, spec, **kw)
+    jac = analytic_jacobian(prof, lls_w, spec, **kw, apply_continuum=False)
     dt = time.perf_counter() - t0
     st = ctx.stats()
     return {"workload": "full IASI 8461 channels, windows variant of the line lists, 1 perturbed profile, "
@@ -228,13 +228,13 @@ def exact_sum(ctx, prof, lls, snd):
     from irsounderlbl.api import forward_model
     sched = ctx.schedule()
     ctx.set_schedule(sched["tile_points"], far_wing_cm=0.0)
-    forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx)
+    forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     t0 = time.perf_counter()
-    _, _, bt0 = forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx)
+    _, _, bt0 = forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     dt = time.perf_counter() - t0
     st = ctx.stats()
     ctx.set_schedule(sched["tile_points"], far_wing_cm=sched["far_wing_cm"])
-    _, _, bt1 = forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx)
+    _, _, bt1 = forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     return {"spectra_per_s": round(1.0 / dt, 4), "linesum_ms": round(st["ms_linesum"], 3),
             "max_abs_dBT_vs_default_K": float(np.max(np.abs(bt1 - bt0)))}
 
@@ -284,7 +284,7 @@ def main():
         ctx.upload(ll)
 
     def step():
-        return forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx)
+        return forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
 
     for i in range(args.warmup):
         step()

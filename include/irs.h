@@ -16,14 +16,16 @@
  *
  * Reference interfaces replaced (paths relative to the reference repository):
  *   irs_forward            forward_model            src/Sensor/sounder.jl:233-360
- *   irs_faddeeva32         faddeeva_voigt / _voigt_H (Re w) and the Im w of the VP_Y term
+ *   irs_faddeeva32         faddeeva_voigt / _voigt_H (Re w) and the Im w of lm_dispersive_kernel!
  *                                                   src/CrossSections/voigt.jl:158-172,
- *                                                   src/CrossSections/line_mixing.jl:966-1005
+ *                                                   src/CrossSections/line_mixing.jl:1166-1186
  *   irs_xsec_layer         compute_voigt_cross_sections (+ _reject_weak_lines,
  *                          compute_voigt_lm_cross_sections)
  *                                                   src/CrossSections/voigt.jl:333-462,
  *                                                   src/CrossSections/line_mixing.jl:1351-1369
- *   irs_rte                schwarzschild_rte        src/Solver/schwarzschild.jl:255-334
+ *   irs_rte                schwarzschild_rte + level_transmittances + add_reflected_downwelling!
+ *                                                   src/Solver/schwarzschild.jl:130-209,228-274,
+ *                                                   src/Solver/transmittance.jl:26-43
  *   irs_ils_kernel         ils_kernel               src/Sensor/ils.jl:80-114
  *   irs_ils_channels       apply_ils + _resample_to_channels
  *                                                   src/Sensor/ils.jl:131-153,
@@ -60,11 +62,26 @@ extern "C" {
 #define IRS_APOD_NB_MEDIUM  2   /* :norton_beer_medium */
 #define IRS_APOD_NB_STRONG  3   /* :norton_beer_strong */
 
-#define IRS_SRC_TOON 0          /* source_function=:toon  schwarzschild.jl:286-298 */
-#define IRS_SRC_CIM  1          /* source_function=:cim   schwarzschild.jl:299-318 */
+#define IRS_SRC_TOON 0          /* source_function=:toon  schwarzschild.jl:43-47,161-173 */
+#define IRS_SRC_CIM  1          /* source_function=:cim   schwarzschild.jl:59-64,174-193 */
 
 #define IRS_TMETHOD_LOGP_AT_PCG   0   /* T_method=:logp_at_pcg   layers.jl:133-178 */
 #define IRS_TMETHOD_MASS_WEIGHTED 1   /* T_method=:mass_weighted */
+
+/* continua (sounder.jl:245-246,310-322; continuum.jl:163-327).  The CIA
+ * tables are not bundled (HITRAN does not allow redistribution): until one is
+ * loaded with irs_cia_set / irs_cia_load_file its pair adds zero, as the
+ * reference does when the data file is absent (continuum.jl:254-258). */
+#define IRS_CONT_H2O    1       /* :h2o     MT-CKD 4.3 self + foreign (bundled table) */
+#define IRS_CONT_CO2    2       /* :co2     MT-CKD CO2 line-coupling continuum (bundled table) */
+#define IRS_CONT_CO2CIA 4       /* :co2_cia HITRAN CO2-CO2 CIA, scaled by n_CO2 n_air */
+#define IRS_CONT_N2     8       /* :n2      HITRAN N2-N2 CIA */
+#define IRS_CONT_O2    16       /* :o2      HITRAN O2-O2 CIA */
+#define IRS_CONT_ALL   31
+
+#define IRS_CIA_CO2CO2 0
+#define IRS_CIA_N2N2   1
+#define IRS_CIA_O2O2   2
 
 #define IRS_LM_NONE 0           /* line_mixing=nothing */
 #define IRS_LM_VPY  1           /* first-order (VP_Y) mixing on the CO2 list, Y per line */
@@ -123,7 +140,8 @@ typedef struct {
     int    line_mixing;         /* IRS_LM_NONE */
     int    T_method;            /* IRS_TMETHOD_LOGP_AT_PCG */
     int    source_function;     /* IRS_SRC_CIM */
-    int    reserved0;
+    int    continua;            /* apply_continuum + continua: bit mask of IRS_CONT_* added to
+                                   every layer (0 = apply_continuum=false); default IRS_CONT_ALL */
 } irs_fm_options;
 
 /* Timing and work counters of the last irs_forward / irs_xsec_layer call. */
@@ -204,6 +222,26 @@ int irs_channel_od(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const 
 int irs_channel_transmittance(irs_ctx *ctx, const irs_profile *profiles, int n_prof,
                               const irs_sounder *snd, const irs_fm_options *opt, float *T_lev,
                               float *cim_G);
+
+/* HITRAN CIA table of one pair (IRS_CIA_*): n_blocks temperature blocks, block
+ * b with knots nu[off[b] .. off[b+1]) and cross sections sigma (cm^5 molec^-2,
+ * negative values are clamped to 0 as continuum.jl:131-134 does) at T[b].
+ * n_blocks = 0 removes the table.  Replaces _load_cia_table + DataTable
+ * (src/CrossSections/continuum.jl:106-148). */
+int irs_cia_set(irs_ctx *ctx, int pair, int n_blocks, const double *T, const int64_t *off,
+                const double *nu, const double *sigma);
+/* Reads a HITRAN .cia text file (header line per block: pair nu_min nu_max n T ...,
+ * then "nu sigma" rows) into the pair's table (continuum.jl:112-145). */
+int irs_cia_load_file(irs_ctx *ctx, int pair, const char *path);
+/* Continuum of one layer on the grid (nu0, dnu, n_nu), FP64 on the device:
+ * tau_c[i] = dz_cm * the sum of the selected continua (absorption coefficients in
+ * cm^-1) at the given VMRs, pressure and temperature.  With one bit and dz_cm = 1
+ * this is h2o_continuum / co2_continuum / co2_cia / n2_cia / o2_cia
+ * (continuum.jl:163-327); forward_model uses the mid-layer state and
+ * dz = _dp_to_dz_cm(dp, p_mid, T_mid) (sounder.jl:310-322; optical_depth.jl:123-130). */
+int irs_continuum_layer(irs_ctx *ctx, double nu0, double dnu, int64_t n_nu, int continua,
+                        double vmr_h2o, double vmr_co2, double vmr_n2, double vmr_o2, double p_hpa,
+                        double T, double dz_cm, double *tau_c);
 
 /* Copies a sorted line list to the device; returns a handle. */
 int irs_linelist_upload(irs_ctx *ctx, const irs_linelist *ll, int32_t *handle);

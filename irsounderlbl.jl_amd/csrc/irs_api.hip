@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -19,7 +20,9 @@
 #include <vector>
 
 #include "../../include/irs.h"
+#include "../data/mtckd_table.h"
 #include "../data/tips2024_table.h"
+#include "irs_cont.h"
 #include "irs_internal.h"
 #include "irs_jac.h"
 
@@ -72,6 +75,8 @@ struct GroupPlan {
                                    // 3 level channel transmittances + CIM sums
     const void *chod, *chlev, *chg;
     double chod_floor;
+    int continua;                  // IRS_CONT_* added to tau after the line sum (0: none)
+    int cia_gen;                   // CIA table generation (graphs capture the table pointers)
 };
 
 struct GraphEntry {
@@ -106,6 +111,11 @@ struct irs_ctx {
     std::map<std::string, DevBuf> orders;                 // shape -> heaviest-first tile order
     std::map<int, int> capacity;                          // P -> resident persistent workgroups
     DevBuf d_queue, d_prof, d_chod, d_chlev, d_chg, d_pre;
+    DevBuf d_ckd;                      // MT-CKD tables: H2O [2001][4], then CO2 [5003][2] = {S*xfac, tdep}
+    DevBuf d_cia[3];                   // per CIA pair: T, numin, numax [nb], off [nb+1], nu, sig [knots]
+    int cia_nb[3] = {0, 0, 0};
+    int64_t cia_nk[3] = {0, 0, 0};
+    int cia_gen = 0;
     PinBuf h_items, h_prof, h_out;
     std::vector<GraphEntry> graphs;
 };
@@ -275,8 +285,8 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 
 // One item per (layer, species) of a profile.
 struct LayerProps {
-    std::vector<double> p_mid, dp, T_ave;
-    std::vector<std::vector<double>> vmr_cg, p_cg, T_cg;
+    std::vector<double> p_mid, dp, T_ave, T_mid;
+    std::vector<std::vector<double>> vmr_cg, p_cg, T_cg, vmr_mid;
 };
 
 int layer_properties(irs_ctx *ctx, const irs_profile *pr, int T_method, LayerProps &lp) {
@@ -295,12 +305,16 @@ int layer_properties(irs_ctx *ctx, const irs_profile *pr, int T_method, LayerPro
         lp.dp[k] = std::fabs(p[k] - p[k + 1]);
         lp.T_ave[k] = cg_temperature_mass(T[k], T[k + 1], p[k], p[k + 1]);
     }
+    lp.T_mid.resize(L);
+    for (int k = 0; k < L; ++k) lp.T_mid[k] = interp_logp(p, T, nlev, lp.p_mid[k]);   // layers.jl:142
     lp.vmr_cg.assign(pr->n_species, std::vector<double>(L));
     lp.p_cg.assign(pr->n_species, std::vector<double>(L));
     lp.T_cg.assign(pr->n_species, std::vector<double>(L));
+    lp.vmr_mid.assign(pr->n_species, std::vector<double>(L));
     for (int s = 0; s < pr->n_species; ++s) {
         const double *v = pr->vmr + (size_t)s * nlev;
         for (int k = 0; k < L; ++k) {
+            lp.vmr_mid[s][k] = interp_logp(p, v, nlev, lp.p_mid[k]);                      // layers.jl:145-148
             lp.vmr_cg[s][k] = cg_column_vmr(v[k], v[k + 1], p[k], p[k + 1]);
             lp.p_cg[s][k] = cg_pressure(v[k], v[k + 1], p[k], p[k + 1]);
             lp.T_cg[s][k] = T_method == IRS_TMETHOD_MASS_WEIGHTED ? lp.T_ave[k]
@@ -308,6 +322,53 @@ int layer_properties(irs_ctx *ctx, const irs_profile *pr, int T_method, LayerPro
         }
     }
     return IRS_OK;
+}
+
+// Continuum state of each layer (sounder.jl:311-315; optical_depth.jl:123-130):
+// mid-layer VMRs (H2O 0 and CO2 415 ppm when the profile has no such species),
+// p_mid, T_mid, dz and the mid-layer weight of the upper level.
+void cont_layers(const irs_profile *pr, const LayerProps &lp, ContLayer *out) {
+    const int L = pr->n_levels - 1;
+    int sh = -1, sc = -1;
+    for (int s = 0; s < pr->n_species; ++s) {
+        if (pr->species[s] == 1) sh = s;
+        if (pr->species[s] == 2) sc = s;
+    }
+    for (int k = 0; k < L; ++k) {
+        ContLayer &c = out[k];
+        std::memset(&c, 0, sizeof(c));
+        c.vh2o = sh >= 0 ? lp.vmr_mid[sh][k] : 0.0;
+        c.vco2 = sc >= 0 ? lp.vmr_mid[sc][k] : 4.15e-4;
+        c.vn2 = 0.78084 * (1.0 - c.vh2o);                 // sounder.jl:315,319-320
+        c.vo2 = 0.20946 * (1.0 - c.vh2o);
+        c.p_mid = lp.p_mid[k];
+        c.T_mid = lp.T_mid[k];
+        const double H = 8.314462 * c.T_mid / (0.028964 * 9.80665);
+        c.dz = H * lp.dp[k] / c.p_mid * 100.0;
+        c.wM = std::log(lp.p_mid[k] / pr->p_hpa[k]) / std::log(pr->p_hpa[k + 1] / pr->p_hpa[k]);
+    }
+}
+
+// Continuum launch arguments for the context's tables; the caller sets tau,
+// the grid, the layer states and the optional derivative outputs.
+ContArgs cont_args(const irs_ctx *ctx, int continua) {
+    ContArgs a{};
+    a.continua = continua;
+    a.h2o = (const double *)ctx->d_ckd.p;
+    a.co2 = a.h2o + (size_t)IRS_CKD_H2O_N * 4;
+    for (int pr = 0; pr < 3; ++pr) {
+        CiaDev &c = a.cia[pr];
+        c.nb = ctx->cia_nb[pr];
+        if (!c.nb) continue;
+        const double *b = (const double *)ctx->d_cia[pr].p;
+        c.T = b;
+        c.numin = b + c.nb;
+        c.numax = b + 2 * c.nb;
+        c.off = (const int64_t *)(b + 3 * c.nb);
+        c.nu = b + 4 * c.nb + 1;
+        c.sig = c.nu + ctx->cia_nk[pr];
+    }
+    return a;
 }
 
 int check_list(irs_ctx *ctx, int32_t h) {
@@ -698,6 +759,7 @@ void irs_default_options(irs_fm_options *o) {
     o->line_mixing = IRS_LM_NONE;
     o->T_method = IRS_TMETHOD_LOGP_AT_PCG;
     o->source_function = IRS_SRC_CIM;
+    o->continua = IRS_CONT_ALL;      // apply_continuum = true, all five continua (sounder.jl:245-246)
 }
 
 int irs_device_info(int device, char *buf, int buflen) {
@@ -734,6 +796,21 @@ int irs_create(int device, irs_ctx **out) {
         irs_destroy(ctx);
         return IRS_ENOMEM;
     }
+    {
+        std::vector<double> ck((size_t)IRS_CKD_H2O_N * 4 + (size_t)IRS_CKD_CO2_N * 2);
+        for (int r = 0; r < IRS_CKD_H2O_N; ++r)
+            for (int c = 0; c < 4; ++c) ck[(size_t)r * 4 + c] = irs_ckd_h2o[r][c];
+        double *c2 = ck.data() + (size_t)IRS_CKD_H2O_N * 4;
+        for (int r = 0; r < IRS_CKD_CO2_N; ++r) {
+            c2[2 * r] = irs_ckd_co2[r][1] * irs_ckd_co2[r][2];   // S x xfac (continuum.jl:90)
+            c2[2 * r + 1] = irs_ckd_co2[r][3];
+        }
+        if (ensure(ctx->d_ckd, sizeof(double) * ck.size()) != hipSuccess ||
+            hipMemcpy(ctx->d_ckd.p, ck.data(), sizeof(double) * ck.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            irs_destroy(ctx);
+            return IRS_ENOMEM;
+        }
+    }
     if (const char *e = std::getenv("IRS_LS_P")) {
         int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) ctx->sched.tile_points = v;
@@ -765,6 +842,8 @@ void irs_destroy(irs_ctx *ctx) {
     ctx->d_pre.release();
     ctx->d_chlev.release();
     ctx->d_chg.release();
+    ctx->d_ckd.release();
+    for (auto &b : ctx->d_cia) b.release();
     ctx->h_items.release();
     ctx->h_prof.release();
     ctx->h_out.release();
@@ -1131,6 +1210,18 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.n_rec = pa.n_rec;
     la.rec3 = pa.rec3;
     ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
+    if (pl.continua) {
+        ContArgs co = cont_args(ctx, pl.continua);
+        co.tau = (float *)pl.tau;
+        co.n = pl.n;
+        co.n_layers = L;
+        co.n_prof = pl.gn;
+        co.an = pl.an;
+        co.sn = pl.sn;
+        co.den = pl.den;
+        co.lay = (const ContLayer *)((const double *)pl.prof + NL * (2 * L + 4));
+        ENQ(launch_continuum(co, st));
+    }
     if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
     if (pl.mode >= 1) {
         ChodArgs ca{};
@@ -1334,6 +1425,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         return fail(ctx, IRS_EINVAL, "source_function must be :toon or :cim");
     if (opt.apodization < 0 || opt.apodization > 3) return fail(ctx, IRS_EINVAL, "Unknown apodization");
     if (opt.line_mixing != IRS_LM_NONE && opt.line_mixing != IRS_LM_VPY) return fail(ctx, IRS_EINVAL, "bad line_mixing");
+    if (opt.continua < 0 || opt.continua > IRS_CONT_ALL) return fail(ctx, IRS_EINVAL, "bad continua mask");
     auto t0 = std::chrono::steady_clock::now();
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
 
@@ -1422,7 +1514,9 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     double ms_prep = 0, ms_ls = 0, ms_rte = 0, ms_ils = 0, ms_graph = 0;
     bool tuned_now = false;
 
-    const size_t prof_doubles = (size_t)group * (2 * L + 1 + 3);
+    // per profile: T_lev [L+1], T_ave [L], {T_sfc, mu, eps}, then the continuum layer
+    // states [L] (8 doubles each) at offset group * (2L + 4)
+    const size_t prof_doubles = (size_t)group * (2 * L + 1 + 3) + (size_t)group * L * 8;
     HIPCHK(ensure(ctx->d_items, sizeof(ItemDev) * (size_t)group * L * ns));
     HIPCHK(ensure(ctx->d_rec0, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     HIPCHK(ensure(ctx->d_rec1, sizeof(float4) * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
@@ -1452,6 +1546,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         ItemDev *items = (ItemDev *)ctx->h_items.p;
         double *hp = (double *)ctx->h_prof.p;
         double *h_tlev = hp, *h_tave = hp + (size_t)gn * (L + 1), *h_scal = hp + (size_t)gn * (2 * L + 1);
+        ContLayer *h_cont = (ContLayer *)(hp + (size_t)gn * (2 * L + 4));
         int64_t rec = 0;
         for (int q = 0; q < gn; ++q) {
             const irs_profile &pr = profiles[g0 + q];
@@ -1463,6 +1558,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
             h_scal[3 * q] = pr.T_sfc > 0.0 ? pr.T_sfc : pr.T[0];   // sounder.jl:337
             h_scal[3 * q + 1] = pr.mu;
             h_scal[3 * q + 2] = pr.eps_sfc;
+            if (opt.continua) cont_layers(&pr, lp, h_cont + (size_t)q * L);
             for (int k = 0; k < L; ++k)
                 for (int s = 0; s < ns; ++s) {
                     ItemDev &it = items[((size_t)q * L + k) * ns + s];
@@ -1528,7 +1624,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         pl.h_prof = ctx->h_prof.p; pl.h_out = ctx->h_out.p;
         pl.n = n; pl.nch = nch; pl.half = half; pl.n_items = (int64_t)gn * L * ns; pl.max_lines = max_lines;
         pl.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)gn * L * ns);
-        pl.prof_bytes = (int64_t)(sizeof(double) * (size_t)gn * (2 * L + 1 + 3));
+        pl.prof_bytes = (int64_t)(sizeof(double) * ((size_t)gn * (2 * L + 1 + 3) + (opt.continua ? (size_t)gn * L * 8 : 0)));
         pl.gn = gn; pl.L = L; pl.ns = ns; pl.P = P;
         pl.pblocks = persistent ? persistent_blocks(ctx, P) : 0;
         pl.tiles = (int)((n + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
@@ -1540,6 +1636,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         pl.chlev = mode >= 2 ? ctx->d_chlev.p : nullptr;
         pl.chg = mode >= 2 ? ctx->d_chg.p : nullptr;
         pl.chod_floor = chod_floor;
+        pl.continua = opt.continua;
+        pl.cia_gen = opt.continua ? ctx->cia_gen : 0;
         if (ctx->sched.use_graph) {
             rc = launch_group_graph(ctx, pl);
             if (rc) return rc;
@@ -1646,6 +1744,147 @@ int irs_get_schedule(const irs_ctx *ctx, irs_schedule *sc) {
     return IRS_OK;
 }
 
+int irs_cia_set(irs_ctx *ctx, int pair, int n_blocks, const double *T, const int64_t *off, const double *nu,
+                const double *sigma) {
+    if (!ctx) return IRS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (pair < 0 || pair > 2) return fail(ctx, IRS_EINVAL, "CIA pair must be 0 (CO2-CO2), 1 (N2-N2) or 2 (O2-O2)");
+    if (n_blocks < 0 || n_blocks > IRS_CIA_MAXB)
+        return fail(ctx, IRS_EINVAL, "CIA table needs 0.." + std::to_string(IRS_CIA_MAXB) + " blocks");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto &ge : ctx->graphs)
+        if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
+    ctx->graphs.clear();
+    ctx->cia_gen += 1;
+    ctx->cia_nb[pair] = 0;
+    ctx->cia_nk[pair] = 0;
+    if (n_blocks == 0) return IRS_OK;
+    if (!T || !off || !nu || !sigma) return fail(ctx, IRS_EINVAL, "null argument");
+    if (off[0] != 0) return fail(ctx, IRS_EINVAL, "CIA block offsets must start at 0");
+    for (int b = 0; b < n_blocks; ++b) {
+        if (off[b + 1] <= off[b]) return fail(ctx, IRS_EINVAL, "every CIA block needs at least one knot");
+        for (int64_t i = off[b] + 1; i < off[b + 1]; ++i)
+            if (!(nu[i] > nu[i - 1])) return fail(ctx, IRS_EINVAL, "CIA block wavenumbers must increase");
+        if (!std::isfinite(T[b])) return fail(ctx, IRS_EINVAL, "bad CIA block temperature");
+    }
+    // blocks in ascending temperature, stable so that equal temperatures keep
+    // their file order (the first one wins the bracketing, continuum.jl:346-350)
+    std::vector<int> ord(n_blocks);
+    for (int b = 0; b < n_blocks; ++b) ord[b] = b;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return T[x] < T[y]; });
+    const int64_t nk = off[n_blocks];
+    std::vector<double> buf((size_t)4 * n_blocks + 1 + 2 * (size_t)nk);
+    int64_t *boff = (int64_t *)(buf.data() + 3 * n_blocks);
+    double *bnu = buf.data() + 4 * n_blocks + 1, *bsig = bnu + nk;
+    int64_t o = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+        const int src = ord[b];
+        buf[b] = T[src];
+        buf[n_blocks + b] = nu[off[src]];
+        buf[2 * n_blocks + b] = nu[off[src + 1] - 1];
+        boff[b] = o;
+        for (int64_t i = off[src]; i < off[src + 1]; ++i, ++o) {
+            bnu[o] = nu[i];
+            bsig[o] = sigma[i] > 0.0 ? sigma[i] : 0.0;
+        }
+    }
+    boff[n_blocks] = o;
+    HIPCHK(ensure(ctx->d_cia[pair], sizeof(double) * buf.size()));
+    HIPCHK(hipMemcpy(ctx->d_cia[pair].p, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+    ctx->cia_nb[pair] = n_blocks;
+    ctx->cia_nk[pair] = nk;
+    return IRS_OK;
+}
+
+int irs_cia_load_file(irs_ctx *ctx, int pair, const char *path) {
+    if (!ctx) return IRS_EINVAL;
+    if (!path) return fail(ctx, IRS_EINVAL, "null path");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(ctx, IRS_ENOTFOUND, std::string("cannot open CIA file ") + path);
+    std::vector<double> T, nu, sig;
+    std::vector<int64_t> off;
+    std::vector<std::pair<double, double>> cur;
+    auto flush = [&]() {
+        if (cur.empty()) return;
+        // HITRAN sometimes stores a block in descending wavenumber (continuum.jl:116-121)
+        std::stable_sort(cur.begin(), cur.end(),
+                         [](const std::pair<double, double> &a, const std::pair<double, double> &b) { return a.first < b.first; });
+        off.push_back((int64_t)nu.size());
+        for (auto &c : cur) { nu.push_back(c.first); sig.push_back(c.second); }
+        cur.clear();
+    };
+    char line[1024];
+    double curT = NAN;
+    int bad = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        const char *c = line;
+        while (*c == ' ' || *c == '\t') ++c;
+        if (*c == '\0' || *c == '\n' || *c == '\r') continue;
+        if (std::isalpha((unsigned char)*c)) {           // header: pair numin numax n T sigma_max ...
+            flush();
+            char name[256];
+            double a0, a1, tT;
+            long nn;
+            if (std::sscanf(c, "%255s %lf %lf %ld %lf", name, &a0, &a1, &nn, &tT) != 5) { bad = 1; break; }
+            curT = tT;
+            T.push_back(curT);
+        } else {
+            double x, y;
+            if (T.empty() || std::sscanf(c, "%lf %lf", &x, &y) != 2) { bad = 1; break; }
+            cur.push_back({x, std::max(0.0, y)});
+        }
+    }
+    std::fclose(f);
+    if (bad) return fail(ctx, IRS_EINVAL, std::string("malformed CIA file ") + path);
+    flush();
+    // headers without data rows do not make a block
+    if (off.size() != T.size()) return fail(ctx, IRS_EINVAL, std::string("CIA block without data in ") + path);
+    off.push_back((int64_t)nu.size());
+    if (T.empty()) return fail(ctx, IRS_EINVAL, std::string("no CIA blocks in ") + path);
+    if ((int)T.size() > IRS_CIA_MAXB)
+        return fail(ctx, IRS_EINVAL, "CIA table needs 0.." + std::to_string(IRS_CIA_MAXB) + " blocks");
+    return irs_cia_set(ctx, pair, (int)T.size(), T.data(), off.data(), nu.data(), sig.data());
+}
+
+int irs_continuum_layer(irs_ctx *ctx, double nu0, double dnu, int64_t n_nu, int continua, double vmr_h2o,
+                        double vmr_co2, double vmr_n2, double vmr_o2, double p_hpa, double T, double dz_cm,
+                        double *tau_c) {
+    if (!ctx) return IRS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (!tau_c || n_nu <= 0 || !(dnu > 0.0)) return fail(ctx, IRS_EINVAL, "bad grid");
+    if (continua < 0 || continua > IRS_CONT_ALL) return fail(ctx, IRS_EINVAL, "bad continua mask");
+    if (!(p_hpa > 0.0) || !(T > 0.0)) return fail(ctx, IRS_EINVAL, "non-positive pressure or temperature");
+    ContLayer cl{};
+    cl.vh2o = vmr_h2o;
+    cl.vco2 = vmr_co2;
+    cl.vn2 = vmr_n2;
+    cl.vo2 = vmr_o2;
+    cl.p_mid = p_hpa;
+    cl.T_mid = T;
+    cl.dz = dz_cm;
+    HIPCHK(ensure(ctx->d_tmp4, sizeof(ContLayer)));
+    HIPCHK(ensure(ctx->d_tmp5, sizeof(double) * n_nu));
+    HIPCHK(hipMemcpyAsync(ctx->d_tmp4.p, &cl, sizeof(cl), hipMemcpyHostToDevice, ctx->stream));
+    const Grid g = make_grid(nu0, dnu, nu0 + (double)(n_nu - 1) * dnu);
+    ContArgs co = cont_args(ctx, continua);
+    co.n = n_nu;
+    co.n_layers = 1;
+    co.n_prof = 1;
+    co.an = g.an;
+    co.sn = g.sn;
+    co.den = g.den;
+    co.lay = (const ContLayer *)ctx->d_tmp4.p;
+    co.tau64 = (double *)ctx->d_tmp5.p;
+    if (continua) {
+        HIPCHK(launch_continuum(co, ctx->stream));
+        HIPCHK(hipMemcpyAsync(tau_c, ctx->d_tmp5.p, sizeof(double) * n_nu, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        std::memset(tau_c, 0, sizeof(double) * n_nu);
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IRS_OK;
+}
+
 int irs_state_size(const irs_state_spec *sp, int n_levels) {
     if (!sp || n_levels <= 0) return -1;
     int n = sp->include_temperature ? n_levels : 0;
@@ -1665,6 +1904,7 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     if (!(pr->mu > 0.0 && pr->mu <= 1.0)) return fail(ctx, IRS_EINVAL, "μ must be in (0, 1]");
     if (opt.source_function != IRS_SRC_TOON && opt.source_function != IRS_SRC_CIM)
         return fail(ctx, IRS_EINVAL, "source_function must be :toon or :cim");
+    if (opt.continua < 0 || opt.continua > IRS_CONT_ALL) return fail(ctx, IRS_EINVAL, "bad continua mask");
     auto t0 = std::chrono::steady_clock::now();
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
     const Grid g = internal_grid(snd, &opt, nullptr);
@@ -1793,6 +2033,34 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     sa.vhi = sa.vlo + NL * std::max(nr, 1);
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     HIPCHK(launch_grad_linesum(sa, ctx->stream));
+    if (opt.continua) {
+        // continuum optical depth and its analytic T / H2O / CO2 derivatives
+        // (vmr_jacobian.jl:288-349: the VMR terms only with vmr_coupling, as there)
+        std::vector<ContLayer> cl(L);
+        cont_layers(pr, lp, cl.data());
+        HIPCHK(ensure(ctx->d_tmp4, sizeof(ContLayer) * L));
+        HIPCHK(hipMemcpyAsync(ctx->d_tmp4.p, cl.data(), sizeof(ContLayer) * L, hipMemcpyHostToDevice, ctx->stream));
+        ContArgs co = cont_args(ctx, opt.continua);
+        co.tau = sa.tau;
+        co.n = n;
+        co.n_layers = L;
+        co.n_prof = 1;
+        co.an = g.an;
+        co.sn = g.sn;
+        co.den = g.den;
+        co.lay = (const ContLayer *)ctx->d_tmp4.p;
+        if (sp->include_temperature) {
+            co.dtlo = sa.dtlo;
+            co.dthi = sa.dthi;
+        }
+        for (int r = 0; r < nr && sp->vmr_coupling; ++r) {
+            const int spid = sp->retrieved[r];
+            float *lo = sa.vlo + (size_t)r * NL, *hi = sa.vhi + (size_t)r * NL;
+            if (spid == 1) { co.vlo_h = lo; co.vhi_h = hi; }
+            if (spid == 2) { co.vlo_c = lo; co.vhi_c = hi; }
+        }
+        HIPCHK(launch_continuum(co, ctx->stream));
+    }
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     // ---- RTE Jacobian
     std::vector<double> tlev(Tl, Tl + nlev), tave(L), frac(L);

@@ -3,9 +3,9 @@
 // rte_kernel: one thread per internal grid point.  Layers are swept from the
 // top of the atmosphere down, forming the level-to-space transmittance as the
 // running product T_k = T_{k+1} exp(-tau_k/mu) (src/Solver/transmittance.jl:
-// 97-114) and adding each layer's emission with the CIM or Toon source
-// (src/Solver/schwarzschild.jl:168-188, 286-318), then the surface term and,
-// for eps < 1, the reflected downwelling term (schwarzschild.jl:353-399).
+// 26-43) and adding each layer's emission with the CIM or Toon source
+// (src/Solver/schwarzschild.jl:161-193), then the surface term and, for
+// eps < 1, the reflected downwelling term (schwarzschild.jl:203-206,228-274).
 // tau is read once (FP32 from the line sum, or FP64 for irs_rte).
 //
 // ils_channels_kernel: one workgroup per sounder channel: the direct ILS
@@ -16,11 +16,11 @@
 #include "irs_devmath.h"
 #include "irs_internal.h"
 
-__device__ __forceinline__ double cim_corr(double s, double e) {   // schwarzschild.jl:184-188
+__device__ __forceinline__ double cim_corr(double s, double e) {   // schwarzschild.jl:59-64
     return s < 0.06 ? s / 6.0 : 1.0 - 2.0 * (e / (e - 1.0) + 1.0 / s);
 }
 
-__device__ __forceinline__ double lit_corr(double s, double e) {   // schwarzschild.jl:168-172
+__device__ __forceinline__ double lit_corr(double s, double e) {   // schwarzschild.jl:43-47
     return s < 1e-4 ? s * (0.5 - s * (1.0 / 3.0 - s * 0.125)) : (1.0 - e) / s - e;
 }
 

@@ -18,11 +18,11 @@ __all__ = ["AtmosphericProfile", "ColumnScale", "CrISInstrument", "FullProfile",
            "nadir_geometry", "scan_angle_to_local_zenith", "sounder_grid", "wavenumber_grid"]
 
 from .api import (IRSError, Context, analytic_jacobian, apply_ils_channels,  # noqa: E402
-                  brightness_temperature, compute_voigt_cross_sections,
-                  compute_voigt_lm_cross_sections, forward_model, ils_kernel, planck_radiance,
-                  schwarzschild_rte)
+                  brightness_temperature, co2_cia, co2_continuum, compute_voigt_cross_sections,
+                  compute_voigt_lm_cross_sections, forward_model, h2o_continuum, ils_kernel,
+                  n2_cia, o2_cia, planck_radiance, schwarzschild_rte)
 
 __all__ += ["IRSError", "Context", "analytic_jacobian", "apply_ils_channels",
-            "brightness_temperature", "compute_voigt_cross_sections",
-            "compute_voigt_lm_cross_sections", "forward_model", "ils_kernel", "planck_radiance",
-            "schwarzschild_rte"]
+            "brightness_temperature", "co2_cia", "co2_continuum", "compute_voigt_cross_sections",
+            "compute_voigt_lm_cross_sections", "forward_model", "h2o_continuum", "ils_kernel",
+            "n2_cia", "o2_cia", "planck_radiance", "schwarzschild_rte"]

@@ -4,8 +4,10 @@ Names, argument meaning, defaults and error behaviour follow the reference:
   forward_model                   src/Sensor/sounder.jl:233-360
   compute_voigt_cross_sections    src/CrossSections/voigt.jl:375-462 (+ DPTMIN :333-355)
   compute_voigt_lm_cross_sections src/CrossSections/line_mixing.jl:1351-1369
-  schwarzschild_rte               src/Solver/schwarzschild.jl:255-334
+  schwarzschild_rte               src/Solver/schwarzschild.jl:130-209,228-274
   ils_kernel / apply_ils          src/Sensor/ils.jl:80-153
+  h2o_continuum, co2_continuum, co2_cia, n2_cia, o2_cia
+                                  src/CrossSections/continuum.jl:163-327
   planck_radiance / brightness_temperature   src/Solver/planck.jl:11-57
   analytic_jacobian               src/Estimation/vmr_jacobian.jl:152-505
 Every numeric call runs in the HIP library; there is no CPU fallback.  A
@@ -16,6 +18,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import weakref
 from typing import Dict, Optional
 
 import numpy as np
@@ -30,6 +33,9 @@ LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "csrc", "libirs.so"))
 APODIZATION = {"gaussian": 0, "norton_beer_weak": 1, "norton_beer_medium": 2, "norton_beer_strong": 3}
 SOURCE_FUNCTION = {"toon": 0, "cim": 1}
 T_METHOD = {"logp_at_pcg": 0, "mass_weighted": 1}
+CONTINUA = {"h2o": 1, "co2": 2, "co2_cia": 4, "n2": 8, "o2": 16}
+DEFAULT_CONTINUA = ("h2o", "co2", "co2_cia", "n2", "o2")      # sounder.jl:246
+CIA_PAIRS = {"CO2-CO2": 0, "N2-N2": 1, "O2-O2": 2}
 
 IRS_OK, IRS_EINVAL, IRS_EHIP, IRS_ENOMEM, IRS_ENOTFOUND, IRS_ENODEV = 0, -1, -2, -3, -4, -5
 
@@ -63,7 +69,7 @@ class IrsOptions(C.Structure):
     _fields_ = [("internal_dnu", f64), ("high_res_factor", i32), ("cutoff", f64),
                 ("lm_cutoff", f64), ("dptmn", f64), ("x_far", f64), ("with_ils", i32),
                 ("apodization", i32), ("line_mixing", i32), ("T_method", i32),
-                ("source_function", i32), ("reserved0", i32)]
+                ("source_function", i32), ("continua", i32)]
 
 
 class IrsStateSpec(C.Structure):
@@ -96,7 +102,8 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_mlp_n_params", "irs_mlp_input_pad", "irs_mlp_get_params", "irs_mlp_set_params",
            "irs_mlp_forward", "irs_mlp_loss_grad", "irs_mlp_set_data", "irs_mlp_train",
            "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
-           "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian")
+           "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian", "irs_cia_set",
+           "irs_cia_load_file", "irs_continuum_layer")
 
 _lib = None
 
@@ -165,6 +172,9 @@ def load_library(path: str = LIB_PATH):
     L.irs_emu_last_error.restype = C.c_char_p
     L.irs_emu_forward.argtypes = [P, P, P, P, P, i32, i32, P, P, P]
     L.irs_emu_jacobian.argtypes = [P, P, P, P, P, i32, P, P, P, P, P]
+    L.irs_cia_set.argtypes = [P, i32, i32, P, P, P, P]
+    L.irs_cia_load_file.argtypes = [P, i32, C.c_char_p]
+    L.irs_continuum_layer.argtypes = [P, f64, f64, i64, i32, f64, f64, f64, f64, f64, f64, f64, P]
     for name in EXPORTS:
         if name not in ("irs_destroy", "irs_default_options", "irs_last_error", "irs_mlp_destroy",
                         "irs_mlp_last_error", "irs_mlp_n_params", "irs_emu_destroy", "irs_emu_last_error"):
@@ -201,9 +211,12 @@ class Context:
         if rc != IRS_OK:
             raise IRSError(rc, f"irs_create failed ({rc}): no usable gfx950 device {device}")
         self.h = h
-        self._lists: Dict[int, tuple] = {}
+        self._lists: Dict[int, tuple] = {}     # id(list) -> (handle, weakref, finalizer)
 
     def close(self):
+        for _, _, fin in list(getattr(self, "_lists", {}).values()):
+            fin.detach()
+        self._lists = {}
         if getattr(self, "h", None):
             self.lib.irs_destroy(self.h)
             self.h = None
@@ -243,16 +256,52 @@ class Context:
         self.check(self.lib.irs_set_schedule(self.h, C.byref(sc)))
 
     def upload(self, ll: HITRANLinelist) -> int:
+        """Device handle of a line list, uploaded on first use.  The cache holds
+        the list weakly: when the list is garbage collected its device copy is
+        freed (irs_linelist_free)."""
         key = id(ll)
-        if key in self._lists and self._lists[key][1] is ll:
-            return self._lists[key][0]
+        hit = self._lists.get(key)
+        if hit is not None and hit[1]() is ll:
+            return hit[0]
         s = IrsLinelist(len(ll), _p(ll.mol_id), _p(ll.iso_id), _p(ll.wavenumber), _p(ll.intensity),
                         _p(ll.air_broad), _p(ll.self_broad), _p(ll.lower_energy), _p(ll.temp_depend),
                         _p(ll.pressure_shift), _p(ll.y_lm))
         h = C.c_int32()
         self.check(self.lib.irs_linelist_upload(self.h, C.byref(s), C.byref(h)))
-        self._lists[key] = (h.value, ll)
+        fin = weakref.finalize(ll, Context._release_list, weakref.ref(self), key, h.value)
+        self._lists[key] = (h.value, weakref.ref(ll), fin)
         return h.value
+
+    @staticmethod
+    def _release_list(ctx_ref, key, handle):
+        ctx = ctx_ref()
+        if ctx is None or not getattr(ctx, "h", None):
+            return
+        ent = ctx._lists.get(key)
+        if ent is not None and ent[0] == handle:
+            del ctx._lists[key]
+        ctx.lib.irs_linelist_free(ctx.h, handle)
+
+    def set_cia(self, pair, blocks):
+        """CIA table of one collision pair ("CO2-CO2", "N2-N2", "O2-O2"): blocks
+        is a list of (T, nu[], sigma[]) (cm^5 molec^-2); an empty list removes
+        it, so the pair adds zero (the reference's behaviour without the data
+        file, src/CrossSections/continuum.jl:254-258)."""
+        pr = CIA_PAIRS[pair] if isinstance(pair, str) else int(pair)
+        T = np.array([float(b[0]) for b in blocks] or [0.0])
+        nus = [np.ascontiguousarray(b[1], np.float64) for b in blocks]
+        sig = [np.ascontiguousarray(b[2], np.float64) for b in blocks]
+        off = np.zeros(len(blocks) + 1, np.int64)
+        for i, x in enumerate(nus):
+            off[i + 1] = off[i] + len(x)
+        nu = np.concatenate(nus) if nus else np.zeros(1)
+        sg = np.concatenate(sig) if sig else np.zeros(1)
+        self.check(self.lib.irs_cia_set(self.h, pr, len(blocks), _p(T), _p(off), _p(nu), _p(sg)))
+
+    def load_cia(self, pair, path: str):
+        """Reads a HITRAN .cia file into the pair's table (continuum.jl:106-148)."""
+        pr = CIA_PAIRS[pair] if isinstance(pair, str) else int(pair)
+        self.check(self.lib.irs_cia_load_file(self.h, pr, os.fsencode(path)))
 
 
 _default_ctx: Optional[Context] = None
@@ -309,10 +358,64 @@ def compute_voigt_lm_cross_sections(nu_grid: WavenumberGrid, ll_co2: HITRANLinel
     return out
 
 
+def _continuum(bit, nu_grid, vmr, p_hPa, T, ctx):
+    ctx = ctx or default_context()
+    out = np.zeros(nu_grid.n)
+    v = [0.0, 0.0, 0.0, 0.0]
+    v[{1: 0, 2: 1, 4: 1, 8: 2, 16: 3}[bit]] = float(vmr)
+    ctx.check(ctx.lib.irs_continuum_layer(ctx.h, nu_grid.nu_min, nu_grid.dnu, nu_grid.n, bit, v[0], v[1], v[2],
+                                          v[3], float(p_hPa), float(T), 1.0, _p(out)))
+    return out
+
+
+def h2o_continuum(nu_grid: WavenumberGrid, vmr_h2o: float, p_hPa: float, T: float, *,
+                  ctx: Optional[Context] = None) -> np.ndarray:
+    """MT-CKD 4.3 H2O self + foreign continuum absorption coefficient (cm^-1)
+    (continuum.jl:163-190), evaluated on the device."""
+    return _continuum(1, nu_grid, vmr_h2o, p_hPa, T, ctx)
+
+
+def co2_continuum(nu_grid: WavenumberGrid, vmr_co2: float, p_hPa: float, T: float, *,
+                  ctx: Optional[Context] = None) -> np.ndarray:
+    """MT-CKD CO2 line-coupling continuum (cm^-1) (continuum.jl:209-232)."""
+    return _continuum(2, nu_grid, vmr_co2, p_hPa, T, ctx)
+
+
+def co2_cia(nu_grid: WavenumberGrid, vmr_co2: float, p_hPa: float, T: float, *,
+            ctx: Optional[Context] = None) -> np.ndarray:
+    """CO2-CO2 CIA scaled by n_CO2 n_air (cm^-1); zeros without a loaded table
+    (continuum.jl:249-270)."""
+    return _continuum(4, nu_grid, vmr_co2, p_hPa, T, ctx)
+
+
+def n2_cia(nu_grid: WavenumberGrid, vmr_n2: float, p_hPa: float, T: float, *,
+           ctx: Optional[Context] = None) -> np.ndarray:
+    """N2-N2 CIA (cm^-1) (continuum.jl:279-298)."""
+    return _continuum(8, nu_grid, vmr_n2, p_hPa, T, ctx)
+
+
+def o2_cia(nu_grid: WavenumberGrid, vmr_o2: float, p_hPa: float, T: float, *,
+           ctx: Optional[Context] = None) -> np.ndarray:
+    """O2-O2 CIA (cm^-1) (continuum.jl:307-327)."""
+    return _continuum(16, nu_grid, vmr_o2, p_hPa, T, ctx)
+
+
+def _continua_mask(apply_continuum, continua) -> int:
+    if not apply_continuum:
+        return 0
+    m = 0
+    for c in continua:
+        key = c.lstrip(":") if isinstance(c, str) else c
+        if key not in CONTINUA:
+            raise ValueError(f"unknown continuum :{key}; expected one of {list(CONTINUA)}")
+        m |= CONTINUA[key]
+    return m
+
+
 def schwarzschild_rte(nu_grid: WavenumberGrid, tau_layers: np.ndarray, T_levels, T_sfc: float, *,
                       mu: float = 1.0, eps_sfc: float = 1.0, source_function: str = "toon",
                       T_ave=None, ctx: Optional[Context] = None) -> np.ndarray:
-    """TOA radiance; tau_layers is (n_nu, n_layers) (schwarzschild.jl:255-334)."""
+    """TOA radiance; tau_layers is (n_nu, n_layers) (schwarzschild.jl:130-209)."""
     tau = np.asfortranarray(tau_layers, np.float64)
     n_nu, L = tau.shape
     if n_nu != nu_grid.n:
@@ -393,7 +496,7 @@ def brightness_temperature(nu, R):
 
 def _options(internal_dnu=0.001, high_res_factor=None, cutoff=25.0, lm_cutoff=None, dptmn=1e-6,
              x_far=122.0, with_ils=True, apodization="gaussian", line_mixing=None,
-             T_method="logp_at_pcg", source_function="cim"):
+             T_method="logp_at_pcg", source_function="cim", continua_mask=0):
     o = IrsOptions()
     load_library().irs_default_options(C.byref(o))
     o.internal_dnu = 0.0 if internal_dnu is None else float(internal_dnu)
@@ -419,6 +522,7 @@ def _options(internal_dnu=0.001, high_res_factor=None, cutoff=25.0, lm_cutoff=No
     if sf not in SOURCE_FUNCTION:
         raise ValueError(f"source_function must be :toon or :cim, got :{sf}")
     o.source_function = SOURCE_FUNCTION[sf]
+    o.continua = int(continua_mask)
     return o
 
 
@@ -450,18 +554,17 @@ def forward_model(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
                   geom: Optional[ViewingGeometry] = None, T_sfc=None, eps_sfc: float = 1.0,
                   internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
                   cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
-                  apply_continuum: bool = False, with_ils: bool = True,
+                  apply_continuum: bool = True, continua=DEFAULT_CONTINUA, with_ils: bool = True,
                   apodization="gaussian", line_mixing=None, T_method="logp_at_pcg",
                   source_function="cim", x_far: float = 122.0, ctx: Optional[Context] = None):
     """forward_model(prof, linelists; ...) -> (nu, R, BT)   (src/Sensor/sounder.jl:233-360).
 
     `prof` may be one AtmosphericProfile or a list of them (a batch on one
     device launch sequence); geom, T_sfc and eps_sfc may then be lists too.
-    Continuum absorption is not part of this engine: apply_continuum=True
-    raises (the reference default is True; pass False explicitly)."""
-    if apply_continuum:
-        raise NotImplementedError("continuum absorption is not implemented in the GPU engine; "
-                                  "pass apply_continuum=False")
+    apply_continuum / continua add the MT-CKD H2O and CO2 continua and the
+    CIA pairs whose tables are loaded (Context.load_cia) per layer, as the
+    reference does by default (sounder.jl:245-246,310-322)."""
+    cmask = _continua_mask(apply_continuum, continua)
     inst = iasi if iasi is not None else (sounder if sounder is not None else IASIInstrument())
     batch = isinstance(prof, (list, tuple))
     profs = list(prof) if batch else [prof]
@@ -472,7 +575,7 @@ def forward_model(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     eps = eps_sfc if isinstance(eps_sfc, (list, tuple, np.ndarray)) else [eps_sfc] * nb
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
-                   apodization, line_mixing, T_method, source_function)
+                   apodization, line_mixing, T_method, source_function, cmask)
     b = _Batch(ctx, profs, linelists, tsf, eps, [g.mu for g in geoms])
     snd = _sounder(inst)
     grid = sounder_grid(inst)
@@ -490,7 +593,8 @@ def analytic_jacobian(prof: AtmosphericProfile, linelists: Dict[GasSpecies, HITR
                       T_sfc=None, eps_sfc: float = 1.0, observable: str = "bt",
                       internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
                       cutoff: float = 25.0, lm_cutoff: Optional[float] = None,
-                      apply_continuum: bool = False, with_ils: bool = True, apodization="gaussian",
+                      apply_continuum: bool = True, continua=DEFAULT_CONTINUA, with_ils: bool = True,
+                      apodization="gaussian",
                       line_mixing=None, T_method="logp_at_pcg", source_function="cim",
                       vmr_coupling: bool = True, x_far: float = 122.0,
                       ctx: Optional[Context] = None) -> Jacobian:
@@ -505,9 +609,7 @@ def analytic_jacobian(prof: AtmosphericProfile, linelists: Dict[GasSpecies, HITR
     equals projecting before the convolution)."""
     if observable not in ("bt", "radiance"):
         raise ValueError(f"observable must be :bt or :radiance, got :{observable}")
-    if apply_continuum:
-        raise NotImplementedError("continuum absorption is not implemented in the GPU engine; "
-                                  "pass apply_continuum=False")
+    cmask = _continua_mask(apply_continuum, continua)
     if prof.n_levels != spec.n_levels:
         raise ValueError(f"profile has {prof.n_levels} levels, spec expects {spec.n_levels}")
     inst = iasi if sounder is None else sounder
@@ -515,7 +617,7 @@ def analytic_jacobian(prof: AtmosphericProfile, linelists: Dict[GasSpecies, HITR
     geom = geom if geom is not None else nadir_geometry()
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, 0.0, x_far, with_ils,
-                   apodization, line_mixing, T_method, source_function)
+                   apodization, line_mixing, T_method, source_function, cmask)
     # device blocks: ColumnScale natively, PartialColumns as full log-VMR columns
     blocks = spec.vmr_blocks
     if len(blocks) > 4:
@@ -579,6 +681,7 @@ def channel_optical_depths(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
                            cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
                            with_ils: bool = True, apodization="gaussian", line_mixing=None,
                            T_method="logp_at_pcg", x_far: float = 122.0, cim: bool = False,
+                           apply_continuum: bool = True, continua=DEFAULT_CONTINUA,
                            ctx: Optional[Context] = None):
     """Per-layer channel optical depths along each profile's viewing path
     (geom, default nadir), [n_prof, n_layers, n_channels] float32
@@ -591,7 +694,7 @@ def channel_optical_depths(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     profs = list(prof) if batch else [prof]
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
-                   apodization, line_mixing, T_method, "cim")
+                   apodization, line_mixing, T_method, "cim", _continua_mask(apply_continuum, continua))
     nb = len(profs)
     geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
     mus = [1.0 if g is None else g.mu for g in geoms]
@@ -636,7 +739,8 @@ def channel_transmittance(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
                           internal_dnu: Optional[float] = 0.001, high_res_factor: Optional[int] = None,
                           cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
                           with_ils: bool = True, apodization="gaussian", line_mixing=None,
-                          T_method="logp_at_pcg", x_far: float = 122.0, ctx: Optional[Context] = None):
+                          T_method="logp_at_pcg", x_far: float = 122.0, apply_continuum: bool = True,
+                          continua=DEFAULT_CONTINUA, ctx: Optional[Context] = None):
     """ILS-convolved level-to-space transmittances along each profile's path,
     T_lev [n_prof, n_layers+1, n_channels], and the CIM sums
     G [n_prof, n_layers, n_channels] (irs_channel_transmittance): the
@@ -646,7 +750,7 @@ def channel_transmittance(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     profs = list(prof) if batch else [prof]
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
-                   apodization, line_mixing, T_method, "cim")
+                   apodization, line_mixing, T_method, "cim", _continua_mask(apply_continuum, continua))
     nb = len(profs)
     geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
     mus = [1.0 if g is None else g.mu for g in geoms]

@@ -218,10 +218,12 @@ class ChannelODEmulator:
     def train(profiles: Sequence, linelists: Dict[GasSpecies, HITRANLinelist], sounder: Sounder, *,
               n_pc: int = 256, steps: int = 3000, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
               floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None, targets=None,
-              relevance_floor: Optional[float] = None, **kw):
+              relevance_floor: Optional[float] = None, apply_continuum: bool = False, **kw):
         """Train on scenes `profiles` (AtmosphericProfile or (profile, T_sfc, mu)
         tuples); the targets along each scene's path come from the GPU LBL
-        engine unless `targets` = (tau_ch [N, L, n_ch], g [N, L, n_ch]) is given."""
+        engine unless `targets` = (tau_ch [N, L, n_ch], g [N, L, n_ch]) is given.
+        The LBL runs without continuum unless apply_continuum is True; the spot
+        check and fallback of forward_emulated must use the same setting."""
         scenes = [_scene(p) for p in profiles]
         profs = [s[0] for s in scenes]
         mus = [s[2] for s in scenes]
@@ -229,7 +231,8 @@ class ChannelODEmulator:
         t0 = time.perf_counter()
         if targets is None:
             targets = channel_optical_depths(profs, linelists, sounder=sounder, geom=geoms, floor=floor,
-                                             line_mixing=line_mixing, cim=True, ctx=ctx, **kw)
+                                             line_mixing=line_mixing, cim=True, ctx=ctx,
+                                             apply_continuum=apply_continuum, **kw)
         tau_ch, g = targets
         t1 = time.perf_counter()
         N, L, nch = g.shape
@@ -472,9 +475,10 @@ class EmulatedResult:
 def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: Sounder,
                      geom=None, T_sfc=None, eps_sfc=1.0, threshold: float = 0.05,
                      spot: Optional[SpotChecker] = None, ctx: Optional[Context] = None,
-                     line_mixing="vpy", **kw) -> EmulatedResult:
+                     line_mixing="vpy", apply_continuum: bool = False, **kw) -> EmulatedResult:
     """Emulator + per-scene LBL spot check + full-LBL fallback (BT budget
-    `threshold` K on the spot channels)."""
+    `threshold` K on the spot channels).  apply_continuum as in training."""
+    kw["apply_continuum"] = apply_continuum
     nb = len(profs)
     geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
     geoms = [g if g is not None else ViewingGeometry(0.0, 0.0, 1.0) for g in geoms]

@@ -10,8 +10,10 @@ hand-written gfx950 kernels.  The reference's own types (`AtmosphericProfile`,
 
 Line mixing: VP_Y needs the first-order coefficient Y of every CO2 line at the
 layer temperature; pass `line_mixing = Y::Vector{Float64}` (atm^-1, one per
-line of the CO2 list, in list order).  Continuum is not computed on the device:
-`apply_continuum=true` raises, as do the other unsupported keywords.
+line of the CO2 list, in list order).  Continuum (`apply_continuum`, `continua`)
+is computed on the device from the bundled MT-CKD tables; the HITRAN CIA pairs
+add zero until a table is loaded with `load_cia!`, as in the reference when the
+data files are absent.
 """
 module IRSounderLBLAMD
 
@@ -49,7 +51,7 @@ end
 mutable struct COptions
     internal_dnu::Float64; high_res_factor::Cint; cutoff::Float64; lm_cutoff::Float64
     dptmn::Float64; x_far::Float64; with_ils::Cint; apodization::Cint; line_mixing::Cint
-    T_method::Cint; source_function::Cint; reserved0::Cint
+    T_method::Cint; source_function::Cint; continua::Cint
 end
 
 struct CStateSpec
@@ -61,6 +63,8 @@ end
 
 const APOD = Dict(:gaussian => 0, :norton_beer_weak => 1, :norton_beer_medium => 2,
                   :norton_beer_strong => 3)
+const CONTINUA = Dict(:h2o => 1, :co2 => 2, :co2_cia => 4, :n2 => 8, :o2 => 16)
+const CIA_PAIR = Dict(:co2_cia => 0, :n2 => 1, :o2 => 2)
 
 # ---- context and line-list cache --------------------------------------------
 mutable struct Context
@@ -108,8 +112,19 @@ function upload!(ctx::Context, ll::HITRANLinelist, y::Union{Nothing, Vector{Floa
     ctx.lists[key] = h[]
 end
 
+function continua_mask(apply_continuum::Bool, continua)
+    apply_continuum || return Cint(0)
+    m = 0
+    for c in continua
+        haskey(CONTINUA, c) || error("unknown continuum :$c; expected one of $(collect(keys(CONTINUA)))")
+        m |= CONTINUA[c]
+    end
+    return Cint(m)
+end
+
 function options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with_ils,
-                 apodization, line_mixing, T_method, source_function, x_far = 122.0)
+                 apodization, line_mixing, T_method, source_function, x_far = 122.0,
+                 apply_continuum = false, continua = ())
     o = COptions(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
     ccall((:irs_default_options, libirs), Cvoid, (Ref{COptions},), o)
     o.internal_dnu = internal_dnu === nothing ? 0.0 : internal_dnu
@@ -125,7 +140,20 @@ function options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with
     o.T_method = T_method === :logp_at_pcg ? 0 : 1
     source_function in (:toon, :cim) || error("source_function must be :toon or :cim, got :$source_function")
     o.source_function = source_function === :toon ? 0 : 1
+    o.continua = continua_mask(apply_continuum, continua)
     return o
+end
+
+"""
+    load_cia!(pair, path; ctx = context())
+
+Load a HITRAN `.cia` file for `pair` (`:co2_cia`, `:n2` or `:o2`, the `continua`
+symbols of the pair) into the device table (src/CrossSections/continuum.jl:106-148).
+"""
+function load_cia!(pair::Symbol, path::AbstractString; ctx::Context = context())
+    haskey(CIA_PAIR, pair) || error("CIA pair must be :co2_cia, :n2 or :o2, got :$pair")
+    check(ctx, ccall((:irs_cia_load_file, libirs), Cint, (Ptr{Cvoid}, Cint, Cstring),
+                     ctx.h, CIA_PAIR[pair], path))
 end
 
 csounder(s::Sounder) = CSounder(s.ν_min, s.ν_max, s.Δν, s.opd_max, s.fwhm_gauss, s.n_channels)
@@ -151,8 +179,8 @@ profiles (with `geom`, `T_sfc`, `ε_sfc` scalars or vectors): one call, one
 device launch sequence; R and BT are then n_channels x n_profiles.
 """
 function forward_model(prof, linelists::Dict{GasSpecies, HITRANLinelist};
-                       sounder::Union{Sounder, Nothing} = nothing,
-                       iasi::Sounder = IASIInstrument(), geom = nadir_geometry(),
+                       sounder::Sounder = IASIInstrument(),
+                       iasi::Union{Sounder, Nothing} = nothing, geom = nadir_geometry(),
                        T_sfc = nothing, ε_sfc = 1.0,
                        internal_dnu::Union{Float64, Nothing} = 0.001,
                        high_res_factor::Union{Int, Nothing} = nothing,
@@ -161,14 +189,13 @@ function forward_model(prof, linelists::Dict{GasSpecies, HITRANLinelist};
                        with_ils::Bool = true, apodization::Symbol = :gaussian,
                        line_mixing = nothing, T_method::Symbol = :logp_at_pcg,
                        source_function::Symbol = :cim, backend = nothing)
-    apply_continuum && error("continuum absorption is not computed by libirs; pass apply_continuum=false")
-    snd = sounder === nothing ? iasi : sounder
+    snd = iasi === nothing ? sounder : iasi       # sounder.jl:254: `iasi=` wins when given
     profs = prof isa AbstractVector ? prof : [prof]
     nb = length(profs)
     pick(x, q) = x isa AbstractVector ? x[q] : x
     ctx = context()
     o = options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with_ils, apodization,
-                line_mixing, T_method, source_function)
+                line_mixing, T_method, source_function, apply_continuum, continua)
     keep = Any[]
     cps = Vector{CProfile}(undef, nb)
     for q in 1:nb
@@ -210,13 +237,12 @@ function analytic_jacobian(prof::AtmosphericProfile, linelists::Dict{GasSpecies,
                            source_function::Symbol = :cim, vmr_coupling::Bool = true,
                            backend = nothing)::Jacobian
     observable in (:bt, :radiance) || error("observable must be :bt or :radiance, got :$observable")
-    apply_continuum && error("continuum absorption is not computed by libirs; pass apply_continuum=false")
     length(prof.temperature) == spec.n_levels ||
         error("profile has $(length(prof.temperature)) levels, spec expects $(spec.n_levels)")
-    snd = sounder === nothing ? iasi : sounder
+    snd = sounder === nothing ? iasi : sounder    # vmr_jacobian.jl:177: `sounder=` wins when given
     ctx = context()
     o = options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn = 0.0, with_ils,
-                apodization, line_mixing, T_method, source_function)
+                apodization, line_mixing, T_method, source_function, apply_continuum, continua)
     keep = Any[]
     cp, _ = cprofile(ctx, prof, linelists, line_mixing, T_sfc, ε_sfc, geom.μ, keep)
     blocks = spec.vmr_blocks

@@ -22,6 +22,7 @@
 #endif
 
 #include "../irsounderlbl.jl_amd/data/tips2024_table.h"
+#include "../irsounderlbl.jl_amd/data/mtckd_table.h"
 
 #define SQRT_LN2    0.83255461115769775635   /* sqrt(log(2.0))  voigt.jl:4 */
 #define INV_SQRT_PI 0.56418958354775628695   /* inv(sqrt(pi))   voigt.jl:5 */
@@ -558,6 +559,159 @@ void orc_layer_properties(const orc_profile *prof, int T_method, double *p_mid, 
 }
 
 /* ========================================================================== *
+ * Continuum (src/CrossSections/continuum.jl:138-363; sounder.jl:310-322;
+ * optical_depth.jl:123-130)
+ * ========================================================================== */
+#define CK_KB    1.380649e-23
+#define CK_C2    1.4387769
+#define CK_PREF  1013.0
+#define CK_TREF  296.0
+#define CK_TEFF  246.0
+
+/* linear interpolation on a table's ascending knots with Flat extrapolation
+ * (Interpolations.linear_interpolation, continuum.jl:70-72,93-94) */
+static double tab_interp(const double (*t)[4], int n, int col, double nu, const double *scale_col) {
+    if (nu <= t[0][0]) return t[0][col] * (scale_col ? t[0][(int)scale_col[0]] : 1.0);
+    if (nu >= t[n - 1][0]) return t[n - 1][col] * (scale_col ? t[n - 1][(int)scale_col[0]] : 1.0);
+    int lo = 0, hi = n - 1;                  /* searchsortedlast */
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (t[mid][0] <= nu) lo = mid; else hi = mid;
+    }
+    double v0 = t[lo][col], v1 = t[hi][col];
+    if (scale_col) {
+        v0 *= t[lo][(int)scale_col[0]];
+        v1 *= t[hi][(int)scale_col[0]];
+    }
+    double w = (nu - t[lo][0]) / (t[hi][0] - t[lo][0]);
+    return v0 * (1.0 - w) + v1 * w;
+}
+
+double orc_radfn(double nu, double T) {         /* continuum.jl:150-155 */
+    double x = CK_C2 * nu / T;
+    if (x <= 0.01) return 0.5 * x * nu;
+    if (x <= 10.0) return nu * (1.0 - exp(-x)) / (1.0 + exp(-x));
+    return nu;
+}
+
+double orc_h2o_continuum(double nu, double vmr, double p, double T) {   /* continuum.jl:163-190 */
+    double rho_rat = (p / CK_PREF) * (CK_TREF / T);
+    double n_h2o = vmr * p * 100.0 / (CK_KB * T) * 1e-6;
+    double Cs = tab_interp(irs_ckd_h2o, IRS_CKD_H2O_N, 1, nu, NULL);
+    double Cf = tab_interp(irs_ckd_h2o, IRS_CKD_H2O_N, 2, nu, NULL);
+    double tx = tab_interp(irs_ckd_h2o, IRS_CKD_H2O_N, 3, nu, NULL);
+    double rad = orc_radfn(nu, T);
+    double ks = Cs * pow(CK_TREF / T, tx) * vmr * rho_rat * rad * n_h2o;
+    double kf = Cf * (1.0 - vmr) * rho_rat * rad * n_h2o;
+    return ks + kf;
+}
+
+double orc_co2_continuum(double nu, double vmr, double p, double T) {   /* continuum.jl:209-232 */
+    static const double xfac_col = 2.0;         /* S x xfac folded at load (continuum.jl:90) */
+    double rho_rat = (p / CK_PREF) * (CK_TREF / T);
+    double n_co2 = vmr * p * 100.0 / (CK_KB * T) * 1e-6;
+    double pref = 1e-20 * n_co2 * rho_rat;
+    double S = tab_interp(irs_ckd_co2, IRS_CKD_CO2_N, 1, nu, &xfac_col);
+    double td = tab_interp(irs_ckd_co2, IRS_CKD_CO2_N, 3, nu, NULL);
+    double tcor = td == 0.0 ? 1.0 : pow(T / CK_TEFF, td);
+    return S * tcor * pref * orc_radfn(nu, T);
+}
+
+typedef struct {
+    int nb;
+    double *T;
+    int64_t *off;
+    double *nu, *sig;
+} cia_tab;
+static cia_tab g_cia[3];
+
+int orc_cia_set(int pair, int n_blocks, const double *T, const int64_t *off, const double *nu,
+                const double *sigma) {
+    if (pair < 0 || pair > 2 || n_blocks < 0) return -1;
+    cia_tab *c = &g_cia[pair];
+    free(c->T); free(c->off); free(c->nu); free(c->sig);
+    memset(c, 0, sizeof(*c));
+    if (n_blocks == 0) return 0;
+    int64_t nk = off[n_blocks];
+    c->T = malloc(sizeof(double) * n_blocks);
+    c->off = malloc(sizeof(int64_t) * (n_blocks + 1));
+    c->nu = malloc(sizeof(double) * nk);
+    c->sig = malloc(sizeof(double) * nk);
+    memcpy(c->T, T, sizeof(double) * n_blocks);
+    memcpy(c->off, off, sizeof(int64_t) * (n_blocks + 1));
+    memcpy(c->nu, nu, sizeof(double) * nk);
+    for (int64_t i = 0; i < nk; ++i) c->sig[i] = sigma[i] > 0.0 ? sigma[i] : 0.0;   /* :134 */
+    c->nb = n_blocks;
+    return 0;
+}
+
+/* one block's interpolant, zero outside its knots (extrapolation_bc = 0, :123) */
+static double cia_block(const cia_tab *c, int b, double nu) {
+    const double *x = c->nu + c->off[b], *y = c->sig + c->off[b];
+    int64_t n = c->off[b + 1] - c->off[b];
+    if (n < 1 || nu < x[0] || nu > x[n - 1]) return 0.0;
+    if (n == 1) return y[0];
+    int64_t lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) / 2;
+        if (x[mid] <= nu) lo = mid; else hi = mid;
+    }
+    double w = (nu - x[lo]) / (x[hi] - x[lo]);
+    return y[lo] * (1.0 - w) + y[hi] * w;
+}
+
+double orc_cia_sigma(int pair, double nu, double T) {   /* _cia_sigma, continuum.jl:336-363 */
+    const cia_tab *c = &g_cia[pair];
+    double T_lo = -INFINITY, s_lo = 0.0, T_hi = INFINITY, s_hi = 0.0;
+    int any = 0;
+    for (int b = 0; b < c->nb; ++b) {
+        const double *x = c->nu + c->off[b];
+        int64_t n = c->off[b + 1] - c->off[b];
+        if (n < 1 || nu < x[0] || nu > x[n - 1]) continue;
+        any = 1;
+        double s = cia_block(c, b, nu);
+        if (c->T[b] <= T && c->T[b] > T_lo) { T_lo = c->T[b]; s_lo = s; }
+        if (c->T[b] >= T && c->T[b] < T_hi) { T_hi = c->T[b]; s_hi = s; }
+    }
+    if (!any) return 0.0;
+    if (!isfinite(T_lo)) return s_hi;
+    if (!isfinite(T_hi)) return s_lo;
+    if (T_lo == T_hi) return s_lo;
+    double w = (T - T_lo) / (T_hi - T_lo);
+    return (1.0 - w) * s_lo + w * s_hi;
+}
+
+double orc_cia_k(int pair, double nu, double vmr, double p, double T) {   /* continuum.jl:249-327 */
+    if (g_cia[pair].nb == 0) return 0.0;
+    double n_x = vmr * p * 100.0 / (CK_KB * T) * 1e-6;
+    double nn;
+    if (pair == 0) nn = n_x * (p * 100.0 / (CK_KB * T) * 1e-6);   /* n_CO2 n_air */
+    else nn = n_x * n_x;
+    return orc_cia_sigma(pair, nu, T) * nn;
+}
+
+double orc_dp_to_dz_cm(double dp, double p_mid, double T) {       /* optical_depth.jl:123-130 */
+    double H = 8.314462 * T / (0.028964 * 9.80665);
+    return H * dp / p_mid * 100.0;
+}
+
+void orc_continuum_layer(double nu0, double dnu, int64_t n_nu, int continua, double vh, double vc,
+                         double pm, double Tm, double dp, double *tau_c) {
+    double dz = orc_dp_to_dz_cm(dp, pm, Tm);
+    double vdry = 1.0 - vh;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_nu; ++i) {
+        double nu = orc_grid_node(nu0, dnu, i), t = 0.0;
+        if (continua & ORC_CONT_H2O) t += orc_h2o_continuum(nu, vh, pm, Tm) * dz;
+        if (continua & ORC_CONT_CO2) t += orc_co2_continuum(nu, vc, pm, Tm) * dz;
+        if (continua & ORC_CONT_CO2CIA) t += orc_cia_k(0, nu, vc, pm, Tm) * dz;
+        if (continua & ORC_CONT_N2) t += orc_cia_k(1, nu, 0.78084 * vdry, pm, Tm) * dz;
+        if (continua & ORC_CONT_O2) t += orc_cia_k(2, nu, 0.20946 * vdry, pm, Tm) * dz;
+        tau_c[i] = t;
+    }
+}
+
+/* ========================================================================== *
  * Planck / BT (src/Solver/planck.jl:11-71)
  * ========================================================================== */
 #define C1_PL 1.191042953e-5
@@ -576,8 +730,8 @@ double orc_dbdt(double nu, double T) {
 }
 
 /* ========================================================================== *
- * RTE (src/Solver/schwarzschild.jl:168-215, 255-334, 353-399;
- *      src/Solver/transmittance.jl:97-114)
+ * RTE (src/Solver/schwarzschild.jl:43-64, 130-209, 228-274;
+ *      src/Solver/transmittance.jl:26-43)
  * ========================================================================== */
 double orc_lit_correction(double s) {
     if (s < 1e-4) return s * (0.5 - s * (1.0 / 3.0 - s * 0.125));
@@ -632,7 +786,7 @@ int orc_schwarzschild_rte(double nu0g, double dnu, int64_t n_nu, const double *t
                     acc += ba * dT + (bt - ba) * ck;
                 }
             }
-            if (eps < 1.0) {   /* schwarzschild.jl:353-399 */
+            if (eps < 1.0) {   /* schwarzschild.jl:203-206,228-274 */
                 double down = 0.0, g = 1.0;
                 for (int k = 0; k < L; ++k) {
                     double s = tau[(size_t)k * n_nu + i] / mu;
@@ -818,6 +972,25 @@ int orc_forward(const orc_profile *prof, const orc_sounder *snd, const orc_optio
             double *tk = tau + (size_t)k * n;
             for (int64_t i = 0; i < n; ++i) tk[i] += sig[i] * coef;
         }
+    }
+    if (opt->continua) {                                       /* sounder.jl:310-322 */
+        double *vh_mid = malloc(sizeof(double) * L), *vc_mid = malloc(sizeof(double) * L);
+        int sh = -1, sc = -1;
+        for (int s = 0; s < ns; ++s) {
+            if (prof->species[s] == 1) sh = s;
+            if (prof->species[s] == 2) sc = s;
+        }
+        if (sh >= 0) orc_interp_logp(prof->p_hpa, prof->vmr + (size_t)sh * prof->n_levels, prof->n_levels, p_mid, L, vh_mid);
+        else for (int k = 0; k < L; ++k) vh_mid[k] = 0.0;
+        if (sc >= 0) orc_interp_logp(prof->p_hpa, prof->vmr + (size_t)sc * prof->n_levels, prof->n_levels, p_mid, L, vc_mid);
+        else for (int k = 0; k < L; ++k) vc_mid[k] = 4.15e-4;
+        for (int k = 0; k < L; ++k) {
+            orc_continuum_layer(lo, d, n, opt->continua, vh_mid[k], vc_mid[k], p_mid[k], T_mid[k], dp[k], sig);
+            double *tk = tau + (size_t)k * n;
+            for (int64_t i = 0; i < n; ++i) tk[i] += sig[i];
+        }
+        free(vh_mid);
+        free(vc_mid);
     }
     double t1 = now_s();
     double T_sfc = opt->T_sfc > 0.0 ? opt->T_sfc : prof->T[0];
@@ -1226,6 +1399,57 @@ int orc_jacobian(const orc_profile *prof, const orc_sounder *snd, const orc_opti
             }
         }
     }
+    /* continuum and its central-difference derivatives (vmr_jacobian.jl:288-349):
+     * T part when temperature is retrieved, VMR parts for retrieved H2O / CO2 with
+     * vmr_coupling; chained to the levels by the mid-layer weight wM */
+    double *CVlo = calloc(NL * (nr ? nr : 1), sizeof(double)), *CVhi = calloc(NL * (nr ? nr : 1), sizeof(double));
+    if (opt->continua) {
+        double *vh_mid = malloc(sizeof(double) * L), *vco_mid = malloc(sizeof(double) * L);
+        int sh = -1, sc = -1;
+        for (int s = 0; s < ns; ++s) {
+            if (prof->species[s] == 1) sh = s;
+            if (prof->species[s] == 2) sc = s;
+        }
+        if (sh >= 0) orc_interp_logp(P, prof->vmr + (size_t)sh * nlev, nlev, p_mid, L, vh_mid);
+        else for (int k = 0; k < L; ++k) vh_mid[k] = 0.0;
+        if (sc >= 0) orc_interp_logp(P, prof->vmr + (size_t)sc * nlev, nlev, p_mid, L, vco_mid);
+        else for (int k = 0; k < L; ++k) vco_mid[k] = 4.15e-4;
+        double *a = malloc(sizeof(double) * n), *b = malloc(sizeof(double) * n);
+        for (int k = 0; k < L; ++k) {
+            const double vh = vh_mid[k], vcx = vco_mid[k], Tm = T_mid[k], pm = p_mid[k], dp_ = dpk[k];
+            const double wM = log(pm / P[k]) / log(P[k + 1] / P[k]);
+            orc_continuum_layer(lo, d, n, opt->continua, vh, vcx, pm, Tm, dp_, a);
+            for (int64_t i = 0; i < n; ++i) tau[(size_t)k * n + i] += a[i];
+            if (spec->include_temperature) {
+                const double hT = 0.05;
+                orc_continuum_layer(lo, d, n, opt->continua, vh, vcx, pm, Tm + hT, dp_, a);
+                orc_continuum_layer(lo, d, n, opt->continua, vh, vcx, pm, Tm - hT, dp_, b);
+                for (int64_t i = 0; i < n; ++i) {
+                    double g = (a[i] - b[i]) / (2 * hT);
+                    DTlo[(size_t)k * n + i] += g * (1.0 - wM);
+                    DThi[(size_t)k * n + i] += g * wM;
+                }
+            }
+            for (int r = 0; r < nr && spec->vmr_coupling; ++r) {
+                const int spid = spec->retrieved[r];
+                if (spid != 1 && spid != 2) continue;
+                const double v = spid == 1 ? vh : vcx, h = fmax(fabs(v), 1e-30) * 1e-6;
+                if (spid == 1) {
+                    orc_continuum_layer(lo, d, n, opt->continua, vh + h, vcx, pm, Tm, dp_, a);
+                    orc_continuum_layer(lo, d, n, opt->continua, vh - h, vcx, pm, Tm, dp_, b);
+                } else {
+                    orc_continuum_layer(lo, d, n, opt->continua, vh, vcx + h, pm, Tm, dp_, a);
+                    orc_continuum_layer(lo, d, n, opt->continua, vh, vcx - h, pm, Tm, dp_, b);
+                }
+                for (int64_t i = 0; i < n; ++i) {
+                    double g = (a[i] - b[i]) / (2 * h);
+                    CVlo[((size_t)r * L + k) * n + i] = g * (1.0 - wM);
+                    CVhi[((size_t)r * L + k) * n + i] = g * wM;
+                }
+            }
+        }
+        free(a); free(b); free(vh_mid); free(vco_mid);
+    }
     double T_sfc = opt->T_sfc > 0.0 ? opt->T_sfc : Tl[0];
     double *Ihi = malloc(sizeof(double) * n), *dtau = malloc(sizeof(double) * NL);
     double *dTl = malloc(sizeof(double) * (NL + n)), *dTs = malloc(sizeof(double) * n), *de = malloc(sizeof(double) * n);
@@ -1286,6 +1510,9 @@ int orc_jacobian(const orc_profile *prof, const orc_sounder *snd, const orc_opti
                     if (j >= 1 && vc[(size_t)sidx * L + j - 1] != 0.0)
                         v += dtau[(size_t)(j - 1) * n + i] * Vhi[((size_t)r * L + j - 1) * n + i];
                 }
+                /* continuum term, not gated by the CG VMR (vmr_jacobian.jl:451-454) */
+                if (j < L) v += dtau[(size_t)j * n + i] * CVlo[((size_t)r * L + j) * n + i];
+                if (j >= 1) v += dtau[(size_t)(j - 1) * n + i] * CVhi[((size_t)r * L + j - 1) * n + i];
                 tmp[i] = v;
             }
             if (cs) {
@@ -1305,7 +1532,8 @@ int orc_jacobian(const orc_profile *prof, const orc_sounder *snd, const orc_opti
 #undef EMIT
     free(col); free(ch); free(kern); free(idx); free(nuc); free(Rc); free(dbt);
     free(Ihi); free(dtau); free(dTl); free(dTs); free(de);
-    free(tau); free(DTlo); free(DThi); free(Vlo); free(Vhi); free(sg); free(gT); free(gp); free(gv);
+    free(tau); free(DTlo); free(DThi); free(Vlo); free(Vhi); free(CVlo); free(CVhi);
+    free(sg); free(gT); free(gp); free(gv);
     free(p_mid); free(T_mid); free(dpk); free(T_ave); free(vc); free(pc); free(tc);
     return 0;
 }

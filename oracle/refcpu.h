@@ -142,7 +142,32 @@ typedef struct {
     double eps_sfc;
     double mu;
     int    ils_direct_full;    /* 1: convolve every internal point (reference loop) */
+    int    continua;           /* bit mask of ORC_CONT_* (0: apply_continuum=false) */
 } orc_options;
+
+/* ---- continuum (src/CrossSections/continuum.jl:138-363) ------------------- */
+#define ORC_CONT_H2O    1      /* :h2o      MT-CKD 4.3 self + foreign */
+#define ORC_CONT_CO2    2      /* :co2      MT-CKD CO2 line-coupling continuum */
+#define ORC_CONT_CO2CIA 4      /* :co2_cia  HITRAN CO2-CO2 CIA (scaled to air) */
+#define ORC_CONT_N2     8      /* :n2       HITRAN N2-N2 CIA */
+#define ORC_CONT_O2    16      /* :o2       HITRAN O2-O2 CIA */
+double orc_radfn(double nu, double T);
+/* absorption coefficients in cm^-1 at one wavenumber */
+double orc_h2o_continuum(double nu, double vmr_h2o, double p_hpa, double T);
+double orc_co2_continuum(double nu, double vmr_co2, double p_hpa, double T);
+/* CIA tables: pair 0 CO2-CO2, 1 N2-N2, 2 O2-O2.  Block b holds knots
+ * nu[off[b] .. off[b+1]) ascending with sigma >= 0 (cm^5 molec^-2) at T[b].
+ * n_blocks = 0 removes the table (the absent-data case: zero contribution). */
+int    orc_cia_set(int pair, int n_blocks, const double *T, const int64_t *off, const double *nu,
+                   const double *sigma);
+double orc_cia_sigma(int pair, double nu, double T);
+/* CIA absorption of one pair (cm^-1) for the pair's number densities */
+double orc_cia_k(int pair, double nu, double vmr, double p_hpa, double T);
+double orc_dp_to_dz_cm(double dp_hpa, double p_mid_hpa, double T);
+/* continuum optical depth of one layer on the grid (sounder.jl:310-322):
+ * tau_c[i] = dz(T_mid) * sum of the selected continua at (vmr_h2o, vmr_co2, p_mid, T_mid) */
+void   orc_continuum_layer(double nu0, double dnu, int64_t n_nu, int continua, double vmr_h2o,
+                           double vmr_co2, double p_mid, double T_mid, double dp, double *tau_c);
 
 typedef struct {
     double nu_lo, dnu;          /* internal grid */
@@ -209,7 +234,7 @@ typedef struct {
 } orc_state_spec;
 
 int orc_state_size(const orc_state_spec *spec, int n_levels);
-/* analytic_jacobian (vmr_jacobian.jl:152-505) without continuum; K is
+/* analytic_jacobian (vmr_jacobian.jl:152-505) with the continua of opt->continua; K is
  * column-major [n_state][n_ch]; y0 [n_ch]; x0 [n_state]. */
 int orc_jacobian(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
                  const orc_state_spec *spec, double *K, double *y0, double *x0);

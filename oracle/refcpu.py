@@ -40,7 +40,7 @@ class OrcOptions(C.Structure):
                 ("lm_cutoff", f64), ("dptmn", f64), ("x_far", f64), ("with_ils", i32),
                 ("apodization", i32), ("line_mixing", i32), ("T_method", i32),
                 ("source_function", i32), ("T_sfc", f64), ("eps_sfc", f64), ("mu", f64),
-                ("ils_direct_full", i32)]
+                ("ils_direct_full", i32), ("continua", i32)]
 
 
 class OrcInfo(C.Structure):
@@ -246,10 +246,70 @@ class Problem:
 
 def options(internal_dnu=0.001, high_res_factor=0, cutoff=25.0, lm_cutoff=None, dptmn=1e-6,
             x_far=122.0, with_ils=True, apodization=0, line_mixing=0, T_method=0,
-            source_function=1, T_sfc=0.0, eps_sfc=1.0, mu=1.0, ils_direct_full=False):
+            source_function=1, T_sfc=0.0, eps_sfc=1.0, mu=1.0, ils_direct_full=False, continua=0):
+    """continua: bit mask (1 h2o, 2 co2, 4 co2_cia, 8 n2, 16 o2); 0 = no continuum."""
     return OrcOptions(internal_dnu, high_res_factor, cutoff, cutoff if lm_cutoff is None else lm_cutoff,
                       dptmn, x_far, int(with_ils), apodization, line_mixing, T_method,
-                      source_function, T_sfc, eps_sfc, mu, int(ils_direct_full))
+                      source_function, T_sfc, eps_sfc, mu, int(ils_direct_full), int(continua))
+
+
+# ---- continuum (src/CrossSections/continuum.jl) ------------------------------------
+def _declare_cont():
+    L = lib()
+    if getattr(L, "_cont_declared", False):
+        return L
+    for nm in ("orc_h2o_continuum", "orc_co2_continuum"):
+        getattr(L, nm).argtypes = [f64, f64, f64, f64]
+        getattr(L, nm).restype = f64
+    L.orc_radfn.argtypes = [f64, f64]
+    L.orc_radfn.restype = f64
+    L.orc_cia_set.argtypes = [i32, i32, P, P, P, P]
+    L.orc_cia_set.restype = i32
+    L.orc_cia_sigma.argtypes = [i32, f64, f64]
+    L.orc_cia_sigma.restype = f64
+    L.orc_cia_k.argtypes = [i32, f64, f64, f64, f64]
+    L.orc_cia_k.restype = f64
+    L.orc_dp_to_dz_cm.argtypes = [f64, f64, f64]
+    L.orc_dp_to_dz_cm.restype = f64
+    L.orc_continuum_layer.argtypes = [f64, f64, i64, i32, f64, f64, f64, f64, f64, P]
+    L.orc_continuum_layer.restype = None
+    L._cont_declared = True
+    return L
+
+
+def h2o_continuum(nu, vmr, p_hpa, T):
+    L = _declare_cont()
+    return np.array([L.orc_h2o_continuum(float(x), vmr, p_hpa, T) for x in np.atleast_1d(nu)])
+
+
+def co2_continuum(nu, vmr, p_hpa, T):
+    L = _declare_cont()
+    return np.array([L.orc_co2_continuum(float(x), vmr, p_hpa, T) for x in np.atleast_1d(nu)])
+
+
+def cia_k(pair, nu, vmr, p_hpa, T):
+    L = _declare_cont()
+    return np.array([L.orc_cia_k(pair, float(x), vmr, p_hpa, T) for x in np.atleast_1d(nu)])
+
+
+def cia_set(pair, blocks):
+    """blocks: list of (T, nu[], sigma[]); [] removes the table."""
+    L = _declare_cont()
+    T = np.array([float(b[0]) for b in blocks] or [0.0])
+    off = np.zeros(len(blocks) + 1, np.int64)
+    for i, b in enumerate(blocks):
+        off[i + 1] = off[i] + len(b[1])
+    nu = np.concatenate([np.asarray(b[1], np.float64) for b in blocks]) if blocks else np.zeros(1)
+    sg = np.concatenate([np.asarray(b[2], np.float64) for b in blocks]) if blocks else np.zeros(1)
+    if L.orc_cia_set(pair, len(blocks), _ptr(T), _ptr(off), _ptr(nu), _ptr(sg)) != 0:
+        raise ValueError("orc_cia_set failed")
+
+
+def continuum_layer(nu0, dnu, n, continua, vmr_h2o, vmr_co2, p_mid, T_mid, dp):
+    L = _declare_cont()
+    out = np.zeros(n)
+    L.orc_continuum_layer(nu0, dnu, n, continua, vmr_h2o, vmr_co2, p_mid, T_mid, dp, _ptr(out))
+    return out
 
 
 def forward(prof, linelists, sounder, opts=None, layer_mask=None, want_tau=False, **kw):

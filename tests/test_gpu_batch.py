@@ -37,7 +37,7 @@ def _run(ctx, scene, idx=None, **kw):
     p = [x[0] for x in sel]
     geoms = [ViewingGeometry(float(np.degrees(np.arccos(x[2]))), 0.0, x[2]) for x in sel]
     return forward_model(p, sub, sounder=snd, geom=geoms, T_sfc=[x[1] for x in sel],
-                         line_mixing="vpy", ctx=ctx, **kw)
+                         line_mixing="vpy", ctx=ctx, **kw, apply_continuum=False)
 
 
 @pytest.fixture()

@@ -27,7 +27,7 @@ def test_channel_od_matches_oracle(ctx, lo, mu, windows):
     floor = 1e-4
     geom = ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu)
     got, g = channel_optical_depths(prof, sub, sounder=snd, geom=geom, floor=floor, line_mixing="vpy",
-                                    cim=True, ctx=ctx)
+                                    cim=True, ctx=ctx, apply_continuum=False)
     _, _, info, tau = o.forward(prof, sub, snd, want_tau=True, line_mixing=1)
     _, kern = o.ils_kernel(info.dnu, snd.opd_max, snd.fwhm_gauss)
     ch = lo + 0.25 * np.arange(got.shape[1])
@@ -40,7 +40,7 @@ def test_channel_od_matches_oracle(ctx, lo, mu, windows):
     gerr = np.abs(g - gref)
     assert np.all((gerr <= 2e-5 + 1e-4 * np.abs(gref)) | near), float(np.max(np.where(near, 0, gerr)))
     # level transmittances and raw CIM sums (the emulator's target)
-    Tl, G = channel_transmittance(prof, sub, sounder=snd, geom=geom, line_mixing="vpy", ctx=ctx)
+    Tl, G = channel_transmittance(prof, sub, sounder=snd, geom=geom, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     assert np.max(np.abs(Tl - Tch)) < 2e-6 + 1e-4 * np.abs(Tch).max()
     Gr = channel_od_ref.last_G
     assert np.max(np.abs(G - Gr)) < 1e-6 + 1e-4 * np.abs(Gr).max()
@@ -59,13 +59,13 @@ def test_channel_od_transparent_and_flat(ctx):
     snd = Sounder.make(700.0, 703.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
     prof = syn.base_profile(False)
     empty = {GasSpecies.CO2: filter_linelist(syn.linelist_config0(), 10.0, 11.0)}
-    got = channel_optical_depths(prof, empty, sounder=snd, ctx=ctx)
+    got = channel_optical_depths(prof, empty, sounder=snd, ctx=ctx, apply_continuum=False)
     assert np.all(got == 0.0)
     ll = {GasSpecies.CO2: filter_linelist(syn.linelist_config0(), 660.0, 745.0)}
     pp = [x[0] for x in syn.perturbed_profiles(3, seed=7, with_o3=False)]
-    b = channel_optical_depths(pp, ll, sounder=snd, ctx=ctx)
+    b = channel_optical_depths(pp, ll, sounder=snd, ctx=ctx, apply_continuum=False)
     for q in range(3):
-        assert np.array_equal(b[q], channel_optical_depths(pp[q], ll, sounder=snd, ctx=ctx))
+        assert np.array_equal(b[q], channel_optical_depths(pp[q], ll, sounder=snd, ctx=ctx, apply_continuum=False))
 
 
 # ---- BF16 MFMA MLP vs the FP64 reference (oracle/mlp_ref.py) -----------------
@@ -166,7 +166,7 @@ def test_emulator_trains_and_beats_mean_predictor(ctx, small_emu):
     profs = [p[0] for p in hold]
     geoms = [ViewingGeometry(float(np.degrees(np.arccos(p[2]))), 0.0, p[2]) for p in hold]
     tsf = [p[1] for p in hold]
-    _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     _, bt_emu = emu.forward(profs, geom=geoms, T_sfc=tsf)
     err_trained = np.abs(bt_emu - bt_lbl).max()
     saved = emu.mlp.get_params()
@@ -185,7 +185,7 @@ def test_emulator_spot_check_and_fallback(ctx, small_emu):
     geoms = [ViewingGeometry(float(np.degrees(np.arccos(p[2]))), 0.0, p[2]) for p in hold]
     tsf = [p[1] for p in hold]
     spot = SpotChecker(sub, snd, blocks=((1206.0, 8), (1214.0, 8)))
-    _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx)
+    _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     # spot channels computed on sub-grids equal the full LBL at those channels
     bt_spot = spot(profs, geoms, tsf, [1.0] * len(profs), ctx, line_mixing="vpy")
     assert np.max(np.abs(bt_spot - bt_lbl[:, spot.channels])) < 1e-4
@@ -249,7 +249,7 @@ def test_emulator_jacobian_matches_finite_differences(ctx, small_emu):
     from irsounderlbl.api import analytic_jacobian
     from irsounderlbl.types import StateVectorSpec
     ref = analytic_jacobian(prof, sub, StateVectorSpec(50, [GasSpecies.H2O], include_emissivity=False),
-                            sounder=snd, geom=geom, T_sfc=T_sfc, line_mixing="vpy", vmr_coupling=False, ctx=ctx)
+                            sounder=snd, geom=geom, T_sfc=T_sfc, line_mixing="vpy", vmr_coupling=False, ctx=ctx, apply_continuum=False)
     KT_ref = ref.K[:, :50]
     rel = np.linalg.norm(jac.K_T - KT_ref) / np.linalg.norm(KT_ref)
     relq = np.linalg.norm(jac.K_lnq_H2O - ref.K[:, 50:100]) / np.linalg.norm(ref.K[:, 50:100])

@@ -44,7 +44,7 @@ def test_jacobian_small_problem(ctx, sf, src, log_vmr):
     T_sfc, eps, mu = 292.0, 0.97, 0.9
     spec = StateVectorSpec(prof.n_levels, [GasSpecies.H2O, GasSpecies.CO2], log_vmr=log_vmr)
     jac = analytic_jacobian(prof, lls, spec, sounder=snd, geom=_geom(mu), T_sfc=T_sfc, eps_sfc=eps,
-                            source_function=src, ctx=ctx, **KW)
+                            source_function=src, ctx=ctx, **KW, apply_continuum=False)
     Kr, y0r, x0r = o.jacobian(prof, lls, snd, retrieved=[GasSpecies.H2O, GasSpecies.CO2],
                               T_sfc=T_sfc, eps_sfc=eps, mu=mu, source_function=sf, log_vmr=log_vmr, **KW)
     assert jac.K.shape == Kr.shape == (33, spec.n)
@@ -68,7 +68,7 @@ def test_jacobian_radiance_and_reduced_blocks(ctx):
     spec = StateVectorSpec(n, [ColumnScale(GasSpecies.CO2), PartialColumns(GasSpecies.H2O, B)],
                            include_temperature=False, include_tsfc=False, include_emissivity=False,
                            log_vmr=False)
-    jac = analytic_jacobian(prof, lls, spec, sounder=snd, observable="radiance", ctx=ctx, **KW)
+    jac = analytic_jacobian(prof, lls, spec, sounder=snd, observable="radiance", ctx=ctx, **KW, apply_continuum=False)
     Kr, y0r, _ = o.jacobian(prof, lls, snd, retrieved=[GasSpecies.CO2, GasSpecies.H2O],
                             include_temperature=False, include_tsfc=False, include_emissivity=False,
                             log_vmr=True, observable="radiance", **KW)
@@ -88,7 +88,7 @@ def test_jacobian_no_coupling_and_mass_weighted(ctx):
     for tm, tmi in (("mass_weighted", 1), ("logp_at_pcg", 0)):
         for cpl in (False, True):
             jac = analytic_jacobian(prof, lls, spec, sounder=snd, T_method=tm, vmr_coupling=cpl,
-                                    ctx=ctx, **KW)
+                                    ctx=ctx, **KW, apply_continuum=False)
             Kr, _, _ = o.jacobian(prof, lls, snd, retrieved=[GasSpecies.H2O], T_method=tmi,
                                   vmr_coupling=cpl, **KW)
             err = _cmp(jac.K, Kr, 1e-3 * np.abs(Kr).max())
@@ -105,7 +105,7 @@ def test_jacobian_line_mixing_window(ctx):
     snd = Sounder.make(700.0, 703.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
     spec = StateVectorSpec(prof.n_levels, [GasSpecies.CO2, GasSpecies.H2O])
     jac = analytic_jacobian(prof, sub, spec, sounder=snd, line_mixing="vpy", internal_dnu=0.002,
-                            ctx=ctx)
+                            ctx=ctx, apply_continuum=False)
     Kr, y0r, _ = o.jacobian(prof, sub, snd, retrieved=[GasSpecies.CO2, GasSpecies.H2O],
                             line_mixing=1, internal_dnu=0.002)
     assert np.max(np.abs(jac.y0 - y0r)) < 1e-3
@@ -115,7 +115,7 @@ def test_jacobian_line_mixing_window(ctx):
         assert err < COL_TOL, (blk, err)
     # y0 is the forward model without DPTMIN
     _, _, bt = forward_model(prof, sub, sounder=snd, line_mixing="vpy", dptmn=0.0,
-                             internal_dnu=0.002, ctx=ctx)
+                             internal_dnu=0.002, ctx=ctx, apply_continuum=False)
     assert np.max(np.abs(bt - jac.y0)) < 1e-6
 
 
@@ -136,7 +136,7 @@ def test_jacobian_coarse_far_wing_matches_every_pair(ctx):
 
     def run():
         return analytic_jacobian(prof, sub, spec, sounder=snd, geom=_geom(mu), T_sfc=T_sfc,
-                                 line_mixing="vpy", internal_dnu=0.001, ctx=ctx)
+                                 line_mixing="vpy", internal_dnu=0.001, ctx=ctx, apply_continuum=False)
     try:
         ctx.set_schedule(4, far_wing_cm=0.0)
         exact = run()
@@ -156,9 +156,9 @@ def test_jacobian_errors(ctx):
     prof, lls, snd = small_problem()
     spec = StateVectorSpec(prof.n_levels, [GasSpecies.O3])
     with pytest.raises(ValueError):
-        analytic_jacobian(prof, lls, spec, sounder=snd, ctx=ctx, **KW)
+        analytic_jacobian(prof, lls, spec, sounder=snd, ctx=ctx, **KW, apply_continuum=False)
     with pytest.raises(ValueError):
         analytic_jacobian(prof, lls, StateVectorSpec(prof.n_levels), sounder=snd, observable="tb",
-                          ctx=ctx, **KW)
+                          ctx=ctx, **KW, apply_continuum=False)
     with pytest.raises(ValueError):
-        analytic_jacobian(prof, lls, StateVectorSpec(prof.n_levels + 1), sounder=snd, ctx=ctx, **KW)
+        analytic_jacobian(prof, lls, StateVectorSpec(prof.n_levels + 1), sounder=snd, ctx=ctx, **KW, apply_continuum=False)

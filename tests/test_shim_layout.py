@@ -1,0 +1,119 @@
+"""The Julia ccall shim and the Python ctypes mirror lay out every struct of
+include/irs.h field for field (names, order and C types), and the shim keeps
+the reference's keyword signatures.  Julia is not installed here, so the
+shim is checked as text; the ctypes side is checked on the loaded classes.
+"""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "irs.h")
+SHIM = os.path.join(ROOT, "irsounderlbl.jl_amd", "julia", "IRSounderLBLAMD.jl")
+
+# C struct -> (Julia struct, ctypes class)
+PAIRS = {"irs_linelist": ("CLinelist", "IrsLinelist"), "irs_sounder": ("CSounder", "IrsSounder"),
+         "irs_profile": ("CProfile", "IrsProfile"), "irs_fm_options": ("COptions", "IrsOptions"),
+         "irs_state_spec": ("CStateSpec", "IrsStateSpec")}
+
+C_KIND = {"int64_t": "i64", "int": "i32", "int32_t": "i32", "double": "f64", "float": "f32",
+          "int8_t": "i8"}
+JL_KIND = {"Int64": "i64", "Cint": "i32", "Int32": "i32", "Float64": "f64", "Float32": "f32", "Int8": "i8"}
+
+
+def c_structs():
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for body, name in re.findall(r"typedef struct\s*\{(.*?)\}\s*(\w+)\s*;", text, flags=re.S):
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            decl = decl.replace("const ", "")
+            m = re.match(r"(\w+)\s*(\**)\s*(.*)", decl)
+            base, ptr, names = m.group(1), m.group(2), m.group(3)
+            for nm in names.split(","):
+                nm = nm.strip()
+                star = ptr + ("*" if nm.startswith("*") else "")
+                nm = nm.lstrip("*").strip()
+                fields.append((nm, "ptr" if star else C_KIND[base]))
+        out[name] = fields
+    return out
+
+
+def jl_structs():
+    text = open(SHIM).read()
+    out = {}
+    for kind, name, body in re.findall(r"(mutable struct|struct)\s+(\w+)\s*\n(.*?)\nend", text, flags=re.S):
+        if name not in {v[0] for v in PAIRS.values()}:
+            continue
+        fields = []
+        for part in re.split(r"[;\n]", body):
+            part = part.strip()
+            if "::" not in part:
+                continue
+            nm, ty = [x.split("#")[0].strip() for x in part.split("::", 1)]
+            fields.append((nm, "ptr" if ty.startswith("Ptr{") else JL_KIND[ty]))
+        out[name] = fields
+    return out
+
+
+def test_header_structs_parse():
+    cs = c_structs()
+    for c in PAIRS:
+        assert c in cs and len(cs[c]) >= 6, c
+    assert ("continua", "i32") in cs["irs_fm_options"]
+
+
+@pytest.mark.parametrize("cname", sorted(PAIRS))
+def test_julia_shim_struct_matches_header(cname):
+    cs, js = c_structs(), jl_structs()
+    jname = PAIRS[cname][0]
+    assert js[jname] == cs[cname], (cname, js[jname], cs[cname])
+
+
+@pytest.mark.parametrize("cname", sorted(PAIRS))
+def test_ctypes_struct_matches_header(cname):
+    from irsounderlbl import api
+    cls = getattr(api, PAIRS[cname][1])
+    kinds = {C.c_int64: "i64", C.c_int32: "i32", C.c_double: "f64", C.c_float: "f32", C.c_int8: "i8",
+             C.c_void_p: "ptr"}
+    got = [(nm, kinds[ty]) for nm, ty in cls._fields_]
+    assert got == c_structs()[cname]
+
+
+def test_shim_keyword_precedence_follows_reference():
+    """forward_model: sounder::Sounder = IASIInstrument(), iasi = nothing, and
+    iasi wins when given (src/Sensor/sounder.jl:235-236,254); analytic_jacobian:
+    sounder = nothing, iasi = IASIInstrument(), and sounder wins when given
+    (src/Estimation/vmr_jacobian.jl:155-156,177)."""
+    text = open(SHIM).read()
+    fm = text.split("function forward_model(")[1].split("\nend")[0]
+    assert re.search(r"sounder::Sounder\s*=\s*IASIInstrument\(\)", fm)
+    assert re.search(r"iasi::Union\{Sounder, Nothing\}\s*=\s*nothing", fm)
+    assert re.search(r"snd\s*=\s*iasi\s*===\s*nothing\s*\?\s*sounder\s*:\s*iasi", fm)
+    assert "apply_continuum::Bool = true" in fm
+    aj = text.split("function analytic_jacobian(")[1].split("\nend")[0]
+    assert re.search(r"sounder::Union\{Sounder, Nothing\}\s*=\s*nothing", aj)
+    assert re.search(r"iasi::Sounder\s*=\s*IASIInstrument\(\)", aj)
+    assert re.search(r"snd\s*=\s*sounder\s*===\s*nothing\s*\?\s*iasi\s*:\s*sounder", aj)
+    # every libirs symbol the shim calls is declared in the header
+    declared = set(re.findall(r"\b(irs_[a-z_0-9]+)\s*\(", open(HEADER).read()))
+    for sym in re.findall(r"\(:(irs_[a-z_0-9]+),\s*libirs\)", text):
+        assert sym in declared, sym
+
+
+def test_python_keyword_precedence():
+    """The Python mirror resolves the instrument the same way (host logic)."""
+    import inspect
+    from irsounderlbl import api
+    src = inspect.getsource(api.forward_model)
+    assert "inst = iasi if iasi is not None else (sounder if sounder is not None else IASIInstrument())" in src
+    src = inspect.getsource(api.analytic_jacobian)
+    assert "inst = iasi if sounder is None else sounder" in src
+    sig = inspect.signature(api.forward_model)
+    assert sig.parameters["apply_continuum"].default is True
+    assert sig.parameters["continua"].default == ("h2o", "co2", "co2_cia", "n2", "o2")
