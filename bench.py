@@ -24,9 +24,19 @@ with the BT error against the GPU LBL (itself within 1e-9 K of the FP64 oracle).
 "lbl_jacobian" times one full-IASI analytic Jacobian (102 columns) of a
 perturbed profile on the windows variant of the line lists.
 
+"fp64_every_pair" times the same spectrum through the reference kernel's
+arithmetic on the device (every pair, FP64 line parameters and sums, FP64
+optical depths and RTE) and reports the BT difference of the FP32 engine to it.
+
+"roofline" is the line-sum kernel: FP32 FLOP/s from the committed rocprofv3
+counters of that kernel (SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes per launch) over
+the launch time measured here; the reference's every-pair FLOP count over the
+same time is reported beside it.  "hbm" gives the RTE and channel-ILS kernels'
+bytes per launch over their measured times against 8 TB/s.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-accuracy]
                     [--no-batch] [--batch N] [--no-exact] [--no-emulator] [--no-jacobian]
-                    [--emu-train N] [--emu-eval N]
+                    [--no-fp64] [--emu-train N] [--emu-eval N]
 """
 import argparse
 import json
@@ -39,64 +49,182 @@ sys.path.insert(0, os.path.join(ROOT, "irsounderlbl.jl_amd"))
 
 import numpy as np  # noqa: E402
 
-PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 peak
+PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 peak (packed FP32)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6
+PEAK_HBM_GBPS = 8000.0
 FLOP_PER_VOIGT_PAIR = 14       # nominal, far-wing expression of the reference kernel
 FLOP_PER_DISP_PAIR = 9
-VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions/s (4 SIMD16 per CU, 4 passes each)
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(prof, lls, snd, layer):
-    """The FP64 oracle on the host: one layer's line sum (all species) plus the
-    full RTE and channel ILS, scaled to one spectrum."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prof, lls, snd):
+    """The FP64 oracle (the reference algorithm restated in C, OpenMP over grid
+    points inside each cross section, sequential layers) on this host:
+    * config 0 (20k lines, 645-800 cm^-1): one full spectrum, with the
+      reference's single-threaded full-grid direct ILS (measured);
+    * config 1 (the headline workload): the line sums of the bottom, middle and
+      top layers (all three species, VP_Y), x 49/3; the full RTE and channel ILS
+      measured; the reference's single-threaded full-grid direct ILS timed on
+      2000 points and scaled to the grid ("reference structure"); with the
+      channel-only ILS ("optimized");
+    * config 2: the middle layer of two perturbed profiles (per-profile rate)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu
+    from irsounderlbl import synthetic as syn
+    from irsounderlbl.types import GasSpecies, Sounder
     L = prof.n_levels - 1
-    mask = np.zeros(L, np.uint8)
-    mask[layer] = 1
+    threads = refcpu.lib().orc_num_threads()
+    out = {"cores": threads, "host_threads": os.cpu_count(), "cpu_model": _cpu_model(), "kind": "port"}
+    # config 0, full spectrum
+    p0, ll0 = syn.base_profile(False), syn.linelist_config0()
+    s0 = Sounder.make(645.0, 800.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
     t0 = time.perf_counter()
-    _, _, info = refcpu.forward(prof, lls, snd, layer_mask=mask, line_mixing=1)
-    wall = time.perf_counter() - t0
-    per_spec = info.seconds_xsec * L + info.seconds_rte + info.seconds_ils
-    return {"value": 1.0 / per_spec, "unit": "spectra/s", "cores": refcpu.lib().orc_num_threads(),
-            "kind": "port",
-            "sample": f"FP64 oracle, layer {layer} of {L} line sums (all 3 species, VP_Y) + full "
-                      f"RTE + channel ILS; {wall:.1f} s wall; per-spectrum time = layer time x {L} + "
-                      f"RTE + ILS"}
+    _, bt0, info0 = refcpu.forward(p0, {GasSpecies.CO2: ll0}, s0, ils_direct_full=True)
+    ORACLE_CACHE["config0_bt"] = bt0
+    out["config0_full_spectrum_s"] = round(time.perf_counter() - t0, 2)
+    out["config0_seconds"] = {"linesum": round(info0.seconds_xsec, 2), "rte": round(info0.seconds_rte, 3),
+                              "ils_full_grid_1thread": round(info0.seconds_ils, 2)}
+    # config 1: three sampled layers
+    layers = (0, L // 2, L - 1)
+    t_ls = []
+    for k in layers:
+        mask = np.zeros(L, np.uint8)
+        mask[k] = 1
+        _, _, info = refcpu.forward(prof, lls, snd, layer_mask=mask, line_mixing=1)
+        t_ls.append(info.seconds_xsec)
+    ls = float(np.mean(t_ls)) * L
+    rte, ils_ch = info.seconds_rte, info.seconds_ils
+    # the reference's full-grid direct ILS, single-threaded, timed on 2000 points
+    n_hi = info.n_hi
+    idx = np.linspace(0, n_hi - 1, 2000).astype(np.int64)
+    _, kern = refcpu.ils_kernel(info.dnu, snd.opd_max, snd.fwhm_gauss, 0)
+    spec = np.ones(n_hi)
+    refcpu.lib().orc_set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        refcpu.apply_ils_at(spec, info.dnu, kern, idx)
+        t_ils_pt = (time.perf_counter() - t0) / len(idx)
+    finally:
+        refcpu.lib().orc_set_num_threads(threads)
+    ils_full = t_ils_pt * n_hi
+    ref_struct = ls + rte + ils_full
+    optimized = ls + rte + ils_ch
+    out.update({
+        "value": 1.0 / ref_struct, "unit": "spectra/s",
+        "sample": f"config 1: FP64 oracle line sums of layers {layers} (3 species, VP_Y) x {L}/3 = {ls:.1f} s, "
+                  f"RTE {rte:.2f} s, the reference's single-threaded full-grid ILS {ils_full:.1f} s "
+                  f"(2000 of {n_hi} points timed); value = 1 / their sum",
+        "config1_seconds": {"linesum": round(ls, 2), "linesum_per_layer": [round(x, 3) for x in t_ls],
+                            "rte": round(rte, 3), "ils_full_grid_1thread": round(ils_full, 2),
+                            "ils_channels_only": round(ils_ch, 3)},
+        "config1_reference_structure_spectra_per_s": 1.0 / ref_struct,
+        "config1_optimized_spectra_per_s": 1.0 / optimized,
+        "config1_linesum_only_spectra_per_s": 1.0 / ls})
+    # config 2: two perturbed profiles, middle layer, per-profile rate
+    pp = syn.perturbed_profiles(2, seed=2002)
+    t2 = []
+    for pr, _, mu in pp:
+        mask = np.zeros(L, np.uint8)
+        mask[L // 2] = 1
+        _, _, info = refcpu.forward(pr, lls, snd, layer_mask=mask, line_mixing=1, mu=mu)
+        t2.append(info.seconds_xsec * L + info.seconds_rte + ils_full)
+    out["config2_per_profile_spectra_per_s"] = 2.0 / float(np.sum(t2))
+    out["config2_sample"] = "2 perturbed profiles: middle-layer line sums x 49 + RTE + full-grid single-thread ILS"
+    return out
+
+
+# GPU results that the FP64 oracle checks in the CPU leg: (section, key, gpu
+# value, callable that computes the oracle's value, kind).  Only the CPU leg
+# runs the oracle, after every GPU figure is taken.
+CHECKS = []
+ORACLE_CACHE = {}
+
+
+def check_bt(section, key, bt, oracle_bt):
+    CHECKS.append((section, key, np.asarray(bt), oracle_bt, "bt"))
 
 
 def accuracy_sample(ctx, prof, lls, lls_w):
-    """Errors vs the FP64 oracle on bounded samples: BT on a 10 cm^-1 channel
-    window (1205-1215 cm^-1) of the windows variant of the line lists (the
-    benchmark lists saturate every channel at 217 K, where BT would not test
-    the optical depths; here BT is 250-270 K), and the CO2 cross section with
-    line mixing of layer 24 on a 10 cm^-1 grid window in the 15 um band."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import refcpu
-    from irsounderlbl.api import compute_voigt_lm_cross_sections, forward_model
+    """GPU side of the accuracy samples: BT on a 10 cm^-1 channel window
+    (1205-1215 cm^-1) of the windows variant of the line lists (the benchmark
+    lists saturate every channel at 217 K, where BT would not test the optical
+    depths; here BT is 250-270 K), with and without continuum, and the CO2
+    cross section with line mixing of layer 24 on a 10 cm^-1 grid window in the
+    15 um band.  The oracle values are computed in the CPU leg."""
+    from irsounderlbl.api import compute_voigt_lm_cross_sections, forward_model, layer_properties
     from irsounderlbl.types import GasSpecies, Sounder, filter_linelist, wavenumber_grid
-    out = {}
     lo, hi = 1205.0, 1215.0
     snd = Sounder.make(lo, hi, 0.25, opd_max=2.0, fwhm_gauss=0.5)
     sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls_w.items()}
     _, _, bt = forward_model(prof, sub, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
-    _, bto, _ = refcpu.forward(prof, sub, snd, line_mixing=1)
-    d = np.abs(bt - bto)
-    out["bt_1205-1215_windows_variant"] = {"max_abs_K": float(d.max()), "rms_K": float(np.sqrt(np.mean(d * d))),
-                                           "bt_range_K": [float(bto.min()), float(bto.max())]}
-    lp = refcpu.layer_properties(prof)
+    check_bt("accuracy_vs_fp64", "bt_1205-1215_windows_variant", bt,
+             lambda o: o.forward(prof, sub, snd, line_mixing=1)[1])
+    _, _, btc = forward_model(prof, sub, sounder=snd, line_mixing="vpy", ctx=ctx)
+    check_bt("accuracy_vs_fp64", "bt_1205-1215_windows_variant_with_continuum", btc,
+             lambda o: o.forward(prof, sub, snd, line_mixing=1, continua=31)[1])
+    lp = layer_properties(prof)
     T = lp["T_cg"][GasSpecies.CO2][24]
     p = lp["p_cg"][GasSpecies.CO2][24] / 1013.25
     g = wavenumber_grid(700.0, 710.0, 0.001)
     co2 = filter_linelist(lls[GasSpecies.CO2], 674.0, 736.0)
     got = compute_voigt_lm_cross_sections(g, co2, T, p, ctx=ctx)
-    ref = refcpu.voigt_lm_xsec(co2, g.nu_min, g.dnu, g.n, T, p)
-    m = ref > 1e-6 * ref.max()
-    out["sigma_co2_layer24_700-710"] = {"max_rel": float(np.max(np.abs(got[m] - ref[m]) / ref[m]))}
-    return out
+    CHECKS.append(("accuracy_vs_fp64", "sigma_co2_layer24_700-710", got,
+                   lambda o: o.voigt_lm_xsec(co2, g.nu_min, g.dnu, g.n, T, p), "sigma"))
+    return {}
+
+
+def config0_figure(ctx):
+    """Config 0 on the GPU (20k lines, 645-800 cm^-1, 621 channels): spectra/s,
+    checked against the oracle's full config-0 spectrum of the CPU leg."""
+    from irsounderlbl import synthetic as syn
+    from irsounderlbl.api import forward_model
+    from irsounderlbl.types import GasSpecies, Sounder
+    p0, ll0 = syn.base_profile(False), {GasSpecies.CO2: syn.linelist_config0()}
+    s0 = Sounder.make(645.0, 800.0, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    forward_model(p0, ll0, sounder=s0, ctx=ctx, apply_continuum=False)
+    t0 = time.perf_counter()
+    n = 20
+    for _ in range(n):
+        _, _, bt = forward_model(p0, ll0, sounder=s0, ctx=ctx, apply_continuum=False)
+    dt = (time.perf_counter() - t0) / n
+    check_bt("config0", "bt_err_vs_fp64_oracle_K", bt,
+             lambda o: ORACLE_CACHE["config0_bt"] if "config0_bt" in ORACLE_CACHE else o.forward(p0, ll0, s0)[1])
+    return {"workload": "config 0: one 50-level profile, 20k CO2-like lines, 645-800 cm^-1, 621 channels",
+            "spectra_per_s": round(1.0 / dt, 2), "ms_per_spectrum": round(dt * 1e3, 3)}
+
+
+def run_checks(out):
+    """The CPU leg's oracle checks of the GPU results registered in CHECKS."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcpu
+    for section, key, got, oracle, kind in CHECKS:
+        ref = oracle(refcpu)
+        if kind == "sigma":
+            m = ref > 1e-6 * ref.max()
+            val = {"max_rel": float(np.max(np.abs(got[m] - ref[m]) / ref[m]))}
+        else:
+            d = np.abs(np.asarray(got) - np.asarray(ref))
+            val = {"max_abs_K": float(d.max()), "rms_K": float(np.sqrt(np.mean(d * d))),
+                   "bt_range_K": [float(np.min(ref)), float(np.max(ref))]}
+        sec = out.setdefault(section, {})
+        if isinstance(sec.get(key), dict):
+            sec[key].update(val)
+        else:
+            sec[key] = val
 
 
 def batched(ctx, lls, snd, n_prof):
@@ -119,6 +247,23 @@ def batched(ctx, lls, snd, n_prof):
     st = ctx.stats()
     ctx.set_schedule(4)
     flops = FLOP_PER_VOIGT_PAIR * st["active_pairs"] + FLOP_PER_DISP_PAIR * st["dispersive_pairs"]
+    # the FP64 oracle (CPU leg) on two of the profiles, on two channel windows
+    # (each with the lines that reach its internal grid)
+    from irsounderlbl.types import Sounder, filter_linelist, sounder_grid
+    ch_all = sounder_grid(snd).nu
+    got, calls = [], []
+    for q in (0, n_prof // 2):
+        for lo, hi in ((1205.0, 1208.0), (2410.0, 2413.0)):
+            sw = Sounder.make(lo, hi, snd.dnu, opd_max=snd.opd_max, fwhm_gauss=snd.fwhm_gauss)
+            sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls.items()}
+            c0 = int(round((lo - snd.nu_min) / snd.dnu))
+            assert abs(ch_all[c0] - lo) < 1e-9
+            n_w = int(round((hi - lo) / snd.dnu)) + 1
+            got.append(BT[q, c0:c0 + n_w])
+            calls.append((profs[q], sub, sw, pp[q][2], tsf[q]))
+    check_bt("batched", "bt_err_vs_fp64_oracle_K", np.concatenate(got),
+             lambda o: np.concatenate([o.forward(pr, sub, sw, line_mixing=1, mu=mu, T_sfc=ts)[1]
+                                       for pr, sub, sw, mu, ts in calls]))
     return {"workload": f"{n_prof} perturbed profiles (one scan line), config-1 line lists + VP_Y, "
                         "DPTMIN live, one forward_model call",
             "value": round(n_prof / dt, 4), "unit": "spectra/s", "seconds": round(dt, 3),
@@ -126,7 +271,9 @@ def batched(ctx, lls, snd, n_prof):
             "groups": st["linesum_launches"], "schedule": {"tile_points": st["tile_points"],
                                                          "persistent": st["persistent"]},
             "nominal_tflops": round(flops / (st["ms_device"] * 1e-3) / 1e12, 2),
-            "bt_finite": bool(np.all(np.isfinite(BT)))}
+            "bt_finite": bool(np.all(np.isfinite(BT))),
+            "bt_err_vs_fp64_oracle_K": {"profiles": [0, n_prof // 2],
+                                        "channels": "1205-1208 and 2410-2413 cm^-1 (13 + 13 channels per profile)"}}
 
 
 def emulator_figure(ctx, lls, snd, n_train, n_eval):
@@ -239,15 +386,106 @@ def exact_sum(ctx, prof, lls, snd):
             "max_abs_dBT_vs_default_K": float(np.max(np.abs(bt1 - bt0)))}
 
 
-def pmc_from_profiles():
-    """Per-launch counter figures of the line sum from the committed rocprofv3
-    summary (HBM bytes from FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel
-    calibration; VALU wave instructions from SQ_INSTS_VALU)."""
-    p = os.path.join(ROOT, "profiles", "linesum_full_iasi_pmc.json")
+def fp64_figure(ctx, prof, lls, snd, bt32, value32):
+    """One full-range spectrum through the every-pair FP64 path (the reference
+    kernel's arithmetic on the device); second call timed."""
+    from irsounderlbl.api import forward_model
+    sched = ctx.schedule()
+    ctx.set_schedule(sched["tile_points"], fp64_every_pair=True)
+    try:
+        log("bench: FP64 every-pair spectrum")
+        forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
+        t0 = time.perf_counter()
+        _, _, bt64 = forward_model(prof, lls, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+    finally:
+        ctx.set_schedule(sched["tile_points"], far_wing_cm=sched["far_wing_cm"])
+    d = np.abs(bt32 - bt64)
+    # the same comparison where the spectrum is partly transparent (the headline
+    # lists saturate every channel at 217 K): the windows variant of the lists
+    from irsounderlbl import synthetic as syn
+    lls_w = syn.linelists_config1(windows=True)
+    ctx.set_schedule(sched["tile_points"], fp64_every_pair=True)
+    try:
+        _, _, bt64w = forward_model(prof, lls_w, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
+    finally:
+        ctx.set_schedule(sched["tile_points"], far_wing_cm=sched["far_wing_cm"])
+    _, _, bt32w = forward_model(prof, lls_w, sounder=snd, line_mixing="vpy", ctx=ctx, apply_continuum=False)
+    dw = np.abs(bt32w - bt64w)
+    return {"workload": "the headline spectrum, every pair in FP64 (FP64 line parameters, Faddeeva, sums, optical "
+                        "depths and RTE)",
+            "windows_variant": {"fp32_engine_max_abs_dBT_K": float(dw.max()),
+                                "fp32_engine_rms_dBT_K": float(np.sqrt(np.mean(dw * dw))),
+                                "bt_range_K": [float(bt64w.min()), float(bt64w.max())]},
+            "spectra_per_s": round(1.0 / dt, 4), "linesum_ms": round(st["ms_linesum"], 1),
+            "fp32_engine_max_abs_dBT_K": float(d.max()), "fp32_engine_rms_dBT_K": float(np.sqrt(np.mean(d * d))),
+            "fp32_engine_speedup": round(value32 * dt, 1)}
+
+
+def pmc_from_profiles(name):
+    """Per-launch counter figures from a committed rocprofv3 summary under profiles/."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f)
     return {}
+
+
+def roofline_figure(stats, ls_avg):
+    """Line-sum roofline: FP32 FLOP per launch from the committed counters of the
+    same launch (SQ_INSTS_VALU_FLOPS_FP32 counts per wave instruction; x 64
+    lanes) over the launch time measured here with HIP events on the context
+    stream."""
+    pmc = pmc_from_profiles("linesum_full_iasi_pmc.json")
+    kernel = ("linesum_persistent_kernel" if stats["persistent"] else "linesum_kernel") + f"<{stats['tile_points']}>"
+    nominal = FLOP_PER_VOIGT_PAIR * stats["active_pairs"] + FLOP_PER_DISP_PAIR * stats["dispersive_pairs"]
+    t = ls_avg * 1e-3
+    flop = 64.0 * pmc["valu_flops_fp32_per_launch"] if pmc.get("valu_flops_fp32_per_launch") else None
+    achieved = flop / t / 1e12 if flop else None
+    roof = {"bound": "valu", "achieved": round(achieved, 3) if achieved else None, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4) if achieved else None,
+            "traffic": pmc.get("hbm_bytes_per_launch"), "kernel": kernel, "flop_per_launch": flop,
+            "launch_ms_avg": round(ls_avg, 3),
+            "peak_non_packed": PEAK_FP32_NONPACKED_TFLOPS,
+            "frac_non_packed": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4) if achieved else None,
+            "source": "flop_per_launch = 64 x SQ_INSTS_VALU_FLOPS_FP32 of this launch "
+                      "(profiles/linesum_full_iasi_pmc.json); traffic = FETCH_SIZE + WRITE_SIZE scaled by a "
+                      "device copy of known size",
+            "valu_issue": pmc.get("valu_issue"),
+            "algorithmic_equivalent": {
+                "flop_per_launch": nominal, "tflops": round(nominal / t / 1e12, 2),
+                "frac_of_peak": round(nominal / t / 1e12 / PEAK_FP32_TFLOPS, 3),
+                "pairs_reference": {"voigt": stats["active_pairs"], "dispersive": stats["dispersive_pairs"]},
+                "executed": {"fine_pairs": stats["fine_pairs"], "coarse_far_wing_node_pairs": stats["coarse_pairs"],
+                             "fine_pairs_vpy": stats["fine_pairs_lm"],
+                             "coarse_node_pairs_vpy": stats["coarse_pairs_lm"],
+                             "line_slices_expanded": stats["expanded_line_slices"]},
+                "note": "the reference's every-pair FLOP count (14 per Voigt pair, 9 per dispersive pair) over "
+                        "the same launch time; above 1 because the kernel evaluates far fewer pairs (coarse far "
+                        "wings, per-slice expansions, mid-core series), so it is not a kernel-efficiency figure"}}
+    return roof
+
+
+def hbm_figure(stats, n_hi, n_layers, n_ch, half):
+    """RTE and channel-ILS kernels: algorithmic bytes per launch (tau read as
+    FP32 + R_hi written; R_hi read + taps + R/BT written) and the committed
+    counter bytes, over the launch times measured here, against 8 TB/s."""
+    pmc = pmc_from_profiles("rte_ils_full_iasi_pmc.json")
+    out = {}
+    for key, ms, alg in (("rte", stats["ms_rte"], n_hi * n_layers * 4 + n_hi * 8),
+                         ("ils", stats["ms_ils"], n_hi * 8 + (2 * half + 1) * 8 + n_ch * 16)):
+        k = pmc.get(key, {})
+        t = ms * 1e-3
+        d = {"kernel": k.get("kernel"), "launch_ms": round(ms, 4), "algorithmic_bytes": alg,
+             "achieved": round(alg / t / 1e9, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+             "frac": round(alg / t / 1e9 / PEAK_HBM_GBPS, 4), "traffic": k.get("hbm_bytes_per_launch")}
+        if k.get("hbm_bytes_per_launch"):
+            d["traffic_GBps"] = round(k["hbm_bytes_per_launch"] / t / 1e9, 1)
+        if k.get("bound_note"):
+            d["note"] = k["bound_note"]
+        out[key] = d
+    return out
 
 
 def main():
@@ -262,6 +500,7 @@ def main():
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-emulator", action="store_true")
     ap.add_argument("--no-jacobian", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--emu-train", type=int, default=2048)
     ap.add_argument("--emu-eval", type=int, default=256)
     args = ap.parse_args()
@@ -289,7 +528,6 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"bench: warmup {i + 1}/{args.warmup} done")
-    import ctypes
     ls_ms = []
     stats = None
     t0 = time.perf_counter()
@@ -302,37 +540,9 @@ def main():
     log(f"bench: {args.steps} steps in {t1 - t0:.3f} s")
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = 1e3 / ms_step
-    # Algorithmic work (the roofline numerator): the reference's every-pair sum,
-    # 14 FLOP per active Voigt pair and 9 per dispersive pair (pairs counted on
-    # the host-side prep from the active line sets), per launch of the line sum.
-    # The kernel reaches the same sum with far fewer evaluations (coarse far
-    # wings, EDGE-line expansion, asymptotic mid-core series); what it executed
-    # is reported beside it, with the measured VALU issue rate of the launch.
-    nominal = FLOP_PER_VOIGT_PAIR * stats["active_pairs"] + FLOP_PER_DISP_PAIR * stats["dispersive_pairs"]
     ls_avg = float(np.mean(ls_ms))
-    achieved = nominal / (ls_avg * 1e-3) / 1e12
-    prof_pmc = pmc_from_profiles()
-    executed = {"fine_pairs": stats["fine_pairs"], "coarse_far_wing_node_pairs": stats["coarse_pairs"],
-                "fine_pairs_vpy": stats["fine_pairs_lm"], "coarse_node_pairs_vpy": stats["coarse_pairs_lm"],
-                "line_slices_expanded": stats["expanded_line_slices"]}
-    if prof_pmc.get("valu_insts_per_launch"):
-        rate = prof_pmc["valu_insts_per_launch"] / (ls_avg * 1e-3)
-        executed["valu_wave_insts_per_launch"] = prof_pmc["valu_insts_per_launch"]
-        executed["valu_issue_frac"] = round(rate / VALU_ISSUE_PEAK, 3)
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": prof_pmc.get("hbm_bytes_per_launch"),
-            "kernel": ("linesum_persistent_kernel" if stats["persistent"] else "linesum_kernel")
-                      + f"<{stats['tile_points']}>", "flop_per_launch": nominal,
-            "launch_ms_avg": round(ls_avg, 3),
-            "pairs_reference": {"voigt": stats["active_pairs"], "dispersive": stats["dispersive_pairs"]},
-            "executed": executed,
-            "note": "achieved = the reference's every-pair FLOP count (14/Voigt pair, 9/dispersive pair) "
-                    "per line-sum launch / launch time; above 1 the kernel beats the brute-force FP32 "
-                    "roofline by evaluating far fewer pairs (coarse far wings beyond 1.5 cm^-1, per-slice "
-                    "expansion of cutoff-boundary lines, mid-core series); executed work and the VALU "
-                    "issue fraction (vs 256 CUs x 4 SIMD x 2.4 GHz / 4 wave64 issues, from the committed "
-                    "PMC profile) are the kernel-efficiency figures"}
+    half = int(np.ceil(16.0 / 0.001))
+    roof = roofline_figure(stats, ls_avg)
     out = {"metric": "IASI spectra/s on 1 GPU (LBL vs emulator) at BT err ≤0.005/0.05 K vs FP64",
            "value": round(value, 4), "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
@@ -343,8 +553,12 @@ def main():
                       "path": "plain FP32 LBL", "profiles_per_step": 1},
            "roofline": roof,
            "far_wing_on_fine_grid": None if args.no_exact else exact_sum(ctx, prof, lls, snd),
+           "hbm": hbm_figure(stats, stats["n_hi"], stats["n_layers"], len(BT), half),
            "kernels_ms": {"prep": round(stats["ms_prep"], 3), "linesum": round(ls_avg, 3),
-                          "rte": round(stats["ms_rte"], 3), "ils": round(stats["ms_ils"], 3)}}
+                          "rte": round(stats["ms_rte"], 3), "ils": round(stats["ms_ils"], 3)},
+           "host_ms_per_step": round(ms_step - stats["ms_device"], 3)}
+    if not args.no_fp64:
+        out["fp64_every_pair"] = fp64_figure(ctx, prof, lls, snd, BT, value)
     if not args.no_batch:
         out["batched"] = batched(ctx, lls, snd, args.batch)
     if not args.no_jacobian:
@@ -352,11 +566,14 @@ def main():
     if not args.no_emulator:
         out["emulator"] = emulator_figure(ctx, lls_w, snd, args.emu_train, args.emu_eval)
     if not args.no_accuracy:
-        log("bench: accuracy sample vs FP64 oracle")
-        out["accuracy_vs_fp64"] = accuracy_sample(ctx, prof, lls, lls_w)
+        log("bench: accuracy samples and config 0 (GPU side)")
+        accuracy_sample(ctx, prof, lls, lls_w)
+        out["config0"] = config0_figure(ctx)
     if not args.no_cpu:
-        log("bench: cpu baseline (oracle, one layer)")
-        out["cpu_baseline"] = cpu_baseline(prof, lls, snd, layer=24)
+        log("bench: cpu baseline (FP64 oracle: config 0 full, config 1 sampled layers, config 2 two profiles)")
+        out["cpu_baseline"] = cpu_baseline(prof, lls, snd)
+        log("bench: FP64 oracle checks of the GPU results")
+        run_checks(out)
     print(json.dumps(out), flush=True)
     ctx.close()
 

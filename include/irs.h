@@ -170,6 +170,7 @@ typedef struct {
     int64_t coarse_pairs_lm;
     int64_t expanded_line_slices; /* lines x 64-point slices evaluated by local expansion
                                      (cutoff-boundary and far FAST lines) */
+    double  ms_continuum;       /* continuum kernel (0 with apply_continuum = false) */
 } irs_stats;
 
 /* Line-sum schedule of irs_forward.  No counterpart in the reference: it
@@ -188,6 +189,10 @@ typedef struct {
                                    (interior lines beyond 500 grid steps, cutoff-boundary
                                    lines) by local expansion in the slice; 0: every pair on
                                    the internal grid */
+    int fp64_every_pair;        /* 1: irs_forward runs the reference's every-pair line sum in
+                                   FP64 (FP64 optical depths, one profile at a time, no graph):
+                                   the plain FP64 figure beside the FP32 engine */
+    int reserved;
 } irs_schedule;
 
 int         irs_version(void);

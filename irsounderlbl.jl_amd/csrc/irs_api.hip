@@ -67,8 +67,9 @@ struct PinBuf {
 // cached by the byte image of this struct (zero-filled before use).
 struct GroupPlan {
     const void *items, *rec0, *rec1, *rec2, *rec3, *keys, *ktab, *tau, *rhi, *counters, *queue, *lists, *tips, *order;
-    const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out, *pre;
+    const void *prof, *rch, *bt, *chidx, *chw, *chnu, *taps, *h_items, *h_prof, *h_out, *pre, *pblk;
     int64_t n, nch, half, n_items, max_lines, items_bytes, prof_bytes, n_rec, ktab_lo, ktab_n;
+    int64_t ch_idx0, ch_stride;    // equally spaced exact-node channels (stride 0: general ILS path)
     int gn, L, ns, P, pblocks, tiles, source_function, with_ils, far_d, far_hc;
     double nu_lo, dnu, an, sn, den;
     int mode;                      // 0 radiance + BT, 1 channel optical depths (2: + CIM weights),
@@ -106,11 +107,13 @@ struct irs_ctx {
     DevBuf j_items, j_g, j_tau, j_dt, j_v, j_rte, j_cols, j_desc, j_w, j_K, j_fac;
     std::map<TapsKey, DevBuf> taps;
     irs_stats stats{};
-    irs_schedule sched{4, 1, 0, 0, 1.5};      // persistent heaviest-first queue: 26.7 vs 27.9 ms per full-IASI step
+    irs_schedule sched{4, 1, 0, 0, 1.5, 0, 0};      // persistent heaviest-first queue: 26.7 vs 27.9 ms per full-IASI step
     std::map<std::string, std::pair<int, int>> tuned;     // shape -> (P, persistent)
     std::map<std::string, DevBuf> orders;                 // shape -> heaviest-first tile order
     std::map<int, int> capacity;                          // P -> resident persistent workgroups
     DevBuf d_queue, d_prof, d_chod, d_chlev, d_chg, d_pre;
+    DevBuf d_pblk;                     // pedestal scan block sums [sum_items ceil(n/256)][3]
+    DevBuf d_rec64, d_tau64;           // every-pair FP64 path: line records, FP64 optical depths
     DevBuf d_ckd;                      // MT-CKD tables: H2O [2001][4], then CO2 [5003][2] = {S*xfac, tdep}
     DevBuf d_cia[3];                   // per CIA pair: T, numin, numax [nb], off [nb+1], nu, sig [knots]
     int cia_nb[3] = {0, 0, 0};
@@ -545,11 +548,15 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
                 double nu_lo, double dnu, int64_t n_hi, float *out) {
     int64_t rec = 0;
     int max_lines = 0;
+    int64_t pb = 0;
     for (auto &it : items) {
         it.rec_off = rec;
         rec += it.n;
+        it.pb_off = pb;
+        pb += (it.n + 255) / 256;
         max_lines = std::max(max_lines, it.n);
     }
+    HIPCHK(ensure(ctx->d_pblk, sizeof(double) * 3 * std::max<int64_t>(pb, 1)));
     HIPCHK(ensure(ctx->d_rec0, sizeof(float4) * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_rec1, sizeof(float4) * std::max<int64_t>(rec, 1)));
     HIPCHK(ensure(ctx->d_rec2, sizeof(float4) * std::max<int64_t>(rec, 1)));
@@ -576,6 +583,7 @@ int run_linesum(irs_ctx *ctx, std::vector<ItemDev> &items, int n_prof, int n_lay
     pa.counters = (unsigned long long *)ctx->d_counters.p;
     pa.n_hi = n_hi;
     pa.pre = (double *)ctx->d_pre.p;
+    pa.pblk = (double *)ctx->d_pblk.p;
     pa.n_rec = std::max<int64_t>(rec, 1);
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(launch_prep(pa, max_lines, ctx->stream));
@@ -705,6 +713,20 @@ void channel_positions(const Grid &gi, const Grid &gc, std::vector<int64_t> &idx
         idx[c] = k;
         w[c] = (v - gi.at(k)) / (gi.at(k + 1) - gi.at(k));
     }
+}
+
+// Stride of the channel nodes when every channel is an exact internal node and
+// the nodes are equally spaced (the case of every forward_model grid); 0 otherwise.
+int64_t channel_stride(const std::vector<int64_t> &idx, const std::vector<double> &w) {
+    if (idx.empty()) return 0;
+    for (double x : w)
+        if (x != 0.0) return 0;
+    if (idx.size() == 1) return 1;
+    const int64_t S = idx[1] - idx[0];
+    if (S < 1 || S > (1 << 20)) return 0;
+    for (size_t c = 1; c < idx.size(); ++c)
+        if (idx[c] - idx[c - 1] != S) return 0;
+    return S;
 }
 
 double cg_mass_frac(double p1, double p2) {   // schwarzschild_jacobian.jl:47-53
@@ -843,6 +865,9 @@ void irs_destroy(irs_ctx *ctx) {
     ctx->d_chlev.release();
     ctx->d_chg.release();
     ctx->d_ckd.release();
+    ctx->d_pblk.release();
+    ctx->d_rec64.release();
+    ctx->d_tau64.release();
     for (auto &b : ctx->d_cia) b.release();
     ctx->h_items.release();
     ctx->h_prof.release();
@@ -1180,6 +1205,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     pa.counters = (unsigned long long *)pl.counters;
     pa.n_hi = pl.n;
     pa.pre = (double *)pl.pre;
+    pa.pblk = (double *)pl.pblk;
     pa.n_rec = pl.n_rec;
     if (timing) ENQ(hipEventRecord(ctx->ev[0], st));
     ENQ(launch_prep(pa, (int)pl.max_lines, st));
@@ -1210,6 +1236,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.n_rec = pa.n_rec;
     la.rec3 = pa.rec3;
     ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
+    if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
     if (pl.continua) {
         ContArgs co = cont_args(ctx, pl.continua);
         co.tau = (float *)pl.tau;
@@ -1222,7 +1249,7 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
         co.lay = (const ContLayer *)((const double *)pl.prof + NL * (2 * L + 4));
         ENQ(launch_continuum(co, st));
     }
-    if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
+    if (timing) ENQ(hipEventRecord(ctx->ev[7], st));
     if (pl.mode >= 1) {
         ChodArgs ca{};
         ca.tau = (const float *)pl.tau;
@@ -1277,20 +1304,36 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     ra.n_prof = pl.gn;
     ENQ(launch_rte(ra, st));
     if (timing) ENQ(hipEventRecord(ctx->ev[3], st));
-    IlsArgs ia{};
-    ia.spec = (const double *)pl.rhi;
-    ia.n = pl.n;
-    ia.taps = (const double *)pl.taps;
-    ia.half = pl.with_ils ? pl.half : 0;
-    ia.dnu = pl.with_ils ? pl.dnu : 1.0;
-    ia.ch_idx = (const int64_t *)pl.chidx;
-    ia.ch_w = (const double *)pl.chw;
-    ia.ch_nu = (const double *)pl.chnu;
-    ia.n_ch = pl.nch;
-    ia.r_out = (double *)pl.rch;
-    ia.bt_out = (double *)pl.bt;
-    ia.n_prof = pl.gn;
-    ENQ(launch_ils_channels(ia, st));
+    if (pl.ch_stride > 0) {
+        IlsBlockArgs ib{};
+        ib.spec = (const double *)pl.rhi;
+        ib.n = pl.n;
+        ib.taps = (const double *)pl.taps;
+        ib.half = pl.with_ils ? pl.half : 0;
+        ib.dnu = pl.with_ils ? pl.dnu : 1.0;
+        ib.idx0 = pl.ch_idx0;
+        ib.stride = (int)pl.ch_stride;
+        ib.ch_nu = (const double *)pl.chnu;
+        ib.n_ch = pl.nch;
+        ib.r_out = (double *)pl.rch;
+        ib.bt_out = (double *)pl.bt;
+        ENQ(launch_ils_block(ib, pl.gn, st));
+    } else {
+        IlsArgs ia{};
+        ia.spec = (const double *)pl.rhi;
+        ia.n = pl.n;
+        ia.taps = (const double *)pl.taps;
+        ia.half = pl.with_ils ? pl.half : 0;
+        ia.dnu = pl.with_ils ? pl.dnu : 1.0;
+        ia.ch_idx = (const int64_t *)pl.chidx;
+        ia.ch_w = (const double *)pl.chw;
+        ia.ch_nu = (const double *)pl.chnu;
+        ia.n_ch = pl.nch;
+        ia.r_out = (double *)pl.rch;
+        ia.bt_out = (double *)pl.bt;
+        ia.n_prof = pl.gn;
+        ENQ(launch_ils_channels(ia, st));
+    }
     if (timing) ENQ(hipEventRecord(ctx->ev[4], st));
     double *hout = (double *)pl.h_out;
     ENQ(hipMemcpyAsync(hout, pl.rch, sizeof(double) * pl.nch * NL, hipMemcpyDeviceToHost, st));
@@ -1354,6 +1397,7 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
     pa.counters = (unsigned long long *)base.counters;
     pa.n_hi = base.n;
     pa.pre = (double *)base.pre;
+    pa.pblk = (double *)base.pblk;
     pa.n_rec = base.n_rec;
     HIPCHK(launch_prep(pa, (int)base.max_lines, st));
     const int cand_P[3] = {4, 8, 16};
@@ -1406,6 +1450,125 @@ static int autotune_group(irs_ctx *ctx, const GroupPlan &base, const std::vector
             }
         }
     ctx->tuned[key] = {bestP, bestPers};
+    return IRS_OK;
+}
+
+// One profile through the every-pair FP64 line sum (k_ls64.hip), FP64 optical
+// depths, continuum, the FP64 RTE and the channel ILS: the reference kernel's
+// arithmetic on the device.  Eager, on the context stream.
+static int run_profile_fp64(irs_ctx *ctx, const ItemDev *items_h, int L, int ns, int64_t n, const Grid &g,
+                            int max_lines, int64_t n_rec, const double *h_prof, const double *taps, int64_t half,
+                            double ils_dnu, int source_function, int continua, int64_t ch_stride, int64_t ch_idx0,
+                            int64_t nch, double *R, double *BT, double &ms_prep, double &ms_ls, double &ms_cont,
+                            double &ms_rte, double &ms_ils) {
+    hipStream_t st = ctx->stream;
+    const size_t n_items = (size_t)L * ns;
+    const size_t prof_doubles = (size_t)(2 * L + 4) + (size_t)L * 8;
+    HIPCHK(ensure(ctx->d_rec64, sizeof(double) * 8 * (size_t)std::max<int64_t>(n_rec, 1)));
+    HIPCHK(ensure(ctx->d_tau64, sizeof(double) * (size_t)n * L));
+    HIPCHK(hipMemcpyAsync(ctx->d_items.p, items_h, sizeof(ItemDev) * n_items, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->d_prof.p, h_prof, sizeof(double) * prof_doubles, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_reset_counters((unsigned *)ctx->d_queue.p, (unsigned long long *)ctx->d_counters.p, IRS_NCNT, st));
+    PrepArgs pa{};
+    pa.items = (const ItemDev *)ctx->d_items.p;
+    pa.lists = (const LineListDev *)ctx->d_lists.p;
+    pa.tips_q = (const double *)ctx->d_tips.p;
+    pa.n_items = (int)n_items;
+    pa.nu_lo = g.at(0);
+    pa.dnu = g.step;
+    pa.rec0 = (float4 *)ctx->d_rec0.p;
+    pa.rec1 = (float4 *)ctx->d_rec1.p;
+    pa.rec2 = (float4 *)ctx->d_rec2.p;
+    pa.counters = (unsigned long long *)ctx->d_counters.p;
+    pa.n_hi = n;
+    pa.n_rec = std::max<int64_t>(n_rec, 1);
+    pa.rec64 = (double *)ctx->d_rec64.p;
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(launch_prep(pa, max_lines, st));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    Linesum64Args la{};
+    la.items = pa.items;
+    la.rec64 = pa.rec64;
+    la.keys = (const int32_t *)ctx->d_keys.p;
+    la.tau = (double *)ctx->d_tau64.p;
+    la.n_hi = n;
+    la.n_layers = L;
+    la.n_species = ns;
+    la.dnu = g.step;
+    la.an = g.an;
+    la.sn = g.sn;
+    la.den = g.den;
+    HIPCHK(launch_linesum64(la, st));
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    const double *prof = (const double *)ctx->d_prof.p;
+    if (continua) {
+        ContArgs co = cont_args(ctx, continua);
+        co.tau64 = (double *)ctx->d_tau64.p;
+        co.add64 = 1;
+        co.n = n;
+        co.n_layers = L;
+        co.n_prof = 1;
+        co.an = g.an;
+        co.sn = g.sn;
+        co.den = g.den;
+        co.lay = (const ContLayer *)(prof + (2 * L + 4));
+        HIPCHK(launch_continuum(co, st));
+    }
+    HIPCHK(hipEventRecord(ctx->ev[7], st));
+    RteArgs ra{};
+    ra.tau64 = (const double *)ctx->d_tau64.p;
+    ra.n = n;
+    ra.n_layers = L;
+    ra.T_lev = prof;
+    ra.T_ave = prof + (L + 1);
+    ra.scal = prof + (2 * L + 1);
+    ra.source_function = source_function;
+    ra.an = g.an;
+    ra.sn = g.sn;
+    ra.den = g.den;
+    ra.out = (double *)ctx->d_rhi.p;
+    ra.n_prof = 1;
+    HIPCHK(launch_rte(ra, st));
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    if (ch_stride > 0) {
+        IlsBlockArgs ib{};
+        ib.spec = (const double *)ctx->d_rhi.p;
+        ib.n = n;
+        ib.taps = taps;
+        ib.half = half;
+        ib.dnu = ils_dnu;
+        ib.idx0 = ch_idx0;
+        ib.stride = (int)ch_stride;
+        ib.ch_nu = (const double *)ctx->d_chnu.p;
+        ib.n_ch = nch;
+        ib.r_out = (double *)ctx->d_rch.p;
+        ib.bt_out = (double *)ctx->d_bt.p;
+        HIPCHK(launch_ils_block(ib, 1, st));
+    } else {
+        IlsArgs ia{};
+        ia.spec = (const double *)ctx->d_rhi.p;
+        ia.n = n;
+        ia.taps = taps;
+        ia.half = half;
+        ia.dnu = ils_dnu;
+        ia.ch_idx = (const int64_t *)ctx->d_chidx.p;
+        ia.ch_w = (const double *)ctx->d_chw.p;
+        ia.ch_nu = (const double *)ctx->d_chnu.p;
+        ia.n_ch = nch;
+        ia.r_out = (double *)ctx->d_rch.p;
+        ia.bt_out = (double *)ctx->d_bt.p;
+        ia.n_prof = 1;
+        HIPCHK(launch_ils_channels(ia, st));
+    }
+    HIPCHK(hipEventRecord(ctx->ev[4], st));
+    HIPCHK(hipMemcpyAsync(R, ctx->d_rch.p, sizeof(double) * nch, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(BT, ctx->d_bt.p, sizeof(double) * nch, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ms_prep += elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    ms_ls += elapsed_ms(ctx->ev[1], ctx->ev[2]);
+    ms_cont += elapsed_ms(ctx->ev[2], ctx->ev[7]);
+    ms_rte += elapsed_ms(ctx->ev[7], ctx->ev[3]);
+    ms_ils += elapsed_ms(ctx->ev[3], ctx->ev[4]);
     return IRS_OK;
 }
 
@@ -1504,6 +1667,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     if (const char *e = std::getenv("IRS_GROUP_BYTES")) budget = std::max(1.0, std::atof(e));
     int gmax = (int)std::max(1.0, std::min((double)n_prof, budget / std::max(bytes_per_prof, 1.0)));
     if (const char *e = std::getenv("IRS_PROFILE_GROUP")) gmax = std::max(1, std::min(n_prof, std::atoi(e)));
+    const bool fp64 = ctx->sched.fp64_every_pair && mode == 0;
+    if (fp64) gmax = 1;                    // the every-pair FP64 path runs one profile at a time
     const int n_groups = (n_prof + gmax - 1) / gmax;
     const int group = (n_prof + n_groups - 1) / n_groups;
 
@@ -1511,7 +1676,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     std::vector<ItemDev> proto(ns);
     std::vector<const ItemDev *> per_species;
     int P = ctx->sched.tile_points, persistent = ctx->sched.persistent;
-    double ms_prep = 0, ms_ls = 0, ms_rte = 0, ms_ils = 0, ms_graph = 0;
+    double ms_prep = 0, ms_ls = 0, ms_rte = 0, ms_ils = 0, ms_graph = 0, ms_cont = 0;
     bool tuned_now = false;
 
     // per profile: T_lev [L+1], T_ave [L], {T_sfc, mu, eps}, then the continuum layer
@@ -1524,6 +1689,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     HIPCHK(ensure(ctx->d_rec3, sizeof(float4) * 6 * std::max<int64_t>((int64_t)group * L * lines_per_prof, 1)));
     const int64_t n_rec_cap = std::max<int64_t>((int64_t)group * L * lines_per_prof, 1);
     HIPCHK(ensure(ctx->d_pre, sizeof(double) * 3 * n_rec_cap));
+    HIPCHK(ensure(ctx->d_pblk, sizeof(double) * 3 * (n_rec_cap / 256 + (int64_t)group * L * ns + 1)));
     HIPCHK(ensure(ctx->d_tau, sizeof(float) * (size_t)n * L * group));
     HIPCHK(ensure(ctx->d_rhi, sizeof(double) * (size_t)n * group));
     HIPCHK(ensure(ctx->d_rch, sizeof(double) * nch * group));
@@ -1547,7 +1713,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         double *hp = (double *)ctx->h_prof.p;
         double *h_tlev = hp, *h_tave = hp + (size_t)gn * (L + 1), *h_scal = hp + (size_t)gn * (2 * L + 1);
         ContLayer *h_cont = (ContLayer *)(hp + (size_t)gn * (2 * L + 4));
-        int64_t rec = 0;
+        int64_t rec = 0, pblk = 0;
         for (int q = 0; q < gn; ++q) {
             const irs_profile &pr = profiles[g0 + q];
             LayerProps lp;
@@ -1581,6 +1747,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                     }
                     it.rec_off = rec;
                     rec += it.n;
+                    it.pb_off = pblk;
+                    pblk += (it.n + 255) / 256;
                     if (q == 0 && k == L / 2 && it.active) proto[s] = it;
                 }
         }
@@ -1598,6 +1766,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                     tp.ktab_n = ctx->ktab_n; tp.tau = ctx->d_tau.p;
                     tp.counters = ctx->d_counters.p; tp.queue = ctx->d_queue.p; tp.lists = ctx->d_lists.p;
                     tp.tips = ctx->d_tips.p; tp.h_items = ctx->h_items.p; tp.pre = ctx->d_pre.p;
+                    tp.pblk = ctx->d_pblk.p;
                     tp.n_rec = n_rec_cap;
                     tp.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)L * ns);
                     tp.n = n; tp.L = L; tp.ns = ns; tp.max_lines = max_lines; tp.nu_lo = nu_lo; tp.dnu = dnu;
@@ -1608,6 +1777,17 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                 P = ctx->tuned[key].first;
                 persistent = ctx->tuned[key].second;
             }
+        }
+        if (fp64) {
+            const int64_t ch_stride = channel_stride(cidx, cw);
+            rc = run_profile_fp64(ctx, items, L, ns, n, g, max_lines, rec, (const double *)ctx->h_prof.p, taps,
+                                  opt.with_ils ? half : 0, opt.with_ils ? dnu : 1.0, opt.source_function,
+                                  opt.continua, ch_stride, ch_stride > 0 ? cidx[0] : 0, nch, R + (size_t)g0 * nch,
+                                  BT + (size_t)g0 * nch, ms_prep, ms_ls, ms_cont, ms_rte, ms_ils);
+            if (rc) return rc;
+            read_counters(ctx);
+            ctx->stats.linesum_launches += 1;
+            continue;
         }
         const int32_t *order = nullptr;
         rc = tile_order(ctx, per_species, n, nu_lo, dnu, P, &order);
@@ -1620,7 +1800,9 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         pl.queue = ctx->d_queue.p; pl.lists = ctx->d_lists.p; pl.tips = ctx->d_tips.p; pl.order = order;
         pl.prof = ctx->d_prof.p; pl.rch = ctx->d_rch.p; pl.bt = ctx->d_bt.p; pl.chidx = ctx->d_chidx.p;
         pl.chw = ctx->d_chw.p; pl.chnu = ctx->d_chnu.p; pl.taps = taps; pl.h_items = ctx->h_items.p;
-        pl.pre = ctx->d_pre.p; pl.n_rec = n_rec_cap;
+        pl.ch_stride = channel_stride(cidx, cw);
+        pl.ch_idx0 = pl.ch_stride > 0 ? cidx[0] : 0;
+        pl.pre = ctx->d_pre.p; pl.pblk = ctx->d_pblk.p; pl.n_rec = n_rec_cap;
         pl.h_prof = ctx->h_prof.p; pl.h_out = ctx->h_out.p;
         pl.n = n; pl.nch = nch; pl.half = half; pl.n_items = (int64_t)gn * L * ns; pl.max_lines = max_lines;
         pl.items_bytes = (int64_t)(sizeof(ItemDev) * (size_t)gn * L * ns);
@@ -1651,7 +1833,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         } else {
             ms_prep += elapsed_ms(ctx->ev[0], ctx->ev[1]);
             ms_ls += elapsed_ms(ctx->ev[1], ctx->ev[2]);
-            ms_rte += elapsed_ms(ctx->ev[2], ctx->ev[3]);
+            ms_cont += elapsed_ms(ctx->ev[2], ctx->ev[7]);
+            ms_rte += elapsed_ms(ctx->ev[7], ctx->ev[3]);
             ms_ils += elapsed_ms(ctx->ev[3], ctx->ev[4]);
         }
         if (mode == 3) {
@@ -1675,7 +1858,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     ctx->stats.ms_linesum = ms_ls;
     ctx->stats.ms_rte = ms_rte;
     ctx->stats.ms_ils = ms_ils;
-    ctx->stats.ms_device = ctx->sched.use_graph ? ms_graph : ms_prep + ms_ls + ms_rte + ms_ils;
+    ctx->stats.ms_continuum = ms_cont;
+    ctx->stats.ms_device = (ctx->sched.use_graph && !fp64) ? ms_graph : ms_prep + ms_ls + ms_cont + ms_rte + ms_ils;
     ctx->stats.n_hi = n;
     ctx->stats.n_layers = L;
     ctx->stats.tile_points = P;
@@ -1734,6 +1918,8 @@ int irs_set_schedule(irs_ctx *ctx, const irs_schedule *sc) {
     ctx->sched.use_graph = sc->use_graph ? 1 : 0;
     ctx->sched.autotune = sc->autotune ? 1 : 0;
     ctx->sched.far_wing_cm = sc->far_wing_cm > 0.0 ? sc->far_wing_cm : 0.0;
+    ctx->sched.fp64_every_pair = sc->fp64_every_pair ? 1 : 0;
+    ctx->sched.reserved = 0;
     ctx->tuned.clear();
     return IRS_OK;
 }
@@ -2151,6 +2337,7 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     HIPCHK(hipMemcpyAsync(ctx->d_chidx.p, cidx.data(), sizeof(int64_t) * nch, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_chw.p, cw.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_chnu.p, cnu.data(), sizeof(double) * nch, hipMemcpyHostToDevice, ctx->stream));
+    const int64_t ch_stride = channel_stride(cidx, cw);
     IlsArgs ia{};
     ia.spec = ra.I;
     ia.n = n;
@@ -2163,7 +2350,23 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     ia.n_ch = nch;
     ia.r_out = (double *)ctx->d_rch.p;
     ia.bt_out = (double *)ctx->d_bt.p;
-    HIPCHK(launch_ils_channels(ia, ctx->stream));
+    IlsBlockArgs ib{};
+    ib.n = n;
+    ib.taps = taps;
+    ib.half = ia.half;
+    ib.dnu = ia.dnu;
+    ib.idx0 = ch_stride > 0 ? cidx[0] : 0;
+    ib.stride = (int)ch_stride;
+    ib.ch_nu = ia.ch_nu;
+    ib.n_ch = nch;
+    if (ch_stride > 0) {
+        ib.spec = ra.I;
+        ib.r_out = ia.r_out;
+        ib.bt_out = ia.bt_out;
+        HIPCHK(launch_ils_block(ib, 1, ctx->stream));
+    } else {
+        HIPCHK(launch_ils_channels(ia, ctx->stream));
+    }
     std::vector<double> Rc(nch), BTc(nch), fac(nch);
     HIPCHK(hipMemcpyAsync(Rc.data(), ctx->d_rch.p, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(BTc.data(), ctx->d_bt.p, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream));
@@ -2186,7 +2389,15 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     ja.factor = (const double *)ctx->j_fac.p;
     ja.n_ch = nch;
     ja.K = (double *)ctx->j_K.p;
-    HIPCHK(launch_jac_ils(ja, ncol, ctx->stream));
+    if (ch_stride > 0) {
+        ib.spec = ca.out;
+        ib.factor = (const double *)ctx->j_fac.p;
+        ib.r_out = (double *)ctx->j_K.p;
+        ib.bt_out = nullptr;
+        HIPCHK(launch_ils_block(ib, ncol, ctx->stream));
+    } else {
+        HIPCHK(launch_jac_ils(ja, ncol, ctx->stream));
+    }
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
     HIPCHK(hipMemcpyAsync(K, ctx->j_K.p, sizeof(double) * (size_t)nch * ncol, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

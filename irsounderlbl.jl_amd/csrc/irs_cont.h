@@ -46,7 +46,8 @@ struct ContArgs {
     //   dtlo += d tau_c/d T_mid (1 - wM),  dthi += d tau_c/d T_mid wM
     //   vlo_h/vhi_h, vlo_c/vhi_c: the same for the H2O / CO2 level VMR
     float *dtlo, *dthi, *vlo_h, *vhi_h, *vlo_c, *vhi_c;
-    double *tau64;                     // optional FP64 output instead of tau (layer export), [n_layers][n]
+    double *tau64;                     // optional FP64 tau instead of tau, [n_prof][n_layers][n]
+    int add64;                         // 1: tau64 += tau_c (FP64 forward), 0: tau64 = tau_c (layer export)
 };
 
 hipError_t launch_continuum(const ContArgs &a, hipStream_t s);

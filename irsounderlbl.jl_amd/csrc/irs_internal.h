@@ -48,6 +48,7 @@ struct ItemDev {
     double  T, p_atm, vmr_self, coef, dptmn, cutoff, x_far;
     double  lm_cutoff;      // cm^-1 (the VP_Y window; lm_cutoff / dnu = Clm)
     int64_t ktab_off;       // offset of the list's key table (LinesumArgs::ktab)
+    int64_t pb_off;         // first 256-line block of this item in PrepArgs::pblk
 };
 
 struct PrepArgs {
@@ -63,6 +64,20 @@ struct PrepArgs {
     double *pre;                   // optional [3][n_rec]: per-item inclusive prefix sums of the
     int64_t n_rec;                 // pedestal amplitude P, P c and P c^2 (c: centre, grid steps)
     float4 *rec3;                  // optional [n_rec][6]: mid-core records (Voigt, then dispersive)
+    double *pblk;                  // with pre: [sum over items of ceil(n/256)][3] block sums of the scan
+    double *rec64;                 // optional [n_rec][8] FP64 line records of the every-pair FP64 sum:
+                                   // {shifted centre, f, y, Snorm (0 if rejected), Hc, YSn, fd, yd}
+};
+
+// Every-pair FP64 line sum (k_ls64.hip), one profile: tau [n_layers][n_hi] FP64.
+struct Linesum64Args {
+    const ItemDev *items;          // [n_layers][n_species]
+    const double *rec64;           // PrepArgs::rec64
+    const int32_t *keys;
+    double *tau;
+    int64_t n_hi;
+    int n_layers, n_species;
+    double dnu, an, sn, den;       // grid node i = (an + i*sn)/den
 };
 
 struct LinesumArgs {
@@ -115,6 +130,22 @@ struct IlsArgs {
     int n_prof;                    // blockIdx.y = profile; spec is [n_prof][n] (0 is taken as 1)
 };
 
+// Channel ILS on equally spaced exact-node channels (ils_block_kernel):
+// channel c is the ILS correlation of column col at node idx0 + c * stride.
+struct IlsBlockArgs {
+    const double *spec;            // [n_cols][n]
+    int64_t n;
+    const double *taps;            // [2*half+1]
+    int64_t half;
+    double dnu;
+    int64_t idx0;
+    int stride;
+    const double *ch_nu;           // for bt_out
+    const double *factor;          // optional per-channel factor (dBT/dR of the Jacobian columns)
+    int64_t n_ch;
+    double *r_out, *bt_out;        // [n_cols][n_ch]; bt_out optional
+};
+
 // Channel optical depths (k_chod.hip)
 struct ChodArgs {
     const float *tau;              // [n_prof][n_layers][n]
@@ -149,3 +180,5 @@ hipError_t launch_rte(const RteArgs &a, hipStream_t s);
 hipError_t launch_ils_taps(const double *A, const double *L, int n_pts, double dL, double dnu,
                            int64_t half, double *taps, hipStream_t s);
 hipError_t launch_ils_channels(const IlsArgs &a, hipStream_t s);
+hipError_t launch_ils_block(const IlsBlockArgs &a, int n_cols, hipStream_t s);
+hipError_t launch_linesum64(const Linesum64Args &a, hipStream_t s);

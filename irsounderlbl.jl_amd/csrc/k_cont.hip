@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) continuum_kernel(const ContArgs a) {
         }
         const double tc = k_sum * cl.dz;
         const size_t o = ((size_t)q * a.n_layers + k) * (size_t)a.n + (size_t)i;
-        if (a.tau64) a.tau64[o] = tc;
+        if (a.tau64) a.tau64[o] = a.add64 ? a.tau64[o] + tc : tc;
         else if (a.tau) a.tau[o] += (float)tc;
         if (jac && q == 0) {
             const size_t oj = (size_t)k * (size_t)a.n + (size_t)i;

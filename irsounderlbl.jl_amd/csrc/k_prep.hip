@@ -114,6 +114,7 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
     const ItemDev it = a.items[item];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     unsigned long long my_active = 0, my_pairs = 0, my_dpairs = 0;
+    double ped_P = 0.0, ped_c = 0.0;           // the record's pedestal amplitude and centre (grid steps)
     if (it.active && j < it.n) {
         const LineListDev L = a.lists[it.list];
         const double T = it.T, p = it.p_atm, vs = it.vmr_self, coef = it.coef;
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
         r1.x = 0.0f; r2.y = 1.0f; r2.z = 1.0f; r2.w = 0.0f;
         // VP_Y dispersive weights with the S-file conventions (line_mixing.jl:1193-1220)
         bool disp = false;
-        double yd = 0.0, sd = 1.0, ysn = 0.0;
+        double yd = 0.0, sd = 1.0, ysn = 0.0, fd_ = 0.0, ysn1 = 0.0;
         if (it.lm && L.ylm && L.ylm[j] != 0.0) {
             const double Y = L.ylm[j];
             const double md = lm_mass(mol, iso);
@@ -188,6 +189,8 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
                 r2.z = (float)(gld * fd);
                 r2.w = (float)(coef * Y * p * st * fd * IRS_INV_SQRT_PI);
                 disp = true;
+                fd_ = fd;
+                ysn1 = Y * p * st * fd * IRS_INV_SQRT_PI;
                 yd = gld * fd;
                 sd = a.dnu * fd;
                 ysn = coef * Y * p * st * fd * IRS_INV_SQRT_PI;
@@ -197,6 +200,19 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
         a.rec0[o] = r0;
         a.rec1[o] = r1;
         a.rec2[o] = r2;
+        if (a.rec64) {
+            double *q = a.rec64 + o * 8;
+            q[0] = c;
+            q[1] = f;
+            q[2] = y;
+            q[3] = keep ? snorm : 0.0;
+            q[4] = hc;
+            q[5] = disp ? ysn1 : 0.0;             // per molecule (the FP64 sum multiplies by coef)
+            q[6] = disp ? fd_ : 0.0;
+            q[7] = disp ? yd : 0.0;
+        }
+        ped_P = (double)r1.y;
+        ped_c = (double)__float_as_int(r0.x) + (double)r0.y;
         if (a.rec3) {
             float4 *c = a.rec3 + 6 * o;
             if (keep) midcore_coeffs(y, a.dnu * f, coef * snorm * y * IRS_INV_SQRT_PI, it.x_far, false, c);
@@ -220,6 +236,31 @@ __global__ void __launch_bounds__(256) prep_kernel(const PrepArgs a) {
             const double l2 = fmax(lo, 0.0), h2 = fmin(hi, (double)(a.n_hi - 1));
             if (h2 >= l2) my_dpairs = (unsigned long long)(h2 - l2 + 1.0);
         }
+    }
+    // pedestal prefix sums, first pass: inclusive scan of P, P c, P c^2 over the
+    // block's 256 lines (the record values, so the moments match the records);
+    // block totals go to pblk and are carried across blocks by the offsets pass
+    if (a.pre) {
+        __shared__ double wtot[3][4];
+        double s0 = ped_P, s1 = ped_P * ped_c, s2 = ped_P * ped_c * ped_c;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (int o = 1; o < 64; o <<= 1) {
+            const double w0 = __shfl_up(s0, o, 64), w1 = __shfl_up(s1, o, 64), w2 = __shfl_up(s2, o, 64);
+            if (lane >= o) { s0 += w0; s1 += w1; s2 += w2; }
+        }
+        if (lane == 63) { wtot[0][wv] = s0; wtot[1][wv] = s1; wtot[2][wv] = s2; }
+        __syncthreads();
+        for (int w = 0; w < wv; ++w) { s0 += wtot[0][w]; s1 += wtot[1][w]; s2 += wtot[2][w]; }
+        if (it.active && j < it.n) {
+            double *p0 = a.pre + it.rec_off + j;
+            p0[0] = s0;
+            p0[a.n_rec] = s1;
+            p0[2 * a.n_rec] = s2;
+        }
+        const int64_t nb = it.active ? ((int64_t)it.n + 255) / 256 : 0;
+        if (threadIdx.x < 3 && (int64_t)blockIdx.x < nb)
+            a.pblk[(it.pb_off + blockIdx.x) * 3 + threadIdx.x] =
+                wtot[threadIdx.x][0] + wtot[threadIdx.x][1] + wtot[threadIdx.x][2] + wtot[threadIdx.x][3];
     }
     // block reduction of the work counters, one atomic per counter and block
     __shared__ unsigned long long red[3][4];
@@ -254,56 +295,35 @@ __global__ void __launch_bounds__(256) keys_kernel(const double *nu, int64_t n, 
     }
 }
 
-// Per item, inclusive prefix sums over its lines (list order) of the pedestal
-// amplitude P_j, P_j c_j and P_j c_j^2, c_j the centre in grid steps (FP64), so
-// that a tile obtains the three pedestal moments of any line range by two
-// lookups.  One workgroup per item, 256-line chunks, wave-shuffle scan.
-// Per-item inclusive prefix sums of P, P c, P c^2 (FP64), one workgroup per
-// item: each thread scans PS_K consecutive lines in registers, the thread
-// totals are scanned across the wave (shuffles) and the block (LDS), so a
-// workgroup advances 256 * PS_K lines per step.
-#define PS_K 8
-__global__ void __launch_bounds__(256) pedestal_scan_kernel(const PrepArgs a) {
+// Pedestal prefix sums, second and third passes.  prep_kernel leaves in pre
+// the inclusive scan of P, P c, P c^2 (c the centre in grid steps, FP64) within
+// each 256-line block of an item and the block totals in pblk; the offsets pass
+// turns an item's block totals into exclusive prefix sums (one workgroup per
+// item, 256 blocks per step), and the fix-up pass adds each block's offset, so
+// that pre holds per-item inclusive prefix sums over the item's lines: a tile
+// obtains the three pedestal moments of any line range by two lookups.
+__global__ void __launch_bounds__(256) pedestal_offsets_kernel(const PrepArgs a) {
     __shared__ double wsum[3][4];
     const ItemDev it = a.items[blockIdx.x];
     if (!it.active || it.n == 0) return;
+    const int64_t nb = ((int64_t)it.n + 255) / 256;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    double *p0 = a.pre + it.rec_off, *p1 = p0 + a.n_rec, *p2 = p1 + a.n_rec;
+    double *pb = a.pblk + it.pb_off * 3;
     double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-    for (int64_t j0 = 0; j0 < it.n; j0 += 256 * PS_K) {
-        const int64_t jt = j0 + (int64_t)t * PS_K;
-        double v0[PS_K], v1[PS_K], v2[PS_K];
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < PS_K; ++k) {
-            const int64_t j = jt + k;
-            double P = 0.0, c = 0.0;
-            if (j < it.n) {
-                const float4 r0 = a.rec0[it.rec_off + j];
-                P = (double)a.rec1[it.rec_off + j].y;
-                c = (double)__float_as_int(r0.x) + (double)r0.y;
-            }
-            s0 += P;
-            s1 += P * c;
-            s2 += P * c * c;
-            v0[k] = s0;
-            v1[k] = s1;
-            v2[k] = s2;
-        }
-        double u0 = s0, u1 = s1, u2 = s2;             // inclusive scan of the thread totals
+    for (int64_t b0 = 0; b0 < nb; b0 += 256) {
+        const int64_t b = b0 + t;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (b < nb) { v0 = pb[3 * b]; v1 = pb[3 * b + 1]; v2 = pb[3 * b + 2]; }
+        double u0 = v0, u1 = v1, u2 = v2;
         for (int o = 1; o < 64; o <<= 1) {
             const double w0 = __shfl_up(u0, o, 64), w1 = __shfl_up(u1, o, 64), w2 = __shfl_up(u2, o, 64);
             if (lane >= o) { u0 += w0; u1 += w1; u2 += w2; }
         }
         if (lane == 63) { wsum[0][wv] = u0; wsum[1][wv] = u1; wsum[2][wv] = u2; }
         __syncthreads();
-        double b0 = c0 + (u0 - s0), b1 = c1 + (u1 - s1), b2 = c2 + (u2 - s2);   // offset of this thread
-        for (int w = 0; w < wv; ++w) { b0 += wsum[0][w]; b1 += wsum[1][w]; b2 += wsum[2][w]; }
-#pragma unroll
-        for (int k = 0; k < PS_K; ++k) {
-            const int64_t j = jt + k;
-            if (j < it.n) { p0[j] = b0 + v0[k]; p1[j] = b1 + v1[k]; p2[j] = b2 + v2[k]; }
-        }
+        double e0 = c0 + (u0 - v0), e1 = c1 + (u1 - v1), e2 = c2 + (u2 - v2);   // exclusive
+        for (int w = 0; w < wv; ++w) { e0 += wsum[0][w]; e1 += wsum[1][w]; e2 += wsum[2][w]; }
+        if (b < nb) { pb[3 * b] = e0; pb[3 * b + 1] = e1; pb[3 * b + 2] = e2; }
         c0 += wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
         c1 += wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
         c2 += wsum[2][0] + wsum[2][1] + wsum[2][2] + wsum[2][3];
@@ -311,13 +331,28 @@ __global__ void __launch_bounds__(256) pedestal_scan_kernel(const PrepArgs a) {
     }
 }
 
+__global__ void __launch_bounds__(256) pedestal_fixup_kernel(const PrepArgs a) {
+    const ItemDev it = a.items[blockIdx.y];
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (!it.active || blockIdx.x == 0 || j >= it.n) return;      // block 0 has offset 0
+    const double *pb = a.pblk + (it.pb_off + blockIdx.x) * 3;
+    double *p0 = a.pre + it.rec_off + j;
+    p0[0] += pb[0];
+    p0[a.n_rec] += pb[1];
+    p0[2 * a.n_rec] += pb[2];
+}
+
 hipError_t launch_prep(const PrepArgs &a, int max_lines, hipStream_t s) {
     if (a.n_items <= 0 || max_lines <= 0) return hipSuccess;
+    if (a.pre && !a.pblk) return hipErrorInvalidValue;
     dim3 grid((max_lines + 255) / 256, a.n_items);
     hipLaunchKernelGGL(prep_kernel, grid, dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !a.pre) return e;
-    hipLaunchKernelGGL(pedestal_scan_kernel, dim3(a.n_items), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(pedestal_offsets_kernel, dim3(a.n_items), dim3(256), 0, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (max_lines > 256) hipLaunchKernelGGL(pedestal_fixup_kernel, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -85,12 +85,12 @@ class IrsStats(C.Structure):
                 ("core_pairs_bound", i64), ("ms_device", f64), ("graph_launches", i64),
                 ("tile_points", i32), ("persistent", i32), ("autotuned", i32), ("reserved1", i32),
                 ("fine_pairs", i64), ("coarse_pairs", i64), ("fine_pairs_lm", i64),
-                ("coarse_pairs_lm", i64), ("expanded_line_slices", i64)]
+                ("coarse_pairs_lm", i64), ("expanded_line_slices", i64), ("ms_continuum", f64)]
 
 
 class IrsSchedule(C.Structure):
     _fields_ = [("tile_points", i32), ("persistent", i32), ("use_graph", i32), ("autotune", i32),
-                ("far_wing_cm", f64)]
+                ("far_wing_cm", f64), ("fp64_every_pair", i32), ("reserved", i32)]
 
 
 EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_last_error",
@@ -245,14 +245,15 @@ class Context:
         return {k: getattr(sc, k) for k, _ in IrsSchedule._fields_}
 
     def set_schedule(self, tile_points: int = 4, persistent: bool = True, use_graph: bool = False,
-                     autotune: bool = False, far_wing_cm: float = 1.5):
+                     autotune: bool = False, far_wing_cm: float = 1.5, fp64_every_pair: bool = False):
         """Line-sum schedule of forward_model: points per thread (4/8/16), the
         persistent heaviest-first work-queue kernel, hipGraph capture and
         replay of each profile group, autotuning on the first call of a shape,
         and the distance beyond which far wings are summed on a coarse grid
-        and interpolated (0: every pair on the internal grid)."""
+        and interpolated (0: every pair on the internal grid).
+        fp64_every_pair runs the reference's every-pair sum in FP64 instead."""
         sc = IrsSchedule(int(tile_points), int(bool(persistent)), int(bool(use_graph)), int(bool(autotune)),
-                         float(far_wing_cm))
+                         float(far_wing_cm), int(bool(fp64_every_pair)), 0)
         self.check(self.lib.irs_set_schedule(self.h, C.byref(sc)))
 
     def upload(self, ll: HITRANLinelist) -> int:

@@ -193,6 +193,7 @@ void orc_layer_properties(const orc_profile *prof, int T_method, double *p_mid, 
                           double *T_ave);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,8 @@ def lib():
                                        C.POINTER(OrcInfo)]
         L.orc_layer_properties.argtypes = [C.POINTER(OrcProfile), i32, P, P, P, P, P, P, P]
         L.orc_num_threads.restype = i32
+        L.orc_set_num_threads.argtypes = [i32]
+        L.orc_set_num_threads.restype = None
         _lib = L
     return _lib
 

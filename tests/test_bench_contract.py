@@ -26,8 +26,16 @@ def test_committed_profile_matches_default_schedule():
     with open(os.path.join(ROOT, "profiles", "linesum_full_iasi_pmc.json")) as f:
         J = json.load(f)
     assert J["kernel"] == kernel
-    for k in ("hbm_bytes_per_launch", "valu_insts_per_launch", "launch_ms_avg_trace"):
+    for k in ("hbm_bytes_per_launch", "valu_insts_per_launch", "launch_ms_avg_trace", "valu_flops_fp32_per_launch"):
         assert J[k] > 0
+    # the counter-based FP32 FLOP/s of the committed profile is a fraction of the peak
+    tflops = 64 * J["valu_flops_fp32_per_launch"] / (J["launch_ms_avg_trace"] * 1e-3) / 1e12
+    assert 0 < tflops < 157.3
+    with open(os.path.join(ROOT, "profiles", "rte_ils_full_iasi_pmc.json")) as f:
+        H = json.load(f)
+    assert H["rte"]["kernel"].startswith("rte_kernel") and H["ils"]["kernel"].startswith("ils_block_kernel")
+    for k in ("rte", "ils"):
+        assert H[k]["hbm_bytes_per_launch"] > 0 and H[k]["launch_ms_avg_trace"] > 0
     with open(os.path.join(ROOT, "profiles", "linesum_full_iasi_kernel_trace.md")) as f:
         trace = f.read()
     assert f"void {kernel}(LinesumArgs)" in trace

@@ -187,7 +187,7 @@ def test_emulator_spot_check_and_fallback(ctx, small_emu):
     spot = SpotChecker(sub, snd, blocks=((1206.0, 8), (1214.0, 8)))
     _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
     # spot channels computed on sub-grids equal the full LBL at those channels
-    bt_spot = spot(profs, geoms, tsf, [1.0] * len(profs), ctx, line_mixing="vpy")
+    bt_spot = spot(profs, geoms, tsf, [1.0] * len(profs), ctx, line_mixing="vpy", apply_continuum=False)
     assert np.max(np.abs(bt_spot - bt_lbl[:, spot.channels])) < 1e-4
     r_all = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=0.0, spot=spot, ctx=ctx)
     assert r_all.fell_back.all() and np.array_equal(r_all.BT, bt_lbl)

@@ -89,3 +89,37 @@ def test_forward_windows_variant_bt(ctx, lo, hi):
         d = np.abs(BT - BTo)
         print(f"windows {lo}-{hi} lm={lm}: BT {BTo.min():.1f}-{BTo.max():.1f} K, max|d| {d.max():.2e} K")
         assert d.max() < BT_TOL
+
+
+@pytest.mark.parametrize("lo,hi,lm", [(1205.0, 1213.0, None), (778.0, 784.0, "vpy")])
+def test_fp64_every_pair_forward_matches_oracle(ctx, lo, hi, lm):
+    """The every-pair FP64 line sum (the reference kernel's arithmetic on the
+    device, fp64_every_pair schedule) against the FP64 oracle: the two compute
+    the same sums in the same precision, so BT agrees to ~1e-8 K (the bar is
+    1e-6 K), and the FP32 engine lies within its own 0.005 K bar of both."""
+    from irsounderlbl.api import forward_model
+    from irsounderlbl.types import ViewingGeometry
+    lls = syn.linelists_config1(windows=True)
+    prof = syn.base_profile(True)
+    snd = Sounder.make(lo, hi, 0.25, opd_max=2.0, fwhm_gauss=0.5)
+    sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls.items()}
+    mu, eps = 0.8, 0.97
+    geom = ViewingGeometry(float(np.degrees(np.arccos(mu))), 0.0, mu)
+    kw = dict(sounder=snd, geom=geom, eps_sfc=eps, line_mixing=lm, ctx=ctx)
+    try:
+        ctx.set_schedule(4, fp64_every_pair=True)
+        _, R64, BT64 = forward_model(prof, sub, apply_continuum=False, **kw)
+        st = ctx.stats()
+        _, _, BT64c = forward_model(prof, sub, **kw)                    # with continuum
+    finally:
+        ctx.set_schedule(4)
+    BT32 = forward_model(prof, sub, apply_continuum=False, **kw)[2]
+    _, BTo, _ = refcpu.forward(prof, sub, snd, line_mixing=1 if lm else 0, mu=mu, eps_sfc=eps)
+    _, BToc, _ = refcpu.forward(prof, sub, snd, line_mixing=1 if lm else 0, mu=mu, eps_sfc=eps, continua=31)
+    d64 = np.abs(BT64 - BTo).max()
+    print(f"FP64 every pair {lo}-{hi}: max|dBT| vs oracle {d64:.2e} K (with continuum "
+          f"{np.abs(BT64c - BToc).max():.2e}), FP32 vs FP64 {np.abs(BT32 - BT64).max():.2e} K, "
+          f"line sum {st['ms_linesum']:.1f} ms")
+    assert d64 < 1e-6
+    assert np.abs(BT64c - BToc).max() < 1e-6
+    assert np.abs(BT32 - BT64).max() < BT_TOL
