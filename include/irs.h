@@ -270,6 +270,27 @@ int irs_xsec_layer(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t
                    double cutoff, double lm_cutoff, double x_far, int line_mixing,
                    double *sigma);
 
+/* Size of the fixed device scratch array irs_xsec_layer_noalloc works in: one
+ * call needs about 172 bytes per line of the list plus 4 bytes per grid point
+ * (each part rounded up to 256 bytes), e.g. a 300 000-line list on the
+ * 2 147 001-point full IASI grid. */
+#define IRS_NOALLOC_SCRATCH_BYTES (64ll << 20)
+
+/* irs_xsec_layer without any device allocation: the same arguments, kernels
+ * and result (bit-identical sigma), with every device buffer taken from a
+ * fixed IRS_NOALLOC_SCRATCH_BYTES array of the library that each GPU holds
+ * until the process ends.  Returns IRS_EINVAL, naming the bytes needed, when
+ * the list and grid do not fit.  The list must have been uploaded before (the
+ * upload allocates).  All calls on one GPU share that GPU's array: calls that
+ * overlap in time on one GPU (several threads, several contexts) race and must
+ * be serialised by the caller.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+int irs_xsec_layer_noalloc(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t n_nu,
+                           double T, double p_atm, double vmr_self, double coef, double dptmn,
+                           double cutoff, double lm_cutoff, double x_far, int line_mixing,
+                           double *sigma);
+
 /* Schwarzschild RTE on a host tau [n_nu x n_layers]; T_ave required for CIM. */
 int irs_rte(irs_ctx *ctx, double nu0, double dnu, int64_t n_nu, const double *tau, int n_layers,
             const double *T_lev, double T_sfc, double mu, double eps_sfc, int source_function,

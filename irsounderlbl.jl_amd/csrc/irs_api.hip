@@ -29,6 +29,14 @@
 hipError_t launch_keys(const double *nu, int64_t n, double nu_lo, double dnu, int32_t *keys,
                        hipStream_t s);
 
+// Device scratch of irs_xsec_layer_noalloc: every device buffer of one
+// cross-section evaluation (list descriptor, work item, counters, keys, line
+// records, pedestal sums, the FP32 output) laid out in this fixed array, so
+// that the call allocates nothing.  Each GPU holds its own copy for the life of
+// the process; host code uses the address hipGetSymbolAddress returns for the
+// current device (irs_ctx::scratch), never the array's name.
+static __device__ __align__(256) unsigned char scratch[IRS_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 struct DevBuf {
@@ -119,6 +127,7 @@ struct irs_ctx {
     int cia_nb[3] = {0, 0, 0};
     int64_t cia_nk[3] = {0, 0, 0};
     int cia_gen = 0;
+    unsigned char *scratch = nullptr;  // this device's copy of the no-allocation scratch array
     PinBuf h_items, h_prof, h_out;
     std::vector<GraphEntry> graphs;
 };
@@ -833,6 +842,11 @@ int irs_create(int device, irs_ctx **out) {
             return IRS_ENOMEM;
         }
     }
+    // the device address of the no-allocation scratch array on this device
+    if (hipGetSymbolAddress((void **)&ctx->scratch, HIP_SYMBOL(scratch)) != hipSuccess || !ctx->scratch) {
+        irs_destroy(ctx);
+        return IRS_EHIP;
+    }
     if (const char *e = std::getenv("IRS_LS_P")) {
         int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) ctx->sched.tile_points = v;
@@ -1050,6 +1064,131 @@ int irs_xsec_layer(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t
     ctx->stats.n_layers = 1;
     ctx->stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return IRS_OK;
+}
+
+int irs_xsec_layer_noalloc(irs_ctx *ctx, int32_t handle, double nu0, double dnu, int64_t n_nu, double T, double p_atm,
+                           double vmr_self, double coef, double dptmn, double cutoff, double lm_cutoff, double x_far,
+                           int line_mixing, double *sigma) {
+    if (!ctx) return IRS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    int rc = check_list(ctx, handle);
+    if (rc) return rc;
+    if (!sigma || n_nu <= 0 || !(dnu > 0.0)) return fail(ctx, IRS_EINVAL, "bad grid");
+    if (!(T > 0.0) || !(p_atm >= 0.0)) return fail(ctx, IRS_EINVAL, "bad T or p");
+    if (!(cutoff > 0.0) || !(lm_cutoff > 0.0)) return fail(ctx, IRS_EINVAL, "cutoff must be > 0");
+    if (line_mixing != IRS_LM_NONE && line_mixing != IRS_LM_VPY) return fail(ctx, IRS_EINVAL, "bad line_mixing");
+    if (!(coef > 0.0)) return fail(ctx, IRS_EINVAL, "coef must be > 0");
+    if (!ctx->scratch) return fail(ctx, IRS_EHIP, "no scratch array on this device");
+    const ListHost &lh = ctx->lists[handle];
+    const int64_t n = lh.n;
+    // scratch layout, every part 256-byte aligned
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_list = 0, o_item = o_list + al(sizeof(LineListDev)), o_cnt = o_item + al(sizeof(ItemDev));
+    const size_t o_queue = o_cnt + al(sizeof(unsigned long long) * IRS_NCNT), o_keys = o_queue + 256;
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    const size_t o_r0 = o_keys + al(sizeof(int32_t) * nn), o_r1 = o_r0 + al(sizeof(float4) * nn);
+    const size_t o_r2 = o_r1 + al(sizeof(float4) * nn), o_r3 = o_r2 + al(sizeof(float4) * nn);
+    const size_t o_pre = o_r3 + al(sizeof(float4) * 6 * nn), o_pblk = o_pre + al(sizeof(double) * 3 * nn);
+    const size_t o_tau = o_pblk + al(sizeof(double) * 3 * ((nn + 255) / 256));
+    const size_t need = o_tau + al(sizeof(float) * (size_t)n_nu);
+    if (need > (size_t)IRS_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, IRS_EINVAL, "irs_xsec_layer_noalloc: " + std::to_string(n) + " lines and " +
+                                         std::to_string(n_nu) + " grid points need " + std::to_string(need) +
+                                         " bytes of scratch, more than its " +
+                                         std::to_string((long long)IRS_NOALLOC_SCRATCH_BYTES) + "; use irs_xsec_layer");
+    unsigned char *S = ctx->scratch;
+    hipStream_t st = ctx->stream;
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    // the same item as irs_xsec_layer (records carry coef x 2^60)
+    const bool lm = line_mixing == IRS_LM_VPY && lh.has_lm;
+    const double vs = lm ? 0.0 : vmr_self;
+    const double scale = 1152921504606846976.0;   // 2^60
+    ItemDev it = make_item(lh, 0, T, p_atm, vs, coef * scale, dptmn, cutoff, lm_cutoff, x_far, lm, dnu);
+    it.dptmn = dptmn * scale;
+    it.key_off = 0;
+    it.ktab_off = 0;
+    it.rec_off = 0;
+    it.pb_off = 0;
+    HIPCHK(hipMemcpyAsync(S + o_list, &lh.dev, sizeof(LineListDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S + o_item, &it, sizeof(ItemDev), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_reset_counters((unsigned *)(S + o_queue), (unsigned long long *)(S + o_cnt), IRS_NCNT, st));
+    HIPCHK(launch_keys(lh.dev.nu, n, nu0, dnu, (int32_t *)(S + o_keys), st));
+    PrepArgs pa{};
+    pa.items = (const ItemDev *)(S + o_item);
+    pa.lists = (const LineListDev *)(S + o_list);
+    pa.tips_q = (const double *)ctx->d_tips.p;
+    pa.n_items = 1;
+    pa.nu_lo = nu0;
+    pa.dnu = dnu;
+    pa.rec0 = (float4 *)(S + o_r0);
+    pa.rec1 = (float4 *)(S + o_r1);
+    pa.rec2 = (float4 *)(S + o_r2);
+    pa.rec3 = (float4 *)(S + o_r3);
+    pa.counters = (unsigned long long *)(S + o_cnt);
+    pa.n_hi = n_nu;
+    pa.pre = (double *)(S + o_pre);
+    pa.pblk = (double *)(S + o_pblk);
+    pa.n_rec = (int64_t)nn;
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(launch_prep(pa, (int)n, st));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    LinesumArgs la{};
+    la.items = pa.items;
+    la.rec0 = pa.rec0;
+    la.rec1 = pa.rec1;
+    la.rec2 = pa.rec2;
+    la.rec3 = pa.rec3;
+    la.keys = (const int32_t *)(S + o_keys);
+    la.ktab = nullptr;                     // window bounds by binary search in the keys
+    la.out = (float *)(S + o_tau);
+    la.n_hi = n_nu;
+    la.n_layers = 1;
+    la.n_species = 1;
+    la.nu_lo = nu0;
+    la.dnu = dnu;
+    la.n_prof = 1;
+    const int P = ctx->sched.tile_points;
+    la.tiles = (int)((n_nu + (int64_t)IRS_LS_THREADS * P - 1) / ((int64_t)IRS_LS_THREADS * P));
+    la.tile_order = nullptr;
+    la.queue = (unsigned *)(S + o_queue);
+    far_params(ctx, dnu, P, la.far_d, la.far_hc);
+    la.work = pa.counters + 3;
+    la.pre = pa.pre;
+    la.n_rec = pa.n_rec;
+    HIPCHK(launch_linesum(la, P, ctx->sched.persistent ? linesum_persistent_capacity(P) : 0, st));
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    // FP32 output into the first half of the caller's FP64 buffer, widened in
+    // place from the end (element i's double covers floats 2i and 2i+1, both >= i)
+    HIPCHK(hipMemcpyAsync(sigma, S + o_tau, sizeof(float) * (size_t)n_nu, hipMemcpyDeviceToHost, st));
+    unsigned long long cs[IRS_NCNT];
+    HIPCHK(hipMemcpyAsync(cs, S + o_cnt, sizeof(cs), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double inv = 1.0 / (coef * scale);
+    unsigned char *bytes = (unsigned char *)sigma;
+    for (int64_t i = n_nu - 1; i >= 0; --i) {
+        float v;
+        std::memcpy(&v, bytes + 4 * i, sizeof(v));
+        const double d = (double)v * inv;
+        std::memcpy(bytes + 8 * i, &d, sizeof(d));
+    }
+    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < IRS_NCNT; ++i) c[i % 8] += cs[i];
+    ctx->stats.active_lines = (int64_t)c[0];
+    ctx->stats.active_pairs = (int64_t)c[1];
+    ctx->stats.dispersive_pairs = (int64_t)c[2];
+    ctx->stats.expanded_line_slices = (int64_t)c[3];
+    ctx->stats.fine_pairs = (int64_t)c[4];
+    ctx->stats.coarse_pairs = (int64_t)c[5];
+    ctx->stats.fine_pairs_lm = (int64_t)c[6];
+    ctx->stats.coarse_pairs_lm = (int64_t)c[7];
+    ctx->stats.ms_prep = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.ms_linesum = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+    ctx->stats.n_hi = n_nu;
+    ctx->stats.n_layers = 1;
+    ctx->stats.linesum_launches = 1;
+    ctx->stats.tile_points = P;
+    ctx->stats.persistent = ctx->sched.persistent;
     return IRS_OK;
 }
 

@@ -103,7 +103,7 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_mlp_forward", "irs_mlp_loss_grad", "irs_mlp_set_data", "irs_mlp_train",
            "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
            "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian", "irs_cia_set",
-           "irs_cia_load_file", "irs_continuum_layer")
+           "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc")
 
 _lib = None
 
@@ -130,6 +130,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_linelist_free.argtypes = [P, C.c_int32]
     L.irs_xsec_layer.argtypes = [P, C.c_int32, f64, f64, i64, f64, f64, f64, f64, f64, f64, f64,
                                  f64, i32, P]
+    L.irs_xsec_layer_noalloc.argtypes = L.irs_xsec_layer.argtypes
     L.irs_rte.argtypes = [P, f64, f64, i64, P, i32, P, f64, f64, f64, i32, P, P]
     L.irs_faddeeva32.argtypes = [P, P, P, i64, i32, P, P]
     L.irs_ils_half.argtypes = [f64, C.POINTER(i64)]
