@@ -218,6 +218,15 @@ int         irs_get_schedule(const irs_ctx *ctx, irs_schedule *schedule);
  * / T^ch_{k+1} (layer-alone form under an opaque column), clamped to [0, 1]. */
 int irs_channel_od(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
                    const irs_fm_options *opt, double floor, float *tau_ch, float *cim_g);
+/* irs_channel_od with cim_g and the normalised ILS first moment of each layer's
+ * transmittance step (no counterpart in the reference; equally spaced node
+ * channels only): ils_mom[k] = ILS (x) [(nu - nu_c)(T_{k+1} - T_k)] / T^ch_{k+1}
+ * where T^ch_{k+1} >= floor, else 0, so that Planck's slope across the ILS enters
+ * the channel RTE: B(T, nu) (T_{k+1} - T_k) -> B(T, nu_c) (T^ch_{k+1} - T^ch_k)
+ * + dB/dnu(T, nu_c) ils_mom[k] T^ch_{k+1}. */
+int irs_channel_od_moments(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                           const irs_fm_options *opt, double floor, float *tau_ch, float *cim_g,
+                           float *ils_mom);
 /* The emulator's target (no counterpart in the reference): ILS-convolved
  * level-to-space transmittances along each profile's path T_lev
  * [n_prof][n_layers+1][n_channels] (index n_layers = top, 1) and the CIM sums
@@ -366,34 +375,38 @@ int         irs_mlp_train(irs_mlp *mlp, int steps, int batch, float lr, uint64_t
 /* J[n][n_out][n_in] = dy/dx (forward-mode tangents) */
 int         irs_mlp_input_jacobian(irs_mlp *mlp, const float *x, int64_t n, float *J);
 
-/* Emulated forward model: two targets per layer and channel decoded as
- * x_k(c) = t_mean[k][c] + inv_w[k][c] sum_m (y_m * coef_scale[m]) basis[m][c] (inv_w
- * [n_layers][2 n_channels], NULL: 1; first target
- * on columns 0..n_ch-1, second on n_ch..2 n_ch-1) with
- * y = MLP((features - x_mean) / x_std), x_mean and x_std [n_layers][n_in] (per layer):
- *   kind = 2: the layer channel transmittance t_k = exp(-tau_ch_k) and the CIM
- *             weight g_k of irs_channel_od (T_k = t_k T_{k+1}, G_k = g_k T_{k+1});
- *   kind = 3: the level transmittance T_k and the CIM sum G_k of irs_channel_transmittance;
- * then the channel RTE
- *   R = eps B(T_sfc) T_0 + sum_k B(T_ave,k)(T_{k+1} - T_k) + (B(T_{k+1}) - B(T_ave,k)) G_k
- * (CIM source only; for eps < 1 the reflected term is approximate).  The
- * n_tpc argument of irs_emu_create is this kind. */
+/* Emulated forward model.  Per layer k and channel c three targets, the layer
+ * channel transmittance t_k, the CIM weight g_k and the normalised ILS first
+ * moment m_k of irs_channel_od_moments, decoded in the layer's own basis:
+ *   y_k(c) = mean[k][c] + inv_w[k][c] sum_m a_{k,m} basis[k][m][c]
+ *   (mean, inv_w [n_layers][3 n_channels], basis [n_layers][n_pc][3 n_channels];
+ *   t on columns 0..n_ch-1, g on n_ch..2 n_ch-1, m on 2 n_ch..3 n_ch-1), with
+ *   a_k = A_k phi(z_k) + coef_scale MLP(z_k[0 .. n_in-1]),
+ *   z_k = (features_k - x_mean_k) / x_std_k  (n_feat raw features per layer,
+ *        x_mean and x_std [n_layers][n_feat]),
+ *   phi = [1, z_i (i < n_local), z_a z_b (a <= b < n_local), z_i (n_local <= i < n_feat)],
+ *   A [n_layers][n_phi][n_pc], n_phi = 1 + n_local + n_local (n_local + 1) / 2 + n_feat - n_local;
+ * then the channel RTE with Planck linear in nu across the ILS:
+ *   R = eps [B(T_sfc) T_0 + B'(T_sfc) M_s] + sum_k T_{k+1} [B(T_ave,k)(1 - t_k)
+ *       + (B(T_{k+1}) - B(T_ave,k)) g_k + B'(T_ave,k) m_k],  M_s = -sum_k m_k T_{k+1},
+ *   B' = dB/dnu at the channel centre, T_L = 1, T_k = t_k T_{k+1}
+ * (CIM source only; for eps < 1 the reflected term is approximate).  n_feat <= 24. */
 typedef struct irs_emu irs_emu;
-int         irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_tpc,
-                           const float *x_mean, const float *x_std, const float *coef_scale,
-                           const float *basis, const float *t_mean, const float *inv_w,
+int         irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_feat, int n_local,
+                           const float *x_mean, const float *x_std, const float *A, float coef_scale,
+                           const float *basis, const float *mean, const float *inv_w,
                            const double *ch_nu, irs_emu **out);
 void        irs_emu_destroy(irs_emu *emu);
 const char *irs_emu_last_error(const irs_emu *emu);
-/* features[n_prof][L][n_in] (raw), T_lev[n_prof][L+1], T_ave[n_prof][L], scal[n_prof][3] =
+/* features[n_prof][L][n_feat] (raw), T_lev[n_prof][L+1], T_ave[n_prof][L], scal[n_prof][3] =
  * {T_sfc, mu, eps}; R, BT [n_prof][n_channels]; t_out[n_prof][L][n_channels] (decoded T_k) optional */
 int         irs_emu_forward(irs_emu *emu, const float *features, const double *T_lev, const double *T_ave,
                             const double *scal, int n_prof, int source_function, double *R, double *BT,
                             float *t_out);
 
-/* Emulator Jacobians (chain rule through the MLP input Jacobian, the decode and
- * the adjoint of the channel RTE; kind-2 emulators, eps = 1 exact):
- * BT [n_prof][n_ch]; dfeat [n_prof][L][n_in][n_ch] = dBT/d(raw feature);
+/* Emulator Jacobians (chain rule through the MLP input Jacobian, the regression,
+ * the decode and the adjoint of the channel RTE; the reflected term of eps < 1 held fixed):
+ * BT [n_prof][n_ch]; dfeat [n_prof][L][n_feat][n_ch] = dBT/d(raw feature);
  * dTave [n_prof][L][n_ch], dTlev [n_prof][L+1][n_ch], dTsfc [n_prof][n_ch]: the
  * Planck terms of dBT/dT. */
 int         irs_emu_jacobian(irs_emu *emu, const float *features, const double *T_lev,

@@ -83,6 +83,8 @@ struct GroupPlan {
     int mode;                      // 0 radiance + BT, 1 channel optical depths (2: + CIM weights),
                                    // 3 level channel transmittances + CIM sums
     const void *chod, *chlev, *chg;
+    const void *chrows, *chsums;   // banded channel-OD scratch of one profile (exact-node channels)
+    const void *chmom, *chmsums;   // ILS first moments [gn][L][nch] and their tap sums + moment taps
     double chod_floor;
     int continua;                  // IRS_CONT_* added to tau after the line sum (0: none)
     int cia_gen;                   // CIA table generation (graphs capture the table pointers)
@@ -121,6 +123,8 @@ struct irs_ctx {
     std::map<int, int> capacity;                          // P -> resident persistent workgroups
     DevBuf d_queue, d_prof, d_chod, d_chlev, d_chg, d_pre;
     DevBuf d_pblk;                     // pedestal scan block sums [sum_items ceil(n/256)][3]
+    DevBuf d_chrows, d_chsums;         // channel optical depths on exact-node channels: per-point rows, tap sums
+    DevBuf d_chmom, d_chmsums;         // ILS first moments of the level transmittance steps
     DevBuf d_rec64, d_tau64;           // every-pair FP64 path: line records, FP64 optical depths
     DevBuf d_ckd;                      // MT-CKD tables: H2O [2001][4], then CO2 [5003][2] = {S*xfac, tdep}
     DevBuf d_cia[3];                   // per CIA pair: T, numin, numax [nb], off [nb+1], nu, sig [knots]
@@ -878,6 +882,10 @@ void irs_destroy(irs_ctx *ctx) {
     ctx->d_pre.release();
     ctx->d_chlev.release();
     ctx->d_chg.release();
+    ctx->d_chrows.release();
+    ctx->d_chsums.release();
+    ctx->d_chmom.release();
+    ctx->d_chmsums.release();
     ctx->d_ckd.release();
     ctx->d_pblk.release();
     ctx->d_rec64.release();
@@ -1409,7 +1417,12 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
             ca.g_out = (float *)pl.chg;
             ca.raw_G = pl.mode == 3;
         }
-        ENQ(launch_chod(ca, pl.gn, st));
+        ca.m_out = (float *)pl.chmom;
+        if (pl.chrows)
+            ENQ(launch_chod_banded(ca, pl.gn, pl.ch_idx0, (int)pl.ch_stride, (float *)pl.chrows, (double *)pl.chsums,
+                                   (double *)pl.chmsums, st));
+        else
+            ENQ(launch_chod(ca, pl.gn, st));
         if (timing) {
             ENQ(hipEventRecord(ctx->ev[3], st));
             ENQ(hipEventRecord(ctx->ev[4], st));
@@ -1422,8 +1435,11 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
             return hipSuccess;
         }
         ENQ(hipMemcpyAsync((void *)pl.h_out, pl.chod, sizeof(float) * pl.nch * L * NL, hipMemcpyDeviceToHost, st));
-        if (pl.mode == 2)
+        if (pl.mode == 2 || pl.mode == 4)
             ENQ(hipMemcpyAsync((float *)pl.h_out + pl.nch * L * NL, pl.chg, sizeof(float) * pl.nch * L * NL,
+                               hipMemcpyDeviceToHost, st));
+        if (pl.mode == 4)
+            ENQ(hipMemcpyAsync((float *)pl.h_out + 2 * pl.nch * L * NL, pl.chmom, sizeof(float) * pl.nch * L * NL,
                                hipMemcpyDeviceToHost, st));
         return hipSuccess;
     }
@@ -1713,11 +1729,11 @@ static int run_profile_fp64(irs_ctx *ctx, const ItemDev *items_h, int L, int ns,
 
 static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
                        const irs_fm_options *opt_in, int mode, double chod_floor, double *R, double *BT,
-                       float *tau_ch, float *cim_g) {
+                       float *tau_ch, float *cim_g, float *ils_mom = nullptr) {
     if (!ctx) return IRS_EINVAL;
     (void)hipSetDevice(ctx->device);
     if (!profiles || n_prof <= 0 || !snd || (mode == 0 && (!R || !BT)) || (mode >= 1 && !tau_ch) ||
-        (mode >= 2 && !cim_g))
+        (mode >= 2 && !cim_g) || (mode == 4 && !ils_mom))
         return fail(ctx, IRS_EINVAL, "null argument");
     irs_fm_options opt;
     if (opt_in) opt = *opt_in; else irs_default_options(&opt);
@@ -1839,7 +1855,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     HIPCHK(ensure_pinned(ctx->h_items, sizeof(ItemDev) * (size_t)group * L * ns));
     HIPCHK(ensure_pinned(ctx->h_prof, sizeof(double) * prof_doubles));
     HIPCHK(ensure_pinned(ctx->h_out, std::max(sizeof(double) * 2 * nch * group,
-                                              sizeof(float) * (2 * L + 1) * nch * group)));
+                                              sizeof(float) * (3 * L + 1) * nch * group)));
     if (mode >= 1) HIPCHK(ensure(ctx->d_chod, sizeof(float) * nch * L * group));
     if (mode >= 2) {
         HIPCHK(ensure(ctx->d_chlev, sizeof(float) * nch * (L + 1) * group));
@@ -1957,6 +1973,21 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         pl.chlev = mode >= 2 ? ctx->d_chlev.p : nullptr;
         pl.chg = mode >= 2 ? ctx->d_chg.p : nullptr;
         pl.chod_floor = chod_floor;
+        if (mode >= 1 && pl.ch_stride > 0) {
+            const int64_t nr = chod_banded_rows(L, mode >= 2);
+            HIPCHK(ensure(ctx->d_chrows, sizeof(float) * (size_t)nr * n));
+            HIPCHK(ensure(ctx->d_chsums, sizeof(double) * (size_t)nr * nch));
+            pl.chrows = ctx->d_chrows.p;
+            pl.chsums = ctx->d_chsums.p;
+            if (mode == 4) {
+                HIPCHK(ensure(ctx->d_chmom, sizeof(float) * (size_t)nch * L * group));
+                HIPCHK(ensure(ctx->d_chmsums, sizeof(double) * ((size_t)(L + 1) * nch + 2 * half + 1)));
+                pl.chmom = ctx->d_chmom.p;
+                pl.chmsums = ctx->d_chmsums.p;
+            }
+        } else if (mode == 4) {
+            return fail(ctx, IRS_EINVAL, "ILS moments need equally spaced channels on internal grid nodes");
+        }
         pl.continua = opt.continua;
         pl.cia_gen = opt.continua ? ctx->cia_gen : 0;
         if (ctx->sched.use_graph) {
@@ -1982,8 +2013,11 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                         sizeof(float) * nch * L * gn);
         } else if (mode >= 1) {
             std::memcpy(tau_ch + (size_t)g0 * L * nch, ctx->h_out.p, sizeof(float) * nch * L * gn);
-            if (mode == 2)
+            if (mode == 2 || mode == 4)
                 std::memcpy(cim_g + (size_t)g0 * L * nch, (const float *)ctx->h_out.p + (size_t)nch * L * gn,
+                            sizeof(float) * nch * L * gn);
+            if (mode == 4)
+                std::memcpy(ils_mom + (size_t)g0 * L * nch, (const float *)ctx->h_out.p + (size_t)2 * nch * L * gn,
                             sizeof(float) * nch * L * gn);
         } else {
             const double *hout = (const double *)ctx->h_out.p;
@@ -2018,6 +2052,12 @@ int irs_channel_od(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const 
                    const irs_fm_options *opt, double floor, float *tau_ch, float *cim_g) {
     if (ctx && !(floor > 0.0 && floor < 1.0)) return fail(ctx, IRS_EINVAL, "floor must be in (0, 1)");
     return run_forward(ctx, profiles, n_prof, snd, opt, cim_g ? 2 : 1, floor, nullptr, nullptr, tau_ch, cim_g);
+}
+
+int irs_channel_od_moments(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
+                           const irs_fm_options *opt, double floor, float *tau_ch, float *cim_g, float *ils_mom) {
+    if (ctx && !(floor > 0.0 && floor < 1.0)) return fail(ctx, IRS_EINVAL, "floor must be in (0, 1)");
+    return run_forward(ctx, profiles, n_prof, snd, opt, 4, floor, nullptr, nullptr, tau_ch, cim_g, ils_mom);
 }
 
 int irs_layer_properties(const irs_profile *pr, int T_method, double *p_mid, double *dp, double *T_ave,

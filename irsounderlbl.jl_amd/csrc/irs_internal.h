@@ -165,6 +165,8 @@ struct ChodArgs {
     const float *lev_in;           // = lev_out, read by the CIM pass
     float *g_out;                  // optional [n_prof][n_layers][n_ch] CIM channel weight
     int raw_G;                     // 1: store G_k unnormalised
+    float *m_out;                  // optional [n_prof][n_layers][n_ch] normalised ILS first moment
+                                   // ILS (x) [(nu - nu_c)(T_{k+1} - T_k)] / T^ch_{k+1} (exact-node channels only)
 };
 
 // launchers (k_*.hip)
@@ -176,6 +178,12 @@ hipError_t launch_linesum(const LinesumArgs &a, int P, int persistent_blocks, hi
 hipError_t launch_reset_counters(unsigned *queue, unsigned long long *counters, int n_counters, hipStream_t s);
 int linesum_persistent_capacity(int P);
 hipError_t launch_chod(const ChodArgs &a, int n_prof, hipStream_t s);
+// exact-node channels idx_c = idx0 + c stride: rows [chod_banded_rows][n] FP32 and
+// sums [chod_banded_rows][n_ch] FP64 scratch of one profile
+int64_t chod_banded_rows(int n_layers, bool cim);
+// msums (needed with m_out): [(n_layers + 1) n_ch + 2 half + 1] FP64 scratch
+hipError_t launch_chod_banded(const ChodArgs &a, int n_prof, int64_t idx0, int stride, float *rows, double *sums,
+                              double *msums, hipStream_t s);
 hipError_t launch_rte(const RteArgs &a, hipStream_t s);
 hipError_t launch_ils_taps(const double *A, const double *L, int n_pts, double dL, double dnu,
                            int64_t half, double *taps, hipStream_t s);

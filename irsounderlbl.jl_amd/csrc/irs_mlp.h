@@ -31,38 +31,34 @@ hipError_t launch_gelu_tangent(const float *Z, const float *dZ, int64_t n, uint1
 hipError_t launch_sum_partials(const float *partial, int n, float scale, float *out, hipStream_t s);
 hipError_t launch_broadcast_row(const float *row, int B, int N, float *out, hipStream_t s);
 
-// Emulated forward model (k_emu.hip): decode of the level channel
-// transmittances and CIM sums from PCA coefficients fused with the channel RTE.
+// Emulated forward model (k_emu.hip): decode of the three per-layer targets
+// (t_k, g_k, m_k) from per-layer PCA coefficients fused with the channel RTE.
 struct EmuRteArgs {
-    const float *coef;           // [n_prof][L][n_pc]   (MLP output)
-    const float *coef_scale;     // [n_pc]: coefficient = MLP output * coef_scale
-    const float *basis;          // [n_pc][2 n_ch]: two targets per layer and channel
-    const float *t_mean;         // [L][2 n_ch]
-    const float *inv_w;          // [L][2 n_ch] decode weights (null: 1)
-    int target_kind;             // 0: (t_k, g_k) ratios, 1: (T_k, G_k) levels
+    const float *coef;           // [n_prof][L][n_pc] PCA coefficients a_{k,m}
+    const float *basis;          // [L][n_pc][3 n_ch]: the layer's basis over (t, g, m)
+    const float *mean;           // [L][3 n_ch]
+    const float *inv_w;          // [L][3 n_ch] decode weights
     const double *ch_nu;         // [n_ch]
     const double *T_lev, *T_ave; // [n_prof][L+1], [n_prof][L]
     const double *scal;          // [n_prof][3] = {T_sfc, mu, eps}
-    int n_prof, L, n_pc, n_ch, source_function;
+    int n_prof, L, n_pc, n_ch;
     double *R, *BT;              // [n_prof][n_ch]
     float *t_out;                // optional [n_prof][L][n_ch] decoded level transmittances T_k
-    // Jacobian outputs (optional): dBT/dt_k [n_prof][L][n_ch] and dBT/dT_lev, dBT/dT_ave source terms
-    double *dbt_dt;              // [n_prof][L][n_ch]
-    double *dbt_dtlev;           // [n_prof][L+1][n_ch]
-    double *dbt_dtave;           // [n_prof][L][n_ch]
 };
 
+hipError_t launch_emu_coef(const float *y, const float *phi, const float *A, float scale, int64_t r0, int64_t rows,
+                           int L, int n_phi, int n_pc, float *coef, hipStream_t s);
 hipError_t launch_emu_rte(const EmuRteArgs &a, hipStream_t s);
 
 // Emulator Jacobian outputs (all multiplied by dBT/dR)
 struct EmuAdjArgs {
     double *BT;                  // [n_prof][n_ch]
-    float *dt, *dg;              // dBT/dt_k, dBT/dg_k          [n_prof][L][n_ch]
+    float *dt, *dg, *dm;         // dBT/dt_k, dBT/dg_k, dBT/dm_k [n_prof][L][n_ch]
     float *dTave;                // dBT/dT_ave,k (Planck)       [n_prof][L][n_ch]
     float *dTlev;                // dBT/dT_lev,j (Planck)       [n_prof][L+1][n_ch]
     float *dTsfc;                // dBT/dT_sfc                  [n_prof][n_ch]
-    const float *Jmlp;           // dy/dx of the MLP            [n_prof][L][n_pc][n_in]
-    float *dfeat;                // dBT/dfeature                [n_prof][L][n_in][n_ch]
+    const float *Jcoef;          // d a_{k,m} / d feature_i     [n_prof][L][n_pc][n_feat]
+    float *dfeat;                // dBT/dfeature                [n_prof][L][n_feat][n_ch]
 };
 
-hipError_t launch_emu_adjoint(const EmuRteArgs &a, const EmuAdjArgs &j, int n_in, hipStream_t s);
+hipError_t launch_emu_adjoint(const EmuRteArgs &a, const EmuAdjArgs &j, int n_feat, hipStream_t s);

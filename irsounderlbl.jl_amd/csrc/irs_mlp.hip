@@ -44,17 +44,19 @@ struct irs_mlp {
 
 struct irs_emu {
     irs_mlp *mlp = nullptr;
-    int L = 0, n_pc = 0, n_ch = 0, n_in = 0, n_tpc = 1;
-    std::vector<float> x_mean, x_std;
-    float *coef_scale = nullptr;     // [n_pc]
-    float *basis = nullptr;          // [n_pc][n_ch]
-    float *t_mean = nullptr;         // [L][n_ch]
-    float *inv_w = nullptr;          // [L][2 n_ch] or null
-    double *ch_nu = nullptr;         // [n_ch]
-    float *coef = nullptr;           // [cap rows][n_pc]
-    double *prof = nullptr;          // T_lev, T_ave, scal
-    double *R = nullptr, *BT = nullptr, *jac = nullptr;
-    int64_t cap_rows = 0, cap_prof = 0, cap_jac = 0;
+    int L = 0, n_pc = 0, n_ch = 0, n_feat = 0, n_local = 0, n_phi = 0;
+    std::vector<float> x_mean, x_std;  // [L][n_feat]
+    std::vector<float> A_host;         // [L][n_phi][n_pc] regression of the coefficients on phi(z)
+    float coef_scale = 1.0f;           // a = A phi + coef_scale MLP(z)
+    float *A = nullptr;                // device copy of A_host
+    float *basis = nullptr;            // [L][n_pc][3 n_ch]
+    float *mean = nullptr;             // [L][3 n_ch]
+    float *inv_w = nullptr;            // [L][3 n_ch]
+    double *ch_nu = nullptr;           // [n_ch]
+    float *coef = nullptr, *phi = nullptr;
+    double *prof = nullptr;            // T_lev, T_ave, scal
+    double *R = nullptr, *BT = nullptr;
+    int64_t cap_rows = 0, cap_phi = 0, cap_prof = 0, cap_out = 0;
     std::string err;
 };
 
@@ -423,13 +425,14 @@ int irs_mlp_input_jacobian(irs_mlp *m, const float *x, int64_t n, float *J) {
 }
 
 // ---- emulator ----------------------------------------------------------------
-int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_tpc, const float *x_mean,
-                   const float *x_std, const float *coef_scale, const float *basis, const float *t_mean,
-                   const float *inv_w, const double *ch_nu, irs_emu **out) {
+int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_feat, int n_local,
+                   const float *x_mean, const float *x_std, const float *A, float coef_scale, const float *basis,
+                   const float *mean, const float *inv_w, const double *ch_nu, irs_emu **out) {
     if (!out) return IRS_EINVAL;
     *out = nullptr;
-    if (!mlp || !x_mean || !x_std || !coef_scale || !basis || !t_mean || !ch_nu || n_layers < 1 || n_pc < 1 ||
-        n_channels < 1 || n_pc != mlp->n_out || (n_tpc != 2 && n_tpc != 3))
+    if (!mlp || !x_mean || !x_std || !A || !basis || !mean || !inv_w || !ch_nu || n_layers < 1 || n_pc < 1 ||
+        n_channels < 1 || n_pc != mlp->n_out || n_local < 0 || n_local > n_feat || n_feat < mlp->n_in ||
+        n_feat > 24 || !(coef_scale > 0.0f))
         return IRS_EINVAL;
     (void)hipSetDevice(mlp->device);
     irs_emu *e = new irs_emu();
@@ -437,20 +440,23 @@ int irs_emu_create(irs_mlp *mlp, int n_layers, int n_pc, int n_channels, int n_t
     e->L = n_layers;
     e->n_pc = n_pc;
     e->n_ch = n_channels;
-    e->n_in = mlp->n_in;
-    e->n_tpc = n_tpc;
-    const size_t W = (size_t)2 * n_channels;
-    e->x_mean.assign(x_mean, x_mean + (size_t)n_layers * mlp->n_in);
-    e->x_std.assign(x_std, x_std + (size_t)n_layers * mlp->n_in);
-    bool ok = dalloc(&e->coef_scale, n_pc) == hipSuccess && dalloc(&e->basis, (size_t)n_pc * W) == hipSuccess &&
-              dalloc(&e->t_mean, (size_t)n_layers * W) == hipSuccess &&
-              dalloc(&e->ch_nu, n_channels) == hipSuccess &&
-              (!inv_w || dalloc(&e->inv_w, (size_t)n_layers * W) == hipSuccess);
-    ok = ok && hipMemcpy(e->coef_scale, coef_scale, sizeof(float) * n_pc, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(e->basis, basis, sizeof(float) * (size_t)n_pc * W, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(e->t_mean, t_mean, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(e->ch_nu, ch_nu, sizeof(double) * n_channels, hipMemcpyHostToDevice) == hipSuccess &&
-         (!inv_w || hipMemcpy(e->inv_w, inv_w, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess);
+    e->n_feat = n_feat;
+    e->n_local = n_local;
+    e->n_phi = 1 + n_local + n_local * (n_local + 1) / 2 + (n_feat - n_local);
+    e->coef_scale = coef_scale;
+    const size_t W = (size_t)3 * n_channels, nA = (size_t)n_layers * e->n_phi * n_pc;
+    e->x_mean.assign(x_mean, x_mean + (size_t)n_layers * n_feat);
+    e->x_std.assign(x_std, x_std + (size_t)n_layers * n_feat);
+    e->A_host.assign(A, A + nA);
+    bool ok = dalloc(&e->A, nA) == hipSuccess && dalloc(&e->basis, (size_t)n_layers * n_pc * W) == hipSuccess &&
+              dalloc(&e->mean, (size_t)n_layers * W) == hipSuccess &&
+              dalloc(&e->inv_w, (size_t)n_layers * W) == hipSuccess && dalloc(&e->ch_nu, n_channels) == hipSuccess;
+    ok = ok && hipMemcpy(e->A, A, sizeof(float) * nA, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->basis, basis, sizeof(float) * (size_t)n_layers * n_pc * W, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->mean, mean, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->inv_w, inv_w, sizeof(float) * (size_t)n_layers * W, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(e->ch_nu, ch_nu, sizeof(double) * n_channels, hipMemcpyHostToDevice) == hipSuccess;
+    for (size_t i = 0; ok && i < e->x_std.size(); ++i) ok = e->x_std[i] > 0.0f;
     if (!ok) {
         irs_emu_destroy(e);
         return IRS_ENOMEM;
@@ -463,7 +469,7 @@ void irs_emu_destroy(irs_emu *e) {
     if (!e) return;
     (void)hipSetDevice(e->mlp->device);
     (void)hipStreamSynchronize(e->mlp->stream);
-    void *ptrs[] = {e->coef_scale, e->basis, e->t_mean, e->ch_nu, e->coef, e->prof, e->R, e->BT, e->jac, e->inv_w};
+    void *ptrs[] = {e->A, e->basis, e->mean, e->inv_w, e->ch_nu, e->coef, e->phi, e->prof, e->R, e->BT};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete e;
@@ -495,21 +501,50 @@ hipError_t grow(T **p, int64_t &cap, int64_t need) {
     return r;
 }
 
-// MLP on standardized features -> coef [rows][n_pc] on the device (physical units)
-int emu_coefficients(irs_emu *e, const float *features, int64_t rows) {
+// Standardised features z [rows][n_feat] and the regression terms phi(z) [rows][n_phi]:
+// 1, z_i (i < n_local), z_a z_b (a <= b < n_local), z_i (n_local <= i < n_feat).
+void emu_inputs(const irs_emu *e, const float *features, int64_t rows, std::vector<float> &z, std::vector<float> &phi) {
+    const int nf = e->n_feat, nl = e->n_local;
+    z.resize((size_t)rows * nf);
+    phi.resize((size_t)rows * e->n_phi);
+    for (int64_t r = 0; r < rows; ++r) {
+        const int l = (int)(r % e->L);
+        float *zr = z.data() + r * nf, *pr = phi.data() + r * e->n_phi;
+        for (int i = 0; i < nf; ++i)
+            zr[i] = (features[r * nf + i] - e->x_mean[(size_t)l * nf + i]) / e->x_std[(size_t)l * nf + i];
+        int j = 0;
+        pr[j++] = 1.0f;
+        for (int i = 0; i < nl; ++i) pr[j++] = zr[i];
+        for (int a = 0; a < nl; ++a)
+            for (int b = a; b < nl; ++b) pr[j++] = zr[a] * zr[b];
+        for (int i = nl; i < nf; ++i) pr[j++] = zr[i];
+    }
+}
+
+// coefficients a [rows][n_pc] = A phi + coef_scale MLP(z[:, :n_in]) on the device
+int emu_coefficients(irs_emu *e, const float *features, int64_t rows, std::vector<float> *z_out = nullptr) {
     irs_mlp *m = e->mlp;
     int64_t cap = e->cap_rows;
     ECHK(grow(&e->coef, cap, rows * e->n_pc));
     e->cap_rows = cap;
-    std::vector<float> xs((size_t)rows * e->n_in);
-    for (int64_t r = 0; r < rows; ++r)
-        for (int i = 0; i < e->n_in; ++i)
-            xs[r * e->n_in + i] = (features[r * e->n_in + i] - e->x_mean[(r % e->L) * e->n_in + i]) /
-                                  e->x_std[(r % e->L) * e->n_in + i];
+    cap = e->cap_phi;
+    ECHK(grow(&e->phi, cap, rows * e->n_phi));
+    e->cap_phi = cap;
+    std::vector<float> z, phi;
+    emu_inputs(e, features, rows, z, phi);
+    std::vector<float> zin;
+    const float *zm = z.data();
+    if (m->n_in != e->n_feat) {
+        zin.resize((size_t)rows * m->n_in);
+        for (int64_t r = 0; r < rows; ++r)
+            for (int i = 0; i < m->n_in; ++i) zin[r * m->n_in + i] = z[r * e->n_feat + i];
+        zm = zin.data();
+    }
+    ECHK(hipMemcpyAsync(e->phi, phi.data(), sizeof(float) * phi.size(), hipMemcpyHostToDevice, m->stream));
     for (int64_t r0 = 0; r0 < rows; r0 += m->max_batch) {
         const int64_t nr = std::min<int64_t>(m->max_batch, rows - r0);
         const int B = (int)((nr + 63) / 64 * 64);
-        int rc = stage_inputs(m, xs.data() + r0 * e->n_in, nr, B);
+        int rc = stage_inputs(m, zm + r0 * m->n_in, nr, B);
         if (rc) {
             e->err = m->err;
             return rc;
@@ -519,21 +554,54 @@ int emu_coefficients(irs_emu *e, const float *features, int64_t rows) {
             e->err = m->err;
             return rc;
         }
-        ECHK(hipMemcpyAsync(e->coef + r0 * e->n_pc, m->Y, sizeof(float) * nr * e->n_pc, hipMemcpyDeviceToDevice,
-                            m->stream));
+        ECHK(launch_emu_coef(m->Y, e->phi, e->A, e->coef_scale, r0, nr, e->L, e->n_phi, e->n_pc, e->coef, m->stream));
     }
-    // synchronous here: xs is pageable host memory freed on return
+    // synchronous here: z and phi are pageable host memory freed on return
     ECHK(hipStreamSynchronize(m->stream));
+    if (z_out) z_out->swap(z);
     return IRS_OK;
+}
+
+int emu_upload_profiles(irs_emu *e, const double *T_lev, const double *T_ave, const double *scal, int n_prof) {
+    irs_mlp *m = e->mlp;
+    const int L = e->L;
+    const int64_t pd = (int64_t)n_prof * (2 * L + 1 + 3);
+    int64_t cp = e->cap_prof;
+    ECHK(grow(&e->prof, cp, pd));
+    e->cap_prof = cp;
+    double *dp = e->prof;
+    ECHK(hipMemcpyAsync(dp, T_lev, sizeof(double) * n_prof * (L + 1), hipMemcpyHostToDevice, m->stream));
+    ECHK(hipMemcpyAsync(dp + (size_t)n_prof * (L + 1), T_ave, sizeof(double) * n_prof * L, hipMemcpyHostToDevice,
+                        m->stream));
+    ECHK(hipMemcpyAsync(dp + (size_t)n_prof * (2 * L + 1), scal, sizeof(double) * n_prof * 3, hipMemcpyHostToDevice,
+                        m->stream));
+    return IRS_OK;
+}
+
+EmuRteArgs emu_args(const irs_emu *e, int n_prof) {
+    EmuRteArgs a{};
+    a.coef = e->coef;
+    a.basis = e->basis;
+    a.mean = e->mean;
+    a.inv_w = e->inv_w;
+    a.ch_nu = e->ch_nu;
+    a.T_lev = e->prof;
+    a.T_ave = e->prof + (size_t)n_prof * (e->L + 1);
+    a.scal = e->prof + (size_t)n_prof * (2 * e->L + 1);
+    a.n_prof = n_prof;
+    a.L = e->L;
+    a.n_pc = e->n_pc;
+    a.n_ch = e->n_ch;
+    return a;
 }
 
 }  // namespace
 
 extern "C" {
 
-// features [n_prof][L][n_in] (raw), T_lev [n_prof][L+1], T_ave [n_prof][L],
+// features [n_prof][L][n_feat] (raw), T_lev [n_prof][L+1], T_ave [n_prof][L],
 // scal [n_prof][3] = {T_sfc, mu, eps}; R, BT [n_prof][n_ch].  Optional
-// t_out [n_prof][L][n_ch] (decoded layer channel transmittances at nadir).
+// t_out [n_prof][L][n_ch] (decoded level transmittances along the path).
 int irs_emu_forward(irs_emu *e, const float *features, const double *T_lev, const double *T_ave,
                     const double *scal, int n_prof, int source_function, double *R, double *BT, float *t_out) {
     if (!e || !features || !T_lev || !T_ave || !scal || !R || !BT || n_prof < 1) return IRS_EINVAL;
@@ -546,49 +614,38 @@ int irs_emu_forward(irs_emu *e, const float *features, const double *T_lev, cons
     const int L = e->L;
     int rc = emu_coefficients(e, features, (int64_t)n_prof * L);
     if (rc) return rc;
-    const int64_t pd = (int64_t)n_prof * (2 * L + 1 + 3);
-    int64_t cp = e->cap_prof;
-    ECHK(grow(&e->prof, cp, pd));
-    e->cap_prof = cp;
-    double *dp = e->prof;
-    ECHK(hipMemcpyAsync(dp, T_lev, sizeof(double) * n_prof * (L + 1), hipMemcpyHostToDevice, m->stream));
-    ECHK(hipMemcpyAsync(dp + (size_t)n_prof * (L + 1), T_ave, sizeof(double) * n_prof * L, hipMemcpyHostToDevice,
-                        m->stream));
-    ECHK(hipMemcpyAsync(dp + (size_t)n_prof * (2 * L + 1), scal, sizeof(double) * n_prof * 3, hipMemcpyHostToDevice,
-                        m->stream));
-    if (e->R) (void)hipFree(e->R);
-    if (e->BT) (void)hipFree(e->BT);
-    e->R = e->BT = nullptr;
-    ECHK(dalloc(&e->R, (size_t)n_prof * e->n_ch));
-    ECHK(dalloc(&e->BT, (size_t)n_prof * e->n_ch));
+    rc = emu_upload_profiles(e, T_lev, T_ave, scal, n_prof);
+    if (rc) return rc;
+    int64_t co = e->cap_out;
+    if (co < (int64_t)n_prof * e->n_ch) {
+        if (e->R) (void)hipFree(e->R);
+        if (e->BT) (void)hipFree(e->BT);
+        e->R = e->BT = nullptr;
+        e->cap_out = 0;
+        ECHK(dalloc(&e->R, (size_t)n_prof * e->n_ch));
+        ECHK(dalloc(&e->BT, (size_t)n_prof * e->n_ch));
+        e->cap_out = (int64_t)n_prof * e->n_ch;
+    }
     float *tdev = nullptr;
     if (t_out) ECHK(dalloc(&tdev, (size_t)n_prof * L * e->n_ch));
-    EmuRteArgs a{};
-    a.coef = e->coef;
-    a.basis = e->basis;
-    a.t_mean = e->t_mean;
-    a.inv_w = e->inv_w;
-    a.ch_nu = e->ch_nu;
-    a.T_lev = dp;
-    a.T_ave = dp + (size_t)n_prof * (L + 1);
-    a.scal = dp + (size_t)n_prof * (2 * L + 1);
-    a.n_prof = n_prof;
-    a.L = L;
-    a.n_pc = e->n_pc;
-    a.n_ch = e->n_ch;
-    a.source_function = source_function;
+    EmuRteArgs a = emu_args(e, n_prof);
     a.R = e->R;
     a.BT = e->BT;
     a.t_out = tdev;
-    a.coef_scale = e->coef_scale;
-    a.target_kind = e->n_tpc == 3 ? 1 : 0;
-    ECHK(launch_emu_rte(a, m->stream));
-    ECHK(hipMemcpyAsync(R, e->R, sizeof(double) * n_prof * e->n_ch, hipMemcpyDeviceToHost, m->stream));
-    ECHK(hipMemcpyAsync(BT, e->BT, sizeof(double) * n_prof * e->n_ch, hipMemcpyDeviceToHost, m->stream));
-    if (t_out)
-        ECHK(hipMemcpyAsync(t_out, tdev, sizeof(float) * n_prof * L * e->n_ch, hipMemcpyDeviceToHost, m->stream));
-    ECHK(hipStreamSynchronize(m->stream));
+    hipError_t er = launch_emu_rte(a, m->stream);
+    if (er == hipSuccess)
+        er = hipMemcpyAsync(R, e->R, sizeof(double) * n_prof * e->n_ch, hipMemcpyDeviceToHost, m->stream);
+    if (er == hipSuccess)
+        er = hipMemcpyAsync(BT, e->BT, sizeof(double) * n_prof * e->n_ch, hipMemcpyDeviceToHost, m->stream);
+    if (er == hipSuccess && t_out)
+        er = hipMemcpyAsync(t_out, tdev, sizeof(float) * n_prof * L * e->n_ch, hipMemcpyDeviceToHost, m->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(m->stream);
+    (void)hipStreamSynchronize(m->stream);
     if (tdev) (void)hipFree(tdev);
+    if (er != hipSuccess) {
+        e->err = std::string("emulated forward model: ") + hipGetErrorString(er);
+        return IRS_EHIP;
+    }
     return IRS_OK;
 }
 
@@ -597,39 +654,55 @@ int irs_emu_forward(irs_emu *e, const float *features, const double *T_lev, cons
 extern "C" {
 
 // Emulator Jacobians for n_prof scenes: BT [n_prof][n_ch]; dBT/dfeature
-// [n_prof][L][n_in][n_ch] (raw features, through the MLP input Jacobian and the
-// decode); the Planck terms dBT/dT_ave [n_prof][L][n_ch], dBT/dT_lev
-// [n_prof][L+1][n_ch], dBT/dT_sfc [n_prof][n_ch].
+// [n_prof][L][n_feat][n_ch] (raw features, through the MLP input Jacobian, the
+// regression and the decode); the Planck terms dBT/dT_ave [n_prof][L][n_ch],
+// dBT/dT_lev [n_prof][L+1][n_ch], dBT/dT_sfc [n_prof][n_ch].
 int irs_emu_jacobian(irs_emu *e, const float *features, const double *T_lev, const double *T_ave,
                      const double *scal, int n_prof, double *BT, float *dfeat, float *dTave, float *dTlev,
                      float *dTsfc) {
     if (!e || !features || !T_lev || !T_ave || !scal || n_prof < 1 || !BT || !dfeat || !dTave || !dTlev || !dTsfc)
         return IRS_EINVAL;
-    if (e->n_tpc != 2) {
-        e->err = "emulator Jacobians are implemented for the (t_k, g_k) targets";
-        return IRS_EINVAL;
-    }
     irs_mlp *m = e->mlp;
     (void)hipSetDevice(m->device);
-    const int L = e->L, nin = e->n_in, npc = e->n_pc, nch = e->n_ch;
+    const int L = e->L, nf = e->n_feat, nl = e->n_local, nin = m->n_in, npc = e->n_pc, nch = e->n_ch;
+    const int nphi = e->n_phi, nq = nl * (nl + 1) / 2;
     const int64_t rows = (int64_t)n_prof * L;
-    int rc = emu_coefficients(e, features, rows);
+    std::vector<float> z;
+    int rc = emu_coefficients(e, features, rows, &z);
     if (rc) return rc;
-    // MLP input Jacobian on standardized inputs, then per raw feature (1/std)
-    std::vector<float> xs((size_t)rows * nin);
+    // MLP input Jacobian on the standardized inputs
+    std::vector<float> zin((size_t)rows * nin);
     for (int64_t r = 0; r < rows; ++r)
-        for (int i = 0; i < nin; ++i)
-            xs[r * nin + i] = (features[r * nin + i] - e->x_mean[(r % L) * nin + i]) / e->x_std[(r % L) * nin + i];
-    std::vector<float> J((size_t)rows * npc * nin);
-    rc = irs_mlp_input_jacobian(m, xs.data(), rows, J.data());
+        for (int i = 0; i < nin; ++i) zin[r * nin + i] = z[r * nf + i];
+    std::vector<float> Jm((size_t)rows * npc * nin);
+    rc = irs_mlp_input_jacobian(m, zin.data(), rows, Jm.data());
     if (rc) {
         e->err = m->err;
         return rc;
     }
-    for (int64_t r = 0; r < rows; ++r)
-        for (int o = 0; o < npc; ++o)
-            for (int i = 0; i < nin; ++i) J[((size_t)r * npc + o) * nin + i] /= e->x_std[(r % L) * nin + i];
-    // device buffers
+    // d a / d feature = (coef_scale dMLP/dz + A dphi/dz) / x_std
+    std::vector<float> J((size_t)rows * npc * nf, 0.0f);
+    for (int64_t r = 0; r < rows; ++r) {
+        const int l = (int)(r % L);
+        const float *zr = z.data() + r * nf;
+        const float *Al = e->A_host.data() + (size_t)l * nphi * npc;
+        for (int o = 0; o < npc; ++o) {
+            float *Jr = J.data() + ((size_t)r * npc + o) * nf;
+            for (int i = 0; i < nin; ++i) Jr[i] = e->coef_scale * Jm[((size_t)r * npc + o) * nin + i];
+            for (int i = 0; i < nl; ++i) Jr[i] += Al[(size_t)(1 + i) * npc + o];
+            int jj = 1 + nl;
+            for (int a = 0; a < nl; ++a)
+                for (int b = a; b < nl; ++b, ++jj) {
+                    const float w = Al[(size_t)jj * npc + o];
+                    Jr[a] += w * zr[b];
+                    Jr[b] += w * zr[a];
+                }
+            for (int i = nl; i < nf; ++i) Jr[i] += Al[(size_t)(1 + nl + nq + (i - nl)) * npc + o];
+            for (int i = 0; i < nf; ++i) Jr[i] /= e->x_std[(size_t)l * nf + i];
+        }
+    }
+    rc = emu_upload_profiles(e, T_lev, T_ave, scal, n_prof);
+    if (rc) return rc;
     std::vector<void *> bufs;
     auto get = [&](size_t bytes) -> void * {
         void *p = nullptr;
@@ -637,50 +710,28 @@ int irs_emu_jacobian(irs_emu *e, const float *features, const double *T_lev, con
         bufs.push_back(p);
         return p;
     };
-    double *dprof = (double *)get(sizeof(double) * n_prof * (2 * L + 1 + 3));
     float *dJ = (float *)get(sizeof(float) * J.size());
     EmuAdjArgs j{};
     j.BT = (double *)get(sizeof(double) * n_prof * nch);
     j.dt = (float *)get(sizeof(float) * rows * nch);
     j.dg = (float *)get(sizeof(float) * rows * nch);
+    j.dm = (float *)get(sizeof(float) * rows * nch);
     j.dTave = (float *)get(sizeof(float) * rows * nch);
     j.dTlev = (float *)get(sizeof(float) * (rows + n_prof) * nch);
     j.dTsfc = (float *)get(sizeof(float) * n_prof * nch);
-    j.dfeat = (float *)get(sizeof(float) * rows * nin * nch);
-    j.Jmlp = dJ;
+    j.dfeat = (float *)get(sizeof(float) * rows * nf * nch);
+    j.Jcoef = dJ;
     bool ok = true;
     for (void *p : bufs) ok = ok && p;
     hipError_t er = hipSuccess;
     if (ok) {
-        er = hipMemcpyAsync(dprof, T_lev, sizeof(double) * n_prof * (L + 1), hipMemcpyHostToDevice, m->stream);
-        if (er == hipSuccess)
-            er = hipMemcpyAsync(dprof + (size_t)n_prof * (L + 1), T_ave, sizeof(double) * n_prof * L,
-                                hipMemcpyHostToDevice, m->stream);
-        if (er == hipSuccess)
-            er = hipMemcpyAsync(dprof + (size_t)n_prof * (2 * L + 1), scal, sizeof(double) * n_prof * 3,
-                                hipMemcpyHostToDevice, m->stream);
-        if (er == hipSuccess)
-            er = hipMemcpyAsync(dJ, J.data(), sizeof(float) * J.size(), hipMemcpyHostToDevice, m->stream);
-        EmuRteArgs a{};
-        a.coef = e->coef;
-        a.coef_scale = e->coef_scale;
-        a.basis = e->basis;
-        a.t_mean = e->t_mean;
-        a.inv_w = e->inv_w;
-        a.ch_nu = e->ch_nu;
-        a.T_lev = dprof;
-        a.T_ave = dprof + (size_t)n_prof * (L + 1);
-        a.scal = dprof + (size_t)n_prof * (2 * L + 1);
-        a.n_prof = n_prof;
-        a.L = L;
-        a.n_pc = npc;
-        a.n_ch = nch;
-        a.source_function = IRS_SRC_CIM;
-        if (er == hipSuccess) er = launch_emu_adjoint(a, j, nin, m->stream);
+        er = hipMemcpyAsync(dJ, J.data(), sizeof(float) * J.size(), hipMemcpyHostToDevice, m->stream);
+        EmuRteArgs a = emu_args(e, n_prof);
+        if (er == hipSuccess) er = launch_emu_adjoint(a, j, nf, m->stream);
         if (er == hipSuccess)
             er = hipMemcpyAsync(BT, j.BT, sizeof(double) * n_prof * nch, hipMemcpyDeviceToHost, m->stream);
         if (er == hipSuccess)
-            er = hipMemcpyAsync(dfeat, j.dfeat, sizeof(float) * rows * nin * nch, hipMemcpyDeviceToHost, m->stream);
+            er = hipMemcpyAsync(dfeat, j.dfeat, sizeof(float) * rows * nf * nch, hipMemcpyDeviceToHost, m->stream);
         if (er == hipSuccess)
             er = hipMemcpyAsync(dTave, j.dTave, sizeof(float) * rows * nch, hipMemcpyDeviceToHost, m->stream);
         if (er == hipSuccess)

@@ -103,7 +103,8 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_mlp_forward", "irs_mlp_loss_grad", "irs_mlp_set_data", "irs_mlp_train",
            "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
            "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian", "irs_cia_set",
-           "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc")
+           "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc",
+           "irs_channel_od_moments")
 
 _lib = None
 
@@ -145,6 +146,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_set_schedule.argtypes = [P, C.POINTER(IrsSchedule)]
     L.irs_channel_od.argtypes = [P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder), C.POINTER(IrsOptions),
                                  f64, P, P]
+    L.irs_channel_od_moments.argtypes = L.irs_channel_od.argtypes + [P]
     L.irs_get_schedule.argtypes = [P, C.POINTER(IrsSchedule)]
     L.irs_jacobian.argtypes = [P, C.POINTER(IrsProfile), C.POINTER(IrsSounder), C.POINTER(IrsOptions),
                                C.POINTER(IrsStateSpec), P, P, P]
@@ -166,7 +168,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_mlp_set_data.argtypes = [P, P, P, i64]
     L.irs_mlp_train.argtypes = [P, i32, i32, C.c_float, C.c_uint64, P]
     L.irs_mlp_input_jacobian.argtypes = [P, P, i64, P]
-    L.irs_emu_create.argtypes = [P, i32, i32, i32, i32, P, P, P, P, P, P, P, C.POINTER(P)]
+    L.irs_emu_create.argtypes = [P, i32, i32, i32, i32, i32, P, P, P, C.c_float, P, P, P, P, C.POINTER(P)]
     L.irs_emu_destroy.argtypes = [P]
     L.irs_emu_destroy.restype = None
     L.irs_emu_last_error.argtypes = [P]
@@ -683,14 +685,16 @@ def channel_optical_depths(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
                            cutoff: float = 25.0, lm_cutoff: Optional[float] = None, dptmn: float = 1e-6,
                            with_ils: bool = True, apodization="gaussian", line_mixing=None,
                            T_method="logp_at_pcg", x_far: float = 122.0, cim: bool = False,
-                           apply_continuum: bool = True, continua=DEFAULT_CONTINUA,
+                           moments: bool = False, apply_continuum: bool = True, continua=DEFAULT_CONTINUA,
                            ctx: Optional[Context] = None):
     """Per-layer channel optical depths along each profile's viewing path
     (geom, default nadir), [n_prof, n_layers, n_channels] float32
     (irs_channel_od): tau_k = ln(T_{k+1}/T_k) of the ILS-convolved
     level-to-space transmittances T_k, or the layer-alone channel optical depth
     where T_{k+1} < floor.  The emulator's training target.  With cim=True also
-    returns the CIM channel weights g (same shape): (tau, g)."""
+    returns the CIM channel weights g (same shape): (tau, g); with moments=True
+    also the normalised ILS first moments m of each layer's transmittance step
+    (irs_channel_od_moments): (tau, g, m)."""
     inst = sounder if sounder is not None else IASIInstrument()
     batch = isinstance(prof, (list, tuple))
     profs = list(prof) if batch else [prof]
@@ -703,7 +707,12 @@ def channel_optical_depths(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     b = _Batch(ctx, profs, linelists, [None] * nb, [1.0] * nb, mus)
     grid = sounder_grid(inst)
     out = np.zeros((nb, profs[0].n_levels - 1, grid.n), np.float32)
-    g = np.zeros_like(out) if cim else None
+    g = np.zeros_like(out) if (cim or moments) else None
+    if moments:
+        m = np.zeros_like(out)
+        ctx.check(ctx.lib.irs_channel_od_moments(ctx.h, b.arr, nb, C.byref(_sounder(inst)), C.byref(opt),
+                                                 float(floor), _p(out), _p(g), _p(m)))
+        return (out, g, m) if batch else (out[0], g[0], m[0])
     ctx.check(ctx.lib.irs_channel_od(ctx.h, b.arr, nb, C.byref(_sounder(inst)), C.byref(opt),
                                      float(floor), _p(out), _p(g)))
     if cim:

@@ -1,24 +1,33 @@
 """Learned emulator of per-layer channel optical depths, with LBL spot check
 and fallback.  No counterpart in the reference.
 
-Pipeline (all numeric work on the GPU through libirs except the PCA, which is
-an FP64 host SVD):
-  1. training target: for every layer k of GPU line-by-line spectra along
-     each scene's viewing path (`channel_optical_depths`), the layer channel
-     transmittance t_k = exp(-tau_ch_k) (ratio of the ILS-convolved
-     level-to-space transmittances) and the CIM weight g_k, with which the
-     channel RTE reproduces the ILS-convolved LBL radiance of the scene (to
-     2e-3 K, the variation of Planck across the ILS);
-  2. PCA: [t_k, g_k] minus its per-layer training mean, projected on the n_pc
-     leading right singular vectors (randomized range finder + FP64 SVD);
-  3. MLP [ln p, T, ln q_H2O, ln q_O3, q_CO2 (ppm), 1/mu, and the overlying
-     column: ln H2O and ln O3 amounts above, T of the layer above]
-     -> 3 x 512 GELU -> n_pc,
-     trained with the BF16 MFMA kernels (FP32 accumulation) and Adam on the
-     coefficients (one global scale, so the loss is the transmittance MSE);
-  4. inference: MLP -> decode -> channel-space RTE (irs_emu_forward);
-  5. per-scene LBL spot check on blocks of channels; scenes whose largest
-     |BT_emulator - BT_LBL| exceeds the budget are recomputed with full LBL.
+Pipeline (numeric work on the GPU through libirs, except the PCA and the
+per-layer least-squares fits, which are FP64/FP32 host linear algebra):
+  1. training targets: for every layer k of GPU line-by-line spectra along each
+     scene's viewing path (irs_channel_od_moments) the layer channel transmittance
+     t_k = T_k / T_{k+1} of the ILS-convolved level-to-space transmittances, the
+     CIM weight g_k and the normalised ILS first moment m_k of the layer's
+     transmittance step; the channel RTE with Planck linear in nu across the ILS
+     reproduces the ILS-convolved LBL radiance from them (0.02 K at worst on the
+     IASI grid, against 0.14 K with Planck at the channel centre alone);
+  2. compression: per layer, the relevance-weighted deviations from the layer's
+     training mean (weight: the largest training transmittance above the layer
+     times the BT sensitivity to radiance, so the basis spends itself where the
+     top of the atmosphere sees the layer) projected on n_pc leading right
+     singular vectors of that layer (randomized range finder + FP64 SVD); the
+     bases come from the first scenes, later scenes are projected as their
+     targets are produced, so the training set is never held in memory whole;
+  3. model of the coefficients: a per-layer quadratic regression in the layer's
+     own 9 features plus a linear term in 8 profile-context features (leading
+     principal components of the standardised T, ln q_H2O, ln q_O3 profiles,
+     T_sfc and 1/mu), and an MLP 9 -> 3 x 512 GELU -> n_pc trained with the BF16
+     MFMA kernels (FP32 accumulation, Adam) on the regression's residual;
+  4. inference: features -> coefficients (regression + MLP) -> per-layer decode ->
+     channel RTE (irs_emu_forward), batched on the device;
+  5. calibration on held-out scenes (per-channel hold-out BT error, the spot
+     channels where it is largest, the spot-error threshold under which every
+     calibration scene is within the budget everywhere) and the per-scene LBL
+     spot check: scenes above the threshold are recomputed with full LBL.
 """
 from __future__ import annotations
 
@@ -32,11 +41,16 @@ import numpy as np
 from . import api
 from .api import (IRS_OK, Context, IRSError, _p, channel_optical_depths, forward_model,
                   layer_properties, load_library)
-from .types import GasSpecies, HITRANLinelist, Sounder, ViewingGeometry, filter_linelist, sounder_grid
+from .types import (AtmosphericProfile, GasSpecies, HITRANLinelist, Sounder, ViewingGeometry, filter_linelist,
+                    sounder_grid)
 
-N_FEATURES = 9
+N_LOCAL = 9
+N_CTX = 8
+N_FEATURES = N_LOCAL + N_CTX
 FEATURE_NAMES = ("ln p_mid [hPa]", "T_ave [K]", "ln vmr H2O", "ln vmr O3", "vmr CO2 [ppm]", "1/mu",
-                 "ln H2O column above [hPa]", "ln O3 column above [hPa]", "T_ave of the layer above [K]")
+                 "ln H2O column above [hPa]", "ln O3 column above [hPa]", "T_ave of the layer above [K]") + \
+                tuple(f"profile context {i}" for i in range(N_CTX))
+W_G, W_M = 0.1, 0.01          # decode weights of the CIM weight and the ILS moment relative to t_k
 
 
 class MLP:
@@ -134,13 +148,6 @@ def layer_features(prof, mu: float = 1.0, T_method: str = "logp_at_pcg") -> np.n
                      np.log(above(v(1)) + 1e-9), np.log(above(v(3)) + 1e-12), T_up], axis=1)
 
 
-def _scene(p):
-    """(profile, T_sfc, mu) from a profile or a (profile, T_sfc, mu) tuple."""
-    if isinstance(p, tuple):
-        return p[0], p[1], p[2]
-    return p, None, 1.0
-
-
 def _randomized_pca(dev: np.ndarray, k: int, oversample: int = 32, power: int = 2, seed: int = 0):
     """Leading k right singular vectors of dev [n, m] (float32 data): range
     finder with power iterations (Halko, Martinsson & Tropp 2011), FP64 SVD of
@@ -161,43 +168,136 @@ def _randomized_pca(dev: np.ndarray, k: int, oversample: int = 32, power: int = 
     return Vt[:k], S[:k]
 
 
+def _scene(p):
+    """(profile, T_sfc, mu) from a profile or a (profile, T_sfc, mu) tuple."""
+    if isinstance(p, tuple):
+        return p[0], p[1], p[2]
+    return p, None, 1.0
+
+
+def _geom(mu):
+    return ViewingGeometry(float(np.degrees(np.arccos(min(mu, 1.0)))), 0.0, mu)
+
+
+class ProfileContext:
+    """Profile-context features: the leading principal components of the
+    standardised profile vector [T levels, ln q_H2O levels, ln q_O3 levels, T_sfc,
+    1/mu] of the training scenes, each scaled to unit variance."""
+
+    def __init__(self, p_mean, p_std, proj):
+        self.p_mean = np.asarray(p_mean, np.float64)
+        self.p_std = np.asarray(p_std, np.float64)
+        self.proj = np.asarray(proj, np.float64)            # [n_ctx, n_vec], includes the 1/std of each PC
+
+    @staticmethod
+    def vector(prof, T_sfc, mu):
+        v = prof.vmr
+        nl = prof.n_levels
+        def lnq(sp):
+            return np.log(np.maximum(v[sp], 1e-30)) if sp in v else np.full(nl, np.log(1e-30))
+        ts = prof.temperature[0] if T_sfc is None else T_sfc
+        return np.concatenate([prof.temperature, lnq(GasSpecies.H2O), lnq(GasSpecies.O3), [ts, 1.0 / mu]])
+
+    @staticmethod
+    def fit(scenes, n_ctx: int = N_CTX):
+        P = np.stack([ProfileContext.vector(*s) for s in scenes])
+        pm, ps = P.mean(0), P.std(0)
+        ps = np.where(ps > 1e-12, ps, 1.0)
+        Z = (P - pm) / ps
+        _, S, Vt = np.linalg.svd(Z, full_matrices=False)
+        V = Vt[:n_ctx]
+        sd = (Z @ V.T).std(0)
+        sd = np.where(sd > 1e-12, sd, 1.0)
+        if len(V) < n_ctx:                                     # fewer scenes than context features
+            V = np.vstack([V, np.zeros((n_ctx - len(V), V.shape[1]))])
+            sd = np.concatenate([sd, np.ones(n_ctx - len(sd))])
+        return ProfileContext(pm, ps, V / sd[:, None] / ps[None, :])
+
+    def __call__(self, prof, T_sfc, mu):
+        return self.proj @ (self.vector(prof, T_sfc, mu) - self.p_mean)
+
+
+def emulator_features(scenes, context: ProfileContext, T_method: str = "logp_at_pcg") -> np.ndarray:
+    """Raw emulator inputs [N, L, N_FEATURES]: layer_features and the profile
+    context (the same for every layer of a scene)."""
+    out = []
+    for prof, T_sfc, mu in scenes:
+        lf = layer_features(prof, mu, T_method)
+        cx = context(prof, T_sfc, mu)
+        out.append(np.concatenate([lf, np.repeat(cx[None], lf.shape[0], 0)], axis=1))
+    return np.stack(out)
+
+
+def _phi(z: np.ndarray, n_local: int = N_LOCAL) -> np.ndarray:
+    """Regression terms [..., n_phi]: 1, z_i (i < n_local), z_a z_b (a <= b < n_local), the rest of z."""
+    zl = z[..., :n_local]
+    iu = np.triu_indices(n_local)
+    q = (zl[..., :, None] * zl[..., None, :])[..., iu[0], iu[1]]
+    return np.concatenate([np.ones(z.shape[:-1] + (1,)), zl, q, z[..., n_local:]], axis=-1)
+
+
+def _bt_sensitivity(nu, T=250.0):
+    """B / (dB/dT) at T (K per unit relative radiance), normalised to mean 1."""
+    x = 1.4387769 * nu / T
+    s = (T / x) * (1.0 - np.exp(-x))
+    return s / s.mean()
+
+
 @dataclass
 class TrainReport:
     n_profiles: int
     n_samples: int
+    n_basis_profiles: int
     seconds_lbl: float
     seconds_pca: float
+    seconds_fit: float
     seconds_train: float
-    pca_rms_t: float          # PCA truncation error of the relevance-weighted targets
-    pca_max_t: float
+    coef_rms_err_regression: float     # training-set coefficient error of the regression alone
+    coef_rms: float
     losses: np.ndarray = field(repr=False, default=None)
 
 
-class ChannelODEmulator:
-    """Trained emulator: MLP + PCA basis + normalisation, on the device."""
+@dataclass
+class Calibration:
+    """Hold-out statistics of an emulator against GPU LBL on calibration scenes."""
+    spot_channels: np.ndarray          # channel indices of the spot check
+    spot_blocks: list                  # [(first channel, n)] of consecutive spot channels
+    threshold: float                   # spot-error threshold (K) of the fallback decision
+    budget: float                      # BT budget (K) the threshold guarantees on the calibration scenes
+    ch_rms: np.ndarray                 # per-channel hold-out BT error, RMS over the scenes
+    ch_max: np.ndarray                 # per-channel largest hold-out error
+    scene_max: np.ndarray              # per-scene largest error over all channels
+    scene_spot: np.ndarray             # per-scene largest error over the spot channels
+    n_scenes: int
 
-    def __init__(self, mlp: MLP, x_mean, x_std, coef_scale, basis, t_mean, ch_nu, T_method="logp_at_pcg",
-                 n_tpc: int = 2, inv_w=None):
+
+class ChannelODEmulator:
+    """Trained emulator: regression + MLP + per-layer PCA bases, on the device."""
+
+    def __init__(self, mlp: MLP, x_mean, x_std, A, coef_scale, basis, mean, inv_w, ch_nu, context: ProfileContext,
+                 T_method="logp_at_pcg"):
         self.mlp = mlp
         self.lib = mlp.lib
         self.x_mean = np.ascontiguousarray(x_mean, np.float32)
         self.x_std = np.ascontiguousarray(x_std, np.float32)
-        self.coef_scale = np.ascontiguousarray(coef_scale, np.float32)
+        self.A = np.ascontiguousarray(A, np.float32)
+        self.coef_scale = float(coef_scale)
         self.basis = np.ascontiguousarray(basis, np.float32)
-        self.t_mean = np.ascontiguousarray(t_mean, np.float32)
+        self.mean = np.ascontiguousarray(mean, np.float32)
+        self.inv_w = np.ascontiguousarray(inv_w, np.float32)
         self.ch_nu = np.ascontiguousarray(ch_nu, np.float64)
-        self.inv_w = None if inv_w is None else np.ascontiguousarray(inv_w, np.float32)
-        self.n_tpc = int(n_tpc)            # target kind: 2 (t_k, g_k) ratios, 3 (T_k, G_k) levels
-        self.n_pc = self.basis.shape[0]
-        self.n_ch = self.basis.shape[1] // 2
+        self.context = context
+        self.L, self.n_pc, w3 = self.basis.shape
+        self.n_ch = w3 // 3
         if len(self.ch_nu) != self.n_ch:
             raise ValueError("basis width does not match the channels")
-        self.L = self.t_mean.shape[0]
         self.T_method = T_method
+        self.calibration: Optional[Calibration] = None
+        self._spot = None
         h = C.c_void_p()
-        rc = self.lib.irs_emu_create(mlp.h, self.L, self.n_pc, self.n_ch, self.n_tpc, _p(self.x_mean), _p(self.x_std),
-                                     _p(self.coef_scale), _p(self.basis), _p(self.t_mean), _p(self.inv_w),
-                                     _p(self.ch_nu), C.byref(h))
+        rc = self.lib.irs_emu_create(mlp.h, self.L, self.n_pc, self.n_ch, N_FEATURES, N_LOCAL, _p(self.x_mean),
+                                     _p(self.x_std), _p(self.A), self.coef_scale, _p(self.basis), _p(self.mean),
+                                     _p(self.inv_w), _p(self.ch_nu), C.byref(h))
         if rc != IRS_OK:
             raise IRSError(rc, f"irs_emu_create failed ({rc})")
         self.h = h
@@ -216,152 +316,240 @@ class ChannelODEmulator:
     # ---- training -----------------------------------------------------------
     @staticmethod
     def train(profiles: Sequence, linelists: Dict[GasSpecies, HITRANLinelist], sounder: Sounder, *,
-              n_pc: int = 256, steps: int = 3000, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
-              floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None, targets=None,
-              relevance_floor: Optional[float] = None, apply_continuum: bool = False, **kw):
+              n_pc: int = 64, n_basis: int = 2048, steps: int = 24000, batch: int = 4096, lr: float = 1e-3,
+              seed: int = 0, floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None,
+              relevance_floor: float = 1e-2, chunk: int = 256, apply_continuum: bool = False, progress=None,
+              T_method: str = "logp_at_pcg", **kw):
         """Train on scenes `profiles` (AtmosphericProfile or (profile, T_sfc, mu)
-        tuples); the targets along each scene's path come from the GPU LBL
-        engine unless `targets` = (tau_ch [N, L, n_ch], g [N, L, n_ch]) is given.
-        The LBL runs without continuum unless apply_continuum is True; the spot
-        check and fallback of forward_emulated must use the same setting."""
+        tuples) from GPU LBL targets along each scene's path.  The LBL runs
+        without continuum unless apply_continuum is True; the spot check and
+        fallback of forward_emulated must use the same setting."""
         scenes = [_scene(p) for p in profiles]
-        profs = [s[0] for s in scenes]
-        mus = [s[2] for s in scenes]
-        geoms = [ViewingGeometry(float(np.degrees(np.arccos(min(m, 1.0)))), 0.0, m) for m in mus]
+        N = len(scenes)
+        n_basis = min(n_basis, N)
+        sg = sounder_grid(sounder)
+        nch = sg.n
         t0 = time.perf_counter()
-        if targets is None:
-            targets = channel_optical_depths(profs, linelists, sounder=sounder, geom=geoms, floor=floor,
-                                             line_mixing=line_mixing, cim=True, ctx=ctx,
-                                             apply_continuum=apply_continuum, **kw)
-        tau_ch, g = targets
-        t1 = time.perf_counter()
-        N, L, nch = g.shape
-        # target weights W (PCA and loss run on W x deviation, the decode divides by
-        # W): the CIM weights enter the radiance through a Planck difference across
-        # half a layer, ~10x less than the transmittances, so they carry 0.1 W.
-        # Optional relevance weights (relevance_floor not None): a layer's channel
-        # values reach the top of the atmosphere through the transmittance above it,
-        # W[k, c] = max over scenes of T_{k+1}(c) (floor relevance_floor); scenes more
-        # transparent than every training scene then see errors amplified by up to
-        # 1/W (measured: without them the maximum held-out error is smaller).
-        if relevance_floor is None:
-            W = np.ones((L, nch), np.float32)
-        else:
-            above = np.zeros((L, nch))
-            for q in range(N):
-                cum = np.cumsum(tau_ch[q, ::-1].astype(np.float64), axis=0)[::-1]     # sum_{j>=k}
-                Ta = np.exp(-np.vstack([cum[1:], np.zeros((1, nch))]))               # T_{k+1}
-                np.maximum(above, Ta, out=above)
-            W = np.maximum(above, relevance_floor).astype(np.float32)
-        Wv = np.concatenate([W, 0.1 * W], axis=1)                                  # [L, 2 nch]
-        t = np.concatenate([np.exp(-tau_ch.astype(np.float32)), g.astype(np.float32)], axis=2)
-        del targets, tau_ch, g
-        t_mean = t.mean(axis=0)
-        t -= t_mean[None]
-        t *= Wv[None]
-        dev = t.reshape(N * L, 2 * nch)
-        V, _ = _randomized_pca(dev, n_pc, seed=seed)
-        Vf = np.zeros((n_pc, 2 * nch), np.float32)           # fewer columns than n_pc: zero rows
-        Vf[:len(V)] = V
-        coef = dev @ Vf.T                                           # [N L, n_pc]
-        rec = coef @ Vf
-        err = dev - rec                                             # relevance-weighted units
-        pca_rms, pca_max = float(np.sqrt(np.mean(err * err))), float(np.abs(err).max())
-        del err, rec
-        t2 = time.perf_counter()
-        feats = np.stack([layer_features(p, m) for p, m in zip(profs, mus)]).astype(np.float64)   # [N, L, F]
-        # per-layer standardisation keeps the bf16 inputs fine-grained (feature 0,
-        # ln p, is the same grid for every scene: global statistics keep the layer identity)
-        x_mean = feats.mean(axis=0)
-        x_std = feats.std(axis=0)
-        g_mean, g_std = feats.reshape(-1, feats.shape[2]).mean(axis=0), feats.reshape(-1, feats.shape[2]).std(axis=0)
+        t_lbl = 0.0
+
+        def targets(sc):
+            nonlocal t_lbl
+            ta = time.perf_counter()
+            od, g, m = channel_optical_depths([s[0] for s in sc], linelists, sounder=sounder,
+                                              geom=[_geom(s[2]) for s in sc], floor=floor, moments=True,
+                                              line_mixing=line_mixing, ctx=ctx, apply_continuum=apply_continuum,
+                                              T_method=T_method, **kw)
+            t_lbl += time.perf_counter() - ta
+            return np.concatenate([np.exp(-od), g, m], axis=2)       # [n, L, 3 nch] float32
+
+        Yb = []
+        for c0 in range(0, n_basis, chunk):
+            Yb.append(targets(scenes[c0:min(c0 + chunk, n_basis)]))
+            if progress:
+                progress(f"emulator targets {min(c0 + chunk, n_basis)}/{N}")
+        Yb = np.concatenate(Yb)
+        L = Yb.shape[1]
+        tp = time.perf_counter()
+        mean = Yb.mean(axis=0)                                        # [L, 3 nch]
+        top = np.ones((n_basis, nch), np.float32)
+        rel = np.zeros((L, nch), np.float32)
+        for k in range(L - 1, -1, -1):
+            rel[k] = top.max(axis=0)
+            top *= np.clip(Yb[:, k, :nch], 0.0, 1.0)
+        del top
+        wr = np.maximum(rel, relevance_floor) * _bt_sensitivity(sg.nu).astype(np.float32)[None]
+        WR = np.concatenate([wr, W_G * wr, W_M * wr], axis=1).astype(np.float32)     # [L, 3 nch]
+        basis = np.zeros((L, n_pc, 3 * nch), np.float32)
+        coef = np.zeros((N, L, n_pc), np.float32)
+        for l in range(L):
+            d = ((Yb[:, l] - mean[l]) * WR[l]).astype(np.float32)
+            V, _ = _randomized_pca(d, n_pc, seed=seed + l)
+            basis[l, :len(V)] = V
+            coef[:n_basis, l] = d @ basis[l].T
+        del Yb, d
+        t_pca = time.perf_counter() - tp
+        for c0 in range(n_basis, N, chunk):
+            sc = scenes[c0:c0 + chunk]
+            Y = targets(sc)
+            for l in range(L):
+                coef[c0:c0 + len(sc), l] = ((Y[:, l] - mean[l]) * WR[l]).astype(np.float32) @ basis[l].T
+            del Y
+            if progress:
+                progress(f"emulator targets {c0 + len(sc)}/{N}")
+        tf = time.perf_counter()
+        context = ProfileContext.fit(scenes)
+        X = emulator_features(scenes, context, T_method)                      # [N, L, F]
+        x_mean, x_std = X.mean(axis=0), X.std(axis=0)
+        g_mean, g_std = X.reshape(-1, N_FEATURES).mean(axis=0), X.reshape(-1, N_FEATURES).std(axis=0)
+        # feature 0 (ln p) is the same grid for every scene: global statistics keep the layer identity
         x_mean[:, 0], x_std[:, 0] = g_mean[0], max(g_std[0], 1e-6)
         x_std = np.where(x_std > 1e-3 * np.maximum(g_std, 1e-12)[None], x_std, np.maximum(g_std, 1e-6)[None])
-        feats_n = ((feats - x_mean[None]) / x_std[None]).reshape(N * L, -1)
-        scale = float(max(coef[:, 0].std(), 1e-12))
-        coef_scale = np.full(n_pc, scale, np.float32)
-        mlp = MLP(N_FEATURES, 512, n_pc, max(batch, 64), seed=seed)
-        mlp.set_data(feats_n.astype(np.float32), (coef / scale).astype(np.float32))
+        Z = (X - x_mean[None]) / x_std[None]
+        Phi = _phi(Z)
+        n_phi = Phi.shape[-1]
+        A = np.zeros((L, n_phi, n_pc))
+        res = np.empty_like(coef)
+        for l in range(L):
+            P = Phi[:, l]
+            G = P.T @ P
+            G += 1e-8 * np.trace(G) / n_phi * np.eye(n_phi)
+            A[l] = np.linalg.solve(G, P.T @ coef[:, l].astype(np.float64))
+            res[:, l] = coef[:, l] - (P @ A[l]).astype(np.float32)
+        reg_err = float(np.sqrt(np.mean(res.astype(np.float64) ** 2)))
+        coef_rms = float(np.sqrt(np.mean(coef.astype(np.float64) ** 2)))
+        del Phi
+        t_fit = time.perf_counter() - tf
+        tt = time.perf_counter()
+        scale = float(max(res.std(), 1e-12))
+        mlp = MLP(N_LOCAL, 512, n_pc, max(batch, 64), seed=seed)
+        mlp.set_data(Z[..., :N_LOCAL].reshape(-1, N_LOCAL).astype(np.float32),
+                     (res / scale).reshape(-1, n_pc).astype(np.float32))
         # Adam at lr for half the steps, then lr/3 and lr/10 for a quarter each
         n1, n2 = steps // 2, steps // 4
         losses = np.concatenate([mlp.train(n1, batch, lr, seed), mlp.train(n2, batch, lr / 3.0, seed + 1),
                                  mlp.train(steps - n1 - n2, batch, lr / 10.0, seed + 2)])
-        t3 = time.perf_counter()
-        sg = sounder_grid(sounder)
-        # the decode undoes the weights: target_k = mean_k + (sum_m a_{k,m} V_m) / W_k
-        emu = ChannelODEmulator(mlp, x_mean, x_std, coef_scale, Vf, t_mean, sg.nu, n_tpc=2, inv_w=1.0 / Wv)
-        rep = TrainReport(N, N * L, t1 - t0, t2 - t1, t3 - t2, pca_rms, pca_max, losses)
+        t_train = time.perf_counter() - tt
+        emu = ChannelODEmulator(mlp, x_mean, x_std, A.astype(np.float32), scale, basis, mean, 1.0 / WR, sg.nu,
+                                context, T_method)
+        rep = TrainReport(N, N * L, n_basis, t_lbl, t_pca, t_fit, t_train, reg_err, coef_rms, losses)
         return emu, rep
 
     # ---- persistence --------------------------------------------------------
     def save(self, path: str):
-        np.savez_compressed(path, params=self.mlp.get_params(), x_mean=self.x_mean, x_std=self.x_std,
-                            coef_scale=self.coef_scale, basis=self.basis.astype(np.float32),
-                            t_mean=self.t_mean, ch_nu=self.ch_nu,
-                            inv_w=self.inv_w if self.inv_w is not None else np.ones_like(self.t_mean),
-                            dims=np.array([self.mlp.n_in, self.mlp.n_hidden, self.mlp.n_out, self.n_tpc]))
+        cal = self.calibration
+        extra = {}
+        if cal is not None:
+            extra = dict(cal_spot=cal.spot_channels, cal_blocks=np.array(cal.spot_blocks, np.int64),
+                         cal_thr=np.array([cal.threshold, cal.budget, cal.n_scenes]), cal_ch_rms=cal.ch_rms,
+                         cal_ch_max=cal.ch_max, cal_scene_max=cal.scene_max, cal_scene_spot=cal.scene_spot)
+        np.savez_compressed(path, params=self.mlp.get_params(), x_mean=self.x_mean, x_std=self.x_std, A=self.A,
+                            coef_scale=np.array([self.coef_scale]), basis=self.basis, mean=self.mean,
+                            inv_w=self.inv_w, ch_nu=self.ch_nu, ctx_mean=self.context.p_mean,
+                            ctx_std=self.context.p_std, ctx_proj=self.context.proj,
+                            dims=np.array([self.mlp.n_in, self.mlp.n_hidden, self.mlp.n_out]), **extra)
 
     @staticmethod
     def load(path: str, max_batch: int = 4096):
         z = np.load(path, allow_pickle=False)
-        n_in, H, n_out, n_tpc = (int(v) for v in z["dims"])
+        n_in, H, n_out = (int(v) for v in z["dims"])
         mlp = MLP(n_in, H, n_out, max_batch)
         mlp.set_params(z["params"])
-        return ChannelODEmulator(mlp, z["x_mean"], z["x_std"], z["coef_scale"], z["basis"], z["t_mean"],
-                                 z["ch_nu"], n_tpc=n_tpc, inv_w=z["inv_w"])
-
-    # ---- Jacobians ----------------------------------------------------------
-    def jacobian(self, profs, *, geom=None, T_sfc=None, eps_sfc=1.0) -> List["EmulatorJacobian"]:
-        """Emulator Jacobians per scene (levels: T and ln q_H2O; T_sfc); the
-        reflected term (eps < 1) is held fixed."""
-        nb = len(profs)
-        geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
-        geoms = [g if g is not None else ViewingGeometry(0.0, 0.0, 1.0) for g in geoms]
-        tsf = T_sfc if isinstance(T_sfc, (list, tuple, np.ndarray)) else [T_sfc] * nb
-        eps = eps_sfc if isinstance(eps_sfc, (list, tuple, np.ndarray)) else [eps_sfc] * nb
-        return _emu_jacobian(self, list(profs), geoms, list(tsf), list(eps))
+        emu = ChannelODEmulator(mlp, z["x_mean"], z["x_std"], z["A"], float(z["coef_scale"][0]), z["basis"],
+                                z["mean"], z["inv_w"], z["ch_nu"],
+                                ProfileContext(z["ctx_mean"], z["ctx_std"], z["ctx_proj"]))
+        if "cal_spot" in z:
+            thr, budget, n = z["cal_thr"]
+            emu.calibration = Calibration(z["cal_spot"], [tuple(b) for b in z["cal_blocks"]], float(thr),
+                                          float(budget), z["cal_ch_rms"], z["cal_ch_max"], z["cal_scene_max"],
+                                          z["cal_scene_spot"], int(n))
+        return emu
 
     # ---- inference ----------------------------------------------------------
-    def _inputs(self, profs, T_sfcs, eps, mus):
-        feats, tlev, tave, scal = [], [], [], []
-        for q, p in enumerate(profs):
-            lp = layer_properties(p, self.T_method)
-            feats.append(layer_features(p, mus[q], self.T_method))
+    def _inputs(self, scenes, eps):
+        X = emulator_features(scenes, self.context, self.T_method)
+        tlev, tave, scal = [], [], []
+        for q, (p, ts, mu) in enumerate(scenes):
             tlev.append(p.temperature)
-            tave.append(lp["T_ave"])
-            scal.append([p.temperature[0] if T_sfcs[q] is None else T_sfcs[q], mus[q], eps[q]])
-        return (np.ascontiguousarray(np.stack(feats), np.float32), np.ascontiguousarray(np.stack(tlev)),
+            tave.append(layer_properties(p, self.T_method)["T_ave"])
+            scal.append([p.temperature[0] if ts is None else ts, mu, eps[q]])
+        return (np.ascontiguousarray(X, np.float32), np.ascontiguousarray(np.stack(tlev)),
                 np.ascontiguousarray(np.stack(tave)), np.ascontiguousarray(np.array(scal, np.float64)))
 
-    def forward(self, profs, *, geom=None, T_sfc=None, eps_sfc=1.0, source_function="cim",
-                want_t: bool = False):
-        """Emulated (R, BT) [n_prof, n_ch] for a list of profiles."""
+    @staticmethod
+    def _scenes(profs, geom, T_sfc, eps_sfc):
         nb = len(profs)
         geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
         mus = [1.0 if g is None else g.mu for g in geoms]
         tsf = T_sfc if isinstance(T_sfc, (list, tuple, np.ndarray)) else [T_sfc] * nb
         eps = eps_sfc if isinstance(eps_sfc, (list, tuple, np.ndarray)) else [eps_sfc] * nb
-        X, tlev, tave, scal = self._inputs(profs, tsf, eps, mus)
+        return [(profs[q], tsf[q], mus[q]) for q in range(nb)], list(eps)
+
+    def forward(self, profs, *, geom=None, T_sfc=None, eps_sfc=1.0, source_function="cim",
+                want_t: bool = False):
+        """Emulated (R, BT) [n_prof, n_ch] for a list of profiles."""
+        scenes, eps = self._scenes(list(profs), geom, T_sfc, eps_sfc)
+        nb = len(scenes)
+        if source_function.lstrip(":") != "cim":
+            raise ValueError("the emulator is trained for source_function=:cim")
+        X, tlev, tave, scal = self._inputs(scenes, eps)
         if X.shape[1] != self.L:
             raise ValueError(f"profiles have {X.shape[1]} layers, the emulator {self.L}")
         R = np.zeros((nb, self.n_ch))
         BT = np.zeros((nb, self.n_ch))
         tout = np.zeros((nb, self.L, self.n_ch), np.float32) if want_t else None
-        if source_function.lstrip(":") != "cim" and self.n_tpc == 2:
-            raise ValueError("the emulator is trained for source_function=:cim")
-        sf = api.SOURCE_FUNCTION[source_function.lstrip(":")]
-        rc = self.lib.irs_emu_forward(self.h, _p(X), _p(tlev), _p(tave), _p(scal), nb, sf, _p(R), _p(BT),
-                                      _p(tout))
+        rc = self.lib.irs_emu_forward(self.h, _p(X), _p(tlev), _p(tave), _p(scal), nb, api.SOURCE_FUNCTION["cim"],
+                                      _p(R), _p(BT), _p(tout))
         if rc != IRS_OK:
             raise IRSError(rc, self.lib.irs_emu_last_error(self.h).decode(errors="replace"))
         return (R, BT, tout) if want_t else (R, BT)
 
+    # ---- Jacobians ----------------------------------------------------------
+    def jacobian(self, profs, *, geom=None, T_sfc=None, eps_sfc=1.0) -> List["EmulatorJacobian"]:
+        """Emulator Jacobians per scene (levels: T and ln q_H2O; T_sfc); the
+        reflected term (eps < 1) is held fixed."""
+        scenes, eps = self._scenes(list(profs), geom, T_sfc, eps_sfc)
+        return _emu_jacobian(self, scenes, eps)
 
-def _feature_level_jacobian(prof, mu, T_method, hT=1e-2, hq=1e-3):
-    """d features / d (level T) and d features / d (level ln q_H2O), and
-    d T_ave / d (level T), by central differences of the host layer code
-    (smooth, O(levels x layers) work)."""
-    from .types import AtmosphericProfile
+    # ---- calibration --------------------------------------------------------
+    def calibrate(self, profs, linelists, *, sounder: Sounder, geom=None, T_sfc=None, eps_sfc=1.0,
+                  budget: float = 0.05, margin: float = 0.2, n_blocks: int = 4, block: int = 16,
+                  ctx: Optional[Context] = None, line_mixing="vpy", apply_continuum: bool = False,
+                  bt_lbl=None, **kw) -> Calibration:
+        """Hold-out statistics on calibration scenes against GPU LBL: the
+        per-channel error, n_blocks spot blocks of `block` consecutive channels
+        where the hold-out error is largest, and the spot-error threshold t: the
+        largest t such that every calibration scene whose spot error is <= t is
+        within budget * (1 - margin) on every channel (0 when none is)."""
+        scenes, eps = self._scenes(list(profs), geom, T_sfc, eps_sfc)
+        _, bt = self.forward([s[0] for s in scenes], geom=[_geom(s[2]) for s in scenes], T_sfc=[s[1] for s in scenes],
+                             eps_sfc=eps)
+        if bt_lbl is None:
+            _, _, bt_lbl = forward_model([s[0] for s in scenes], linelists, sounder=sounder,
+                                         geom=[_geom(s[2]) for s in scenes], T_sfc=[s[1] for s in scenes],
+                                         eps_sfc=eps, ctx=ctx, line_mixing=line_mixing,
+                                         apply_continuum=apply_continuum, **kw)
+        d = np.abs(bt - bt_lbl)
+        ch_rms = np.sqrt(np.mean(d ** 2, axis=0))
+        ch_max = d.max(axis=0)
+        # spot blocks: greedy, largest summed hold-out RMS over `block` consecutive channels
+        score = np.convolve(ch_rms, np.ones(block), mode="valid")             # block starting at c
+        blocks, taken = [], np.zeros(self.n_ch, bool)
+        for _ in range(n_blocks):
+            sc = score.copy()
+            for c0, n in blocks:
+                sc[max(0, c0 - block + 1):c0 + n] = -np.inf
+            if not np.isfinite(sc.max()):
+                break
+            c0 = int(np.argmax(sc))
+            blocks.append((c0, block))
+            taken[c0:c0 + block] = True
+        blocks.sort()
+        spot = np.flatnonzero(taken)
+        se, fe = d[:, spot].max(axis=1), d.max(axis=1)
+        thr = 0.0
+        for t in np.sort(se):
+            if np.all(fe[se <= t] <= budget * (1.0 - margin)):
+                thr = float(t)
+            else:
+                break
+        self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, se, len(scenes))
+        self._spot = None
+        return self.calibration
+
+    def spot_checker(self, linelists, sounder: Sounder) -> "SpotChecker":
+        if self.calibration is None:
+            raise ValueError("the emulator has no calibration (run calibrate)")
+        if self._spot is None or self._spot.linelists is not linelists:
+            sg = sounder_grid(sounder)
+            self._spot = SpotChecker(linelists, sounder,
+                                     blocks=[(float(sg.nu[c0]), n) for c0, n in self.calibration.spot_blocks])
+        return self._spot
+
+
+def _feature_level_jacobian(prof, T_sfc, mu, context: ProfileContext, T_method, hT=1e-2, hq=1e-3):
+    """d features / d (level T), d features / d (level ln q_H2O), d T_ave / d (level T),
+    d features / d T_sfc: central differences of the host layer code for the layer
+    features, analytic for the (linear) profile context."""
     nl = prof.n_levels
     def feats(T=None, vmr=None):
         p = AtmosphericProfile(prof.pressure, prof.temperature if T is None else T, prof.altitude,
@@ -369,26 +557,33 @@ def _feature_level_jacobian(prof, mu, T_method, hT=1e-2, hq=1e-3):
         return layer_features(p, mu, T_method), layer_properties(p, T_method)["T_ave"]
     X0, _ = feats()
     L, nf = X0.shape
-    dXdT = np.zeros((nl, L, nf))
+    dXdT = np.zeros((nl, L, N_FEATURES))
     dAdT = np.zeros((nl, L))
-    dXdq = np.zeros((nl, L, nf))
+    dXdq = np.zeros((nl, L, N_FEATURES))
     for j in range(nl):
         Tp, Tm = prof.temperature.copy(), prof.temperature.copy()
         Tp[j] += hT
         Tm[j] -= hT
         (xp, ap), (xm, am) = feats(T=Tp), feats(T=Tm)
-        dXdT[j] = (xp - xm) / (2 * hT)
+        dXdT[j, :, :nf] = (xp - xm) / (2 * hT)
         dAdT[j] = (ap - am) / (2 * hT)
         if GasSpecies.H2O in prof.vmr:
-            vp = dict(prof.vmr)
-            vm = dict(prof.vmr)
+            vp, vm = dict(prof.vmr), dict(prof.vmr)
             q = prof.vmr[GasSpecies.H2O]
             qp, qm = q.copy(), q.copy()
             qp[j] *= np.exp(hq)
             qm[j] *= np.exp(-hq)
             vp[GasSpecies.H2O], vm[GasSpecies.H2O] = qp, qm
-            dXdq[j] = (feats(vmr=vp)[0] - feats(vmr=vm)[0]) / (2 * hq)
-    return dXdT, dAdT, dXdq
+            dXdq[j, :, :nf] = (feats(vmr=vp)[0] - feats(vmr=vm)[0]) / (2 * hq)
+    # the context is linear in [T levels, ln q_H2O levels, ln q_O3 levels, T_sfc, 1/mu]
+    P = context.proj
+    dXdT[:, :, nf:] = P[:, :nl].T[:, None, :]
+    dXdq[:, :, nf:] = P[:, nl:2 * nl].T[:, None, :]
+    dXdTs = np.zeros((L, N_FEATURES))
+    dXdTs[:, nf:] = P[:, 3 * nl][None, :]
+    if T_sfc is None:                      # T_sfc defaults to the bottom level temperature
+        dXdT[0, :, nf:] += P[:, 3 * nl][None, :]
+    return dXdT, dAdT, dXdq, dXdTs
 
 
 @dataclass
@@ -403,13 +598,12 @@ class EmulatorJacobian:
     dfeat: np.ndarray = field(repr=False, default=None)
 
 
-def _emu_jacobian(emu, profs, geoms, tsf, eps):
-    nb = len(profs)
-    mus = [g.mu for g in geoms]
-    X, tlev, tave, scal = emu._inputs(profs, tsf, eps, mus)
-    L, nin, nch = emu.L, emu.mlp.n_in, emu.n_ch
+def _emu_jacobian(emu, scenes, eps):
+    nb = len(scenes)
+    X, tlev, tave, scal = emu._inputs(scenes, eps)
+    L, nch = emu.L, emu.n_ch
     BT = np.zeros((nb, nch))
-    dfeat = np.zeros((nb, L, nin, nch), np.float32)
+    dfeat = np.zeros((nb, L, N_FEATURES, nch), np.float32)
     dTave = np.zeros((nb, L, nch), np.float32)
     dTlev = np.zeros((nb, L + 1, nch), np.float32)
     dTsfc = np.zeros((nb, nch), np.float32)
@@ -418,29 +612,32 @@ def _emu_jacobian(emu, profs, geoms, tsf, eps):
     if rc != IRS_OK:
         raise IRSError(rc, emu.lib.irs_emu_last_error(emu.h).decode(errors="replace"))
     out = []
-    for q in range(nb):
-        dXdT, dAdT, dXdq = _feature_level_jacobian(profs[q], mus[q], emu.T_method)
+    for q, (prof, ts, mu) in enumerate(scenes):
+        dXdT, dAdT, dXdq, dXdTs = _feature_level_jacobian(prof, ts, mu, emu.context, emu.T_method)
         F = dfeat[q].astype(np.float64)
         KT = np.einsum("kic,jki->cj", F, dXdT) + np.einsum("kc,jk->cj", dTave[q].astype(np.float64), dAdT)
         KT += dTlev[q].T.astype(np.float64)
         Kq = np.einsum("kic,jki->cj", F, dXdq)
-        out.append(EmulatorJacobian(BT[q], KT, Kq, dTsfc[q].astype(np.float64), dfeat[q]))
+        Ks = np.einsum("kic,ki->c", F, dXdTs)
+        if ts is None:                      # T_sfc is the bottom level temperature: its Planck term goes there
+            KT[:, 0] += dTsfc[q].astype(np.float64)
+        else:
+            Ks = Ks + dTsfc[q].astype(np.float64)
+        out.append(EmulatorJacobian(BT[q], KT, Kq, Ks, dfeat[q]))
     return out
 
 
 # ---- spot check + fallback -----------------------------------------------------
-DEFAULT_SPOT_BLOCKS = ((700.0, 16), (960.0, 16), (1050.0, 16), (2300.0, 16))
-
-
 class SpotChecker:
     """LBL of a few blocks of consecutive channels, each on its own internal
     sub-grid (channel block +- the 16 cm^-1 ILS pad) with only the lines that
     reach it (+- 25 cm^-1 + 1): the same channel radiances as the full-range
     LBL at a fraction of its work."""
 
-    def __init__(self, linelists, sounder: Sounder, blocks=DEFAULT_SPOT_BLOCKS):
+    def __init__(self, linelists, sounder: Sounder, blocks=((700.0, 16), (960.0, 16), (1050.0, 16), (2300.0, 16))):
         sg = sounder_grid(sounder)
         self.sounder = sounder
+        self.linelists = linelists
         self.blocks = []
         for lo, n in blocks:
             c0 = int(round((lo - sounder.nu_min) / sounder.dnu))
@@ -470,24 +667,31 @@ class EmulatedResult:
     fell_back: np.ndarray
     spot_err: np.ndarray
     seconds: Dict[str, float]
+    threshold: float = 0.0
 
 
 def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: Sounder,
-                     geom=None, T_sfc=None, eps_sfc=1.0, threshold: float = 0.05,
+                     geom=None, T_sfc=None, eps_sfc=1.0, threshold: Optional[float] = None,
                      spot: Optional[SpotChecker] = None, ctx: Optional[Context] = None,
                      line_mixing="vpy", apply_continuum: bool = False, **kw) -> EmulatedResult:
-    """Emulator + per-scene LBL spot check + full-LBL fallback (BT budget
-    `threshold` K on the spot channels).  apply_continuum as in training."""
+    """Emulator + per-scene LBL spot check + full-LBL fallback.  The spot channels
+    and the spot-error threshold come from the emulator's calibration (calibrate)
+    unless `spot` / `threshold` are given; a scene whose largest spot error
+    exceeds the threshold is recomputed with full LBL.  apply_continuum as in
+    training."""
     kw["apply_continuum"] = apply_continuum
-    nb = len(profs)
-    geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
-    geoms = [g if g is not None else ViewingGeometry(0.0, 0.0, 1.0) for g in geoms]
-    tsf = T_sfc if isinstance(T_sfc, (list, tuple, np.ndarray)) else [T_sfc] * nb
-    eps = eps_sfc if isinstance(eps_sfc, (list, tuple, np.ndarray)) else [eps_sfc] * nb
+    scenes, eps = ChannelODEmulator._scenes(list(profs), geom, T_sfc, eps_sfc)
+    geoms = [_geom(s[2]) for s in scenes]
+    tsf = [s[1] for s in scenes]
     ctx = ctx or api.default_context()
-    spot = spot or SpotChecker(linelists, sounder)
+    if spot is None:
+        spot = emulator.spot_checker(linelists, sounder)
+    if threshold is None:
+        if emulator.calibration is None:
+            raise ValueError("no threshold given and the emulator has no calibration")
+        threshold = emulator.calibration.threshold
     t0 = time.perf_counter()
-    R, BT = emulator.forward(list(profs), geom=geoms, T_sfc=list(tsf), eps_sfc=list(eps))
+    R, BT = emulator.forward(list(profs), geom=geoms, T_sfc=tsf, eps_sfc=eps)
     t1 = time.perf_counter()
     bt_spot = spot(profs, geoms, tsf, eps, ctx, line_mixing=line_mixing, **kw)
     err = np.max(np.abs(BT[:, spot.channels] - bt_spot), axis=1)
@@ -501,4 +705,5 @@ def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: 
         R[idx] = Rf
         BT[idx] = BTf
     t3 = time.perf_counter()
-    return EmulatedResult(R, BT, fb, err, {"emulator": t1 - t0, "spot_check": t2 - t1, "fallback": t3 - t2})
+    return EmulatedResult(R, BT, fb, err, {"emulator": t1 - t0, "spot_check": t2 - t1, "fallback": t3 - t2},
+                          float(threshold))

@@ -72,4 +72,14 @@ def channel_od_ref(tau, nu_lo, dnu, ch_nu, kern, floor, od_max=50.0, mu=1.0):
             gv = Gc[k] / Tch[k + 1] if Tch[k + 1] >= floor else Gac[k]
             gout[k, c] = min(max(gv, 0.0), 1.0)
     channel_od_ref.last_G = Gall
+    # ILS first moments of the level transmittance steps, M_k = ILS (x) [(nu - nu_c)(T_{k+1} - T_k)]
+    mk = np.asarray(kern, np.float64) * (np.arange(len(kern)) - half) * dnu
+    Mall = np.zeros((L, len(ch_nu)))
+    for c, v in enumerate(ch_nu):
+        i = int(round((v - nu_lo) / dnu))
+        lo, hi = i - half, i + half + 1
+        klo, khi = max(0, -lo), len(kern) - max(0, hi - n)
+        Ml = dnu * (lev[:, max(lo, 0):min(hi, n)] @ mk[klo:khi])
+        Mall[:, c] = Ml[1:] - Ml[:-1]
+    channel_od_ref.last_M = Mall
     return out, Tch_all, gout
