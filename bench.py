@@ -13,13 +13,17 @@ U(0, 48.3 deg); DPTMIN live, so every profile has its own active line sets)
 in one forward_model call with the autotuned schedule and hipGraph replay.
 
 A third figure, "emulator", trains the channel-transmittance emulator on GPU
-LBL spectra (the windows variant of the line lists, whose spectrum has
+LBL targets (the windows variant of the line lists, whose spectrum has
 partly transparent windows; the benchmark lists are opaque at every channel)
-of --emu-train perturbed profiles (2048 by default, 24000 Adam steps; T GP(0, 4 K), H2O x
-lognormal(0.6), O3 x lognormal(0.2), CO2 U(390, 430) ppm, scan angles
-U(0, 48.3 deg)), then runs --emu-eval new profiles of the same distribution
-through emulator + 64-channel LBL spot check + full-LBL fallback at 0.05 K,
-with the BT error against the GPU LBL (itself within 1e-9 K of the FP64 oracle).
+of --emu-train perturbed scenes (2048 by default, --emu-steps 24000 Adam steps;
+T GP(0, 4 K), H2O x lognormal(0.6), O3 x lognormal(0.2), CO2 U(390, 430) ppm,
+scan angles U(0, 48.3 deg)), calibrates it on --emu-cal held-out scenes (the
+per-channel hold-out BT error against GPU LBL, the 4 x 16 spot channels where it
+is largest and the spot threshold for a 0.05 K budget), then runs --emu-eval new
+scenes (1024 by default) through emulator + spot check + full-LBL fallback and
+measures the BT error after fallback against GPU LBL on every scene the emulator
+answered (and against the FP64 oracle on two of them).  --emulator-only runs the
+headline steps and this figure alone.
 
 "lbl_jacobian" times one full-IASI analytic Jacobian (102 columns) of a
 perturbed profile on the windows variant of the line lists.
@@ -36,7 +40,8 @@ bytes per launch over their measured times against 8 TB/s.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-accuracy]
                     [--no-batch] [--batch N] [--no-exact] [--no-emulator] [--no-jacobian]
-                    [--no-fp64] [--emu-train N] [--emu-eval N]
+                    [--no-fp64] [--emu-train N] [--emu-cal N] [--emu-eval N] [--emu-steps N]
+                    [--emulator-only] [--no-par]
 """
 import argparse
 import json
@@ -276,71 +281,163 @@ def batched(ctx, lls, snd, n_prof):
                                         "channels": "1205-1208 and 2410-2413 cm^-1 (13 + 13 channels per profile)"}}
 
 
-def emulator_figure(ctx, lls, snd, n_train, n_eval):
+def _bands(nu, v, fn):
+    out = {}
+    for lo, hi in ((645, 760), (760, 1000), (1000, 1100), (1100, 1300), (1300, 1600), (1600, 2000), (2000, 2250),
+                   (2250, 2400), (2400, 2760)):
+        m = (nu >= lo) & (nu < hi)
+        out[f"{lo}-{hi}"] = round(float(fn(v[m])), 4)
+    return out
+
+
+def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value):
+    """Train the emulator on GPU LBL targets of n_train scenes, calibrate it on
+    n_cal held-out scenes (per-channel hold-out BT error vs GPU LBL, spot blocks,
+    spot threshold for a 0.05 K budget), then run n_eval new scenes through
+    emulator + spot check + full-LBL fallback; the BT error after fallback is
+    measured against GPU LBL on every scene the emulator answered (fallen-back
+    scenes are LBL), and against the FP64 oracle on two of them (CPU leg)."""
     from irsounderlbl import synthetic as syn
-    from irsounderlbl.api import forward_model
-    from irsounderlbl.emulator import ChannelODEmulator, SpotChecker, forward_emulated
-    from irsounderlbl.types import ViewingGeometry
+    from irsounderlbl.api import analytic_jacobian, forward_model
+    from irsounderlbl.emulator import ChannelODEmulator, forward_emulated
+    from irsounderlbl.types import GasSpecies, Sounder, StateVectorSpec, ViewingGeometry, filter_linelist, sounder_grid
     kw = dict(sigma_T=4.0, sigma_h2o=0.6, sigma_o3=0.2, co2_range=(390e-6, 430e-6))
-    from irsounderlbl.api import channel_optical_depths
+    def geoms(sc):
+        return [ViewingGeometry(float(np.degrees(np.arccos(min(p[2], 1.0)))), 0.0, p[2]) for p in sc]
     train = syn.perturbed_profiles(n_train, seed=3003, **kw)
-    log(f"bench: emulator training set ({n_train} profiles, GPU LBL) + PCA + MLP")
+    log(f"bench: emulator training ({n_train} scenes of GPU LBL targets, per-layer PCA, regression, MLP)")
     t0 = time.perf_counter()
-    taus, gs = [], []
-    for c0 in range(0, n_train, 256):
-        sc = train[c0:c0 + 256]
-        tt, gg = channel_optical_depths([p[0] for p in sc], lls, sounder=snd,
-                                        geom=[ViewingGeometry(float(np.degrees(np.arccos(min(p[2], 1.0)))), 0.0, p[2])
-                                              for p in sc], floor=1e-4, cim=True, line_mixing="vpy", ctx=ctx, apply_continuum=False)
-        taus.append(tt)
-        gs.append(gg)
-        log(f"bench: emulator targets {c0 + len(sc)}/{n_train}")
-    t_lbl = time.perf_counter() - t0
-    targets = (np.concatenate(taus), np.concatenate(gs))
-    del taus, gs
-    steps = 6000 if n_train <= 512 else 24000        # 2048 profiles: 24000 steps 0.135 K held-out, 12000 0.206 K
-    emu, rep = ChannelODEmulator.train(train, lls, snd, n_pc=256, steps=steps, batch=4096, lr=1e-3, ctx=ctx,
-                                       targets=targets)
-    del targets
+    emu, rep = ChannelODEmulator.train(train, lls, snd, n_pc=64, n_basis=min(2048, n_train), steps=steps,
+                                       batch=4096, lr=1e-3, ctx=ctx, progress=lambda m: log("bench: " + m))
     t_train = time.perf_counter() - t0
+    del train
+    nu = sounder_grid(snd).nu
+    cal = syn.perturbed_profiles(n_cal, seed=5005, **kw)
+    log(f"bench: emulator calibration ({n_cal} held-out scenes vs GPU LBL)")
+    c = emu.calibrate([p[0] for p in cal], lls, sounder=snd, geom=geoms(cal), T_sfc=[p[1] for p in cal],
+                      budget=0.05, margin=0.2, n_blocks=4, block=16, ctx=ctx)
     ev = syn.perturbed_profiles(n_eval, seed=4004, **kw)
-    profs = [p[0] for p in ev]
-    geoms = [ViewingGeometry(float(np.degrees(np.arccos(p[2]))), 0.0, p[2]) for p in ev]
-    tsf = [p[1] for p in ev]
-    log(f"bench: emulator evaluation ({n_eval} profiles)")
-    spot = SpotChecker(lls, snd)
-    forward_emulated(profs[:8], lls, emu, sounder=snd, geom=geoms[:8], T_sfc=tsf[:8], spot=spot, ctx=ctx)
+    profs, gs, tsf = [p[0] for p in ev], geoms(ev), [p[1] for p in ev]
+    log(f"bench: emulator evaluation ({n_eval} scenes: emulator + spot check + fallback)")
+    forward_emulated(profs[:8], lls, emu, sounder=snd, geom=gs[:8], T_sfc=tsf[:8], ctx=ctx)
     t0 = time.perf_counter()
-    _, bt_emu_only = emu.forward(profs, geom=geoms, T_sfc=tsf)
+    _, bt_emu_only = emu.forward(profs, geom=gs, T_sfc=tsf)
     t_emu = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res = forward_emulated(profs, lls, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=0.05, spot=spot, ctx=ctx)
+    res = forward_emulated(profs, lls, emu, sounder=snd, geom=gs, T_sfc=tsf, ctx=ctx)
     t_full = time.perf_counter() - t0
-    log("bench: emulator reference LBL")
-    _, _, bt_lbl = forward_model(profs, lls, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx, apply_continuum=False)
-    d_emu = np.abs(bt_emu_only - bt_lbl)
-    d_fin = np.abs(res.BT - bt_lbl)
+    kept = np.flatnonzero(~res.fell_back)
+    log(f"bench: emulator reference LBL on the {len(kept)} scenes the emulator answered")
+    d_kept = np.zeros((0, len(nu)))
+    if len(kept):
+        _, _, bt_ref = forward_model([profs[i] for i in kept], lls, sounder=snd, geom=[gs[i] for i in kept],
+                                     T_sfc=[tsf[i] for i in kept], line_mixing="vpy", ctx=ctx, apply_continuum=False)
+        d_kept = np.abs(res.BT[kept] - bt_ref)
+    d_all_max = float(d_kept.max()) if len(kept) else 0.0
+    rms_all = float(np.sqrt(np.sum(d_kept ** 2) / (n_eval * len(nu))))
+    # FP64 oracle (CPU leg) on two answered scenes, two channel windows
+    if len(kept):
+        got, calls = [], []
+        for q in (kept[0], kept[len(kept) // 2]):
+            for lo, hi in ((1205.0, 1208.0), (2410.0, 2413.0)):
+                sw = Sounder.make(lo, hi, snd.dnu, opd_max=snd.opd_max, fwhm_gauss=snd.fwhm_gauss)
+                sub = {k: filter_linelist(v, lo - 42.0, hi + 42.0) for k, v in lls.items()}
+                c0 = int(round((lo - snd.nu_min) / snd.dnu))
+                n_w = int(round((hi - lo) / snd.dnu)) + 1
+                got.append(res.BT[q, c0:c0 + n_w])
+                calls.append((profs[q], sub, sw, ev[q][2], tsf[q]))
+        check_bt("emulator", "bt_err_vs_fp64_oracle_K", np.concatenate(got),
+                 lambda o: np.concatenate([o.forward(pr, sub, sw, line_mixing=1, mu=mu, T_sfc=ts)[1]
+                                           for pr, sub, sw, mu, ts in calls]))
+    log("bench: emulator Jacobians")
     t0 = time.perf_counter()
-    jac = emu.jacobian(profs[:4], geom=geoms[:4], T_sfc=tsf[:4])
+    jac = emu.jacobian(profs[:4], geom=gs[:4], T_sfc=tsf[:4])
     t_jac = (time.perf_counter() - t0) / 4
+    ref = analytic_jacobian(profs[0], lls, StateVectorSpec(50, [GasSpecies.H2O], include_emissivity=False),
+                            sounder=snd, geom=gs[0], T_sfc=tsf[0], line_mixing="vpy", vmr_coupling=False, ctx=ctx,
+                            apply_continuum=False)
+    rel_T = float(np.linalg.norm(jac[0].K_T - ref.K[:, :50]) / np.linalg.norm(ref.K[:, :50]))
+    rel_q = float(np.linalg.norm(jac[0].K_lnq_H2O - ref.K[:, 50:100]) / np.linalg.norm(ref.K[:, 50:100]))
+    sc_cal = c.scene_max
+    out = {"workload": f"train {n_train} / calibrate {n_cal} / evaluate {n_eval} perturbed scenes (T GP 4 K, "
+                       f"H2O x lognormal 0.6, O3 x lognormal 0.2, CO2 390-430 ppm, scan 0-48.3 deg), full IASI "
+                       f"8461 channels, windows variant of the line lists; targets (t, g, ILS moment) x 49 layers, "
+                       f"64 PCs per layer, per-layer quadratic + context regression, MLP 9 -> 3x512 GELU -> 64 "
+                       f"(BF16 MFMA) on its residual; spot check 4 x 16 channels",
+           "training": {"seconds": round(t_train, 1), "lbl_targets_seconds": round(rep.seconds_lbl, 1),
+                        "pca_seconds": round(rep.seconds_pca, 1), "regression_seconds": round(rep.seconds_fit, 1),
+                        "mlp_seconds": round(rep.seconds_train, 1), "adam_steps": steps,
+                        "basis_scenes": rep.n_basis_profiles, "final_loss": float(np.mean(rep.losses[-50:])),
+                        "coef_rms_err_regression_over_coef_rms": round(rep.coef_rms_err_regression / rep.coef_rms, 4)},
+           "holdout_per_channel_bt_err_K": {
+               "scenes": n_cal, "rms": round(float(np.sqrt(np.mean(c.ch_rms ** 2))), 4),
+               "channel_rms_percentiles_10_50_90_99": [round(float(x), 4) for x in np.percentile(c.ch_rms, [10, 50, 90, 99])],
+               "channel_max_percentiles_10_50_90_99": [round(float(x), 4) for x in np.percentile(c.ch_max, [10, 50, 90, 99])],
+               "rms_by_band": _bands(nu, c.ch_rms, lambda v: np.sqrt(np.mean(v ** 2))),
+               "scene_max_percentiles_10_50_90": [round(float(x), 4) for x in np.percentile(sc_cal, [10, 50, 90])],
+               "scenes_within_0.05K": round(float(np.mean(sc_cal <= 0.05)), 3)},
+           "calibration": {"spot_blocks_cm-1": [[round(float(nu[c0]), 2), round(float(nu[c0 + n - 1]), 2)]
+                                                for c0, n in c.spot_blocks],
+                           "spot_threshold_K": round(c.threshold, 4), "budget_K": c.budget, "margin": 0.2},
+           "emulator_only_spectra_per_s": round(n_eval / t_emu, 1),
+           "emulator_plus_fallback_spectra_per_s": round(n_eval / t_full, 3),
+           "plain_lbl_spectra_per_s": lbl_value,
+           "speedup_vs_plain_lbl": round(n_eval / t_full / lbl_value, 3),
+           "fallback_rate": round(float(np.mean(res.fell_back)), 4),
+           "seconds": {k: round(v, 3) for k, v in res.seconds.items()},
+           "bt_err_after_fallback_K": {"max": d_all_max, "rms": rms_all, "scenes_answered_by_emulator": int(len(kept)),
+                                       "measured_against": "GPU LBL on every answered scene (fallen-back scenes are LBL)"},
+           "jacobian_seconds_per_profile": round(t_jac, 3), "jacobian_shape": list(jac[0].K_T.shape),
+           "jacobian_vs_lbl_relative_frobenius": {"T": round(rel_T, 4), "lnq_H2O": round(rel_q, 4)}}
     emu.close()
     emu.mlp.close()
-    return {"workload": f"train {n_train} / evaluate {n_eval} perturbed profiles, full IASI 8461 channels, "
-                        f"windows variant of the line lists, "
-                        "256 PCs, MLP 9 -> 3x512 GELU -> 256 (BF16 MFMA), spot check 4 x 16 channels",
-            "training_seconds": round(t_train, 1), "lbl_dataset_seconds": round(t_lbl, 1),
-            "adam_steps": steps,
-            "pca_seconds": round(rep.seconds_pca, 1), "mlp_train_seconds": round(rep.seconds_train, 1),
-            "final_loss": float(np.mean(rep.losses[-50:])),
-            "emulator_only_spectra_per_s": round(n_eval / t_emu, 1),
-            "emulator_plus_fallback_spectra_per_s": round(n_eval / t_full, 3),
-            "fallback_rate": float(np.mean(res.fell_back)),
-            "seconds": {k: round(v, 3) for k, v in res.seconds.items()},
-            "bt_err_emulator_only_K": {"max": float(d_emu.max()), "rms": float(np.sqrt(np.mean(d_emu ** 2))),
-                                       "p99_per_scene_max": float(np.percentile(d_emu.max(axis=1), 99))},
-            "bt_err_after_fallback_K": {"max": float(d_fin.max()), "rms": float(np.sqrt(np.mean(d_fin ** 2)))},
-            "jacobian_seconds_per_profile": round(t_jac, 3),
-            "jacobian_shape": list(jac[0].K_T.shape)}
+    return out
+
+
+def par_figure(ctx, lls):
+    """HITRAN .par ingest (load_hitran_par on the device): the headline line lists
+    (300 000 lines) as one 160-character-record .par buffer in host memory, parsed,
+    compacted and sorted by wavenumber, host to host.  The CPU leg checks the
+    first 20 000 records bit for bit against the oracle reader and times it."""
+    from irsounderlbl.hitran import format_hitran_par, parse_hitran_par_bytes
+    recs = []
+    for ll in lls.values():
+        recs.extend(format_hitran_par(ll, i) for i in range(len(ll)))
+    data = ("\n".join(recs) + "\n").encode()
+    parse_hitran_par_bytes(data[:200000], ctx=ctx)
+    t0 = time.perf_counter()
+    n_rep = 5
+    for _ in range(n_rep):
+        got, skipped = parse_hitran_par_bytes(data, ctx=ctx)
+    dt = (time.perf_counter() - t0) / n_rep
+    sample = ("\n".join(recs[:20000]) + "\n").encode()
+    got_s, _ = parse_hitran_par_bytes(sample, ctx=ctx)
+    PAR_CHECK.append((sample, got_s))
+    return {"workload": f"{len(recs)} records, {len(data) / 1e6:.1f} MB of .par text in host memory -> sorted SoA",
+            "seconds": round(dt, 4), "records_per_s": round(len(got) / dt, 1),
+            "MB_per_s": round(len(data) / dt / 1e6, 1), "lines": len(got), "skipped": skipped,
+            "note": "host-to-host time: PCIe copy of the text, newline scan, per-record parse with exact "
+                    "decimal rounding, compaction, sortedness check (the lists are in order), copy back"}
+
+
+PAR_CHECK = []
+
+
+def par_cpu_leg(out):
+    """Oracle reader on the 20 000-record sample: bit-exact comparison and its rate."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import par_ref
+    for sample, got in PAR_CHECK:
+        t0 = time.perf_counter()
+        ref, _ = par_ref.load_par_text(sample.decode())
+        dt = time.perf_counter() - t0
+        same = all(np.array_equal(getattr(got, b).view(np.uint8), ref[a].view(np.uint8)) for a, b in (
+            ("mol", "mol_id"), ("iso", "iso_id"), ("nu", "wavenumber"), ("S", "intensity"), ("A", "a_coeff"),
+            ("gamma_air", "air_broad"), ("gamma_self", "self_broad"), ("elower", "lower_energy"),
+            ("n_air", "temp_depend"), ("delta_air", "pressure_shift")))
+        sec = out.setdefault("par_ingest", {})
+        sec["oracle_bit_exact_first_20000_records"] = bool(same)
+        sec["cpu_oracle_records_per_s"] = round(len(ref["nu"]) / dt, 1)
 
 
 def lbl_jacobian_figure(ctx, lls_w, snd):
@@ -501,9 +598,17 @@ def main():
     ap.add_argument("--no-emulator", action="store_true")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-par", action="store_true")
     ap.add_argument("--emu-train", type=int, default=2048)
-    ap.add_argument("--emu-eval", type=int, default=256)
+    ap.add_argument("--emu-cal", type=int, default=256)
+    ap.add_argument("--emu-eval", type=int, default=1024)
+    ap.add_argument("--emu-steps", type=int, default=24000)
+    ap.add_argument("--emulator-only", action="store_true",
+                    help="the headline steps and the emulator figure only (the other figures off)")
     args = ap.parse_args()
+    if args.emulator_only:
+        args.no_cpu = args.no_accuracy = args.no_batch = args.no_exact = args.no_jacobian = args.no_fp64 = True
+        args.no_par = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this engine shards profiles across GPUs with no "
               "exchange, and every box here has one GPU; run with --gpus 1.", file=sys.stderr)
@@ -563,8 +668,12 @@ def main():
         out["batched"] = batched(ctx, lls, snd, args.batch)
     if not args.no_jacobian:
         out["lbl_jacobian"] = lbl_jacobian_figure(ctx, lls_w, snd)
+    if not args.no_par:
+        log("bench: .par ingest")
+        out["par_ingest"] = par_figure(ctx, lls)
     if not args.no_emulator:
-        out["emulator"] = emulator_figure(ctx, lls_w, snd, args.emu_train, args.emu_eval)
+        out["emulator"] = emulator_figure(ctx, lls_w, snd, args.emu_train, args.emu_cal, args.emu_eval,
+                                          args.emu_steps, round(value, 4))
     if not args.no_accuracy:
         log("bench: accuracy samples and config 0 (GPU side)")
         accuracy_sample(ctx, prof, lls, lls_w)
@@ -573,6 +682,10 @@ def main():
         log("bench: cpu baseline (FP64 oracle: config 0 full, config 1 sampled layers, config 2 two profiles)")
         out["cpu_baseline"] = cpu_baseline(prof, lls, snd)
         log("bench: FP64 oracle checks of the GPU results")
+        run_checks(out)
+        par_cpu_leg(out)
+    elif args.emulator_only and CHECKS:
+        log("bench: FP64 oracle checks of the emulator results")
         run_checks(out)
     print(json.dumps(out), flush=True)
     ctx.close()

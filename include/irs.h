@@ -300,6 +300,39 @@ int irs_xsec_layer_noalloc(irs_ctx *ctx, int32_t handle, double nu0, double dnu,
                            double cutoff, double lm_cutoff, double x_far, int line_mixing,
                            double *sigma);
 
+/* ---- HITRAN .par ingest ---------------------------------------------------
+ * Output arrays of the parsed records (host memory, `capacity` entries each). */
+typedef struct {
+    int8_t *mol, *iso;
+    double *nu;                /* cm^-1 */
+    double *S;                 /* cm^-1/(molec cm^-2) at 296 K */
+    double *A;                 /* s^-1 */
+    double *elower;            /* cm^-1 */
+    float  *gamma_air, *gamma_self, *n_air, *delta_air;
+} irs_par_lines;
+
+#define IRS_LINELIST_CACHE_VERSION 1
+
+/* load_hitran_par over a buffer (src/HITRAN/fetch.jl:12-61, the record parser
+ * parse_hitran_par, src/HITRAN/linelist.jl:90-103), parsed on the device: each
+ * newline-terminated record (a trailing CR removed) of at least 67 characters
+ * whose ten fixed-width fields parse is kept if nu_min <= nu <= nu_max (pass
+ * -inf / +inf for no bound); records shorter than 67 characters or with a field
+ * that does not parse are skipped.  The kept lines are returned sorted stably by
+ * wavenumber (HITRANLinelist, linelist.jl:51-56).  capacity >= n_bytes / 67 + 1
+ * always suffices.  n_skipped (may be NULL): records not parsed. */
+int irs_par_parse(irs_ctx *ctx, const char *bytes, int64_t n_bytes, double nu_min, double nu_max,
+                  int64_t capacity, const irs_par_lines *out, int64_t *n_lines, int64_t *n_skipped);
+
+/* load_linelist (src/HITRAN/cache.jl:153-175): the .par file at par_path parsed
+ * whole (irs_par_parse) or read from its cache file in cache_dir (NULL or "": no
+ * cache), then the window [nu_min, nu_max] applied.  The cache file
+ * (cache_dir/linelist_<key>.irsll, key = hash of the absolute path, size,
+ * modification time and IRS_LINELIST_CACHE_VERSION) is written atomically when
+ * missing or when rebuild != 0.  IRS_ENOTFOUND names a missing file. */
+int irs_par_load(irs_ctx *ctx, const char *par_path, const char *cache_dir, int rebuild, double nu_min,
+                 double nu_max, int64_t capacity, const irs_par_lines *out, int64_t *n_lines);
+
 /* Schwarzschild RTE on a host tau [n_nu x n_layers]; T_ave required for CIM. */
 int irs_rte(irs_ctx *ctx, double nu0, double dnu, int64_t n_nu, const double *tau, int n_layers,
             const double *T_lev, double T_sfc, double mu, double eps_sfc, int source_function,

@@ -151,6 +151,7 @@ int fail(irs_ctx *ctx, int code, const std::string &msg) {
     return code;
 }
 
+
 hipError_t ensure(DevBuf &b, size_t bytes) {
     if (bytes <= b.cap) return hipSuccess;
     b.release();
@@ -774,6 +775,10 @@ double dbt_dr(double nu, double R) {   // vmr_jacobian.jl:120-125
 }
 
 }  // namespace
+
+hipStream_t ctx_stream(irs_ctx *ctx) { return ctx->stream; }
+int ctx_device(irs_ctx *ctx) { return ctx->device; }
+int ctx_fail(irs_ctx *ctx, int code, const std::string &msg) { return fail(ctx, code, msg); }
 
 // =============================================================================
 extern "C" {

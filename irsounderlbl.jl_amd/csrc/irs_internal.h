@@ -1,6 +1,8 @@
 // Internal declarations shared by the HIP kernels and the C-ABI host code.
 #pragma once
 
+#include <string>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -190,3 +192,28 @@ hipError_t launch_ils_taps(const double *A, const double *L, int n_pts, double d
 hipError_t launch_ils_channels(const IlsArgs &a, hipStream_t s);
 hipError_t launch_ils_block(const IlsBlockArgs &a, int n_cols, hipStream_t s);
 hipError_t launch_linesum64(const Linesum64Args &a, hipStream_t s);
+
+// ---- HITRAN .par ingest (k_par.hip, irs_par.hip) ------------------------------
+struct ParFields {                 // device SoA of parsed records
+    int8_t *mol, *iso;
+    double *nu, *S, *A, *elower;
+    float *gamma_air, *gamma_self, *n_air, *delta_air;
+};
+hipError_t launch_par_count(const unsigned char *b, int64_t n, int64_t *blk, int64_t n_blk, hipStream_t s);
+hipError_t launch_par_newlines(const unsigned char *b, int64_t n, const int64_t *blk_off, int64_t n_blk, int64_t *nl,
+                               hipStream_t s);
+hipError_t launch_par_parse(const unsigned char *b, int64_t n, const int64_t *nl, int64_t n_nl, int64_t n_rec,
+                            const ParFields &f, uint8_t *status, unsigned long long *n_host, hipStream_t s);
+hipError_t launch_par_keep_count(const uint8_t *status, const double *nu, int64_t n_rec, double nu_min, double nu_max,
+                                 int64_t *blk, hipStream_t s);
+hipError_t launch_par_compact(const uint8_t *status, const ParFields &f, int64_t n_rec, double nu_min, double nu_max,
+                              const int64_t *blk_off, const ParFields &g, hipStream_t s);
+hipError_t launch_par_sort(const ParFields &g, int64_t n, int64_t n_pad, uint64_t *key, uint32_t *idx,
+                           unsigned long long *descents, const ParFields &h, int *sorted_out, hipStream_t s);
+
+// context accessors for code outside irs_api.hip
+struct irs_ctx;
+hipStream_t ctx_stream(irs_ctx *ctx);
+int ctx_device(irs_ctx *ctx);
+int ctx_fail(irs_ctx *ctx, int code, const std::string &msg);
+

@@ -26,3 +26,9 @@ __all__ += ["IRSError", "Context", "analytic_jacobian", "apply_ils_channels",
             "brightness_temperature", "co2_cia", "co2_continuum", "compute_voigt_cross_sections",
             "compute_voigt_lm_cross_sections", "forward_model", "h2o_continuum", "ils_kernel",
             "n2_cia", "o2_cia", "planck_radiance", "schwarzschild_rte"]
+
+from .hitran import (HITRANLine, clear_linelist_cache, linelist_cache_dir, load_hitran_par,  # noqa: E402
+                     load_linelist, parse_hitran_par, set_linelist_cache_dir, write_hitran_par)
+
+__all__ += ["HITRANLine", "clear_linelist_cache", "linelist_cache_dir", "load_hitran_par", "load_linelist",
+            "parse_hitran_par", "set_linelist_cache_dir", "write_hitran_par"]

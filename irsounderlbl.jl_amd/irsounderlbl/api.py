@@ -104,7 +104,7 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
            "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian", "irs_cia_set",
            "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc",
-           "irs_channel_od_moments")
+           "irs_channel_od_moments", "irs_par_parse", "irs_par_load")
 
 _lib = None
 

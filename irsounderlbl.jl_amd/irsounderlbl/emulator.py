@@ -404,10 +404,20 @@ class ChannelODEmulator:
         mlp = MLP(N_LOCAL, 512, n_pc, max(batch, 64), seed=seed)
         mlp.set_data(Z[..., :N_LOCAL].reshape(-1, N_LOCAL).astype(np.float32),
                      (res / scale).reshape(-1, n_pc).astype(np.float32))
-        # Adam at lr for half the steps, then lr/3 and lr/10 for a quarter each
+        # Adam at lr for half the steps, then lr/3 and lr/10 for a quarter each (in
+        # segments of at most 3000 steps, each reshuffled with its own seed)
         n1, n2 = steps // 2, steps // 4
-        losses = np.concatenate([mlp.train(n1, batch, lr, seed), mlp.train(n2, batch, lr / 3.0, seed + 1),
-                                 mlp.train(steps - n1 - n2, batch, lr / 10.0, seed + 2)])
+        parts, done = [], 0
+        for n_ph, lr_ph in ((n1, lr), (n2, lr / 3.0), (steps - n1 - n2, lr / 10.0)):
+            k0 = 0
+            while k0 < n_ph:
+                k = min(3000, n_ph - k0)
+                parts.append(mlp.train(k, batch, lr_ph, seed + 7919 * (done + k0)))
+                k0 += k
+                if progress:
+                    progress(f"emulator MLP {done + k0}/{steps} Adam steps, loss {parts[-1][-20:].mean():.3e}")
+            done += n_ph
+        losses = np.concatenate(parts) if parts else np.zeros(0, np.float32)
         t_train = time.perf_counter() - tt
         emu = ChannelODEmulator(mlp, x_mean, x_std, A.astype(np.float32), scale, basis, mean, 1.0 / WR, sg.nu,
                                 context, T_method)

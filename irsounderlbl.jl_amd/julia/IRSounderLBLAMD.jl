@@ -17,7 +17,7 @@ data files are absent.
 """
 module IRSounderLBLAMD
 
-using IRSounderLBL: AtmosphericProfile, HITRANLinelist, GasSpecies, Sounder, IASIInstrument,
+using IRSounderLBL: AtmosphericProfile, HITRANLine, HITRANLinelist, GasSpecies, Sounder, IASIInstrument,
                     ViewingGeometry, nadir_geometry, WavenumberGrid, wavenumber_grid,
                     StateVectorSpec, Jacobian, H2O, CO2
 
@@ -32,6 +32,12 @@ struct CLinelist
     gamma_air::Ptr{Float32}; gamma_self::Ptr{Float32}
     elower::Ptr{Float64}; n_air::Ptr{Float32}; delta_air::Ptr{Float32}
     y_lm::Ptr{Float64}
+end
+
+struct CParLines
+    mol::Ptr{Int8}; iso::Ptr{Int8}
+    nu::Ptr{Float64}; S::Ptr{Float64}; A::Ptr{Float64}; elower::Ptr{Float64}
+    gamma_air::Ptr{Float32}; gamma_self::Ptr{Float32}; n_air::Ptr{Float32}; delta_air::Ptr{Float32}
 end
 
 struct CSounder
@@ -110,6 +116,38 @@ function upload!(ctx::Context, ll::HITRANLinelist, y::Union{Nothing, Vector{Floa
                          (Ptr{Cvoid}, Ref{CLinelist}, Ref{Int32}), ctx.h, c, h))
     end
     ctx.lists[key] = h[]
+end
+
+"""
+    load_linelist(parpath; ν_min=nothing, ν_max=nothing, cache=true, rebuild=false) -> HITRANLinelist
+
+`load_linelist` (src/HITRAN/cache.jl:153-175) with the records parsed on the
+device (`irs_par_load`): the whole .par parsed once, cached as a
+structure-of-arrays file in the reference's `linelist_cache_dir()` (keyed by
+path, size, mtime and the cache version), the window applied in memory.
+"""
+function load_linelist(parpath::AbstractString; ν_min::Union{Float64,Nothing} = nothing,
+                       ν_max::Union{Float64,Nothing} = nothing, cache::Bool = true, rebuild::Bool = false,
+                       ctx::Context = context())
+    isfile(parpath) || error("File not found: $parpath")
+    cap = filesize(parpath) ÷ 67 + 1
+    mol = Vector{Int8}(undef, cap); iso = Vector{Int8}(undef, cap)
+    nu = Vector{Float64}(undef, cap); S = Vector{Float64}(undef, cap); A = Vector{Float64}(undef, cap)
+    el = Vector{Float64}(undef, cap); ga = Vector{Float32}(undef, cap); gs = Vector{Float32}(undef, cap)
+    na = Vector{Float32}(undef, cap); da = Vector{Float32}(undef, cap)
+    lo = something(ν_min, -Inf); hi = something(ν_max, Inf)
+    dir = cache ? IRSounderLBL.linelist_cache_dir() : ""
+    n = Ref{Int64}(0)
+    GC.@preserve mol iso nu S A el ga gs na da dir begin
+        out = CParLines(pointer(mol), pointer(iso), pointer(nu), pointer(S), pointer(A), pointer(el),
+                        pointer(ga), pointer(gs), pointer(na), pointer(da))
+        check(ctx, ccall((:irs_par_load, libirs), Cint,
+                         (Ptr{Cvoid}, Cstring, Cstring, Cint, Float64, Float64, Int64, Ref{CParLines}, Ref{Int64}),
+                         ctx.h, parpath, dir, rebuild, lo, hi, cap, out, n))
+    end
+    lines = [HITRANLine(mol[i], iso[i], nu[i], S[i], A[i], ga[i], gs[i], el[i], na[i], da[i]) for i in 1:n[]]
+    isempty(lines) && return HITRANLinelist(lines, Set{Int}(), lo, hi)
+    return HITRANLinelist(lines)
 end
 
 function continua_mask(apply_continuum::Bool, continua)

@@ -16,7 +16,7 @@ SHIM = os.path.join(ROOT, "irsounderlbl.jl_amd", "julia", "IRSounderLBLAMD.jl")
 # C struct -> (Julia struct, ctypes class)
 PAIRS = {"irs_linelist": ("CLinelist", "IrsLinelist"), "irs_sounder": ("CSounder", "IrsSounder"),
          "irs_profile": ("CProfile", "IrsProfile"), "irs_fm_options": ("COptions", "IrsOptions"),
-         "irs_state_spec": ("CStateSpec", "IrsStateSpec")}
+         "irs_state_spec": ("CStateSpec", "IrsStateSpec"), "irs_par_lines": ("CParLines", "hitran.IrsParLines")}
 
 C_KIND = {"int64_t": "i64", "int": "i32", "int32_t": "i32", "double": "f64", "float": "f32",
           "int8_t": "i8"}
@@ -77,8 +77,10 @@ def test_julia_shim_struct_matches_header(cname):
 
 @pytest.mark.parametrize("cname", sorted(PAIRS))
 def test_ctypes_struct_matches_header(cname):
-    from irsounderlbl import api
-    cls = getattr(api, PAIRS[cname][1])
+    import importlib
+    mod, _, name = ("api." + PAIRS[cname][1]).rpartition(".") if "." not in PAIRS[cname][1] else \
+        PAIRS[cname][1].rpartition(".")
+    cls = getattr(importlib.import_module("irsounderlbl." + mod.split(".")[-1]), name)
     kinds = {C.c_int64: "i64", C.c_int32: "i32", C.c_double: "f64", C.c_float: "f32", C.c_int8: "i8",
              C.c_void_p: "ptr"}
     got = [(nm, kinds[ty]) for nm, ty in cls._fields_]
