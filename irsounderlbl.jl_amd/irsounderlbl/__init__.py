@@ -8,14 +8,15 @@ fallback: if the library or a GPU is missing, calls raise `IRSError`.
 from .types import (AtmosphericProfile, ColumnScale, CrISInstrument, FullProfile, GasSpecies,
                     HITRANLinelist, IASIInstrument, IASINGInstrument, Jacobian, MTGIRSInstrument,
                     PartialColumns, Sounder, StateVectorSpec, ViewingGeometry, WavenumberGrid,
-                    filter_linelist, iasi_scan_angles, nadir_geometry, pack_state,
-                    scan_angle_to_local_zenith, sounder_grid, wavenumber_grid)
+                    airmass_factor, filter_linelist, iasi_scan_angles, iasi_zenith_angles,
+                    nadir_geometry, pack_state, scan_angle_to_local_zenith, sounder_grid,
+                    wavenumber_grid)
 
 __all__ = ["AtmosphericProfile", "ColumnScale", "CrISInstrument", "FullProfile", "GasSpecies",
            "HITRANLinelist", "IASIInstrument", "IASINGInstrument", "Jacobian", "MTGIRSInstrument",
            "PartialColumns", "Sounder", "StateVectorSpec", "pack_state",
-           "ViewingGeometry", "WavenumberGrid", "filter_linelist", "iasi_scan_angles",
-           "nadir_geometry", "scan_angle_to_local_zenith", "sounder_grid", "wavenumber_grid"]
+           "ViewingGeometry", "WavenumberGrid", "airmass_factor", "filter_linelist",
+           "iasi_scan_angles", "iasi_zenith_angles", "nadir_geometry", "scan_angle_to_local_zenith", "sounder_grid", "wavenumber_grid"]
 
 from .api import (IRSError, Context, analytic_jacobian, apply_ils_channels,  # noqa: E402
                   brightness_temperature, co2_cia, co2_continuum, compute_voigt_cross_sections,
@@ -32,3 +33,10 @@ from .hitran import (HITRANLine, clear_linelist_cache, linelist_cache_dir, load_
 
 __all__ += ["HITRANLine", "clear_linelist_cache", "linelist_cache_dir", "load_hitran_par", "load_linelist",
             "parse_hitran_par", "set_linelist_cache_dir", "write_hitran_par"]
+
+from .retrieval import (RetrievalResult, apodized_measurement_covariance, build_sa, dB_dT,  # noqa: E402
+                        exclude_channels, optimal_estimation, scene_measurement_covariance, scene_nedt)
+from .types import unpack_state  # noqa: E402
+
+__all__ += ["RetrievalResult", "apodized_measurement_covariance", "build_sa", "dB_dT", "exclude_channels",
+            "optimal_estimation", "scene_measurement_covariance", "scene_nedt", "unpack_state"]

@@ -201,10 +201,22 @@ def scan_angle_to_local_zenith(scan_angle: float, h_sat_km: float = 817.0,
     return math.degrees(math.asin(s))
 
 
+def airmass_factor(geom: ViewingGeometry) -> float:
+    """Plane-parallel airmass 1/mu (src/Sensor/geometry.jl:48)."""
+    return 1.0 / geom.mu
+
+
 def iasi_scan_angles():
     """The 30 nominal IASI scan angles (src/Sensor/geometry.jl:61-68)."""
     half = np.linspace(48.33 / 15 / 2, 48.33, 15)
     return np.concatenate([-half[::-1], half])
+
+
+def iasi_zenith_angles(h_sat_km: float = 817.0):
+    """Local zenith angles of the 30 IASI scan positions, sign of the scan angle
+    kept (src/Sensor/geometry.jl:110-113)."""
+    return np.array([math.copysign(scan_angle_to_local_zenith(s, h_sat_km=h_sat_km), s)
+                     for s in iasi_scan_angles()])
 
 
 class WavenumberGrid:
@@ -373,6 +385,19 @@ class StateVectorSpec:
                 return b.range
         raise KeyError(f"species {GasSpecies(s).name} is not in the state vector")
 
+    def __repr__(self):
+        blocks = [f"T({self.n_levels})"] if self.include_temperature else []
+        for b in self.vmr_blocks:
+            if b.basis is None:
+                blocks.append(f"{'logVMR' if self.log_vmr else 'VMR'}_{b.species.name}({self.n_levels})")
+            else:
+                blocks.append(f"col_{b.species.name}({b.m})")
+        if self.include_tsfc:
+            blocks.append("T_sfc")
+        if self.include_emissivity:
+            blocks.append("ε")
+        return f"StateVectorSpec(n={self.n}: " + " + ".join(blocks) + ")"
+
     def labels(self):
         """state_labels (state_vector.jl:335-350), 1-based level indices."""
         out = [f"T[{i + 1}]" for i in self.temp_range]
@@ -413,6 +438,30 @@ def pack_state(spec: StateVectorSpec, prof: AtmosphericProfile, T_sfc: Optional[
     if spec.include_emissivity:
         x[spec.emis_index] = eps_sfc
     return x
+
+
+def unpack_state(spec: StateVectorSpec, x, base_prof: AtmosphericProfile):
+    """unpack_state (state_vector.jl:380-403) -> (profile, T_sfc, eps_sfc): fields outside
+    the state come from base_prof; T_sfc falls back to the lowest level temperature and
+    eps_sfc to 1.0 when they are not retrieved."""
+    x = np.asarray(x, np.float64)
+    if len(x) != spec.n:
+        raise ValueError(f"state vector length {len(x)} ≠ spec.n {spec.n}")
+    if spec.include_temperature:
+        T = x[spec.temp_range.start:spec.temp_range.stop].copy()
+    else:
+        T = base_prof.temperature.copy()
+    vmr = {s: v.copy() for s, v in base_prof.vmr.items()}
+    for b in spec.vmr_blocks:
+        xv = x[b.range.start:b.range.stop]
+        if b.basis is None:
+            vmr[b.species] = np.exp(xv) if spec.log_vmr else xv.copy()
+        else:
+            vmr[b.species] = base_prof.vmr[b.species] * np.exp(b.basis @ xv)
+    prof = AtmosphericProfile(base_prof.pressure.copy(), T, base_prof.altitude.copy(), vmr)
+    T_sfc = float(x[spec.tsfc_index]) if spec.include_tsfc else float(T[0])
+    eps_sfc = float(x[spec.emis_index]) if spec.include_emissivity else 1.0
+    return prof, T_sfc, eps_sfc
 
 
 @dataclass
