@@ -85,6 +85,15 @@ extern "C" {
 
 #define IRS_LM_NONE 0           /* line_mixing=nothing */
 #define IRS_LM_VPY  1           /* first-order (VP_Y) mixing on the CO2 list, Y per line */
+#define IRS_LM_RELMAT 2         /* line_mixing=VPYLineMixing(relmat) or VPWLineMixing(relmat):
+                                   the perturbation of irs_fm_options.relmat, per band by
+                                   relmat_band_mode (line_mixing.jl:144-206) */
+
+/* Per-band treatment of a relaxation matrix (irs_fm_options.relmat_band_mode). */
+#define IRS_RELMAT_SKIP 0       /* not evaluated (filtered, or li > 8, or no W0/B0 table) */
+#define IRS_RELMAT_VPY  1       /* dispersive term with Y(T) (line_mixing.jl:1193-1349) */
+#define IRS_RELMAT_VPW  2       /* eigenmodes of D + i p W minus the no-coupling limit
+                                   (line_mixing.jl:892-1091) */
 
 typedef struct irs_ctx irs_ctx;
 
@@ -103,6 +112,26 @@ typedef struct {
     const float  *delta_air;   /* cm^-1/atm */
     const double *y_lm;        /* atm^-1, first-order line-mixing coefficient, or NULL */
 } irs_linelist;
+
+/* HITRAN relaxation-matrix data (HITRANRelmatData, line_mixing.jl:35-117): bands
+ * of S-file lines, each band's lines contiguous, and the W0/B0 tables keyed by
+ * (min(li, lf), max(li, lf)), one entry per (sub-table, Ji, Jip); sub-table 0..8 =
+ * pp pq pr qp qq qr rp rq rr (branch types of the initial and the coupled line). */
+typedef struct {
+    int32_t        n_bands;
+    const int8_t  *li, *lf, *isot, *molecule;   /* [n_bands]; isot 10 for HITRAN's '0' */
+    const int64_t *line_off;                    /* [n_bands + 1] */
+    const int16_t *Ji;                          /* per line */
+    const int8_t  *branch;                      /* -1 P, 0 Q, +1 R */
+    const double  *nu, *gV_air, *E_lower, *Dipo0, *PopuT0, *DipoT;
+    const float   *n_air, *shift;
+    int32_t        n_tables;
+    const int8_t  *table_lli, *table_llf;       /* [n_tables] */
+    const int64_t *entry_off;                   /* [n_tables + 1] */
+    const int8_t  *entry_sub;
+    const int32_t *entry_ji, *entry_jip;
+    const double  *entry_w0, *entry_b0;
+} irs_relmat;
 
 /* Sounder (sounder.jl:33-40) */
 typedef struct {
@@ -142,6 +171,11 @@ typedef struct {
     int    source_function;     /* IRS_SRC_CIM */
     int    continua;            /* apply_continuum + continua: bit mask of IRS_CONT_* added to
                                    every layer (0 = apply_continuum=false); default IRS_CONT_ALL */
+    int32_t relmat;             /* handle from irs_relmat_upload (line_mixing = IRS_LM_RELMAT), -1 */
+    int32_t lm_species;         /* HITRAN molecule id the relaxation matrix applies to (2) */
+    const uint8_t *relmat_band_mode;  /* [n_bands] IRS_RELMAT_* (line_mixing = IRS_LM_RELMAT) */
+    double keep_threshold;      /* 1e-4: VP_W poles below this fraction of the largest |A|
+                                   are dropped (line_mixing.jl:1065-1073) */
 } irs_fm_options;
 
 /* Timing and work counters of the last irs_forward / irs_xsec_layer call. */
@@ -256,6 +290,27 @@ int irs_cia_load_file(irs_ctx *ctx, int pair, const char *path);
 int irs_continuum_layer(irs_ctx *ctx, double nu0, double dnu, int64_t n_nu, int continua,
                         double vmr_h2o, double vmr_co2, double vmr_n2, double vmr_o2, double p_hpa,
                         double T, double dz_cm, double *tau_c);
+
+/* Keeps a relaxation matrix in the context (host memory: the per-layer W(T),
+ * Y(T) and eigenmodes are built on the host, their term sums run on the
+ * device) and builds every band's coupling table (line_mixing.jl:530-578). */
+int irs_relmat_upload(irs_ctx *ctx, const irs_relmat *rm, int32_t *handle);
+int irs_relmat_free(irs_ctx *ctx, int32_t handle);
+
+/* The line-mixing perturbation of a relaxation matrix at one (T, p_atm) on the
+ * grid nodes of (nu0, dnu): dsigma[n_nu] in cm^2/molec, per band by band_mode
+ * (IRS_RELMAT_*): _lm_perturbation of VPYLineMixing / VPWLineMixing
+ * (line_mixing.jl:283-303), without the Voigt baseline and clamp. */
+int irs_lm_perturbation(irs_ctx *ctx, int32_t relmat, const uint8_t *band_mode, double nu0, double dnu,
+                        int64_t n_nu, double T, double p_atm, double lm_cutoff, double x_far,
+                        double keep_threshold, double *dsigma);
+
+/* Host-only: the terms irs_lm_perturbation sums, for a relaxation matrix given
+ * directly (no context or device): up to `capacity` rows of {c, f, y, a, b} (the
+ * term adds a Re w + b Im w of w((nu - c) f + i y) where |nu - c| <= cutoff),
+ * sorted by c; *n_terms is the full count. */
+int irs_relmat_terms(const irs_relmat *rm, const uint8_t *band_mode, double T, double p_atm,
+                     double keep_threshold, int64_t capacity, double *terms, int64_t *n_terms);
 
 /* Copies a sorted line list to the device; returns a handle. */
 int irs_linelist_upload(irs_ctx *ctx, const irs_linelist *ll, int32_t *handle);

@@ -15,7 +15,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -25,6 +27,7 @@
 #include "irs_cont.h"
 #include "irs_internal.h"
 #include "irs_jac.h"
+#include "irs_relmat.h"
 
 hipError_t launch_keys(const double *nu, int64_t n, double nu_lo, double dnu, int32_t *keys,
                        hipStream_t s);
@@ -88,6 +91,11 @@ struct GroupPlan {
     double chod_floor;
     int continua;                  // IRS_CONT_* added to tau after the line sum (0: none)
     int cia_gen;                   // CIA table generation (graphs capture the table pointers)
+    // relaxation-matrix line mixing: the line sum runs once without the line-mixing
+    // species (items_lm1) and once with it alone (items_lm2, into tau_lm); the term sum
+    // then adds max(tau_lm + coef * dsigma, 0) to tau.  Never captured in a graph.
+    const void *items_lm1, *items_lm2, *tau_lm, *lm_terms, *lm_off, *lm_coef;
+    double lm_cutoff, x_far;
 };
 
 struct GraphEntry {
@@ -132,6 +140,8 @@ struct irs_ctx {
     int64_t cia_nk[3] = {0, 0, 0};
     int cia_gen = 0;
     unsigned char *scratch = nullptr;  // this device's copy of the no-allocation scratch array
+    std::vector<std::unique_ptr<RelmatHost>> relmats;   // uploaded relaxation matrices (host data)
+    DevBuf d_items_lm, d_tau_lm, d_lm_terms, d_lm_off, d_lm_coef, d_lm_out;
     PinBuf h_items, h_prof, h_out;
     std::vector<GraphEntry> graphs;
 };
@@ -776,6 +786,19 @@ double dbt_dr(double nu, double R) {   // vmr_jacobian.jl:120-125
 
 }  // namespace
 
+// TIPS-2024 Q(T), linear in T between the 1 K nodes, T / 296 without a table row
+// (src/HITRAN/partition.jl:10-26); the relaxation-matrix host code's Q ratios.
+double relmat_partition(int mol, int iso, double T) {
+    const int r = tips_row_of(mol, iso);
+    if (r < 0) return T / 296.0;
+    const double *v = irs_tips_q[r];
+    long lo = (long)std::floor(T) - IRS_TIPS_T_MIN, hi = lo + 1;
+    lo = std::min<long>(std::max<long>(lo, 0), IRS_TIPS_NT - 1);
+    hi = std::min<long>(std::max<long>(hi, 0), IRS_TIPS_NT - 1);
+    const double fr = T - std::floor(T);
+    return v[lo] * (1.0 - fr) + v[hi] * fr;
+}
+
 hipStream_t ctx_stream(irs_ctx *ctx) { return ctx->stream; }
 int ctx_device(irs_ctx *ctx) { return ctx->device; }
 int ctx_fail(irs_ctx *ctx, int code, const std::string &msg) { return fail(ctx, code, msg); }
@@ -800,6 +823,10 @@ void irs_default_options(irs_fm_options *o) {
     o->T_method = IRS_TMETHOD_LOGP_AT_PCG;
     o->source_function = IRS_SRC_CIM;
     o->continua = IRS_CONT_ALL;      // apply_continuum = true, all five continua (sounder.jl:245-246)
+    o->relmat = -1;
+    o->lm_species = 2;
+    o->relmat_band_mode = nullptr;
+    o->keep_threshold = 1e-4;        // compute_vpw_band_perturbation default (line_mixing.jl:1069)
 }
 
 int irs_device_info(int device, char *buf, int buflen) {
@@ -895,6 +922,8 @@ void irs_destroy(irs_ctx *ctx) {
     ctx->d_pblk.release();
     ctx->d_rec64.release();
     ctx->d_tau64.release();
+    for (DevBuf *b : {&ctx->d_items_lm, &ctx->d_tau_lm, &ctx->d_lm_terms, &ctx->d_lm_off, &ctx->d_lm_coef, &ctx->d_lm_out})
+        b->release();
     for (auto &b : ctx->d_cia) b.release();
     ctx->h_items.release();
     ctx->h_prof.release();
@@ -1004,6 +1033,85 @@ int irs_linelist_free(irs_ctx *ctx, int32_t handle) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->lists[handle].block) (void)hipFree(ctx->lists[handle].block);
     ctx->lists[handle] = ListHost();
+    return IRS_OK;
+}
+
+int irs_relmat_upload(irs_ctx *ctx, const irs_relmat *rm, int32_t *handle) {
+    if (!ctx || !rm || !handle) return fail(ctx, IRS_EINVAL, "null argument");
+    std::unique_ptr<RelmatHost> h(new RelmatHost);
+    std::string e = relmat_build(rm, *h);
+    if (!e.empty()) return fail(ctx, IRS_EINVAL, e);
+    for (size_t i = 0; i < ctx->relmats.size(); ++i)
+        if (!ctx->relmats[i]) {
+            ctx->relmats[i] = std::move(h);
+            *handle = (int32_t)i;
+            return IRS_OK;
+        }
+    ctx->relmats.push_back(std::move(h));
+    *handle = (int32_t)ctx->relmats.size() - 1;
+    return IRS_OK;
+}
+
+int irs_relmat_free(irs_ctx *ctx, int32_t handle) {
+    if (!ctx) return IRS_EINVAL;
+    if (handle < 0 || handle >= (int32_t)ctx->relmats.size() || !ctx->relmats[handle])
+        return fail(ctx, IRS_EINVAL, "bad relaxation-matrix handle");
+    ctx->relmats[handle].reset();
+    return IRS_OK;
+}
+
+// Uploads per-unit term lists (units in order) and their offsets.
+static int upload_lm_terms(irs_ctx *ctx, const std::vector<std::vector<LmTerm>> &units) {
+    std::vector<int64_t> off(units.size() + 1, 0);
+    for (size_t u = 0; u < units.size(); ++u) off[u + 1] = off[u] + (int64_t)units[u].size();
+    std::vector<LmTerm> flat((size_t)std::max<int64_t>(off.back(), 1));
+    for (size_t u = 0; u < units.size(); ++u)
+        std::copy(units[u].begin(), units[u].end(), flat.begin() + off[u]);
+    HIPCHK(ensure(ctx->d_lm_terms, sizeof(LmTerm) * flat.size()));
+    HIPCHK(ensure(ctx->d_lm_off, sizeof(int64_t) * off.size()));
+    HIPCHK(hipMemcpyAsync(ctx->d_lm_terms.p, flat.data(), sizeof(LmTerm) * flat.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_lm_off.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));     // the host vectors go out of scope
+    return IRS_OK;
+}
+
+static int check_relmat(irs_ctx *ctx, int32_t h) {
+    if (h < 0 || h >= (int32_t)ctx->relmats.size() || !ctx->relmats[h])
+        return fail(ctx, IRS_EINVAL, "bad relaxation-matrix handle");
+    return IRS_OK;
+}
+
+int irs_lm_perturbation(irs_ctx *ctx, int32_t relmat, const uint8_t *band_mode, double nu0, double dnu,
+                        int64_t n_nu, double T, double p_atm, double lm_cutoff, double x_far,
+                        double keep_threshold, double *dsigma) {
+    if (!ctx || !dsigma || n_nu <= 0 || !(dnu > 0.0)) return fail(ctx, IRS_EINVAL, "bad arguments");
+    (void)hipSetDevice(ctx->device);
+    int rc = check_relmat(ctx, relmat);
+    if (rc) return rc;
+    std::vector<std::vector<LmTerm>> units(1);
+    std::string e = relmat_terms(*ctx->relmats[relmat], band_mode, T, p_atm, keep_threshold, units[0]);
+    if (!e.empty()) return fail(ctx, IRS_EINVAL, e);
+    rc = upload_lm_terms(ctx, units);
+    if (rc) return rc;
+    HIPCHK(ensure(ctx->d_lm_out, sizeof(double) * (size_t)n_nu));
+    const Grid g = make_grid(nu0, dnu, nu0 + (double)(n_nu - 1) * dnu);
+    LmTermArgs a{};
+    a.terms = (const LmTerm *)ctx->d_lm_terms.p;
+    a.off = (const int64_t *)ctx->d_lm_off.p;
+    a.n = n_nu;
+    a.n_units = 1;
+    a.an = g.an;
+    a.sn = g.sn;
+    a.den = g.den;
+    a.cutoff = lm_cutoff;
+    a.x_far = x_far;
+    a.out64 = (double *)ctx->d_lm_out.p;
+    HIPCHK(launch_lm_terms(a, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dsigma, ctx->d_lm_out.p, sizeof(double) * (size_t)n_nu, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return IRS_OK;
 }
 
@@ -1387,7 +1495,28 @@ static hipError_t enqueue_group(irs_ctx *ctx, const GroupPlan &pl, bool timing) 
     la.pre = pa.pre;
     la.n_rec = pa.n_rec;
     la.rec3 = pa.rec3;
+    if (pl.items_lm1) la.items = (const ItemDev *)pl.items_lm1;
     ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
+    if (pl.items_lm1) {
+        ENQ(launch_reset_counters((unsigned *)pl.queue, nullptr, 0, st));
+        la.items = (const ItemDev *)pl.items_lm2;
+        la.out = (float *)pl.tau_lm;
+        ENQ(launch_linesum(la, pl.P, pl.pblocks, st));
+        LmTermArgs lt{};
+        lt.terms = (const LmTerm *)pl.lm_terms;
+        lt.off = (const int64_t *)pl.lm_off;
+        lt.n = pl.n;
+        lt.n_units = pl.gn * L;
+        lt.an = pl.an;
+        lt.sn = pl.sn;
+        lt.den = pl.den;
+        lt.cutoff = pl.lm_cutoff;
+        lt.x_far = pl.x_far;
+        lt.tau = (float *)pl.tau;
+        lt.tau_lm = (const float *)pl.tau_lm;
+        lt.coef = (const double *)pl.lm_coef;
+        ENQ(launch_lm_terms(lt, st));
+    }
     if (timing) ENQ(hipEventRecord(ctx->ev[2], st));
     if (pl.continua) {
         ContArgs co = cont_args(ctx, pl.continua);
@@ -1732,6 +1861,49 @@ static int run_profile_fp64(irs_ctx *ctx, const ItemDev *items_h, int L, int ns,
     return IRS_OK;
 }
 
+// Line-mixing terms of every (profile, layer) of a group at the line-mixing
+// species' T_cg, p_cg (built on host threads), their upload, the per-unit coef and
+// the two item arrays of the split line sum (items_lm1: every species but s_lm,
+// items_lm2: s_lm alone).
+static int prepare_relmat(irs_ctx *ctx, const ItemDev *items, int gn, int L, int ns, int s_lm,
+                          const irs_fm_options &opt) {
+    const RelmatHost &rm = *ctx->relmats[opt.relmat];
+    const int nu_units = gn * L;
+    std::vector<std::vector<LmTerm>> units((size_t)nu_units);
+    std::vector<double> coef((size_t)nu_units, 0.0);
+    std::vector<std::string> errs((size_t)nu_units);
+    const int nth = std::max(1, std::min<int>(nu_units, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nth; ++w)
+        pool.emplace_back([&, w]() {
+            for (int u = w; u < nu_units; u += nth) {
+                const ItemDev &it = items[(size_t)u * ns + s_lm];
+                if (!it.active) continue;
+                coef[u] = it.coef;
+                errs[u] = relmat_terms(rm, opt.relmat_band_mode, it.T, it.p_atm, opt.keep_threshold, units[u]);
+            }
+        });
+    for (auto &t : pool) t.join();
+    for (const auto &e : errs)
+        if (!e.empty()) return fail(ctx, IRS_EINVAL, e);
+    int rc = upload_lm_terms(ctx, units);
+    if (rc) return rc;
+    const size_t ni = (size_t)gn * L * ns;
+    std::vector<ItemDev> two(2 * ni);
+    for (size_t i = 0; i < ni; ++i) {
+        two[i] = items[i];
+        two[ni + i] = items[i];
+        if ((int)(i % ns) == s_lm) two[i].active = 0;
+        else two[ni + i].active = 0;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_items_lm.p, two.data(), sizeof(ItemDev) * two.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_lm_coef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IRS_OK;
+}
+
 static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, const irs_sounder *snd,
                        const irs_fm_options *opt_in, int mode, double chod_floor, double *R, double *BT,
                        float *tau_ch, float *cim_g, float *ils_mom = nullptr) {
@@ -1747,8 +1919,16 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     if (opt.source_function != IRS_SRC_TOON && opt.source_function != IRS_SRC_CIM)
         return fail(ctx, IRS_EINVAL, "source_function must be :toon or :cim");
     if (opt.apodization < 0 || opt.apodization > 3) return fail(ctx, IRS_EINVAL, "Unknown apodization");
-    if (opt.line_mixing != IRS_LM_NONE && opt.line_mixing != IRS_LM_VPY) return fail(ctx, IRS_EINVAL, "bad line_mixing");
+    if (opt.line_mixing != IRS_LM_NONE && opt.line_mixing != IRS_LM_VPY && opt.line_mixing != IRS_LM_RELMAT)
+        return fail(ctx, IRS_EINVAL, "bad line_mixing");
     if (opt.continua < 0 || opt.continua > IRS_CONT_ALL) return fail(ctx, IRS_EINVAL, "bad continua mask");
+    const bool relmat = opt.line_mixing == IRS_LM_RELMAT;
+    if (relmat) {
+        if (opt.relmat < 0 || opt.relmat >= (int32_t)ctx->relmats.size() || !ctx->relmats[opt.relmat])
+            return fail(ctx, IRS_EINVAL, "bad relaxation-matrix handle");
+        if (ctx->sched.fp64_every_pair && mode == 0)
+            return fail(ctx, IRS_EINVAL, "the every-pair FP64 path has no relaxation-matrix line mixing");
+    }
     auto t0 = std::chrono::steady_clock::now();
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
 
@@ -1822,7 +2002,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
             lines_per_prof += ctx->lists[profiles[0].linelist[s]].n;
             max_lines = std::max<int>(max_lines, (int)ctx->lists[profiles[0].linelist[s]].n);
         }
-    const double bytes_per_prof = (double)L * (lines_per_prof * 168.0 + n * 4.0) + n * 8.0;
+    const double bytes_per_prof = (double)L * (lines_per_prof * 168.0 + n * (relmat ? 8.0 : 4.0)) + n * 8.0;
     double budget = 32.0e9;
     if (const char *e = std::getenv("IRS_GROUP_BYTES")) budget = std::max(1.0, std::atof(e));
     int gmax = (int)std::max(1.0, std::min((double)n_prof, budget / std::max(bytes_per_prof, 1.0)));
@@ -1862,6 +2042,17 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     HIPCHK(ensure_pinned(ctx->h_out, std::max(sizeof(double) * 2 * nch * group,
                                               sizeof(float) * (3 * L + 1) * nch * group)));
     if (mode >= 1) HIPCHK(ensure(ctx->d_chod, sizeof(float) * nch * L * group));
+    // relaxation-matrix line mixing: the species it applies to, its second line-sum output
+    int s_lm = -1;
+    if (relmat) {
+        for (int s = 0; s < ns; ++s)
+            if (profiles[0].species[s] == opt.lm_species) { s_lm = s; break; }
+        if (s_lm >= 0) {
+            HIPCHK(ensure(ctx->d_tau_lm, sizeof(float) * (size_t)n * L * group));
+            HIPCHK(ensure(ctx->d_items_lm, 2 * sizeof(ItemDev) * (size_t)group * L * ns));
+            HIPCHK(ensure(ctx->d_lm_coef, sizeof(double) * (size_t)group * L));
+        }
+    }
     if (mode >= 2) {
         HIPCHK(ensure(ctx->d_chlev, sizeof(float) * nch * (L + 1) * group));
         HIPCHK(ensure(ctx->d_chg, sizeof(float) * nch * L * group));
@@ -1901,7 +2092,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                         it = make_item(ctx->lists[h], h, T, p_atm, vs, coef, opt.dptmn, opt.cutoff,
                                        opt.lm_cutoff, opt.x_far, lm, dnu);
                         // DPTMIN is skipped on the line-mixing CO2 path (sounder.jl:295-299)
-                        if (sp == 2 && opt.line_mixing == IRS_LM_VPY) it.dptmn = 0.0;
+                        if (sp == 2 && opt.line_mixing != IRS_LM_NONE) it.dptmn = 0.0;
                         it.key_off = koff[h];
                         it.ktab_off = ktoff.count(h) ? ktoff[h] : 0;
                     }
@@ -1937,6 +2128,10 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                 P = ctx->tuned[key].first;
                 persistent = ctx->tuned[key].second;
             }
+        }
+        if (s_lm >= 0) {
+            rc = prepare_relmat(ctx, items, gn, L, ns, s_lm, opt);
+            if (rc) return rc;
         }
         if (fp64) {
             const int64_t ch_stride = channel_stride(cidx, cw);
@@ -1995,7 +2190,18 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
         }
         pl.continua = opt.continua;
         pl.cia_gen = opt.continua ? ctx->cia_gen : 0;
-        if (ctx->sched.use_graph) {
+        if (s_lm >= 0) {
+            pl.items_lm1 = ctx->d_items_lm.p;
+            pl.items_lm2 = (const ItemDev *)ctx->d_items_lm.p + (size_t)gn * L * ns;
+            pl.tau_lm = ctx->d_tau_lm.p;
+            pl.lm_terms = ctx->d_lm_terms.p;
+            pl.lm_off = ctx->d_lm_off.p;
+            pl.lm_coef = ctx->d_lm_coef.p;
+            pl.lm_cutoff = opt.lm_cutoff;
+            pl.x_far = opt.x_far;
+        }
+        const bool graph = ctx->sched.use_graph && s_lm < 0;
+        if (graph) {
             rc = launch_group_graph(ctx, pl);
             if (rc) return rc;
         } else {
@@ -2003,7 +2209,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
             if (e != hipSuccess) return fail(ctx, IRS_EHIP, std::string("forward: ") + hipGetErrorString(e));
         }
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (ctx->sched.use_graph) {
+        if (graph) {
             ms_graph += elapsed_ms(ctx->ev[5], ctx->ev[6]);
         } else {
             ms_prep += elapsed_ms(ctx->ev[0], ctx->ev[1]);
@@ -2037,7 +2243,8 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
     ctx->stats.ms_rte = ms_rte;
     ctx->stats.ms_ils = ms_ils;
     ctx->stats.ms_continuum = ms_cont;
-    ctx->stats.ms_device = (ctx->sched.use_graph && !fp64) ? ms_graph : ms_prep + ms_ls + ms_cont + ms_rte + ms_ils;
+    ctx->stats.ms_device = (ctx->sched.use_graph && !fp64 && !relmat) ? ms_graph
+                                                                         : ms_prep + ms_ls + ms_cont + ms_rte + ms_ils;
     ctx->stats.n_hi = n;
     ctx->stats.n_layers = L;
     ctx->stats.tile_points = P;
@@ -2275,6 +2482,9 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     if (opt.source_function != IRS_SRC_TOON && opt.source_function != IRS_SRC_CIM)
         return fail(ctx, IRS_EINVAL, "source_function must be :toon or :cim");
     if (opt.continua < 0 || opt.continua > IRS_CONT_ALL) return fail(ctx, IRS_EINVAL, "bad continua mask");
+    if (opt.line_mixing == IRS_LM_RELMAT &&
+        (opt.relmat < 0 || opt.relmat >= (int32_t)ctx->relmats.size() || !ctx->relmats[opt.relmat]))
+        return fail(ctx, IRS_EINVAL, "bad relaxation-matrix handle");
     auto t0 = std::chrono::steady_clock::now();
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
     const Grid g = internal_grid(snd, &opt, nullptr);
@@ -2401,6 +2611,69 @@ int irs_jacobian(irs_ctx *ctx, const irs_profile *pr, const irs_sounder *snd, co
     sa.dthi = sa.dtlo + NL;
     sa.vlo = (float *)ctx->j_v.p;
     sa.vhi = sa.vlo + NL * std::max(nr, 1);
+    if (opt.line_mixing == IRS_LM_RELMAT) {
+        // the line-mixing perturbation of the relaxation matrix's species and its central
+        // differences in T (0.02 K) and p (1e-4 relative) per layer, the terms in FP64
+        // (cross_section_jacobian.jl:256-257,283-306)
+        int s_lm = -1;
+        for (int s = 0; s < ns; ++s)
+            if (pr->species[s] == opt.lm_species) { s_lm = s; break; }
+        if (s_lm >= 0) {
+            const RelmatHost &rmh = *ctx->relmats[opt.relmat];
+            const double hT = 0.02;
+            std::vector<std::vector<LmTerm>> units((size_t)5 * L);
+            std::vector<std::string> errs((size_t)L);
+            const int nth = std::max(1, std::min<int>(L, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+            std::vector<std::thread> pool;
+            for (int w = 0; w < nth; ++w)
+                pool.emplace_back([&, w]() {
+                    for (int k = w; k < L; k += nth) {
+                        const JacItemDev &it = items[(size_t)k * ns + s_lm];
+                        if (!it.active) continue;
+                        const double T = it.T, pa0 = it.p_atm, hp = std::max(pa0, 1e-6) * 1e-4;
+                        const double Ts[5] = {T, T + hT, T - hT, T, T};
+                        const double ps[5] = {pa0, pa0, pa0, pa0 + hp, pa0 - hp};
+                        for (int v = 0; v < 5 && errs[k].empty(); ++v) {
+                            errs[k] = relmat_terms(rmh, opt.relmat_band_mode, Ts[v], ps[v], opt.keep_threshold,
+                                                   units[(size_t)5 * k + v]);
+                            for (auto &tm : units[(size_t)5 * k + v]) tm.fp64 = 1;
+                        }
+                    }
+                });
+            for (auto &t : pool) t.join();
+            for (const auto &e : errs)
+                if (!e.empty()) return fail(ctx, IRS_EINVAL, e);
+            HIPCHK(ensure(ctx->d_tau_lm, sizeof(float) * 3 * NL));
+            HIPCHK(ensure(ctx->d_lm_out, sizeof(double) * 5 * (size_t)n));
+            HIPCHK(hipMemsetAsync(ctx->d_tau_lm.p, 0, sizeof(float) * 3 * NL, ctx->stream));
+            float *lm0 = (float *)ctx->d_tau_lm.p, *lmT = lm0 + NL, *lmP = lmT + NL;
+            for (int k = 0; k < L; ++k) {
+                const JacItemDev &it = items[(size_t)k * ns + s_lm];
+                if (!it.active) continue;
+                std::vector<std::vector<LmTerm>> five(units.begin() + 5 * k, units.begin() + 5 * k + 5);
+                int rc2 = upload_lm_terms(ctx, five);
+                if (rc2) return rc2;
+                LmTermArgs la{};
+                la.terms = (const LmTerm *)ctx->d_lm_terms.p;
+                la.off = (const int64_t *)ctx->d_lm_off.p;
+                la.n = n;
+                la.n_units = 5;
+                la.an = g.an;
+                la.sn = g.sn;
+                la.den = g.den;
+                la.cutoff = opt.lm_cutoff;
+                la.x_far = opt.x_far;
+                la.out64 = (double *)ctx->d_lm_out.p;
+                HIPCHK(launch_lm_terms(la, ctx->stream));
+                HIPCHK(launch_lm_fd((const double *)ctx->d_lm_out.p, n, it.coef, hT, std::max(it.p_atm, 1e-6) * 1e-4,
+                                    lm0 + (size_t)k * n, lmT + (size_t)k * n, lmP + (size_t)k * n, ctx->stream));
+            }
+            sa.lm0 = lm0;
+            sa.lmT = lmT;
+            sa.lmP = lmP;
+            sa.lm_s = s_lm;
+        }
+    }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     HIPCHK(launch_grad_linesum(sa, ctx->stream));
     if (opt.continua) {

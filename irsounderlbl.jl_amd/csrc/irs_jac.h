@@ -45,6 +45,10 @@ struct JacSumArgs {
     double dnu;                   // internal grid step (cm^-1)
     double nu_lo;                 // wavenumber of grid point 0
     int far_d, far_hc;            // coarse far-wing split (grid steps); far_hc = 0 turns it off
+    // optional [L][n]: relaxation-matrix line-mixing perturbation of species lm_s and its
+    // T and p derivatives (coef folded), added before that species' final clamp
+    const float *lm0, *lmT, *lmP;
+    int lm_s;
 };
 
 struct RteJacArgs {
@@ -86,6 +90,8 @@ struct JacIlsArgs {
 
 hipError_t launch_prep_grad(const JacPrepArgs &a, int max_lines, hipStream_t s);
 hipError_t launch_grad_linesum(const JacSumArgs &a, hipStream_t s);
+hipError_t launch_lm_fd(const double *d, int64_t n, double coef, double hT, double hp, float *lm0, float *lmT,
+                        float *lmP, hipStream_t s);
 hipError_t launch_rte_jac(const RteJacArgs &a, hipStream_t s);
 hipError_t launch_jac_columns(const JacColArgs &a, int n_cols, hipStream_t s);
 hipError_t launch_jac_ils(const JacIlsArgs &a, int n_cols, hipStream_t s);

@@ -684,6 +684,15 @@ __global__ void __launch_bounds__(256, IRS_JAC_MINB) grad_linesum_kernel(const J
                 if (sg <= 0.0) { sg = 0.0; gT = 0.0; gP = 0.0; }
             }
             const int64_t o = (int64_t)layer * a.n_hi + i;
+            if (a.lm0 && s == a.lm_s) {
+                // relaxation-matrix perturbation and its central differences in (T, p),
+                // then the forward's max(sigma + dsigma, 0) clamp
+                // (cross_section_jacobian.jl:292-319)
+                sg += (double)a.lm0[o];
+                gT += (double)a.lmT[o];
+                gP += (double)a.lmP[o];
+                if (sg <= 0.0) { sg = 0.0; gT = 0.0; gP = 0.0; }
+            }
             a.tau[o] += (float)sg;
             a.dtlo[o] += (float)(gT * it.wT_lo);
             a.dthi[o] += (float)(gT * it.wT_hi);
@@ -694,6 +703,25 @@ __global__ void __launch_bounds__(256, IRS_JAC_MINB) grad_linesum_kernel(const J
             }
         }
     }
+}
+
+// Central differences of the line-mixing perturbation of one layer: d [5][n] at
+// (T, p), (T + hT, p), (T - hT, p), (T, p + hp), (T, p - hp); coef folded in.
+__global__ void __launch_bounds__(256) lm_fd_kernel(const double *d, int64_t n, double coef, double hT, double hp,
+                                                    float *lm0, float *lmT, float *lmP) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    lm0[i] = (float)(coef * d[i]);
+    lmT[i] = (float)(coef * (d[n + i] - d[2 * n + i]) / (2.0 * hT));
+    lmP[i] = (float)(coef * (d[3 * n + i] - d[4 * n + i]) / (2.0 * hp));
+}
+
+hipError_t launch_lm_fd(const double *d, int64_t n, double coef, double hT, double hp, float *lm0, float *lmT,
+                        float *lmP, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lm_fd_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, n, coef, hT, hp, lm0,
+                       lmT, lmP);
+    return hipGetLastError();
 }
 
 hipError_t launch_grad_linesum(const JacSumArgs &a, hipStream_t s) {

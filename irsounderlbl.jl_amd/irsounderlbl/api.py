@@ -23,6 +23,8 @@ from typing import Dict, Optional
 
 import numpy as np
 
+from .relmat import HITRANRelmatData, VPWLineMixing, VPYLineMixing, linelist_isotopes
+from .relmat import band_modes_for as relmat_band_modes
 from .types import (AtmosphericProfile, GasSpecies, HITRANLinelist, Jacobian, Sounder,
                     StateVectorSpec, ViewingGeometry, WavenumberGrid, IASIInstrument,
                     nadir_geometry, sounder_grid)
@@ -69,7 +71,46 @@ class IrsOptions(C.Structure):
     _fields_ = [("internal_dnu", f64), ("high_res_factor", i32), ("cutoff", f64),
                 ("lm_cutoff", f64), ("dptmn", f64), ("x_far", f64), ("with_ils", i32),
                 ("apodization", i32), ("line_mixing", i32), ("T_method", i32),
-                ("source_function", i32), ("continua", i32)]
+                ("source_function", i32), ("continua", i32), ("relmat", i32), ("lm_species", i32),
+                ("relmat_band_mode", P), ("keep_threshold", f64)]
+
+
+class IrsRelmat(C.Structure):
+    _fields_ = [("n_bands", i32), ("li", P), ("lf", P), ("isot", P), ("molecule", P), ("line_off", P),
+                ("Ji", P), ("branch", P), ("nu", P), ("gV_air", P), ("E_lower", P), ("Dipo0", P),
+                ("PopuT0", P), ("DipoT", P), ("n_air", P), ("shift", P), ("n_tables", i32),
+                ("table_lli", P), ("table_llf", P), ("entry_off", P), ("entry_sub", P), ("entry_ji", P),
+                ("entry_jip", P), ("entry_w0", P), ("entry_b0", P)]
+
+
+def relmat_struct(data):
+    """ctypes view of a HITRANRelmatData (irs_relmat) and the arrays it points into."""
+    bands = data.bands
+    keys = sorted(data.wtfit.keys())
+    cat = (lambda xs, dt: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros(0), dt))
+    arr = dict(
+        li=np.array([b.li for b in bands], np.int8), lf=np.array([b.lf for b in bands], np.int8),
+        isot=np.array([b.isot for b in bands], np.int8), molecule=np.array([b.molecule for b in bands], np.int8),
+        line_off=np.concatenate([[0], np.cumsum([b.n for b in bands])]).astype(np.int64),
+        Ji=cat([b.Ji for b in bands], np.int16), branch=cat([b.branch for b in bands], np.int8),
+        nu=cat([b.nu for b in bands], np.float64), gV_air=cat([b.gV_air for b in bands], np.float64),
+        E_lower=cat([b.E_lower for b in bands], np.float64), Dipo0=cat([b.Dipo0 for b in bands], np.float64),
+        PopuT0=cat([b.PopuT0 for b in bands], np.float64), DipoT=cat([b.DipoT for b in bands], np.float64),
+        n_air=cat([b.n_air for b in bands], np.float32), shift=cat([b.shift for b in bands], np.float32))
+    ents = [list(data.wtfit[k].entries()) for k in keys]
+    flat = [e for es in ents for e in es]
+    arr.update(table_lli=np.array([k[0] for k in keys], np.int8), table_llf=np.array([k[1] for k in keys], np.int8),
+               entry_off=np.concatenate([[0], np.cumsum([len(e) for e in ents])]).astype(np.int64),
+               entry_sub=np.array([e[0] for e in flat], np.int8), entry_ji=np.array([e[1] for e in flat], np.int32),
+               entry_jip=np.array([e[2] for e in flat], np.int32),
+               entry_w0=np.array([e[3] for e in flat], np.float64), entry_b0=np.array([e[4] for e in flat], np.float64))
+    arr = {k: np.ascontiguousarray(v) for k, v in arr.items()}
+    st = IrsRelmat(len(bands), *[_p(arr[k]) for k in ("li", "lf", "isot", "molecule", "line_off", "Ji", "branch",
+                                                        "nu", "gV_air", "E_lower", "Dipo0", "PopuT0", "DipoT",
+                                                        "n_air", "shift")],
+                   len(keys), *[_p(arr[k]) for k in ("table_lli", "table_llf", "entry_off", "entry_sub", "entry_ji",
+                                                      "entry_jip", "entry_w0", "entry_b0")])
+    return st, arr
 
 
 class IrsStateSpec(C.Structure):
@@ -104,7 +145,8 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_mlp_input_jacobian", "irs_emu_create", "irs_emu_destroy", "irs_emu_last_error",
            "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian", "irs_cia_set",
            "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc",
-           "irs_channel_od_moments", "irs_par_parse", "irs_par_load")
+           "irs_channel_od_moments", "irs_par_parse", "irs_par_load", "irs_relmat_upload",
+           "irs_relmat_free", "irs_lm_perturbation", "irs_relmat_terms")
 
 _lib = None
 
@@ -178,6 +220,10 @@ def load_library(path: str = LIB_PATH):
     L.irs_cia_set.argtypes = [P, i32, i32, P, P, P, P]
     L.irs_cia_load_file.argtypes = [P, i32, C.c_char_p]
     L.irs_continuum_layer.argtypes = [P, f64, f64, i64, i32, f64, f64, f64, f64, f64, f64, f64, P]
+    L.irs_relmat_upload.argtypes = [P, C.POINTER(IrsRelmat), C.POINTER(C.c_int32)]
+    L.irs_relmat_free.argtypes = [P, C.c_int32]
+    L.irs_lm_perturbation.argtypes = [P, C.c_int32, P, f64, f64, i64, f64, f64, f64, f64, f64, P]
+    L.irs_relmat_terms.argtypes = [C.POINTER(IrsRelmat), P, f64, f64, f64, i64, P, C.POINTER(i64)]
     for name in EXPORTS:
         if name not in ("irs_destroy", "irs_default_options", "irs_last_error", "irs_mlp_destroy",
                         "irs_mlp_last_error", "irs_mlp_n_params", "irs_emu_destroy", "irs_emu_last_error"):
@@ -286,6 +332,30 @@ class Context:
             del ctx._lists[key]
         ctx.lib.irs_linelist_free(ctx.h, handle)
 
+    def upload_relmat(self, data) -> int:
+        """Handle of a relaxation matrix (HITRANRelmatData), uploaded on first use and
+        released when the data object is garbage collected (irs_relmat_free)."""
+        key = ("relmat", id(data))
+        hit = self._lists.get(key)
+        if hit is not None and hit[1]() is data:
+            return hit[0]
+        st, _keep = relmat_struct(data)
+        h = C.c_int32()
+        self.check(self.lib.irs_relmat_upload(self.h, C.byref(st), C.byref(h)))
+        fin = weakref.finalize(data, Context._release_relmat, weakref.ref(self), key, h.value)
+        self._lists[key] = (h.value, weakref.ref(data), fin)
+        return h.value
+
+    @staticmethod
+    def _release_relmat(ctx_ref, key, handle):
+        ctx = ctx_ref()
+        if ctx is None or not getattr(ctx, "h", None):
+            return
+        ent = ctx._lists.get(key)
+        if ent is not None and ent[0] == handle:
+            del ctx._lists[key]
+        ctx.lib.irs_relmat_free(ctx.h, handle)
+
     def set_cia(self, pair, blocks):
         """CIA table of one collision pair ("CO2-CO2", "N2-N2", "O2-O2"): blocks
         is a list of (T, nu[], sigma[]) (cm^5 molec^-2); an empty list removes
@@ -347,19 +417,83 @@ def faddeeva32(x, y, *, rule: str = "core", ctx: Optional[Context] = None) -> np
     return (re + 1j * im).reshape(shape)
 
 
-def compute_voigt_lm_cross_sections(nu_grid: WavenumberGrid, ll_co2: HITRANLinelist, T: float,
-                                     p_atm: float, *, cutoff: float = 25.0,
-                                     lm_cutoff: Optional[float] = None, x_far: float = 122.0,
+def compute_voigt_lm_cross_sections(nu_grid: WavenumberGrid, ll_co2: HITRANLinelist, *args,
+                                     cutoff: float = 25.0, lm_cutoff: Optional[float] = None, x_far: float = 122.0,
+                                     min_band_strength: float = 0.0, isotopes=None,
                                      ctx: Optional[Context] = None) -> np.ndarray:
-    """max(max(sigma_voigt, 0) + sigma_disp, 0) with per-line Y (VP_Y)."""
+    """max(max(sigma_voigt, 0) + sigma_disp, 0) (src/CrossSections/line_mixing.jl:1351-1369).
+
+    compute_voigt_lm_cross_sections(nu_grid, ll_co2, relmat, T, p_atm; ...) takes the
+    dispersive term from the relaxation matrix (Y(T) per band, S-file lines, the
+    isotopologues of ll_co2 unless `isotopes` is given);
+    compute_voigt_lm_cross_sections(nu_grid, ll_co2, T, p_atm; ...) from the per-line
+    Y of ll_co2 (y_lm)."""
     ctx = ctx or default_context()
+    lc = cutoff if lm_cutoff is None else lm_cutoff
+    if args and isinstance(args[0], HITRANRelmatData):
+        relmat, T, p_atm = args
+        sv = compute_voigt_cross_sections(nu_grid, ll_co2, T, p_atm, cutoff=cutoff, x_far=x_far, ctx=ctx)
+        iso = linelist_isotopes(ll_co2) if isotopes is None else set(isotopes)
+        d = lm_perturbation(nu_grid, VPYLineMixing(relmat, min_band_strength), T, p_atm, cutoff=lc, x_far=x_far,
+                            isotopes=iso, ctx=ctx)
+        return np.maximum(sv + d, 0.0)
+    T, p_atm = args
     h = ctx.upload(ll_co2)
     out = np.zeros(nu_grid.n)
-    lc = cutoff if lm_cutoff is None else lm_cutoff
     ctx.check(ctx.lib.irs_xsec_layer(ctx.h, h, nu_grid.nu_min, nu_grid.dnu, nu_grid.n, float(T),
                                      float(p_atm), 0.0, 1.0, 0.0, float(cutoff), float(lc),
                                      float(x_far), 1, _p(out)))
     return out
+
+
+def compute_voigt_vpw_cross_sections(nu_grid: WavenumberGrid, ll_co2: HITRANLinelist, relmat, T: float,
+                                     p_atm: float, *, whitelist, cutoff: float = 25.0,
+                                     lm_cutoff: Optional[float] = None, keep_threshold: float = 1e-4,
+                                     min_band_strength: float = 0.0, isotopes=None, x_far: float = 122.0,
+                                     ctx: Optional[Context] = None) -> np.ndarray:
+    """max(sigma_voigt + dsigma, 0) with the eigenmode (VP_W) perturbation of the
+    whitelisted bands and the VP_Y term of the others (line_mixing.jl:1093-1118)."""
+    ctx = ctx or default_context()
+    sv = compute_voigt_cross_sections(nu_grid, ll_co2, T, p_atm, cutoff=cutoff, x_far=x_far, ctx=ctx)
+    iso = linelist_isotopes(ll_co2) if isotopes is None else set(isotopes)
+    d = lm_perturbation(nu_grid, VPWLineMixing(relmat, set(whitelist), min_band_strength), T, p_atm,
+                        cutoff=cutoff if lm_cutoff is None else lm_cutoff, x_far=x_far, isotopes=iso,
+                        keep_threshold=keep_threshold, ctx=ctx)
+    return np.maximum(sv + d, 0.0)
+
+
+def lm_perturbation(nu_grid: WavenumberGrid, line_mixing, T: float, p_atm: float, *, cutoff: float = 25.0,
+                    x_far: float = 122.0, isotopes=None, keep_threshold: float = 1e-4,
+                    ctx: Optional[Context] = None) -> np.ndarray:
+    """The additive line-mixing term dsigma of a VPYLineMixing / VPWLineMixing
+    (_lm_perturbation, line_mixing.jl:283-303), evaluated on the device."""
+    ctx = ctx or default_context()
+    h = ctx.upload_relmat(line_mixing.data)
+    modes = np.ascontiguousarray(relmat_band_modes(line_mixing, None if isotopes is None else set(isotopes)),
+                                 np.uint8)
+    out = np.zeros(nu_grid.n)
+    ctx.check(ctx.lib.irs_lm_perturbation(ctx.h, h, _p(modes) if len(modes) else None, nu_grid.nu_min,
+                                          nu_grid.dnu, nu_grid.n, float(T), float(p_atm), float(cutoff),
+                                          float(x_far), float(keep_threshold), _p(out)))
+    return out
+
+
+def relmat_terms(line_mixing, T: float, p_atm: float, *, isotopes=None, keep_threshold: float = 1e-4) -> np.ndarray:
+    """Host-side terms [n, 5] = (c, f, y, a, b) the device sums for one layer (no GPU)."""
+    lib = load_library()
+    st, _keep = relmat_struct(line_mixing.data)
+    modes = np.ascontiguousarray(relmat_band_modes(line_mixing, None if isotopes is None else set(isotopes)),
+                                 np.uint8)
+    n = C.c_int64()
+    mp = _p(modes) if len(modes) else None
+    if lib.irs_relmat_terms(C.byref(st), mp, float(T), float(p_atm), float(keep_threshold), 0, None,
+                            C.byref(n)) != IRS_OK:
+        raise ValueError("irs_relmat_terms: bad arguments")
+    out = np.zeros((max(n.value, 1), 5))
+    if lib.irs_relmat_terms(C.byref(st), mp, float(T), float(p_atm), float(keep_threshold), n.value, _p(out),
+                            C.byref(n)) != IRS_OK:
+        raise ValueError("irs_relmat_terms: bad arguments")
+    return out[:n.value]
 
 
 def _continuum(bit, nu_grid, vmr, p_hPa, T, ctx):
@@ -512,7 +646,9 @@ def _options(internal_dnu=0.001, high_res_factor=None, cutoff=25.0, lm_cutoff=No
     o.with_ils = int(bool(with_ils))
     o.apodization = _apod(apodization)
     lm = line_mixing
-    if lm in (None, False, 0, "none"):
+    if isinstance(lm, (VPYLineMixing, VPWLineMixing)):
+        o.line_mixing = 2            # completed by _relmat_options
+    elif lm in (None, False, 0, "none"):
         o.line_mixing = 0
     elif lm in ("vpy", "VPY", 1, True):
         o.line_mixing = 1
@@ -528,6 +664,23 @@ def _options(internal_dnu=0.001, high_res_factor=None, cutoff=25.0, lm_cutoff=No
     o.source_function = SOURCE_FUNCTION[sf]
     o.continua = int(continua_mask)
     return o
+
+
+def _relmat_options(o, line_mixing, ctx, linelists):
+    """Relaxation-matrix fields of the options for a VPYLineMixing / VPWLineMixing:
+    the uploaded matrix, its species and the per-band treatment, restricted to the
+    isotopologues of that species' line list (line_mixing.jl:199-206,222-241).
+    Returns the band-mode array (kept alive by the caller)."""
+    if not isinstance(line_mixing, (VPYLineMixing, VPWLineMixing)):
+        return None
+    data = line_mixing.data
+    ll = linelists.get(data.species)
+    iso = linelist_isotopes(ll) if ll is not None else None
+    modes = np.ascontiguousarray(relmat_band_modes(line_mixing, iso), np.uint8)
+    o.relmat = ctx.upload_relmat(data)
+    o.lm_species = int(data.species)
+    o.relmat_band_mode = _p(modes) if len(modes) else None
+    return modes
 
 
 class _Batch:
@@ -580,6 +733,7 @@ def forward_model(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
                    apodization, line_mixing, T_method, source_function, cmask)
+    _modes = _relmat_options(opt, line_mixing, ctx, linelists)     # noqa: F841 (kept alive for the call)
     b = _Batch(ctx, profs, linelists, tsf, eps, [g.mu for g in geoms])
     snd = _sounder(inst)
     grid = sounder_grid(inst)
@@ -622,6 +776,7 @@ def analytic_jacobian(prof: AtmosphericProfile, linelists: Dict[GasSpecies, HITR
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, 0.0, x_far, with_ils,
                    apodization, line_mixing, T_method, source_function, cmask)
+    _modes = _relmat_options(opt, line_mixing, ctx, linelists)     # noqa: F841 (kept alive for the call)
     # device blocks: ColumnScale natively, PartialColumns as full log-VMR columns
     blocks = spec.vmr_blocks
     if len(blocks) > 4:
@@ -701,6 +856,7 @@ def channel_optical_depths(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
                    apodization, line_mixing, T_method, "cim", _continua_mask(apply_continuum, continua))
+    _modes = _relmat_options(opt, line_mixing, ctx, linelists)     # noqa: F841 (kept alive for the call)
     nb = len(profs)
     geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
     mus = [1.0 if g is None else g.mu for g in geoms]
@@ -762,6 +918,7 @@ def channel_transmittance(prof, linelists: Dict[GasSpecies, HITRANLinelist], *,
     ctx = ctx or default_context()
     opt = _options(internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, x_far, with_ils,
                    apodization, line_mixing, T_method, "cim", _continua_mask(apply_continuum, continua))
+    _modes = _relmat_options(opt, line_mixing, ctx, linelists)     # noqa: F841 (kept alive for the call)
     nb = len(profs)
     geoms = geom if isinstance(geom, (list, tuple)) else [geom] * nb
     mus = [1.0 if g is None else g.mu for g in geoms]

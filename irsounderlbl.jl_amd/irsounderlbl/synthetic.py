@@ -143,3 +143,77 @@ def perturbed_profiles(n_prof: int, seed: int = 2002, sigma_T: float = 3.0, sigm
         mu = math.cos(math.radians(scan_angle_to_local_zenith(scan)))
         out.append((AtmosphericProfile(base.pressure.copy(), T, z.copy(), vmr), float(T_sfc), float(mu)))
     return out
+
+
+def relmat_synthetic(seed: int = 3001):
+    """A CO2-like relaxation matrix (no HITRAN relaxation data travels with this
+    package): P, Q and R branches of five bands around 15 um --
+      "S626_01" isot 1, l 0 -> 1 at 667.38 (J even, every branch, 21 J values),
+      "S636_11" isot 2, l 1 -> 1 at 648.48,
+      "S628_12" isot 3, l 1 -> 2 at 720.80 (the isotopologue symmetry rule applies),
+      "S626_21" isot 1, l 2 -> 1 at 741.72 (li > lf: the swapped W0/B0 lookup),
+      "S626_23" isot 1, l 2 -> 3 at 618.03, whose (2, 3) table is absent --
+    with W0/B0 tables (0, 1), (1, 1), (1, 2) over every branch pair: W(296 K)
+    0.02 exp(-|dJ| / 8) cm^-1/atm, temperature exponent 0.7 + 0.1 dJ / 40.  Line
+    intensities 1e-22..1e-19 at 296 K, gamma_air 0.075 - 0.0004 J, n 0.75, shifts
+    -0.002 - 2e-5 J."""
+    from .relmat import HITRANRelmatData, RelmatBand, RelmatLine, W0B0Table
+    rng = np.random.default_rng(seed)
+    Bc = 0.39
+    specs = (("S626_01", 1, 0, 1, 667.38, 2, 1.0), ("S636_11", 2, 1, 1, 648.48, 1, 0.05),
+             ("S628_12", 3, 1, 2, 720.80, 1, 0.01), ("S626_21", 1, 2, 1, 741.72, 2, 0.1),
+             ("S626_23", 1, 2, 3, 618.03, 2, 0.02))
+    bands = []
+    for name, iso, li, lf, nu0, jstep, scale in specs:
+        lines = []
+        for br in (-1, 0, 1):
+            if br == 0 and li == 0 and lf == 0:
+                continue
+            for Ji in range(1 if br == -1 else 0, 41, jstep):
+                if br == -1 and Ji == 0:
+                    continue
+                Jf = Ji + br
+                nu = nu0 + Bc * (Jf * (Jf + 1) - Ji * (Ji + 1)) - 1e-4 * Ji * (Ji + 1) * (br == 0)
+                E = Bc * Ji * (Ji + 1)
+                pop = (2 * Ji + 1) * math.exp(-1.4388 * E / 296.0) / 290.0
+                S = scale * 1e-19 * (2 * Ji + 1) * math.exp(-1.4388 * E / 296.0) * 10 ** rng.uniform(-0.3, 0.3)
+                stim = 1.0 - math.exp(-1.4388 * nu / 296.0)
+                dipoT = math.sqrt(S / (pop * nu * stim))
+                lines.append(RelmatLine(iso, nu, Ji, br, 0.075 - 0.0004 * Ji, float(np.float32(0.75)),
+                                        float(np.float32(-0.002 - 2e-5 * Ji)), E, dipoT * (1.0 + 0.05 * rng.random()),
+                                        pop, dipoT))
+        lines.sort(key=lambda l: l.nu)
+        bands.append(RelmatBand(name, li, lf, iso, min(l.nu for l in lines), max(l.nu for l in lines), lines))
+    wt = {}
+    for key in ((0, 1), (1, 1), (1, 2)):
+        t = W0B0Table()
+        for sub in ("pp", "pq", "pr", "qp", "qq", "qr", "rp", "rq", "rr"):
+            d = getattr(t, sub)
+            for a in range(0, 42):
+                for b in range(0, 42):
+                    dj = abs(a - b)
+                    d[(a, b)] = (math.log(0.02 * math.exp(-dj / 8.0)), 0.7 + 0.1 * dj / 40.0)
+        wt[key] = t
+    return HITRANRelmatData(bands, wt, GasSpecies.CO2)
+
+
+def linelist_from_relmat(rm) -> HITRANLinelist:
+    """The HITRAN-style line list of a relaxation matrix's S-file lines (the Voigt
+    baseline the line-mixing perturbation is added to): S(296) = DipoT^2 PopuT0 nu
+    (1 - exp(-1.4388 nu / 296)), gamma_air, n_air, the pressure shift and E'' of the
+    S-file, gamma_self 0.1; isotopologue 10 written as HITRAN's 0."""
+    mol, iso, nu, S, ga, gs, E, n, d = ([] for _ in range(9))
+    for b in rm.bands:
+        stim = 1.0 - np.exp(-1.4388 * b.nu / 296.0)
+        mol += [b.molecule] * b.n
+        iso += [0 if b.isot == 10 else b.isot] * b.n
+        nu.append(b.nu)
+        S.append(b.DipoT ** 2 * b.PopuT0 * b.nu * stim)
+        ga.append(b.gV_air)
+        gs.append(np.full(b.n, 0.1))
+        E.append(b.E_lower)
+        n.append(b.n_air)
+        d.append(b.shift)
+    cat = np.concatenate
+    return HITRANLinelist(np.array(mol, np.int8), np.array(iso, np.int8), cat(nu), cat(S),
+                          np.float32(cat(ga)), np.float32(cat(gs)), cat(E), np.float32(cat(n)), np.float32(cat(d)))

@@ -8,9 +8,11 @@ hand-written gfx950 kernels.  The reference's own types (`AtmosphericProfile`,
 `HITRANLinelist`, `Sounder`, `ViewingGeometry`, `StateVectorSpec`, `Jacobian`,
 `WavenumberGrid`) are used unchanged; include this file after `using IRSounderLBL`.
 
-Line mixing: VP_Y needs the first-order coefficient Y of every CO2 line at the
-layer temperature; pass `line_mixing = Y::Vector{Float64}` (atm^-1, one per
-line of the CO2 list, in list order).  Continuum (`apply_continuum`, `continua`)
+Line mixing: `line_mixing = VPYLineMixing(relmat)` or `VPWLineMixing(relmat; ...)`
+(the reference's types) uploads the relaxation matrix once and evaluates Y(T),
+the VP_W eigenmodes and their sums per layer in libirs; `line_mixing =
+Y::Vector{Float64}` (atm^-1, one per line of the CO2 list, in list order) applies
+a fixed first-order coefficient per line instead.  Continuum (`apply_continuum`, `continua`)
 is computed on the device from the bundled MT-CKD tables; the HITRAN CIA pairs
 add zero until a table is loaded with `load_cia!`, as in the reference when the
 data files are absent.
@@ -20,6 +22,8 @@ module IRSounderLBLAMD
 using IRSounderLBL: AtmosphericProfile, HITRANLine, HITRANLinelist, GasSpecies, Sounder, IASIInstrument,
                     ViewingGeometry, nadir_geometry, WavenumberGrid, wavenumber_grid,
                     StateVectorSpec, Jacobian, H2O, CO2
+using IRSounderLBL: VPYLineMixing, VPWLineMixing, HITRANRelmatData
+import IRSounderLBL
 
 const libirs = get(ENV, "IRS_LIBRARY",
                    joinpath(@__DIR__, "..", "csrc", "libirs.so"))
@@ -58,7 +62,26 @@ mutable struct COptions
     internal_dnu::Float64; high_res_factor::Cint; cutoff::Float64; lm_cutoff::Float64
     dptmn::Float64; x_far::Float64; with_ils::Cint; apodization::Cint; line_mixing::Cint
     T_method::Cint; source_function::Cint; continua::Cint
+    relmat::Int32; lm_species::Int32; relmat_band_mode::Ptr{UInt8}; keep_threshold::Float64
 end
+
+struct CRelmat
+    n_bands::Int32
+    li::Ptr{Int8}; lf::Ptr{Int8}; isot::Ptr{Int8}; molecule::Ptr{Int8}
+    line_off::Ptr{Int64}
+    Ji::Ptr{Int16}; branch::Ptr{Int8}
+    nu::Ptr{Float64}; gV_air::Ptr{Float64}; E_lower::Ptr{Float64}; Dipo0::Ptr{Float64}
+    PopuT0::Ptr{Float64}; DipoT::Ptr{Float64}
+    n_air::Ptr{Float32}; shift::Ptr{Float32}
+    n_tables::Int32
+    table_lli::Ptr{Int8}; table_llf::Ptr{Int8}
+    entry_off::Ptr{Int64}
+    entry_sub::Ptr{Int8}
+    entry_ji::Ptr{Int32}; entry_jip::Ptr{Int32}
+    entry_w0::Ptr{Float64}; entry_b0::Ptr{Float64}
+end
+
+const SUBTABLES = (:pp, :pq, :pr, :qp, :qq, :qr, :rp, :rq, :rr)
 
 struct CStateSpec
     include_temperature::Cint; n_retrieved::Cint
@@ -163,7 +186,7 @@ end
 function options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with_ils,
                  apodization, line_mixing, T_method, source_function, x_far = 122.0,
                  apply_continuum = false, continua = ())
-    o = COptions(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    o = COptions(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, C_NULL, 0)
     ccall((:irs_default_options, libirs), Cvoid, (Ref{COptions},), o)
     o.internal_dnu = internal_dnu === nothing ? 0.0 : internal_dnu
     o.high_res_factor = high_res_factor === nothing ? 0 : high_res_factor
@@ -172,7 +195,8 @@ function options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with
     haskey(APOD, apodization) ||
         error("Unknown apodization :$apodization; expected :gaussian or a Norton-Beer style")
     o.apodization = APOD[apodization]
-    o.line_mixing = line_mixing === nothing ? 0 : 1
+    o.line_mixing = line_mixing === nothing ? 0 :
+                    (line_mixing isa Union{VPYLineMixing, VPWLineMixing} ? 2 : 1)
     T_method in (:logp_at_pcg, :mass_weighted) ||
         error("T_method must be :logp_at_pcg or :mass_weighted, got :$T_method")
     o.T_method = T_method === :logp_at_pcg ? 0 : 1
@@ -196,11 +220,72 @@ end
 
 csounder(s::Sounder) = CSounder(s.ν_min, s.ν_max, s.Δν, s.opd_max, s.fwhm_gauss, s.n_channels)
 
+# ---- relaxation matrices (line_mixing.jl:35-117) ------------------------------------
+function upload_relmat!(ctx::Context, data::HITRANRelmatData)
+    haskey(ctx.lists, data) && return ctx.lists[data]
+    B = data.bands
+    li = Int8[b.li for b in B]; lf = Int8[b.lf for b in B]; isot = Int8[b.isot for b in B]
+    mol = Int8[b.molecule for b in B]
+    off = Int64[0; cumsum([length(b.lines) for b in B])]
+    L = reduce(vcat, [b.lines for b in B]; init = IRSounderLBL.RelmatLine[])
+    Ji = Int16[l.Ji for l in L]; br = Int8[l.branch for l in L]
+    nu = Float64[l.ν for l in L]; gv = Float64[l.gV_air for l in L]; el = Float64[l.E_lower for l in L]
+    d0 = Float64[l.Dipo0 for l in L]; p0 = Float64[l.PopuT0 for l in L]; dt = Float64[l.DipoT for l in L]
+    na = Float32[l.n_air for l in L]; sh = Float32[l.shift for l in L]
+    keys_ = sort(collect(keys(data.wtfit)))
+    lli = Int8[k[1] for k in keys_]; llf = Int8[k[2] for k in keys_]
+    eoff = Int64[0]; esub = Int8[]; eji = Int32[]; ejip = Int32[]; ew0 = Float64[]; eb0 = Float64[]
+    for k in keys_, (si, sub) in enumerate(SUBTABLES)
+        for ((ji, jip), (w0, b0)) in getfield(data.wtfit[k], sub)
+            push!(esub, si - 1); push!(eji, ji); push!(ejip, jip); push!(ew0, w0); push!(eb0, b0)
+        end
+        sub === :rr && push!(eoff, length(esub))
+    end
+    h = Ref{Int32}(-1)
+    GC.@preserve li lf isot mol off Ji br nu gv el d0 p0 dt na sh lli llf eoff esub eji ejip ew0 eb0 begin
+        c = CRelmat(length(B), pointer(li), pointer(lf), pointer(isot), pointer(mol), pointer(off),
+                    pointer(Ji), pointer(br), pointer(nu), pointer(gv), pointer(el), pointer(d0), pointer(p0),
+                    pointer(dt), pointer(na), pointer(sh), length(keys_), pointer(lli), pointer(llf),
+                    pointer(eoff), pointer(esub), pointer(eji), pointer(ejip), pointer(ew0), pointer(eb0))
+        check(ctx, ccall((:irs_relmat_upload, libirs), Cint, (Ptr{Cvoid}, Ref{CRelmat}, Ref{Int32}), ctx.h, c, h))
+    end
+    ctx.lists[data] = h[]
+end
+
+# Per-band treatment: the band filters of compute_lm_dispersive_correction and
+# _vpw_perturbation (line_mixing.jl:1120-1154,1303-1349), isotopologues of the
+# species' line list (line_mixing.jl:199-206).
+function relmat_band_modes(lm, linelists)
+    data = lm.data
+    iso = haskey(linelists, data.species) ? IRSounderLBL._linelist_isotopes(linelists[data.species]) : nothing
+    wl = lm isa VPWLineMixing ? lm.whitelist : Set{String}()
+    m = zeros(UInt8, length(data.bands))
+    for (i, b) in enumerate(data.bands)
+        Int(b.li) > 8 && continue
+        iso !== nothing && Int(b.isot) ∉ iso && continue
+        lm.min_band_strength > 0.0 && IRSounderLBL._band_eff_strength(b) < lm.min_band_strength && continue
+        haskey(data.wtfit, (Int8(min(b.li, b.lf)), Int8(max(b.li, b.lf)))) || continue
+        m[i] = b.name in wl ? 2 : 1
+    end
+    return m
+end
+
+function relmat_options!(o::COptions, ctx::Context, line_mixing, linelists, keep)
+    line_mixing isa Union{VPYLineMixing, VPWLineMixing} || return o
+    modes = relmat_band_modes(line_mixing, linelists)
+    push!(keep, modes)
+    o.relmat = upload_relmat!(ctx, line_mixing.data)
+    o.lm_species = Int(line_mixing.data.species)
+    o.relmat_band_mode = isempty(modes) ? Ptr{UInt8}(C_NULL) : pointer(modes)
+    return o
+end
+
 function cprofile(ctx, prof::AtmosphericProfile, linelists, line_mixing, T_sfc, ε_sfc, μ, keep)
     species = collect(keys(prof.vmr))
     sp = Cint[Int(s) for s in species]
+    y = line_mixing isa AbstractVector ? line_mixing : nothing
     hs = Int32[haskey(linelists, s) && !isempty(linelists[s].lines) ?
-               upload!(ctx, linelists[s], (s == CO2 && line_mixing !== nothing) ? line_mixing : nothing) :
+               upload!(ctx, linelists[s], (s == CO2 && y !== nothing) ? y : nothing) :
                Int32(-1) for s in species]
     vmr = reduce(vcat, [Float64.(prof.vmr[s]) for s in species])
     p = Float64.(prof.pressure); T = Float64.(prof.temperature)
@@ -235,6 +320,7 @@ function forward_model(prof, linelists::Dict{GasSpecies, HITRANLinelist};
     o = options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn, with_ils, apodization,
                 line_mixing, T_method, source_function, apply_continuum, continua)
     keep = Any[]
+    relmat_options!(o, ctx, line_mixing, linelists, keep)
     cps = Vector{CProfile}(undef, nb)
     for q in 1:nb
         cps[q], _ = cprofile(ctx, profs[q], linelists, line_mixing, pick(T_sfc, q), pick(ε_sfc, q),
@@ -282,6 +368,7 @@ function analytic_jacobian(prof::AtmosphericProfile, linelists::Dict{GasSpecies,
     o = options(; internal_dnu, high_res_factor, cutoff, lm_cutoff, dptmn = 0.0, with_ils,
                 apodization, line_mixing, T_method, source_function, apply_continuum, continua)
     keep = Any[]
+    relmat_options!(o, ctx, line_mixing, linelists, keep)
     cp, _ = cprofile(ctx, prof, linelists, line_mixing, T_sfc, ε_sfc, geom.μ, keep)
     blocks = spec.vmr_blocks
     iscs = [b.basis !== nothing && size(b.basis, 2) == 1 && all(==(1.0), b.basis) for b in blocks]

@@ -921,9 +921,29 @@ int orc_forward_dims(const orc_sounder *snd, const orc_options *opt, orc_forward
     return 0;
 }
 
+static int forward_impl(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                        double *R, double *BT, double *tau_out, const uint8_t *layer_mask,
+                        orc_forward_info *info, int lm_species, const double *lm_add);
+
 int orc_forward(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
                 double *R, double *BT, double *tau_out, const uint8_t *layer_mask,
                 orc_forward_info *info) {
+    return forward_impl(prof, snd, opt, R, BT, tau_out, layer_mask, info, 0, NULL);
+}
+
+/* forward_model with a relaxation-matrix line-mixing perturbation (VPYLineMixing /
+ * VPWLineMixing): lm_add [n_layers][n_hi] is the perturbation of species
+ * lm_species at each layer's T_cg, p_cg; that species' cross section is
+ * max(sigma_voigt + lm_add, 0) (line_mixing.jl:222-241,1093-1118,1351-1369) and
+ * DPTMIN is skipped for CO2 (sounder.jl:295-299). */
+int orc_forward_lm(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                   int lm_species, const double *lm_add, double *R, double *BT, orc_forward_info *info) {
+    return forward_impl(prof, snd, opt, R, BT, NULL, NULL, info, lm_species, lm_add);
+}
+
+static int forward_impl(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                        double *R, double *BT, double *tau_out, const uint8_t *layer_mask,
+                        orc_forward_info *info, int lm_species, const double *lm_add) {
     orc_forward_info inf;
     memset(&inf, 0, sizeof(inf));
     orc_forward_dims(snd, opt, &inf);
@@ -964,9 +984,17 @@ int orc_forward(const orc_profile *prof, const orc_sounder *snd, const orc_optio
                 inf.active_lines += ll->n;
                 for (int64_t j = 0; j < ll->n; ++j) keep[j] = 1;
             } else {
-                int64_t nk = orc_reject_weak_lines(ll, T, p_atm, coef, vmr_self, opt->dptmn, keep);
+                const double dpt = (lm_add && sp == 2) ? 0.0 : opt->dptmn;
+                int64_t nk = orc_reject_weak_lines(ll, T, p_atm, coef, vmr_self, dpt, keep);
                 inf.active_lines += nk;
                 orc_voigt_xsec(ll, keep, lo, d, n, T, p_atm, vmr_self, opt->cutoff, opt->x_far, sig);
+                if (lm_add && sp == lm_species) {
+                    const double *ak = lm_add + (size_t)k * n;
+                    for (int64_t i = 0; i < n; ++i) {
+                        const double v = sig[i] + ak[i];
+                        sig[i] = v > 0.0 ? v : 0.0;
+                    }
+                }
             }
             /* active pairs: lines whose window [c-cut, c+cut] overlaps the grid */
             for (int64_t j = 0; j < ll->n; ++j) {

@@ -184,6 +184,8 @@ int orc_forward(const orc_profile *prof, const orc_sounder *snd, const orc_optio
                 double *R, double *BT, double *tau_out, const uint8_t *layer_mask,
                 orc_forward_info *info);
 /* channel count and internal-grid geometry for the options */
+int orc_forward_lm(const orc_profile *prof, const orc_sounder *snd, const orc_options *opt,
+                   int lm_species, const double *lm_add, double *R, double *BT, orc_forward_info *info);
 int orc_forward_dims(const orc_sounder *snd, const orc_options *opt, orc_forward_info *info);
 
 /* Layer properties (layers.jl:133-178) for the profile; arrays [n_layers] or

@@ -109,6 +109,9 @@ def lib():
         L.orc_forward.argtypes = [C.POINTER(OrcProfile), C.POINTER(OrcSounder),
                                   C.POINTER(OrcOptions), P, P, P, P, C.POINTER(OrcInfo)]
         L.orc_forward.restype = i32
+        L.orc_forward_lm.argtypes = [C.POINTER(OrcProfile), C.POINTER(OrcSounder), C.POINTER(OrcOptions),
+                                     i32, P, P, P, C.POINTER(OrcInfo)]
+        L.orc_forward_lm.restype = i32
         L.orc_forward_dims.argtypes = [C.POINTER(OrcSounder), C.POINTER(OrcOptions),
                                        C.POINTER(OrcInfo)]
         L.orc_layer_properties.argtypes = [C.POINTER(OrcProfile), i32, P, P, P, P, P, P, P]
@@ -329,6 +332,40 @@ def forward(prof, linelists, sounder, opts=None, layer_mask=None, want_tau=False
     if rc != 0:
         raise ValueError(f"orc_forward error {rc}")
     return (R, BT, info, tau) if want_tau else (R, BT, info)
+
+
+def grid_nodes(nu0, dnu, n):
+    """The n nodes of the Julia range nu0:dnu:... (wavenumber_grid.jl:23-27)."""
+    L = lib()
+    return np.array([L.orc_grid_node(nu0, dnu, i) for i in range(n)])
+
+
+def forward_lm(prof, linelists, sounder, lm_species, lm_fn, opts=None, **kw):
+    """FP64 forward model with a relaxation-matrix line-mixing perturbation:
+    lm_fn(nu_nodes, T, p_atm) is the perturbation of species lm_species at one
+    layer's T_cg, p_cg (the caller's restatement); returns (R, BT)."""
+    opts = opts or options(**kw)
+    pb = Problem(prof, linelists)
+    snd = OrcSounder(sounder.nu_min, sounder.nu_max, sounder.dnu, sounder.opd_max, sounder.fwhm_gauss)
+    info = OrcInfo()
+    lib().orc_forward_dims(C.byref(snd), C.byref(opts), C.byref(info))
+    nodes = grid_nodes(info.nu_lo, info.dnu, info.n_hi)
+    lp = layer_properties(prof, opts.T_method)
+    sp = [k for k in prof.vmr.keys() if int(k) == int(lm_species)]
+    L = prof.n_levels - 1
+    add = np.zeros((L, info.n_hi))
+    if sp:
+        s = sp[0]
+        for k in range(L):
+            if lp["vmr_cg"][s][k] != 0.0:
+                add[k] = lm_fn(nodes, lp["T_cg"][s][k], lp["p_cg"][s][k] / 1013.25)
+    R = np.zeros(info.n_ch)
+    BT = np.zeros(info.n_ch)
+    rc = lib().orc_forward_lm(C.byref(pb.prof), C.byref(snd), C.byref(opts), int(lm_species), _ptr(add),
+                              _ptr(R), _ptr(BT), C.byref(info))
+    if rc != 0:
+        raise ValueError(f"orc_forward_lm error {rc}")
+    return R, BT
 
 
 def layer_properties(prof, T_method=0):

@@ -16,7 +16,8 @@ SHIM = os.path.join(ROOT, "irsounderlbl.jl_amd", "julia", "IRSounderLBLAMD.jl")
 # C struct -> (Julia struct, ctypes class)
 PAIRS = {"irs_linelist": ("CLinelist", "IrsLinelist"), "irs_sounder": ("CSounder", "IrsSounder"),
          "irs_profile": ("CProfile", "IrsProfile"), "irs_fm_options": ("COptions", "IrsOptions"),
-         "irs_state_spec": ("CStateSpec", "IrsStateSpec"), "irs_par_lines": ("CParLines", "hitran.IrsParLines")}
+         "irs_state_spec": ("CStateSpec", "IrsStateSpec"), "irs_par_lines": ("CParLines", "hitran.IrsParLines"),
+         "irs_relmat": ("CRelmat", "IrsRelmat")}
 
 C_KIND = {"int64_t": "i64", "int": "i32", "int32_t": "i32", "double": "f64", "float": "f32",
           "int8_t": "i8"}
