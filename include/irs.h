@@ -492,6 +492,25 @@ int         irs_emu_forward(irs_emu *emu, const float *features, const double *T
                             const double *scal, int n_prof, int source_function, double *R, double *BT,
                             float *t_out);
 
+/* Profile-context features of the emulator (n_ctx = n_feat - n_local): context_c =
+ * sum_j proj[c][j] (v_j - mean_j), v = [T levels, ln q_H2O levels, ln q_O3 levels (floor
+ * 1e-30), T_sfc, 1/mu] of the scene (n_vec = 3 n_levels + 2).  No counterpart in the
+ * reference. */
+int         irs_emu_set_context(irs_emu *emu, int n_ctx, int n_vec, const double *mean, const double *proj);
+
+/* Emulator + per-scene LBL spot check + full-LBL fallback in one call (the learned
+ * path behind forward_model's signature; no counterpart in the reference).  The
+ * emulator answers every scene; block b of consecutive channels [block_c0[b],
+ * block_c0[b] + block_n[b]) is recomputed by LBL on its own sub-grid with the line
+ * lists block_lists[b][n_species] (the caller's lists filtered to the block
+ * +- 42 cm^-1); a scene whose largest |BT_emulator - BT_LBL| over the spot channels
+ * exceeds `threshold` is recomputed with full LBL (irs_forward with opt) and
+ * fell_back[q] = 1.  R, BT [n_prof][n_channels]; spot_err [n_prof] may be NULL. */
+int         irs_forward_emulated(irs_ctx *ctx, irs_emu *emu, const irs_profile *profiles, int n_prof,
+                                 const irs_sounder *snd, const irs_fm_options *opt, int n_blocks,
+                                 const int64_t *block_c0, const int32_t *block_n, const int32_t *block_lists,
+                                 double threshold, double *R, double *BT, uint8_t *fell_back, double *spot_err);
+
 /* Emulator Jacobians (chain rule through the MLP input Jacobian, the regression,
  * the decode and the adjoint of the channel RTE; the reflected term of eps < 1 held fixed):
  * BT [n_prof][n_ch]; dfeat [n_prof][L][n_feat][n_ch] = dBT/d(raw feature);

@@ -58,6 +58,10 @@ struct irs_emu {
     double *R = nullptr, *BT = nullptr;
     int64_t cap_rows = 0, cap_phi = 0, cap_prof = 0, cap_out = 0;
     std::string err;
+    // profile-context features (irs_emu_set_context): proj [n_ctx][n_vec] applied to
+    // [T levels, ln q_H2O levels, ln q_O3 levels, T_sfc, 1/mu] - mean
+    int n_ctx = 0, n_vec = 0;
+    std::vector<double> ctx_mean, ctx_proj;
 };
 
 namespace {
@@ -755,3 +759,163 @@ int irs_emu_jacobian(irs_emu *e, const float *features, const double *T_lev, con
 }
 
 }  // extern "C"
+
+// ---- emulator + spot check + fallback behind one call -------------------------------
+extern "C" int irs_emu_set_context(irs_emu *e, int n_ctx, int n_vec, const double *mean, const double *proj) {
+    if (!e || n_ctx < 0 || n_vec < 0 || (n_ctx > 0 && (!mean || !proj))) return IRS_EINVAL;
+    if (e->n_local + n_ctx != e->n_feat) {
+        e->err = "n_ctx must equal n_feat - n_local";
+        return IRS_EINVAL;
+    }
+    e->n_ctx = n_ctx;
+    e->n_vec = n_vec;
+    e->ctx_mean.assign(mean, mean + (size_t)n_vec);
+    e->ctx_proj.assign(proj, proj + (size_t)n_ctx * n_vec);
+    return IRS_OK;
+}
+
+namespace {
+
+// Raw emulator inputs of one scene [L][n_feat] (9 layer features, then the profile
+// context): ln p_mid, T_ave, ln vmr_cg of H2O and O3 (floor 1e-30), CO2 vmr_cg in
+// ppm, 1/mu, ln of the H2O and O3 columns above the layer (sum vmr_cg dp, + 1e-9 /
+// + 1e-12), T_ave of the layer above (its own at the top).
+int emu_scene_features(const irs_emu *e, const irs_profile &pr, int T_method, float *out, double *T_ave,
+                       double &T_sfc) {
+    const int nl = pr.n_levels, L = nl - 1, ns = pr.n_species;
+    std::vector<double> pmid(L), dp(L), vc((size_t)ns * L), pc((size_t)ns * L), tc((size_t)ns * L);
+    int rc = irs_layer_properties(&pr, T_method, pmid.data(), dp.data(), T_ave, vc.data(), pc.data(), tc.data());
+    if (rc) return rc;
+    auto idx = [&](int mol) {
+        for (int s = 0; s < ns; ++s)
+            if (pr.species[s] == mol) return s;
+        return -1;
+    };
+    const int sh = idx(1), so = idx(3), sc = idx(2);
+    auto v = [&](int s, int k) { return s < 0 ? 0.0 : vc[(size_t)s * L + k]; };
+    std::vector<double> ah(L, 0.0), ao(L, 0.0);
+    double acc_h = 0.0, acc_o = 0.0;
+    for (int k = L - 1; k >= 0; --k) {           // sum over the layers above k
+        ah[k] = acc_h;
+        ao[k] = acc_o;
+        acc_h += v(sh, k) * dp[k];
+        acc_o += v(so, k) * dp[k];
+    }
+    T_sfc = pr.T_sfc > 0.0 ? pr.T_sfc : pr.T[0];
+    // context vector
+    std::vector<double> cx(e->n_ctx, 0.0);
+    if (e->n_ctx > 0) {
+        if (e->n_vec != 3 * nl + 2) return IRS_EINVAL;
+        std::vector<double> vec((size_t)e->n_vec);
+        auto lnq = [&](int s, int j) {
+            return s < 0 ? std::log(1e-30) : std::log(std::max(pr.vmr[(size_t)s * nl + j], 1e-30));
+        };
+        for (int j = 0; j < nl; ++j) {
+            vec[j] = pr.T[j];
+            vec[nl + j] = lnq(sh, j);
+            vec[2 * nl + j] = lnq(so, j);
+        }
+        vec[3 * nl] = T_sfc;
+        vec[3 * nl + 1] = 1.0 / pr.mu;
+        for (int c = 0; c < e->n_ctx; ++c) {
+            double a = 0.0;
+            for (int j = 0; j < e->n_vec; ++j) a += e->ctx_proj[(size_t)c * e->n_vec + j] * (vec[j] - e->ctx_mean[j]);
+            cx[c] = a;
+        }
+    }
+    for (int k = 0; k < L; ++k) {
+        float *f = out + (size_t)k * e->n_feat;
+        const double loc[9] = {std::log(pmid[k]), T_ave[k], std::log(std::max(v(sh, k), 1e-30)),
+                               std::log(std::max(v(so, k), 1e-30)), v(sc, k) * 1e6, 1.0 / pr.mu,
+                               std::log(ah[k] + 1e-9), std::log(ao[k] + 1e-12), T_ave[k + 1 < L ? k + 1 : k]};
+        for (int i = 0; i < e->n_local; ++i) f[i] = (float)loc[i];
+        for (int c = 0; c < e->n_ctx; ++c) f[e->n_local + c] = (float)cx[c];
+    }
+    return IRS_OK;
+}
+
+}  // namespace
+
+extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile *profiles, int n_prof,
+                                    const irs_sounder *snd, const irs_fm_options *opt, int n_blocks,
+                                    const int64_t *block_c0, const int32_t *block_n, const int32_t *block_lists,
+                                    double threshold, double *R, double *BT, uint8_t *fell_back, double *spot_err) {
+    if (!ctx || !e || !profiles || n_prof < 1 || !snd || !R || !BT || !fell_back || n_blocks < 0 ||
+        (n_blocks > 0 && (!block_c0 || !block_n || !block_lists)))
+        return IRS_EINVAL;
+    if (e->n_local != 9 || e->n_local + e->n_ctx != e->n_feat) {
+        e->err = "the emulator needs 9 layer features and its profile context (irs_emu_set_context)";
+        return IRS_EINVAL;
+    }
+    irs_fm_options o;
+    if (opt) o = *opt; else irs_default_options(&o);
+    const int L = profiles[0].n_levels - 1, ns = profiles[0].n_species;
+    if (L != e->L) {
+        e->err = "profiles and emulator have different numbers of layers";
+        return IRS_EINVAL;
+    }
+    const int64_t nch = e->n_ch;
+    if (snd->n_channels != nch) {
+        e->err = "sounder and emulator have different channels";
+        return IRS_EINVAL;
+    }
+    // 1. emulator
+    std::vector<float> X((size_t)n_prof * L * e->n_feat);
+    std::vector<double> tlev((size_t)n_prof * (L + 1)), tave((size_t)n_prof * L), scal((size_t)n_prof * 3);
+    for (int q = 0; q < n_prof; ++q) {
+        const irs_profile &pr = profiles[q];
+        if (pr.n_levels - 1 != L || pr.n_species != ns) return IRS_EINVAL;
+        double ts = 0.0;
+        int rc = emu_scene_features(e, pr, o.T_method, X.data() + (size_t)q * L * e->n_feat,
+                                    tave.data() + (size_t)q * L, ts);
+        if (rc) return rc;
+        for (int k = 0; k <= L; ++k) tlev[(size_t)q * (L + 1) + k] = pr.T[k];
+        scal[3 * q] = ts;
+        scal[3 * q + 1] = pr.mu;
+        scal[3 * q + 2] = pr.eps_sfc;
+    }
+    int rc = irs_emu_forward(e, X.data(), tlev.data(), tave.data(), scal.data(), n_prof, o.source_function, R, BT,
+                             nullptr);
+    if (rc) return rc;
+    // 2. spot check: LBL of each block of consecutive channels on its own sub-grid,
+    //    with the line lists that reach it
+    std::vector<double> err((size_t)n_prof, 0.0);
+    std::vector<irs_profile> sub(profiles, profiles + n_prof);
+    for (int b = 0; b < n_blocks; ++b) {
+        const int64_t c0 = block_c0[b];
+        const int32_t nb = block_n[b];
+        if (c0 < 0 || nb < 1 || c0 + nb > nch) return IRS_EINVAL;
+        irs_sounder sb = *snd;
+        sb.nu_min = snd->nu_min + (double)c0 * snd->dnu;
+        sb.nu_max = snd->nu_min + (double)(c0 + nb - 1) * snd->dnu;
+        sb.n_channels = nb;
+        for (auto &p : sub) p.linelist = block_lists + (size_t)b * ns;
+        std::vector<double> rb((size_t)n_prof * nb), bb((size_t)n_prof * nb);
+        rc = irs_forward(ctx, sub.data(), n_prof, &sb, &o, rb.data(), bb.data());
+        if (rc) return rc;
+        for (int q = 0; q < n_prof; ++q)
+            for (int c = 0; c < nb; ++c)
+                err[q] = std::max(err[q], std::fabs(BT[(size_t)q * nch + c0 + c] - bb[(size_t)q * nb + c]));
+    }
+    // 3. full LBL for the scenes over the threshold
+    std::vector<irs_profile> fb;
+    std::vector<int> fq;
+    for (int q = 0; q < n_prof; ++q) {
+        fell_back[q] = err[q] > threshold ? 1 : 0;
+        if (spot_err) spot_err[q] = err[q];
+        if (fell_back[q]) {
+            fb.push_back(profiles[q]);
+            fq.push_back(q);
+        }
+    }
+    if (!fb.empty()) {
+        std::vector<double> rf(fb.size() * nch), bf(fb.size() * nch);
+        rc = irs_forward(ctx, fb.data(), (int)fb.size(), snd, &o, rf.data(), bf.data());
+        if (rc) return rc;
+        for (size_t i = 0; i < fq.size(); ++i) {
+            std::memcpy(R + (size_t)fq[i] * nch, rf.data() + i * nch, sizeof(double) * nch);
+            std::memcpy(BT + (size_t)fq[i] * nch, bf.data() + i * nch, sizeof(double) * nch);
+        }
+    }
+    return IRS_OK;
+}
