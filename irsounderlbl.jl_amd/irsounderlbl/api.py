@@ -146,7 +146,8 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_emu_forward", "irs_channel_transmittance", "irs_emu_jacobian", "irs_cia_set",
            "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc",
            "irs_channel_od_moments", "irs_par_parse", "irs_par_load", "irs_relmat_upload",
-           "irs_relmat_free", "irs_lm_perturbation", "irs_relmat_terms")
+           "irs_relmat_free", "irs_lm_perturbation", "irs_relmat_terms", "irs_emu_set_context",
+           "irs_forward_emulated")
 
 _lib = None
 
@@ -224,6 +225,9 @@ def load_library(path: str = LIB_PATH):
     L.irs_relmat_free.argtypes = [P, C.c_int32]
     L.irs_lm_perturbation.argtypes = [P, C.c_int32, P, f64, f64, i64, f64, f64, f64, f64, f64, P]
     L.irs_relmat_terms.argtypes = [C.POINTER(IrsRelmat), P, f64, f64, f64, i64, P, C.POINTER(i64)]
+    L.irs_emu_set_context.argtypes = [P, i32, i32, P, P]
+    L.irs_forward_emulated.argtypes = [P, P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
+                                       C.POINTER(IrsOptions), i32, P, P, P, f64, P, P, P, P]
     for name in EXPORTS:
         if name not in ("irs_destroy", "irs_default_options", "irs_last_error", "irs_mlp_destroy",
                         "irs_mlp_last_error", "irs_mlp_n_params", "irs_emu_destroy", "irs_emu_last_error"):

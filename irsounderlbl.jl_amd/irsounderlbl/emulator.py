@@ -301,6 +301,11 @@ class ChannelODEmulator:
         if rc != IRS_OK:
             raise IRSError(rc, f"irs_emu_create failed ({rc})")
         self.h = h
+        cm = np.ascontiguousarray(context.p_mean, np.float64)
+        cp = np.ascontiguousarray(context.proj, np.float64)
+        rc = self.lib.irs_emu_set_context(h, cp.shape[0], cp.shape[1], _p(cm), _p(cp))
+        if rc != IRS_OK:
+            raise IRSError(rc, self.lib.irs_emu_last_error(h).decode(errors="replace"))
 
     def close(self):
         if getattr(self, "h", None):
@@ -683,12 +688,52 @@ class EmulatedResult:
 def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: Sounder,
                      geom=None, T_sfc=None, eps_sfc=1.0, threshold: Optional[float] = None,
                      spot: Optional[SpotChecker] = None, ctx: Optional[Context] = None,
-                     line_mixing="vpy", apply_continuum: bool = False, **kw) -> EmulatedResult:
-    """Emulator + per-scene LBL spot check + full-LBL fallback.  The spot channels
-    and the spot-error threshold come from the emulator's calibration (calibrate)
-    unless `spot` / `threshold` are given; a scene whose largest spot error
-    exceeds the threshold is recomputed with full LBL.  apply_continuum as in
-    training."""
+                     line_mixing="vpy", apply_continuum: bool = False, internal_dnu: float = 0.001,
+                     cutoff: float = 25.0, dptmn: float = 1e-6, T_method: str = "logp_at_pcg") -> EmulatedResult:
+    """Emulator + per-scene LBL spot check + full-LBL fallback in one library call
+    (irs_forward_emulated).  The spot channels and the spot-error threshold come from
+    the emulator's calibration (calibrate) unless `spot` / `threshold` are given; a
+    scene whose largest spot error exceeds the threshold is recomputed with full LBL.
+    apply_continuum as in training."""
+    scenes, eps = ChannelODEmulator._scenes(list(profs), geom, T_sfc, eps_sfc)
+    ctx = ctx or api.default_context()
+    if spot is None:
+        spot = emulator.spot_checker(linelists, sounder)
+    if threshold is None:
+        if emulator.calibration is None:
+            raise ValueError("no threshold given and the emulator has no calibration")
+        threshold = emulator.calibration.threshold
+    opt = api._options(internal_dnu, None, cutoff, None, dptmn, 122.0, True, "gaussian", line_mixing, T_method,
+                       "cim", api._continua_mask(apply_continuum, api.DEFAULT_CONTINUA))
+    b = api._Batch(ctx, [sc[0] for sc in scenes], linelists, [sc[1] for sc in scenes], eps, [sc[2] for sc in scenes])
+    species = [GasSpecies(int(v)) for v in b.species]
+    c0 = np.array([int(ch[0]) for ch, _, _ in spot.blocks], np.int64)
+    nb = np.array([len(ch) for ch, _, _ in spot.blocks], np.int32)
+    lists = np.array([[ctx.upload(sub[sp]) if sp in sub and len(sub[sp]) else -1 for sp in species]
+                      for _, _, sub in spot.blocks], np.int32).reshape(-1)
+    n = len(scenes)
+    R = np.zeros((n, emulator.n_ch))
+    BT = np.zeros((n, emulator.n_ch))
+    fb = np.zeros(n, np.uint8)
+    err = np.zeros(n)
+    snd = api._sounder(sounder)
+    t0 = time.perf_counter()
+    rc = emulator.lib.irs_forward_emulated(ctx.h, emulator.h, b.arr, n, C.byref(snd), C.byref(opt), len(c0),
+                                           _p(c0), _p(nb), _p(lists), float(threshold), _p(R), _p(BT), _p(fb),
+                                           _p(err))
+    if rc != IRS_OK:
+        msg = emulator.lib.irs_emu_last_error(emulator.h).decode(errors="replace") or \
+            emulator.lib.irs_last_error(ctx.h).decode(errors="replace")
+        raise (ValueError(msg) if rc == api.IRS_EINVAL else IRSError(rc, msg))
+    return EmulatedResult(R, BT, fb.astype(bool), err, {"total": time.perf_counter() - t0}, float(threshold))
+
+
+def forward_emulated_steps(profs, linelists, emulator: ChannelODEmulator, *, sounder: Sounder,
+                           geom=None, T_sfc=None, eps_sfc=1.0, threshold: Optional[float] = None,
+                           spot: Optional[SpotChecker] = None, ctx: Optional[Context] = None,
+                           line_mixing="vpy", apply_continuum: bool = False, **kw) -> EmulatedResult:
+    """The same decision as forward_emulated from host-side steps (emulator forward,
+    spot-check forward_model calls, fallback forward_model), timing each step."""
     kw["apply_continuum"] = apply_continuum
     scenes, eps = ChannelODEmulator._scenes(list(profs), geom, T_sfc, eps_sfc)
     geoms = [_geom(s[2]) for s in scenes]
