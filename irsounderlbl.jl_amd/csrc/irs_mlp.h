@@ -16,6 +16,8 @@ struct GemmArgs {
     float *C32;                  // optional f32 output
     uint16_t *C16;               // optional bf16 output
     int flags;
+    int splitk;                  // > 1: K in splitk slices, partial products into Cpart [splitk][M][ldc],
+    float *Cpart;                // summed in slice order into C32 (no bias / epilogue flags)
 };
 
 hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
@@ -24,7 +26,8 @@ hipError_t launch_gather_batch(const float *X, const float *T, const int32_t *id
                                int n_out, uint16_t *xb, float *tb, hipStream_t s);
 hipError_t launch_mse_grad(const float *Y, const float *T, int64_t n, float scale, uint16_t *dY, float *partial,
                            hipStream_t s);
-hipError_t launch_colsum_bf16(const uint16_t *dZ, int M, int N, float *db, hipStream_t s);
+hipError_t launch_colsum_bf16(const uint16_t *dZ, int M, int N, float *db, float *scratch, int64_t scratch_floats,
+                              hipStream_t s);
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *w16, int64_t n, float lr, float b1,
                        float b2, float eps, int step, hipStream_t s);
 hipError_t launch_gelu_tangent(const float *Z, const float *dZ, int64_t n, uint16_t *dH, hipStream_t s);

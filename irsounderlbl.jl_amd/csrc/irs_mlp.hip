@@ -40,7 +40,20 @@ struct irs_mlp {
     int32_t *idx = nullptr;
     int64_t idx_cap = 0;
     int step = 0;
+    float *Gpart = nullptr;          // split-K partial products of the weight gradients
 };
+
+// Split-K slices of a weight-gradient GEMM (few 64 x 64 tiles, K = batch): enough
+// slices that tiles x slices reaches 256 workgroups, each slice at least 128 deep.
+#define IRS_GPART_FLOATS (2 * 256 * 64 * 64)
+static int weight_grad_splits(int M, int N, int K) {
+    const int tiles = (M / 64) * (N / 64);
+    int S = 1;
+    while (tiles * S < 256 && K % (2 * S) == 0 && (K / (2 * S)) % 32 == 0 && K / (2 * S) >= 128 &&
+           (int64_t)2 * S * M * N <= IRS_GPART_FLOATS)
+        S *= 2;
+    return S;
+}
 
 struct irs_emu {
     irs_mlp *mlp = nullptr;
@@ -134,8 +147,10 @@ int mlp_backward(irs_mlp *m, int B) {
         gw.ldc = dout;
         gw.alpha = 1.0f;
         gw.C32 = m->G + m->off_W[l];
+        gw.splitk = weight_grad_splits(din, dout, B);
+        gw.Cpart = m->Gpart;
         MCHK(launch_gemm_bf16(gw, true, false, m->stream));
-        MCHK(launch_colsum_bf16(dcur, B, dout, m->G + m->off_b[l], m->stream));
+        MCHK(launch_colsum_bf16(dcur, B, dout, m->G + m->off_b[l], m->Gpart, IRS_GPART_FLOATS, m->stream));
         if (l > 0) {
             // dZ_{l-1} = (dcur . W_l^T) * gelu'(Z_{l-1})     [B x din]
             GemmArgs gx{};
@@ -202,7 +217,8 @@ int irs_mlp_create(int device, int n_in, int n_hidden, int n_out, int max_batch,
     ok = ok && dalloc(&m->X16, B * m->n_in_pad) == hipSuccess && dalloc(&m->Xin, B * n_in) == hipSuccess &&
          dalloc(&m->dY16, B * n_out) == hipSuccess && dalloc(&m->Y, B * n_out) == hipSuccess &&
          dalloc(&m->T, B * n_out) == hipSuccess && dalloc(&m->partial, (B * n_out + 255) / 256) == hipSuccess &&
-         dalloc(&m->loss, 1) == hipSuccess && dalloc(&m->tang, B * n_hidden) == hipSuccess;
+         dalloc(&m->loss, 1) == hipSuccess && dalloc(&m->tang, B * n_hidden) == hipSuccess &&
+         dalloc(&m->Gpart, IRS_GPART_FLOATS) == hipSuccess;
     for (int l = 0; l < 3 && ok; ++l)
         ok = dalloc(&m->H16[l], B * n_hidden) == hipSuccess && dalloc(&m->dZ16[l], B * n_hidden) == hipSuccess &&
              dalloc(&m->Z[l], B * n_hidden) == hipSuccess;
@@ -235,7 +251,7 @@ void irs_mlp_destroy(irs_mlp *m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void *ptrs[] = {m->P, m->G, m->Mo, m->Vo, m->P16, m->X16, m->dY16, m->Y, m->T, m->partial, m->loss, m->tang,
                     m->Xin, m->Xall, m->Tall, m->idx, m->H16[0], m->H16[1], m->H16[2], m->dZ16[0], m->dZ16[1],
-                    m->dZ16[2], m->Z[0], m->Z[1], m->Z[2]};
+                    m->dZ16[2], m->Z[0], m->Z[1], m->Z[2], m->Gpart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);

@@ -46,12 +46,14 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmArgs g) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int ks = g.splitk > 1 ? g.K / g.splitk : g.K;
+    const int kbeg = (int)blockIdx.z * ks, kend = kbeg + ks;
     f32x4_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < g.K; k0 += GK) {
+    for (int k0 = kbeg; k0 < kend; k0 += GK) {
         // stage A(m, k) for m in [m0, m0+64), k in [k0, k0+32): 2048 values, 8 per thread
         if (!TA) {   // A[m * lda + k], k contiguous
             const int m = t >> 2, k = (t & 3) * 8;
@@ -92,6 +94,20 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmArgs g) {
         __syncthreads();
     }
     // epilogue: lane holds C[wm + 16 i + 4 (lane / 16) + r][wn + 16 j + lane % 16]
+    if (g.splitk > 1) {
+        float *cp = g.Cpart + (int64_t)blockIdx.z * g.M * g.ldc;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + rr;
+                    const int n = n0 + wn + 16 * j + (lane & 15);
+                    cp[(int64_t)m * g.ldc + n] = acc[i][j][rr] * g.alpha;
+                }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -114,13 +130,35 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmArgs g) {
             }
 }
 
+// Split-K reduction: C[m][n] = sum over slices in order (deterministic).
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const float *part, int S, int M, int N, int ldc,
+                                                            float *C) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)M * N) return;
+    const int64_t o = (i / N) * ldc + i % N;
+    const int64_t stride = (int64_t)M * ldc;
+    float s = 0.0f;
+    for (int k = 0; k < S; ++k) s += part[k * stride + o];
+    C[o] = s;
+}
+
 hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s) {
     if (g.M % GT || g.N % GT || g.K % GK || g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
-    dim3 grid(g.N / GT, g.M / GT);
+    if (g.splitk > 1 && (!g.Cpart || !g.C32 || g.C16 || g.bias || g.flags || (g.K / g.splitk) % GK ||
+                         g.K % g.splitk))
+        return hipErrorInvalidValue;
+    dim3 grid(g.N / GT, g.M / GT, g.splitk > 1 ? g.splitk : 1);
     if (!ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, g);
     else if (!ta && tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, g);
     else if (ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(256), 0, s, g);
     else hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, g);
+    if (g.splitk > 1) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const int64_t n = (int64_t)g.M * g.N;
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.Cpart,
+                           g.splitk, g.M, g.N, g.ldc, g.C32);
+    }
     return hipGetLastError();
 }
 
@@ -177,13 +215,33 @@ hipError_t launch_mse_grad(const float *Y, const float *T, int64_t n, float scal
     return hipGetLastError();
 }
 
-// db[n] = sum_m dZ[m][n] (dZ given in bf16 as fed to the GEMMs); one thread per column.
-__global__ void colsum_bf16_kernel(const uint16_t *dZ, int M, int N, float *db) {
+// db[n] = sum_m dZ[m][n] (dZ given in bf16 as fed to the GEMMs).  Rows are cut into
+// S slices; a workgroup covers 64 consecutive columns (one per lane, coalesced
+// rows) of four slices, each thread a compensated sum over its slice; the slice
+// partials are then summed in slice order, so the result does not depend on
+// scheduling.
+__global__ void __launch_bounds__(256) colsum_part_kernel(const uint16_t *dZ, int M, int N, int S, float *part) {
+    const int t = threadIdx.x;
+    const int n = blockIdx.x * 64 + (t & 63);
+    const int sl = blockIdx.y * 4 + (t >> 6);
+    if (n >= N || sl >= S) return;
+    const int rows = (M + S - 1) / S, m0 = sl * rows, m1 = min(M, m0 + rows);
+    float s = 0.0f, c = 0.0f;
+    for (int m = m0; m < m1; ++m) {
+        const float y = bf2f(dZ[(int64_t)m * N + n]) - c;
+        const float tt = s + y;
+        c = (tt - s) - y;
+        s = tt;
+    }
+    part[(int64_t)sl * N + n] = s;
+}
+
+__global__ void __launch_bounds__(256) colsum_reduce_kernel(const float *part, int N, int S, float *db) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= N) return;
     float s = 0.0f, c = 0.0f;
-    for (int m = 0; m < M; ++m) {   // compensated: M up to a few thousand
-        const float y = bf2f(dZ[(int64_t)m * N + n]) - c;
+    for (int k = 0; k < S; ++k) {
+        const float y = part[(int64_t)k * N + n] - c;
         const float tt = s + y;
         c = (tt - s) - y;
         s = tt;
@@ -191,8 +249,17 @@ __global__ void colsum_bf16_kernel(const uint16_t *dZ, int M, int N, float *db) 
     db[n] = s;
 }
 
-hipError_t launch_colsum_bf16(const uint16_t *dZ, int M, int N, float *db, hipStream_t s) {
-    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 255) / 256), dim3(256), 0, s, dZ, M, N, db);
+hipError_t launch_colsum_bf16(const uint16_t *dZ, int M, int N, float *db, float *scratch, int64_t scratch_floats,
+                              hipStream_t s) {
+    int S = 1;
+    while (S < 128 && M / (2 * S) >= 16 && (int64_t)2 * S * N <= scratch_floats) S *= 2;
+    S = (S + 3) / 4 * 4;
+    if ((int64_t)S * N > scratch_floats) S = 4;
+    if ((int64_t)S * N > scratch_floats) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(colsum_part_kernel, dim3((N + 63) / 64, S / 4), dim3(256), 0, s, dZ, M, N, S, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(colsum_reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, s, scratch, N, S, db);
     return hipGetLastError();
 }
 

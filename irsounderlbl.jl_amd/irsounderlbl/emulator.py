@@ -510,11 +510,12 @@ class ChannelODEmulator:
                   budget: float = 0.05, margin: float = 0.2, n_blocks: int = 4, block: int = 16,
                   ctx: Optional[Context] = None, line_mixing="vpy", apply_continuum: bool = False,
                   bt_lbl=None, **kw) -> Calibration:
-        """Hold-out statistics on calibration scenes against GPU LBL: the
-        per-channel error, n_blocks spot blocks of `block` consecutive channels
-        where the hold-out error is largest, and the spot-error threshold t: the
-        largest t such that every calibration scene whose spot error is <= t is
-        within budget * (1 - margin) on every channel (0 when none is)."""
+        """Hold-out statistics on calibration scenes against GPU LBL (the per-channel
+        error), n_blocks spot blocks of `block` consecutive channels (select_spot_blocks:
+        a greedy cover of the scenes that exceed budget * (1 - margin) somewhere by
+        blocks whose error flags them) and the spot-error threshold t: the largest t such
+        that every calibration scene whose spot error is <= t is within
+        budget * (1 - margin) on every channel (0 when none is)."""
         scenes, eps = self._scenes(list(profs), geom, T_sfc, eps_sfc)
         _, bt = self.forward([s[0] for s in scenes], geom=[_geom(s[2]) for s in scenes], T_sfc=[s[1] for s in scenes],
                              eps_sfc=eps)
@@ -526,31 +527,17 @@ class ChannelODEmulator:
         d = np.abs(bt - bt_lbl)
         ch_rms = np.sqrt(np.mean(d ** 2, axis=0))
         ch_max = d.max(axis=0)
-        # spot blocks: greedy, largest summed hold-out RMS over `block` consecutive channels
-        score = np.convolve(ch_rms, np.ones(block), mode="valid")             # block starting at c
-        blocks, taken = [], np.zeros(self.n_ch, bool)
-        for _ in range(n_blocks):
-            sc = score.copy()
-            for c0, n in blocks:
-                sc[max(0, c0 - block + 1):c0 + n] = -np.inf
-            if not np.isfinite(sc.max()):
-                break
-            c0 = int(np.argmax(sc))
-            blocks.append((c0, block))
-            taken[c0:c0 + block] = True
-        blocks.sort()
+        cap = budget * (1.0 - margin)
+        blocks = select_spot_blocks(d, ch_rms, n_blocks, block, cap)
+        taken = np.zeros(self.n_ch, bool)
+        for c0, n in blocks:
+            taken[c0:c0 + n] = True
         spot = np.flatnonzero(taken)
         se, fe = d[:, spot].max(axis=1), d.max(axis=1)
-        thr = 0.0
-        for t in np.sort(se):
-            if np.all(fe[se <= t] <= budget * (1.0 - margin)):
-                thr = float(t)
-            else:
-                break
+        thr = spot_threshold(se, fe, cap)
         self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, se, len(scenes))
         self._spot = None
         return self.calibration
-
     def spot_checker(self, linelists, sounder: Sounder) -> "SpotChecker":
         if self.calibration is None:
             raise ValueError("the emulator has no calibration (run calibrate)")
@@ -559,6 +546,54 @@ class ChannelODEmulator:
             self._spot = SpotChecker(linelists, sounder,
                                      blocks=[(float(sg.nu[c0]), n) for c0, n in self.calibration.spot_blocks])
         return self._spot
+
+
+def spot_threshold(se, fe, cap) -> float:
+    """Largest spot error t such that every scene with spot error <= t has its largest
+    error over all channels <= cap (0 when the scene with the smallest spot error
+    already exceeds it)."""
+    thr = 0.0
+    for t in np.sort(se):
+        if np.all(fe[se <= t] <= cap):
+            thr = float(t)
+        else:
+            break
+    return thr
+
+
+def select_spot_blocks(d, ch_rms, n_blocks: int, block: int, cap: float, stride: int = 4, q: float = 0.9):
+    """Spot blocks on an error matrix d [scenes, channels] as a greedy cover of the bad
+    scenes (those exceeding cap on some channel): each step adds the non-overlapping
+    block of `block` channels (starts every `stride` channels) that flags the most bad
+    scenes not flagged yet -- a block flags a scene when the scene's largest error in
+    it exceeds the q-quantile of the good scenes' -- ties broken by the block's summed
+    hold-out RMS error.  Returns [(first, block)] sorted."""
+    n, nch = d.shape
+    if nch < block:
+        return [(0, nch)]
+    from numpy.lib.stride_tricks import sliding_window_view
+    starts = np.arange(0, nch - block + 1, stride)
+    dc = sliding_window_view(d, block, axis=1)[:, starts].max(axis=2)       # [n, m]
+    score = np.convolve(ch_rms, np.ones(block), mode="valid")[starts]
+    bad = d.max(axis=1) > cap
+    good = ~bad
+    lim = np.quantile(dc[good], q, axis=0) if good.any() else np.full(len(starts), np.inf)
+    flags = dc > lim[None, :]                                                  # [n, m]
+    open_ = bad.copy()
+    chosen = []
+    free = np.ones(len(starts), bool)
+    for _ in range(n_blocks):
+        if not free.any():
+            break
+        gain = np.where(free, (flags & open_[:, None]).sum(axis=0), -1)
+        best = gain.max()
+        pick = np.flatnonzero(gain == best)
+        k = int(pick[np.argmax(score[pick])])
+        chosen.append((int(starts[k]), block))
+        open_ &= ~flags[:, k]
+        free &= np.abs(starts - starts[k]) >= block
+    chosen.sort()
+    return chosen
 
 
 def _feature_level_jacobian(prof, T_sfc, mu, context: ProfileContext, T_method, hT=1e-2, hq=1e-3):

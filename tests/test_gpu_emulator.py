@@ -247,8 +247,8 @@ def test_emulator_trains_and_beats_regression_and_mean(ctx, small_emu):
 
 
 def test_emulator_calibrated_fallback_bounds_every_channel(ctx, small_emu):
-    """Calibration on 32 held-out scenes (spot blocks where the hold-out error is
-    largest, the spot threshold under which every calibration scene is within
+    """Calibration on 32 held-out scenes (spot blocks that flag the scenes over
+    0.8 x 0.05 K, the spot threshold under which every calibration scene is within
     0.8 x 0.05 K everywhere); on 32 other held-out scenes no scene that the spot
     check lets through exceeds 0.05 K on any channel, and fallen-back scenes are
     the LBL spectra."""
