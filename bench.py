@@ -15,12 +15,12 @@ in one forward_model call with the autotuned schedule and hipGraph replay.
 A third figure, "emulator", trains the channel-transmittance emulator on GPU
 LBL targets (the windows variant of the line lists, whose spectrum has
 partly transparent windows; the benchmark lists are opaque at every channel)
-of --emu-train perturbed scenes (2048 by default, --emu-steps 24000 Adam steps;
+of --emu-train perturbed scenes (8192 by default, --emu-steps 96000 Adam steps;
 T GP(0, 4 K), H2O x lognormal(0.6), O3 x lognormal(0.2), CO2 U(390, 430) ppm,
 scan angles U(0, 48.3 deg)), calibrates it on --emu-cal held-out scenes (the
 per-channel hold-out BT error against GPU LBL, the 4 x 16 spot channels where it
 is largest and the spot threshold for a 0.05 K budget), then runs --emu-eval new
-scenes (1024 by default) through emulator + spot check + full-LBL fallback and
+scenes (2048 by default) through emulator + spot check + full-LBL fallback and
 measures the BT error after fallback against GPU LBL on every scene the emulator
 answered (and against the FP64 oracle on two of them).  --emulator-only runs the
 headline steps and this figure alone.
@@ -308,7 +308,8 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value):
     log(f"bench: emulator training ({n_train} scenes of GPU LBL targets, per-layer PCA, regression, MLP)")
     t0 = time.perf_counter()
     emu, rep = ChannelODEmulator.train(train, lls, snd, n_pc=64, n_basis=min(2048, n_train), steps=steps,
-                                       batch=4096, lr=1e-3, ctx=ctx, progress=lambda m: log("bench: " + m))
+                                       batch=4096, lr=1e-3, n_hidden=1024, ctx=ctx,
+                                       progress=lambda m: log("bench: " + m))
     t_train = time.perf_counter() - t0
     del train
     nu = sounder_grid(snd).nu
@@ -362,7 +363,8 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value):
     out = {"workload": f"train {n_train} / calibrate {n_cal} / evaluate {n_eval} perturbed scenes (T GP 4 K, "
                        f"H2O x lognormal 0.6, O3 x lognormal 0.2, CO2 390-430 ppm, scan 0-48.3 deg), full IASI "
                        f"8461 channels, windows variant of the line lists; targets (t, g, ILS moment) x 49 layers, "
-                       f"64 PCs per layer, per-layer quadratic + context regression, MLP 9 -> 3x512 GELU -> 64 "
+                       f"64 PCs per layer, per-layer quadratic + context regression on 13 layer features "
+                       f"(incl. Curtis-Godson means of the path above), MLP 13 -> 3x1024 GELU -> 64 "
                        f"(BF16 MFMA) on its residual; spot check 4 x 16 channels",
            "training": {"seconds": round(t_train, 1), "lbl_targets_seconds": round(rep.seconds_lbl, 1),
                         "pca_seconds": round(rep.seconds_pca, 1), "regression_seconds": round(rep.seconds_fit, 1),
@@ -599,10 +601,10 @@ def main():
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-par", action="store_true")
-    ap.add_argument("--emu-train", type=int, default=2048)
-    ap.add_argument("--emu-cal", type=int, default=256)
-    ap.add_argument("--emu-eval", type=int, default=1024)
-    ap.add_argument("--emu-steps", type=int, default=24000)
+    ap.add_argument("--emu-train", type=int, default=8192)
+    ap.add_argument("--emu-cal", type=int, default=512)
+    ap.add_argument("--emu-eval", type=int, default=2048)
+    ap.add_argument("--emu-steps", type=int, default=96000)
     ap.add_argument("--emulator-only", action="store_true",
                     help="the headline steps and the emulator figure only (the other figures off)")
     args = ap.parse_args()

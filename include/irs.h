@@ -500,7 +500,10 @@ int         irs_emu_set_context(irs_emu *emu, int n_ctx, int n_vec, const double
 
 /* Emulator + per-scene LBL spot check + full-LBL fallback in one call (the learned
  * path behind forward_model's signature; no counterpart in the reference).  The
- * emulator answers every scene; block b of consecutive channels [block_c0[b],
+ * emulator answers every scene (n_local must be 13: per layer ln p_mid, T_ave, ln vmr
+ * H2O, ln vmr O3, CO2 ppm, 1/mu, ln H2O and O3 columns above, T_ave of the layer
+ * above, and the H2O-weighted ln p and T, O3-weighted T and mass-weighted T of the
+ * layers above; the library computes them); block b of consecutive channels [block_c0[b],
  * block_c0[b] + block_n[b]) is recomputed by LBL on its own sub-grid with the line
  * lists block_lists[b][n_species] (the caller's lists filtered to the block
  * +- 42 cm^-1); a scene whose largest |BT_emulator - BT_LBL| over the spot channels

@@ -792,10 +792,13 @@ extern "C" int irs_emu_set_context(irs_emu *e, int n_ctx, int n_vec, const doubl
 
 namespace {
 
-// Raw emulator inputs of one scene [L][n_feat] (9 layer features, then the profile
+// Raw emulator inputs of one scene [L][n_feat] (13 layer features, then the profile
 // context): ln p_mid, T_ave, ln vmr_cg of H2O and O3 (floor 1e-30), CO2 vmr_cg in
 // ppm, 1/mu, ln of the H2O and O3 columns above the layer (sum vmr_cg dp, + 1e-9 /
-// + 1e-12), T_ave of the layer above (its own at the top).
+// + 1e-12), T_ave of the layer above (its own at the top), and the Curtis-Godson
+// means of the path above: H2O-weighted ln p_mid and T_ave, O3-weighted T_ave and
+// mass-weighted T_ave ((sum_{j>k} u_j x_j + eps x_k) / (sum_{j>k} u_j + eps), with
+// eps 1e-9, 1e-9, 1e-12, 1e-6, so the layer's own value where nothing is above).
 int emu_scene_features(const irs_emu *e, const irs_profile &pr, int T_method, float *out, double *T_ave,
                        double &T_sfc) {
     const int nl = pr.n_levels, L = nl - 1, ns = pr.n_species;
@@ -809,13 +812,26 @@ int emu_scene_features(const irs_emu *e, const irs_profile &pr, int T_method, fl
     };
     const int sh = idx(1), so = idx(3), sc = idx(2);
     auto v = [&](int s, int k) { return s < 0 ? 0.0 : vc[(size_t)s * L + k]; };
-    std::vector<double> ah(L, 0.0), ao(L, 0.0);
-    double acc_h = 0.0, acc_o = 0.0;
-    for (int k = L - 1; k >= 0; --k) {           // sum over the layers above k
+    // sums over the layers above k: amounts u = vmr dp of H2O and O3 and the mass dp,
+    // and their moments in ln p and T (Curtis-Godson means of the overlying path)
+    std::vector<double> ah(L, 0.0), ao(L, 0.0), ahp(L, 0.0), aht(L, 0.0), aot(L, 0.0), am(L, 0.0), amt(L, 0.0);
+    double acc_h = 0.0, acc_o = 0.0, acc_hp = 0.0, acc_ht = 0.0, acc_ot = 0.0, acc_m = 0.0, acc_mt = 0.0;
+    for (int k = L - 1; k >= 0; --k) {
         ah[k] = acc_h;
         ao[k] = acc_o;
-        acc_h += v(sh, k) * dp[k];
-        acc_o += v(so, k) * dp[k];
+        ahp[k] = acc_hp;
+        aht[k] = acc_ht;
+        aot[k] = acc_ot;
+        am[k] = acc_m;
+        amt[k] = acc_mt;
+        const double uh = v(sh, k) * dp[k], uo = v(so, k) * dp[k], lnp = std::log(pmid[k]);
+        acc_h += uh;
+        acc_o += uo;
+        acc_hp += uh * lnp;
+        acc_ht += uh * T_ave[k];
+        acc_ot += uo * T_ave[k];
+        acc_m += dp[k];
+        acc_mt += dp[k] * T_ave[k];
     }
     T_sfc = pr.T_sfc > 0.0 ? pr.T_sfc : pr.T[0];
     // context vector
@@ -841,9 +857,12 @@ int emu_scene_features(const irs_emu *e, const irs_profile &pr, int T_method, fl
     }
     for (int k = 0; k < L; ++k) {
         float *f = out + (size_t)k * e->n_feat;
-        const double loc[9] = {std::log(pmid[k]), T_ave[k], std::log(std::max(v(sh, k), 1e-30)),
-                               std::log(std::max(v(so, k), 1e-30)), v(sc, k) * 1e6, 1.0 / pr.mu,
-                               std::log(ah[k] + 1e-9), std::log(ao[k] + 1e-12), T_ave[k + 1 < L ? k + 1 : k]};
+        const double lnp = std::log(pmid[k]), Tk = T_ave[k];
+        const double loc[13] = {lnp, Tk, std::log(std::max(v(sh, k), 1e-30)),
+                                std::log(std::max(v(so, k), 1e-30)), v(sc, k) * 1e6, 1.0 / pr.mu,
+                                std::log(ah[k] + 1e-9), std::log(ao[k] + 1e-12), T_ave[k + 1 < L ? k + 1 : k],
+                                (ahp[k] + 1e-9 * lnp) / (ah[k] + 1e-9), (aht[k] + 1e-9 * Tk) / (ah[k] + 1e-9),
+                                (aot[k] + 1e-12 * Tk) / (ao[k] + 1e-12), (amt[k] + 1e-6 * Tk) / (am[k] + 1e-6)};
         for (int i = 0; i < e->n_local; ++i) f[i] = (float)loc[i];
         for (int c = 0; c < e->n_ctx; ++c) f[e->n_local + c] = (float)cx[c];
     }
@@ -859,8 +878,8 @@ extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile 
     if (!ctx || !e || !profiles || n_prof < 1 || !snd || !R || !BT || !fell_back || n_blocks < 0 ||
         (n_blocks > 0 && (!block_c0 || !block_n || !block_lists)))
         return IRS_EINVAL;
-    if (e->n_local != 9 || e->n_local + e->n_ctx != e->n_feat) {
-        e->err = "the emulator needs 9 layer features and its profile context (irs_emu_set_context)";
+    if (e->n_local != 13 || e->n_local + e->n_ctx != e->n_feat) {
+        e->err = "the emulator needs 13 layer features and its profile context (irs_emu_set_context)";
         return IRS_EINVAL;
     }
     irs_fm_options o;

@@ -18,9 +18,9 @@ per-layer least-squares fits, which are FP64/FP32 host linear algebra):
      bases come from the first scenes, later scenes are projected as their
      targets are produced, so the training set is never held in memory whole;
   3. model of the coefficients: a per-layer quadratic regression in the layer's
-     own 9 features plus a linear term in 8 profile-context features (leading
+     own 13 features plus a linear term in 8 profile-context features (leading
      principal components of the standardised T, ln q_H2O, ln q_O3 profiles,
-     T_sfc and 1/mu), and an MLP 9 -> 3 x 512 GELU -> n_pc trained with the BF16
+     T_sfc and 1/mu), and an MLP 13 -> 3 x n_hidden GELU -> n_pc trained with the BF16
      MFMA kernels (FP32 accumulation, Adam) on the regression's residual;
   4. inference: features -> coefficients (regression + MLP) -> per-layer decode ->
      channel RTE (irs_emu_forward), batched on the device;
@@ -44,11 +44,13 @@ from .api import (IRS_OK, Context, IRSError, _p, channel_optical_depths, forward
 from .types import (AtmosphericProfile, GasSpecies, HITRANLinelist, Sounder, ViewingGeometry, filter_linelist,
                     sounder_grid)
 
-N_LOCAL = 9
+N_LOCAL = 13
 N_CTX = 8
 N_FEATURES = N_LOCAL + N_CTX
 FEATURE_NAMES = ("ln p_mid [hPa]", "T_ave [K]", "ln vmr H2O", "ln vmr O3", "vmr CO2 [ppm]", "1/mu",
-                 "ln H2O column above [hPa]", "ln O3 column above [hPa]", "T_ave of the layer above [K]") + \
+                 "ln H2O column above [hPa]", "ln O3 column above [hPa]", "T_ave of the layer above [K]",
+                 "H2O-weighted ln p above", "H2O-weighted T above [K]", "O3-weighted T above [K]",
+                 "mass-weighted T above [K]") + \
                 tuple(f"profile context {i}" for i in range(N_CTX))
 W_G, W_M = 0.1, 0.01          # decode weights of the CIM weight and the ILS moment relative to t_k
 
@@ -127,25 +129,35 @@ class MLP:
 
 
 def layer_features(prof, mu: float = 1.0, T_method: str = "logp_at_pcg") -> np.ndarray:
-    """Per-layer emulator inputs [L, 9]: ln p_mid, T_ave (TBAR), ln vmr_cg of H2O
+    """Per-layer emulator inputs [L, 13]: ln p_mid, T_ave (TBAR), ln vmr_cg of H2O
     and O3 (1e-30 floor), CO2 vmr_cg in ppm, the path factor 1/mu, and the
-    overlying column (the channel optical depth of a layer is defined through
-    the transmittance above it): ln of the H2O and O3 amounts above the layer
-    (sum vmr_cg dp over the layers above, + a small floor) and T_ave of the
-    layer above (its own at the top)."""
+    overlying path (the channel transmittance of a layer is the ratio of the
+    ILS-convolved transmittances below and above it, so it depends on the spectral
+    structure of the path above): ln of the H2O and O3 amounts above the layer
+    (sum vmr_cg dp over the layers above, + a small floor), T_ave of the layer
+    above (its own at the top), and the Curtis-Godson scaling of the path above --
+    the H2O-weighted mean ln p and T, the O3-weighted mean T and the mass-weighted
+    mean T of the layers above (u-weighted means sum u_j x_j / sum u_j, which fall
+    back to the layer's own x where nothing is above)."""
     lp = layer_properties(prof, T_method)
     L = prof.n_levels - 1
     def v(s):
         return lp["vmr_cg"].get(GasSpecies(s), np.zeros(L))
     dp = lp["dp"]
     def above(q):
-        a = np.cumsum((q * dp)[::-1])[::-1]                  # sum_{j>=k}
+        a = np.cumsum(q[::-1])[::-1]                         # sum_{j>=k}
         return np.concatenate([a[1:], [0.0]])                # sum_{j>k}
+    def cg_mean(u, x, eps):
+        return (above(u * x) + eps * x) / (above(u) + eps)
     Ta = lp["T_ave"]
+    lnp = np.log(lp["p_mid"])
     T_up = np.concatenate([Ta[1:], Ta[-1:]])
-    return np.stack([np.log(lp["p_mid"]), Ta, np.log(np.maximum(v(1), 1e-30)),
+    uh, uo = v(1) * dp, v(3) * dp
+    return np.stack([lnp, Ta, np.log(np.maximum(v(1), 1e-30)),
                      np.log(np.maximum(v(3), 1e-30)), v(2) * 1e6, np.full(L, 1.0 / mu),
-                     np.log(above(v(1)) + 1e-9), np.log(above(v(3)) + 1e-12), T_up], axis=1)
+                     np.log(above(uh) + 1e-9), np.log(above(uo) + 1e-12), T_up,
+                     cg_mean(uh, lnp, 1e-9), cg_mean(uh, Ta, 1e-9), cg_mean(uo, Ta, 1e-12),
+                     cg_mean(dp, Ta, 1e-6)], axis=1)
 
 
 def _randomized_pca(dev: np.ndarray, k: int, oversample: int = 32, power: int = 2, seed: int = 0):
@@ -322,6 +334,7 @@ class ChannelODEmulator:
     @staticmethod
     def train(profiles: Sequence, linelists: Dict[GasSpecies, HITRANLinelist], sounder: Sounder, *,
               n_pc: int = 64, n_basis: int = 2048, steps: int = 24000, batch: int = 4096, lr: float = 1e-3,
+              n_hidden: int = 1024,
               seed: int = 0, floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None,
               relevance_floor: float = 1e-2, chunk: int = 256, apply_continuum: bool = False, progress=None,
               T_method: str = "logp_at_pcg", **kw):
@@ -406,7 +419,7 @@ class ChannelODEmulator:
         t_fit = time.perf_counter() - tf
         tt = time.perf_counter()
         scale = float(max(res.std(), 1e-12))
-        mlp = MLP(N_LOCAL, 512, n_pc, max(batch, 64), seed=seed)
+        mlp = MLP(N_LOCAL, n_hidden, n_pc, max(batch, 64), seed=seed)
         mlp.set_data(Z[..., :N_LOCAL].reshape(-1, N_LOCAL).astype(np.float32),
                      (res / scale).reshape(-1, n_pc).astype(np.float32))
         # Adam at lr for half the steps, then lr/3 and lr/10 for a quarter each (in

@@ -208,7 +208,7 @@ def small_emu(ctx):
     pp = syn.perturbed_profiles(264, seed=31, sigma_T=4.0, sigma_h2o=0.4)
     train, hold = pp[:200], pp[200:]
     emu, rep = ChannelODEmulator.train(train, sub, snd, n_pc=64, n_basis=128, chunk=64, steps=3000, batch=1024,
-                                       lr=1e-3, ctx=ctx)
+                                       lr=1e-3, n_hidden=512, ctx=ctx)
     return sub, snd, emu, rep, hold
 
 
