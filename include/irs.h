@@ -508,11 +508,26 @@ int         irs_emu_set_context(irs_emu *emu, int n_ctx, int n_vec, const double
  * lists block_lists[b][n_species] (the caller's lists filtered to the block
  * +- 42 cm^-1); a scene whose largest |BT_emulator - BT_LBL| over the spot channels
  * exceeds `threshold` is recomputed with full LBL (irs_forward with opt) and
- * fell_back[q] = 1.  R, BT [n_prof][n_channels]; spot_err [n_prof] may be NULL. */
+ * fell_back[q] = 1.  With a correction set (irs_emu_set_correction, whose blocks these
+ * must be) every channel of every scene is first corrected from the scene's spot errors,
+ * the spot channels take their LBL values, and the fallback test compares the
+ * correction's statistic with `threshold` instead.  R, BT [n_prof][n_channels];
+ * spot_err [n_prof] (the tested statistic) may be NULL. */
 int         irs_forward_emulated(irs_ctx *ctx, irs_emu *emu, const irs_profile *profiles, int n_prof,
                                  const irs_sounder *snd, const irs_fm_options *opt, int n_blocks,
                                  const int64_t *block_c0, const int32_t *block_n, const int32_t *block_lists,
                                  double threshold, double *R, double *BT, uint8_t *fell_back, double *spot_err);
+
+/* Spot-check correction of the emulator (no counterpart in the reference): the scene's
+ * errors at the spot channels, e = BT_emulator[S] - BT_LBL[S] over the n_blocks blocks
+ * [block_c0[b], block_c0[b] + block_n[b]) (irs_forward_emulated must get the same blocks
+ * in the same order; at most 1024 channels), move every channel by the linear map fitted on calibration scenes,
+ * BT[c] -= sum_s e[s] W[s][c] (W [n_spot][n_channels]).  stat selects the fallback
+ * statistic: 0, max |e|; 1, the largest leave-one-block-out residual
+ * max_b max |e_b - e_{S without b} M_b| with lbo_W the maps M_b [n_spot - n_b][n_b],
+ * concatenated over b.  n_blocks = 0 removes the correction. */
+int         irs_emu_set_correction(irs_emu *emu, int n_blocks, const int64_t *block_c0, const int32_t *block_n,
+                                   const float *W, const float *lbo_W, int stat);
 
 /* Emulator Jacobians (chain rule through the MLP input Jacobian, the regression,
  * the decode and the adjoint of the channel RTE; the reflected term of eps < 1 held fixed):

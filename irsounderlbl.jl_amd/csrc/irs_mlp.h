@@ -53,6 +53,12 @@ hipError_t launch_emu_coef(const float *y, const float *phi, const float *A, flo
                            int L, int n_phi, int n_pc, float *coef, hipStream_t s);
 hipError_t launch_emu_rte(const EmuRteArgs &a, hipStream_t s);
 
+// Spot-check correction: BT[q][c] -= sum_s e[q][s] W[s][c], R = B(nu_c, BT) (at most
+// EMU_MAX_SPOT spot channels).
+#define EMU_MAX_SPOT 1024
+hipError_t launch_emu_correct(const float *e, const float *W, int n_spot, int64_t n_ch, int n_prof,
+                              const double *ch_nu, double *R, double *BT, hipStream_t s);
+
 // Emulator Jacobian outputs (all multiplied by dBT/dR)
 struct EmuAdjArgs {
     double *BT;                  // [n_prof][n_ch]

@@ -75,6 +75,14 @@ struct irs_emu {
     // [T levels, ln q_H2O levels, ln q_O3 levels, T_sfc, 1/mu] - mean
     int n_ctx = 0, n_vec = 0;
     std::vector<double> ctx_mean, ctx_proj;
+    // spot-check correction (irs_emu_set_correction): spot blocks of corr_bn[b] channels,
+    // W [n_spot][n_ch] on the device, the leave-one-block-out maps on the host
+    std::vector<int> corr_bn;
+    std::vector<int64_t> corr_c0;
+    int corr_ns = 0, corr_stat = 0;
+    float *corr_W = nullptr, *corr_e = nullptr;
+    int64_t cap_corr_e = 0;
+    std::vector<float> corr_lbo;
 };
 
 namespace {
@@ -489,7 +497,8 @@ void irs_emu_destroy(irs_emu *e) {
     if (!e) return;
     (void)hipSetDevice(e->mlp->device);
     (void)hipStreamSynchronize(e->mlp->stream);
-    void *ptrs[] = {e->A, e->basis, e->mean, e->inv_w, e->ch_nu, e->coef, e->phi, e->prof, e->R, e->BT};
+    void *ptrs[] = {e->A, e->basis, e->mean, e->inv_w, e->ch_nu, e->coef, e->phi, e->prof, e->R, e->BT, e->corr_W,
+                    e->corr_e};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete e;
@@ -871,6 +880,85 @@ int emu_scene_features(const irs_emu *e, const irs_profile &pr, int T_method, fl
 
 }  // namespace
 
+extern "C" int irs_emu_set_correction(irs_emu *e, int n_blocks, const int64_t *block_c0, const int32_t *block_n,
+                                      const float *W, const float *lbo_W, int stat) {
+    if (!e || n_blocks < 0) return IRS_EINVAL;
+    irs_mlp *m = e->mlp;
+    (void)hipSetDevice(m->device);
+    if (e->corr_W) (void)hipFree(e->corr_W);
+    e->corr_W = nullptr;
+    e->corr_bn.clear();
+    e->corr_c0.clear();
+    e->corr_lbo.clear();
+    e->corr_ns = 0;
+    if (n_blocks == 0) return IRS_OK;
+    if (!block_c0 || !block_n || !W || (stat != 0 && stat != 1) || (stat == 1 && (!lbo_W || n_blocks < 2))) {
+        e->err = "irs_emu_set_correction: block sizes and W are required; stat is 0 or 1 (1 needs lbo_W, 2+ blocks)";
+        return IRS_EINVAL;
+    }
+    int64_t ns = 0, n_lbo = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+        if (block_n[b] < 1 || block_c0[b] < 0 || block_c0[b] + block_n[b] > e->n_ch) return IRS_EINVAL;
+        ns += block_n[b];
+    }
+    for (int b = 0; b < n_blocks; ++b) n_lbo += (ns - block_n[b]) * block_n[b];
+    if (ns > EMU_MAX_SPOT) {
+        e->err = "irs_emu_set_correction: more than EMU_MAX_SPOT (1024) spot channels";
+        return IRS_EINVAL;
+    }
+    hipError_t er = dalloc(&e->corr_W, (size_t)ns * e->n_ch);
+    if (er == hipSuccess)
+        er = hipMemcpy(e->corr_W, W, sizeof(float) * (size_t)ns * e->n_ch, hipMemcpyHostToDevice);
+    if (er != hipSuccess) {
+        if (e->corr_W) (void)hipFree(e->corr_W);
+        e->corr_W = nullptr;
+        e->err = std::string("irs_emu_set_correction: ") + hipGetErrorString(er);
+        return IRS_EHIP;
+    }
+    e->corr_bn.assign(block_n, block_n + n_blocks);
+    e->corr_c0.assign(block_c0, block_c0 + n_blocks);
+    if (stat == 1) e->corr_lbo.assign(lbo_W, lbo_W + n_lbo);
+    e->corr_ns = (int)ns;
+    e->corr_stat = stat;
+    return IRS_OK;
+}
+
+namespace {
+
+// Decision statistic of the corrected emulator for one scene from its spot errors
+// es[n_spot] (blocks in order): stat 0, the largest |error|; stat 1, the largest
+// leave-one-block-out residual |e_b - e_{S\b} M_b| (M_b [n_spot - n_b][n_b] row-major,
+// the map fitted on calibration scenes from the other blocks' errors to block b's).
+double emu_spot_stat(const irs_emu *e, const float *es) {
+    const int nb = (int)e->corr_bn.size(), ns = e->corr_ns;
+    double st = 0.0;
+    if (e->corr_stat == 0) {
+        for (int i = 0; i < ns; ++i) st = std::max(st, (double)std::fabs(es[i]));
+        return st;
+    }
+    std::vector<double> pred;
+    const float *M = e->corr_lbo.data();
+    int off = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int n = e->corr_bn[b], no = ns - n;
+        pred.assign((size_t)n, 0.0);
+        int j = 0;
+        for (int i = 0; i < ns; ++i) {
+            if (i >= off && i < off + n) continue;
+            const double x = es[i];
+            const float *row = M + (size_t)j * n;
+            for (int c = 0; c < n; ++c) pred[c] += x * (double)row[c];
+            ++j;
+        }
+        for (int c = 0; c < n; ++c) st = std::max(st, std::fabs((double)es[off + c] - pred[c]));
+        M += (size_t)no * n;
+        off += n;
+    }
+    return st;
+}
+
+}  // namespace
+
 extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile *profiles, int n_prof,
                                     const irs_sounder *snd, const irs_fm_options *opt, int n_blocks,
                                     const int64_t *block_c0, const int32_t *block_n, const int32_t *block_lists,
@@ -916,6 +1004,19 @@ extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile 
     //    with the line lists that reach it
     std::vector<double> err((size_t)n_prof, 0.0);
     std::vector<irs_profile> sub(profiles, profiles + n_prof);
+    const bool corr = e->corr_ns > 0;
+    const int ns_spot = e->corr_ns;
+    if (corr) {
+        bool ok = n_blocks == (int)e->corr_bn.size();
+        for (int b = 0; ok && b < n_blocks; ++b) ok = block_n[b] == e->corr_bn[b] && block_c0[b] == e->corr_c0[b];
+        if (!ok) {
+            e->err = "the spot blocks differ from those of the emulator's correction (irs_emu_set_correction)";
+            return IRS_EINVAL;
+        }
+    }
+    std::vector<float> es(corr ? (size_t)n_prof * ns_spot : 0);
+    std::vector<double> spot_R(corr ? (size_t)n_prof * ns_spot : 0), spot_BT(spot_R.size());
+    std::vector<int64_t> spot_ch;
     for (int b = 0; b < n_blocks; ++b) {
         const int64_t c0 = block_c0[b];
         const int32_t nb = block_n[b];
@@ -928,9 +1029,52 @@ extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile 
         std::vector<double> rb((size_t)n_prof * nb), bb((size_t)n_prof * nb);
         rc = irs_forward(ctx, sub.data(), n_prof, &sb, &o, rb.data(), bb.data());
         if (rc) return rc;
+        const size_t off = spot_ch.size();
+        for (int c = 0; c < nb; ++c) spot_ch.push_back(c0 + c);
         for (int q = 0; q < n_prof; ++q)
-            for (int c = 0; c < nb; ++c)
-                err[q] = std::max(err[q], std::fabs(BT[(size_t)q * nch + c0 + c] - bb[(size_t)q * nb + c]));
+            for (int c = 0; c < nb; ++c) {
+                const double d = BT[(size_t)q * nch + c0 + c] - bb[(size_t)q * nb + c];
+                err[q] = std::max(err[q], std::fabs(d));
+                if (corr) {
+                    const size_t i = (size_t)q * ns_spot + off + c;
+                    es[i] = (float)d;
+                    spot_R[i] = rb[(size_t)q * nb + c];
+                    spot_BT[i] = bb[(size_t)q * nb + c];
+                }
+            }
+    }
+    // 2b. correction of every channel from the spot errors (the map fitted on
+    //     calibration scenes), then the spot channels take their LBL values
+    if (corr) {
+        irs_mlp *m = e->mlp;
+        (void)hipSetDevice(m->device);
+        for (int q = 0; q < n_prof; ++q) err[q] = emu_spot_stat(e, es.data() + (size_t)q * ns_spot);
+        if (e->cap_corr_e < (int64_t)es.size()) {
+            if (e->corr_e) (void)hipFree(e->corr_e);
+            e->corr_e = nullptr;
+            e->cap_corr_e = 0;
+            ECHK(dalloc(&e->corr_e, es.size()));
+            e->cap_corr_e = (int64_t)es.size();
+        }
+        hipError_t er = hipMemcpyAsync(e->corr_e, es.data(), sizeof(float) * es.size(), hipMemcpyHostToDevice,
+                                       m->stream);
+        if (er == hipSuccess)
+            er = launch_emu_correct(e->corr_e, e->corr_W, ns_spot, nch, n_prof, e->ch_nu, e->R, e->BT, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(R, e->R, sizeof(double) * n_prof * nch, hipMemcpyDeviceToHost, m->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(BT, e->BT, sizeof(double) * n_prof * nch, hipMemcpyDeviceToHost, m->stream);
+        if (er == hipSuccess) er = hipStreamSynchronize(m->stream);
+        if (er != hipSuccess) {
+            (void)hipStreamSynchronize(m->stream);
+            e->err = std::string("spot-check correction: ") + hipGetErrorString(er);
+            return IRS_EHIP;
+        }
+        for (int q = 0; q < n_prof; ++q)
+            for (int i = 0; i < ns_spot; ++i) {
+                R[(size_t)q * nch + spot_ch[i]] = spot_R[(size_t)q * ns_spot + i];
+                BT[(size_t)q * nch + spot_ch[i]] = spot_BT[(size_t)q * ns_spot + i];
+            }
     }
     // 3. full LBL for the scenes over the threshold
     std::vector<irs_profile> fb;

@@ -146,6 +146,35 @@ hipError_t launch_emu_rte(const EmuRteArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Spot-check correction of emulated spectra: with the scene's errors at the n_spot
+// spot channels e_q = BT_emulated[S] - BT_LBL[S], every channel moves by the linear
+// map fitted on calibration scenes, BT[q][c] -= sum_s e_q[s] W[s][c], and R is the
+// Planck radiance of the corrected BT.  One thread per (channel, scene); the
+// scene's errors sit in LDS (broadcast reads), W is read along c (coalesced).
+__global__ void __launch_bounds__(256) emu_correct_kernel(const float *e, const float *W, int n_spot, int64_t n_ch,
+                                                          const double *ch_nu, double *R, double *BT) {
+    __shared__ float s_e[EMU_MAX_SPOT];
+    const int q = blockIdx.y;
+    for (int i = threadIdx.x; i < n_spot; i += 256) s_e[i] = e[(int64_t)q * n_spot + i];
+    __syncthreads();
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n_ch) return;
+    float acc = 0.0f;
+    for (int k = 0; k < n_spot; ++k) acc = fmaf(s_e[k], W[(int64_t)k * n_ch + c], acc);
+    const int64_t o = (int64_t)q * n_ch + c;
+    const double bt = BT[o] - (double)acc;
+    BT[o] = bt;
+    R[o] = bt > 0.0 ? planck64(ch_nu[c], bt) : 0.0;
+}
+
+hipError_t launch_emu_correct(const float *e, const float *W, int n_spot, int64_t n_ch, int n_prof,
+                              const double *ch_nu, double *R, double *BT, hipStream_t s) {
+    if (n_spot < 1 || n_spot > EMU_MAX_SPOT || n_prof < 1 || n_ch < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(emu_correct_kernel, dim3((unsigned)((n_ch + 255) / 256), (unsigned)n_prof), dim3(256), 0, s, e,
+                       W, n_spot, n_ch, ch_nu, R, BT);
+    return hipGetLastError();
+}
+
 // Adjoint of the emulated channel RTE (one profile per workgroup row).  With
 // f_k = B_k (1 - t_k) + (bt_k - B_k) g_k + (B'_k - eps B'_s) m_k the radiance is
 // R = eps B_s T_0 + sum_k T_{k+1} f_k, T_k = t_k T_{k+1}, and Lambda_k = dR/dT_k

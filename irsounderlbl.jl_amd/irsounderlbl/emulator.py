@@ -279,8 +279,14 @@ class Calibration:
     ch_rms: np.ndarray                 # per-channel hold-out BT error, RMS over the scenes
     ch_max: np.ndarray                 # per-channel largest hold-out error
     scene_max: np.ndarray              # per-scene largest error over all channels
-    scene_spot: np.ndarray             # per-scene largest error over the spot channels
+    scene_spot: np.ndarray             # per-scene fallback statistic (largest spot error, or the correction's)
     n_scenes: int
+    # spot-check correction (None: the emulator's spectra are used as they come)
+    W: Optional[np.ndarray] = field(repr=False, default=None)          # [n_spot, n_ch]
+    lbo_W: Optional[np.ndarray] = field(repr=False, default=None)      # leave-one-block-out maps, concatenated
+    stat: int = 0                                                      # 0: max |e_S|, 1: leave-one-block-out residual
+    scene_max_corrected: Optional[np.ndarray] = field(repr=False, default=None)   # out-of-fold, per scene
+    fallback_rate: float = float("nan")                                # calibration scenes over the threshold
 
 
 class ChannelODEmulator:
@@ -335,7 +341,7 @@ class ChannelODEmulator:
     def train(profiles: Sequence, linelists: Dict[GasSpecies, HITRANLinelist], sounder: Sounder, *,
               n_pc: int = 64, n_basis: int = 2048, steps: int = 24000, batch: int = 4096, lr: float = 1e-3,
               n_hidden: int = 1024,
-              seed: int = 0, floor: float = 1e-4, line_mixing="vpy", ctx: Optional[Context] = None,
+              seed: int = 0, floor: float = 1e-8, line_mixing="vpy", ctx: Optional[Context] = None,
               relevance_floor: float = 1e-2, chunk: int = 256, apply_continuum: bool = False, progress=None,
               T_method: str = "logp_at_pcg", **kw):
         """Train on scenes `profiles` (AtmosphericProfile or (profile, T_sfc, mu)
@@ -448,8 +454,11 @@ class ChannelODEmulator:
         extra = {}
         if cal is not None:
             extra = dict(cal_spot=cal.spot_channels, cal_blocks=np.array(cal.spot_blocks, np.int64),
-                         cal_thr=np.array([cal.threshold, cal.budget, cal.n_scenes]), cal_ch_rms=cal.ch_rms,
-                         cal_ch_max=cal.ch_max, cal_scene_max=cal.scene_max, cal_scene_spot=cal.scene_spot)
+                         cal_thr=np.array([cal.threshold, cal.budget, cal.n_scenes, cal.stat, cal.fallback_rate]),
+                         cal_ch_rms=cal.ch_rms, cal_ch_max=cal.ch_max, cal_scene_max=cal.scene_max,
+                         cal_scene_spot=cal.scene_spot)
+            if cal.W is not None:
+                extra.update(cal_W=cal.W, cal_lbo=cal.lbo_W, cal_scene_max_corr=cal.scene_max_corrected)
         np.savez_compressed(path, params=self.mlp.get_params(), x_mean=self.x_mean, x_std=self.x_std, A=self.A,
                             coef_scale=np.array([self.coef_scale]), basis=self.basis, mean=self.mean,
                             inv_w=self.inv_w, ch_nu=self.ch_nu, ctx_mean=self.context.p_mean,
@@ -466,10 +475,15 @@ class ChannelODEmulator:
                                 z["mean"], z["inv_w"], z["ch_nu"],
                                 ProfileContext(z["ctx_mean"], z["ctx_std"], z["ctx_proj"]))
         if "cal_spot" in z:
-            thr, budget, n = z["cal_thr"]
-            emu.calibration = Calibration(z["cal_spot"], [tuple(b) for b in z["cal_blocks"]], float(thr),
-                                          float(budget), z["cal_ch_rms"], z["cal_ch_max"], z["cal_scene_max"],
-                                          z["cal_scene_spot"], int(n))
+            thr, budget, n, stat, rate = z["cal_thr"]
+            corr = "cal_W" in z
+            emu.calibration = Calibration(z["cal_spot"], [tuple(int(v) for v in b) for b in z["cal_blocks"]],
+                                          float(thr), float(budget), z["cal_ch_rms"], z["cal_ch_max"],
+                                          z["cal_scene_max"], z["cal_scene_spot"], int(n),
+                                          W=z["cal_W"] if corr else None, lbo_W=z["cal_lbo"] if corr else None,
+                                          stat=int(stat), scene_max_corrected=z["cal_scene_max_corr"] if corr else None,
+                                          fallback_rate=float(rate))
+            emu._set_correction(emu.calibration)
         return emu
 
     # ---- inference ----------------------------------------------------------
@@ -520,15 +534,22 @@ class ChannelODEmulator:
 
     # ---- calibration --------------------------------------------------------
     def calibrate(self, profs, linelists, *, sounder: Sounder, geom=None, T_sfc=None, eps_sfc=1.0,
-                  budget: float = 0.05, margin: float = 0.2, n_blocks: int = 4, block: int = 16,
+                  budget: float = 0.05, margin: float = 0.2, n_blocks: int = 4, block: int = 64,
                   ctx: Optional[Context] = None, line_mixing="vpy", apply_continuum: bool = False,
-                  bt_lbl=None, **kw) -> Calibration:
+                  bt_lbl=None, correct: bool = True, lam: float = 1e-3, folds: int = 4, **kw) -> Calibration:
         """Hold-out statistics on calibration scenes against GPU LBL (the per-channel
-        error), n_blocks spot blocks of `block` consecutive channels (select_spot_blocks:
-        a greedy cover of the scenes that exceed budget * (1 - margin) somewhere by
-        blocks whose error flags them) and the spot-error threshold t: the largest t such
-        that every calibration scene whose spot error is <= t is within
-        budget * (1 - margin) on every channel (0 when none is)."""
+        error) and the fallback rule that keeps every answered scene within
+        cap = budget * (1 - margin) on every calibration scene.
+
+        correct=True: the emulator's errors are low-rank across channels, so the errors
+        measured at the spot channels predict them everywhere.  n_blocks spot blocks of
+        `block` channels are picked greedily (select_correction_blocks), a ridge map W from
+        the spot errors to every channel's error is fitted on the calibration scenes, and
+        the fallback statistic is either the largest spot error or the largest
+        leave-one-block-out residual of the map, whichever needs fewer fallbacks; its
+        threshold is the largest value under which every calibration scene is within cap
+        after correction, judged out of fold (`folds` folds).  correct=False: spot blocks
+        by select_spot_blocks and the threshold on the uncorrected errors."""
         scenes, eps = self._scenes(list(profs), geom, T_sfc, eps_sfc)
         _, bt = self.forward([s[0] for s in scenes], geom=[_geom(s[2]) for s in scenes], T_sfc=[s[1] for s in scenes],
                              eps_sfc=eps)
@@ -537,20 +558,52 @@ class ChannelODEmulator:
                                          geom=[_geom(s[2]) for s in scenes], T_sfc=[s[1] for s in scenes],
                                          eps_sfc=eps, ctx=ctx, line_mixing=line_mixing,
                                          apply_continuum=apply_continuum, **kw)
-        d = np.abs(bt - bt_lbl)
+        E = bt - bt_lbl
+        d = np.abs(E)
         ch_rms = np.sqrt(np.mean(d ** 2, axis=0))
         ch_max = d.max(axis=0)
         cap = budget * (1.0 - margin)
-        blocks = select_spot_blocks(d, ch_rms, n_blocks, block, cap)
-        taken = np.zeros(self.n_ch, bool)
-        for c0, n in blocks:
-            taken[c0:c0 + n] = True
-        spot = np.flatnonzero(taken)
-        se, fe = d[:, spot].max(axis=1), d.max(axis=1)
-        thr = spot_threshold(se, fe, cap)
-        self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, se, len(scenes))
+        fe = d.max(axis=1)
+        self._set_correction(None)
+        if not correct:
+            blocks = select_spot_blocks(d, ch_rms, n_blocks, block, cap)
+            spot = np.concatenate([np.arange(c0, c0 + n) for c0, n in blocks])
+            se = d[:, spot].max(axis=1)
+            thr = spot_threshold(se, fe, cap)
+            self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, se, len(scenes),
+                                           fallback_rate=float(np.mean(se > thr)))
+        else:
+            blocks = select_correction_blocks(E, n_blocks, block, lam)
+            spot = np.concatenate([np.arange(c0, c0 + n) for c0, n in blocks])
+            bn = [n for _, n in blocks]
+            res, st = _out_of_fold(E, spot, bn, lam, folds)
+            best = None
+            for k in (0, 1):
+                t = spot_threshold(st[k], res, cap)
+                rate = float(np.mean(st[k] > t))
+                if best is None or rate < best[2]:
+                    best = (k, t, rate)
+            k, thr, rate = best
+            W, lbo = fit_spot_correction(E, spot, bn, lam)
+            self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, st[k], len(scenes),
+                                           W=W, lbo_W=lbo, stat=k, scene_max_corrected=res, fallback_rate=rate)
+            self._set_correction(self.calibration)
         self._spot = None
         return self.calibration
+
+    def _set_correction(self, cal: Optional[Calibration]):
+        if cal is None or cal.W is None:
+            rc = self.lib.irs_emu_set_correction(self.h, 0, None, None, None, None, 0)
+        else:
+            c0 = np.ascontiguousarray([c for c, _ in cal.spot_blocks], np.int64)
+            bn = np.ascontiguousarray([n for _, n in cal.spot_blocks], np.int32)
+            W = np.ascontiguousarray(cal.W, np.float32)
+            lbo = np.ascontiguousarray(cal.lbo_W if cal.lbo_W is not None and len(cal.lbo_W) else np.zeros(1),
+                                       np.float32)
+            rc = self.lib.irs_emu_set_correction(self.h, len(bn), _p(c0), _p(bn), _p(W), _p(lbo), int(cal.stat))
+        if rc != IRS_OK:
+            raise IRSError(rc, self.lib.irs_emu_last_error(self.h).decode(errors="replace"))
+
     def spot_checker(self, linelists, sounder: Sounder) -> "SpotChecker":
         if self.calibration is None:
             raise ValueError("the emulator has no calibration (run calibrate)")
@@ -607,6 +660,117 @@ def select_spot_blocks(d, ch_rms, n_blocks: int, block: int, cap: float, stride:
         free &= np.abs(starts - starts[k]) >= block
     chosen.sort()
     return chosen
+
+
+def _ridge(X, Y, lam):
+    """argmin_W |X W - Y|^2 + lam tr(X^T X)/n |W|^2 (FP64)."""
+    G = X.T @ X
+    return np.linalg.solve(G + lam * max(np.trace(G), 1e-30) / len(G) * np.eye(len(G)), X.T @ Y)
+
+
+def fit_spot_correction(E, spot, block_n, lam: float = 1e-3):
+    """Ridge maps from the spot errors E[:, spot] to every channel's error (W
+    [n_spot, n_ch]) and, per spot block b, from the other blocks' errors to block
+    b's (leave-one-block-out maps M_b [n_spot - n_b, n_b], concatenated row-major)."""
+    E = np.asarray(E, np.float64)
+    Es = E[:, spot]
+    W = _ridge(Es, E, lam).astype(np.float32)
+    lbo, off = [], 0
+    for n in block_n:
+        inb = np.zeros(len(spot), bool)
+        inb[off:off + n] = True
+        lbo.append(_ridge(Es[:, ~inb], Es[:, inb], lam).astype(np.float32).ravel() if len(block_n) > 1
+                   else np.zeros(0, np.float32))
+        off += n
+    return W, np.concatenate(lbo)
+
+
+def spot_statistics(es, block_n, lbo):
+    """Fallback statistics of scenes from their spot errors es [n, n_spot]: (largest
+    |error|, largest leave-one-block-out residual) -- the library's irs_forward_emulated
+    computes the same."""
+    es = np.asarray(es, np.float64)
+    raw = np.abs(es).max(axis=1)
+    res = np.zeros(len(es))
+    if len(block_n) > 1:
+        off, o = 0, 0
+        ns = es.shape[1]
+        for n in block_n:
+            inb = np.zeros(ns, bool)
+            inb[off:off + n] = True
+            M = np.asarray(lbo[o:o + (ns - n) * n], np.float64).reshape(ns - n, n)
+            res = np.maximum(res, np.abs(es[:, inb] - es[:, ~inb] @ M).max(axis=1))
+            off += n
+            o += (ns - n) * n
+    return raw, res
+
+
+def _out_of_fold(E, spot, block_n, lam, folds):
+    """Per calibration scene, out of fold: the largest corrected error over all
+    channels (spot channels exact) and both fallback statistics."""
+    E = np.asarray(E, np.float64)
+    n = len(E)
+    k = max(2, min(folds, n // 4)) if n >= 8 else 1
+    parts = np.array_split(np.random.default_rng(0).permutation(n), k) if k > 1 else [np.arange(n)]
+    res = np.zeros(n)
+    st = [np.zeros(n), np.zeros(n)]
+    for f in parts:
+        tr = np.setdiff1d(np.arange(n), f) if k > 1 else f
+        W, lbo = fit_spot_correction(E[tr], spot, block_n, lam)
+        r = E[f] - E[f][:, spot] @ W.astype(np.float64)
+        r[:, spot] = 0.0
+        res[f] = np.abs(r).max(axis=1)
+        raw, lb = spot_statistics(E[f][:, spot], block_n, lbo)
+        st[0][f], st[1][f] = raw, lb
+    return res, st
+
+
+def select_correction_blocks(E, n_blocks: int, block: int, lam: float = 1e-3, gap: Optional[int] = None,
+                             seed: int = 0, stride: Optional[int] = None, max_scenes: int = 512):
+    """Spot blocks for the correction: greedily add the block of `block` channels
+    (starts every `stride` channels, by default `block`; at least `gap` channels from the
+    chosen ones, by default 64, less on a window too narrow for n_blocks such gaps) whose
+    errors, with those already chosen, best predict every channel's error -- the
+    smallest mean over scenes of the largest corrected error, fitted on half of (at most
+    max_scenes of) the calibration scenes and judged on the other half.  Returns
+    [(first, block)] sorted."""
+    E = np.asarray(E, np.float64)
+    n, nch = E.shape
+    if nch <= block:
+        return [(0, nch)]
+    if gap is None:
+        gap = int(min(64, max(0, nch // (2 * max(n_blocks, 1)) - block)))
+    idx = np.random.default_rng(seed).permutation(n)[:max_scenes]
+    n = len(idx)
+    A, B = (idx[: n // 2], idx[n // 2:]) if n >= 4 else (idx, idx)
+    starts = np.arange(0, nch - block + 1, max(stride or block, 1))
+    chosen = []
+    for _ in range(n_blocks):
+        best, bv = None, np.inf
+        for s0 in starts:
+            if any(abs(int(s0) - c) < block + gap for c in chosen):
+                continue
+            S = np.concatenate([np.arange(c, c + block) for c in chosen + [int(s0)]])
+            W = _ridge(E[A][:, S], E[A], lam)
+            r = E[B] - E[B][:, S] @ W
+            r[:, S] = 0.0
+            v = float(np.mean(np.abs(r).max(axis=1)))
+            if v < bv:
+                bv, best = v, int(s0)
+        if best is None:
+            break
+        chosen.append(best)
+    return sorted((c, block) for c in chosen)
+
+
+def correct_spectra(BT, bt_spot, spot, W, nu):
+    """Host restatement of the library's correction: BT -= (BT[:, spot] - bt_spot) W,
+    spot channels set to bt_spot, R the Planck radiance of the corrected BT."""
+    es = (BT[:, spot] - bt_spot).astype(np.float32)
+    BTc = BT - (es @ np.asarray(W, np.float32)).astype(np.float64)
+    BTc[:, spot] = bt_spot
+    R = api.planck_radiance(nu, BTc)
+    return R, BTc, es
 
 
 def _feature_level_jacobian(prof, T_sfc, mu, context: ProfileContext, T_method, hT=1e-2, hq=1e-3):
@@ -733,6 +897,17 @@ class EmulatedResult:
     threshold: float = 0.0
 
 
+def _uses_correction(emulator, spot) -> bool:
+    """The calibration's spot-check correction applies when the spot check runs on
+    the calibration's own blocks."""
+    cal = emulator.calibration
+    if cal is None or cal.W is None:
+        return False
+    if spot is None:
+        return True
+    return [(int(ch[0]), len(ch)) for ch, _, _ in spot.blocks] == [(int(a), int(b)) for a, b in cal.spot_blocks]
+
+
 def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: Sounder,
                      geom=None, T_sfc=None, eps_sfc=1.0, threshold: Optional[float] = None,
                      spot: Optional[SpotChecker] = None, ctx: Optional[Context] = None,
@@ -745,12 +920,16 @@ def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: 
     apply_continuum as in training."""
     scenes, eps = ChannelODEmulator._scenes(list(profs), geom, T_sfc, eps_sfc)
     ctx = ctx or api.default_context()
+    cal = emulator.calibration
+    use_corr = _uses_correction(emulator, spot)
     if spot is None:
         spot = emulator.spot_checker(linelists, sounder)
     if threshold is None:
-        if emulator.calibration is None:
+        if cal is None:
             raise ValueError("no threshold given and the emulator has no calibration")
-        threshold = emulator.calibration.threshold
+        if cal.W is not None and not use_corr:
+            raise ValueError("the calibrated threshold belongs to the calibration's spot blocks; pass a threshold")
+        threshold = cal.threshold
     opt = api._options(internal_dnu, None, cutoff, None, dptmn, 122.0, True, "gaussian", line_mixing, T_method,
                        "cim", api._continua_mask(apply_continuum, api.DEFAULT_CONTINUA))
     b = api._Batch(ctx, [sc[0] for sc in scenes], linelists, [sc[1] for sc in scenes], eps, [sc[2] for sc in scenes])
@@ -766,9 +945,15 @@ def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: 
     err = np.zeros(n)
     snd = api._sounder(sounder)
     t0 = time.perf_counter()
-    rc = emulator.lib.irs_forward_emulated(ctx.h, emulator.h, b.arr, n, C.byref(snd), C.byref(opt), len(c0),
-                                           _p(c0), _p(nb), _p(lists), float(threshold), _p(R), _p(BT), _p(fb),
-                                           _p(err))
+    if cal is not None and cal.W is not None and not use_corr:
+        emulator._set_correction(None)              # other spot blocks: the uncorrected emulator
+    try:
+        rc = emulator.lib.irs_forward_emulated(ctx.h, emulator.h, b.arr, n, C.byref(snd), C.byref(opt), len(c0),
+                                               _p(c0), _p(nb), _p(lists), float(threshold), _p(R), _p(BT), _p(fb),
+                                               _p(err))
+    finally:
+        if cal is not None and cal.W is not None and not use_corr:
+            emulator._set_correction(cal)
     if rc != IRS_OK:
         msg = emulator.lib.irs_emu_last_error(emulator.h).decode(errors="replace") or \
             emulator.lib.irs_last_error(ctx.h).decode(errors="replace")
@@ -798,6 +983,10 @@ def forward_emulated_steps(profs, linelists, emulator: ChannelODEmulator, *, sou
     t1 = time.perf_counter()
     bt_spot = spot(profs, geoms, tsf, eps, ctx, line_mixing=line_mixing, **kw)
     err = np.max(np.abs(BT[:, spot.channels] - bt_spot), axis=1)
+    cal = emulator.calibration
+    if _uses_correction(emulator, spot):
+        R, BT, es = correct_spectra(BT, bt_spot, spot.channels, cal.W, emulator.ch_nu)
+        err = spot_statistics(es, [len(b[0]) for b in spot.blocks], cal.lbo_W)[cal.stat]
     t2 = time.perf_counter()
     fb = err > threshold
     if np.any(fb):

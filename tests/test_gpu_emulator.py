@@ -247,25 +247,40 @@ def test_emulator_trains_and_beats_regression_and_mean(ctx, small_emu):
 
 
 def test_emulator_calibrated_fallback_bounds_every_channel(ctx, small_emu):
-    """Calibration on 32 held-out scenes (spot blocks that flag the scenes over
-    0.8 x 0.05 K, the spot threshold under which every calibration scene is within
-    0.8 x 0.05 K everywhere); on 32 other held-out scenes no scene that the spot
-    check lets through exceeds 0.05 K on any channel, and fallen-back scenes are
-    the LBL spectra."""
+    """Calibration on 32 held-out scenes without and with the spot-check correction.
+    Uncorrected: spot blocks that flag the scenes over 0.8 x 0.05 K and the spot
+    threshold under which every calibration scene is within that everywhere.
+    Corrected: spot blocks whose errors predict every channel's error, the ridge map
+    applied to every scene, and the threshold on the chosen statistic.  On 32 other
+    held-out scenes no scene that the spot check lets through exceeds 0.05 K on any
+    channel, fallen-back scenes are the LBL spectra, and the library's one-call path
+    (irs_forward_emulated) equals the host-side restatement of the same steps."""
     sub, snd, emu, rep, hold = small_emu
     cal, tst = hold[:32], hold[32:]
-    profs, geoms, tsf = _hold(cal)
-    c = emu.calibrate(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, budget=0.05, n_blocks=2, block=8, ctx=ctx)
-    assert len(c.spot_channels) == 16 and c.ch_rms.shape == (49,)
+    pc, gc, tc = _hold(cal)
     profs, geoms, tsf = _hold(tst)
-    r = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, ctx=ctx)
     _, _, bt_lbl = forward_model(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, line_mixing="vpy", ctx=ctx,
                                  apply_continuum=False)
-    d = np.abs(r.BT - bt_lbl).max(axis=1)
-    print(f"calibrated threshold {c.threshold:.4f} K on blocks {c.spot_blocks}: fallback {r.fell_back.mean():.2f}, "
-          f"max |dBT| after fallback {d.max():.4f} K (emulated scenes {np.sum(~r.fell_back)})")
-    assert np.all(d[~r.fell_back] <= 0.05)
-    assert np.array_equal(r.BT[r.fell_back], bt_lbl[r.fell_back])
+    _, bt_raw = emu.forward(profs, geom=geoms, T_sfc=tsf)
+    for correct in (False, True):
+        c = emu.calibrate(pc, sub, sounder=snd, geom=gc, T_sfc=tc, budget=0.05, n_blocks=2, block=8, ctx=ctx,
+                          correct=correct)
+        assert len(c.spot_channels) == 16 and c.ch_rms.shape == (49,)
+        assert (c.W is not None) == correct
+        r = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, ctx=ctx)
+        d = np.abs(r.BT - bt_lbl).max(axis=1)
+        print(f"correct={correct}: threshold {c.threshold:.4f} K (stat {c.stat}) on blocks {c.spot_blocks}: "
+              f"fallback {r.fell_back.mean():.2f}, max |dBT| after fallback {d.max():.4f} K "
+              f"(emulated scenes {np.sum(~r.fell_back)}; uncorrected max {np.abs(bt_raw - bt_lbl).max():.4f} K)")
+        assert np.all(d[~r.fell_back] <= 0.05)
+        assert np.array_equal(r.BT[r.fell_back], bt_lbl[r.fell_back])
+        r2 = forward_emulated_steps(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, ctx=ctx)
+        assert np.array_equal(r.fell_back, r2.fell_back)
+        np.testing.assert_allclose(r.spot_err, r2.spot_err, rtol=0, atol=1e-4)
+        np.testing.assert_allclose(r.BT, r2.BT, rtol=0, atol=1e-4)
+        if correct:
+            # the spot channels carry their LBL values; the others moved by the fitted map
+            np.testing.assert_allclose(r.BT[:, c.spot_channels], bt_lbl[:, c.spot_channels], rtol=0, atol=1e-4)
 
 
 def test_emulator_spot_check_and_fallback(ctx, small_emu):

@@ -33,3 +33,68 @@ def test_greedy_blocks_find_the_channels_that_flag_bad_scenes():
     se, fe = d[:, spot].max(axis=1), d.max(axis=1)
     t = spot_threshold(se, fe, 0.04)
     assert np.all(~bad[se <= t]) and np.sum(se <= t) == np.sum(~bad)
+
+
+# ---- spot-check correction ------------------------------------------------------
+from irsounderlbl.api import brightness_temperature  # noqa: E402
+from irsounderlbl.emulator import (correct_spectra, fit_spot_correction, select_correction_blocks,  # noqa: E402
+                                   spot_statistics)
+
+
+def _low_rank_errors(n, nch=900, seed=1):
+    """Errors that are three fixed spectral patterns, each confined to its own third
+    of the channels, with scene-dependent amplitudes (plus small noise)."""
+    rng = np.random.default_rng(seed)
+    V = np.zeros((3, nch))
+    for j in range(3):
+        V[j, 300 * j:300 * (j + 1)] = np.sin(np.linspace(0, 3 + j, 300)) + 1.5
+    A = rng.normal(0.0, 0.03, (n, 3))
+    return A @ V + rng.normal(0.0, 1e-4, (n, nch))
+
+
+def test_correction_blocks_cover_every_pattern_and_remove_the_error():
+    """Each pattern lives in one third of the spectrum, so a correction must see one
+    block in every third; with them the ridge map predicts every channel's error from
+    the spot errors of new scenes to the noise level."""
+    E = _low_rank_errors(120)
+    blocks = select_correction_blocks(E, 3, 16, lam=1e-6)
+    spot = np.concatenate([np.arange(c, c + n) for c, n in blocks])
+    assert set(spot // 300) == {0, 1, 2}
+    W, lbo = fit_spot_correction(E, spot, [16, 16, 16], lam=1e-6)
+    assert W.shape == (48, 900) and lbo.shape == (3 * 32 * 16,)
+    Et = _low_rank_errors(50, seed=2)
+    r = Et - Et[:, spot] @ W.astype(np.float64)
+    assert np.abs(Et).max() > 0.1
+    assert np.abs(r).max() < 3e-3
+
+
+def test_leave_one_block_out_statistic_flags_an_unexplained_block():
+    """Errors that the calibration's patterns explain give a small leave-one-block-out
+    residual; an error confined to one block that no pattern carries gives a large one,
+    while the largest spot error can stay small."""
+    E = np.concatenate([_low_rank_errors(200, seed=3)[:, :600]] * 1, axis=1)     # two patterns over 600 channels
+    # blocks inside both supports of pattern 0 so that blocks predict each other
+    blocks = [(20, 16), (150, 16), (350, 16), (480, 16)]
+    spot = np.concatenate([np.arange(c, c + n) for c, n in blocks])
+    W, lbo = fit_spot_correction(E, spot, [16] * 4, lam=1e-6)
+    es = _low_rank_errors(20, seed=4)[:, :600][:, spot]
+    raw, res = spot_statistics(es, [16] * 4, lbo)
+    assert np.all(res < 2e-3) and raw.max() > 0.03
+    bad = es.copy()
+    bad[0, 16:32] += 0.02                      # block 1 only
+    raw_b, res_b = spot_statistics(bad, [16] * 4, lbo)
+    assert res_b[0] > 0.015 and raw_b[0] < raw.max() + 0.03
+
+
+def test_correct_spectra_sets_spot_channels_and_radiances():
+    nu = np.linspace(700.0, 710.0, 41)
+    bt = np.full((2, 41), 250.0) + np.linspace(0, 1, 41)
+    spot = np.arange(10, 14)
+    bt_spot = bt[:, spot] - 0.05
+    W = np.zeros((4, 41), np.float32)
+    W[:, :] = 0.25                                   # every channel moves by the mean spot error
+    R, BTc, es = correct_spectra(bt, bt_spot, spot, W, nu)
+    np.testing.assert_allclose(es, 0.05, atol=1e-6)
+    np.testing.assert_allclose(np.delete(BTc, spot, axis=1), np.delete(bt, spot, axis=1) - 0.05, atol=1e-6)
+    assert np.array_equal(BTc[:, spot], bt_spot)
+    np.testing.assert_allclose(brightness_temperature(nu, R[0]), BTc[0], rtol=0, atol=1e-9)
