@@ -290,7 +290,7 @@ def _bands(nu, v, fn):
     return out
 
 
-def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value):
+def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value, sigma_lnq=0.0):
     """Train the emulator on GPU LBL targets of n_train scenes, calibrate it on
     n_cal held-out scenes (per-channel hold-out BT error vs GPU LBL, spot blocks,
     spot threshold for a 0.05 K budget), then run n_eval new scenes through
@@ -304,7 +304,7 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value):
     kw = dict(sigma_T=4.0, sigma_h2o=0.6, sigma_o3=0.2, co2_range=(390e-6, 430e-6))
     def geoms(sc):
         return [ViewingGeometry(float(np.degrees(np.arccos(min(p[2], 1.0)))), 0.0, p[2]) for p in sc]
-    train = syn.perturbed_profiles(n_train, seed=3003, **kw)
+    train = syn.perturbed_profiles(n_train, seed=3003, sigma_lnq=sigma_lnq, **kw)
     log(f"bench: emulator training ({n_train} scenes of GPU LBL targets, per-layer PCA, regression, MLP)")
     t0 = time.perf_counter()
     emu, rep = ChannelODEmulator.train(train, lls, snd, n_pc=64, n_basis=min(2048, n_train), steps=steps,
@@ -366,7 +366,8 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value):
     rel_q = float(np.linalg.norm(jac[0].K_lnq_H2O - ref.K[:, 50:100]) / np.linalg.norm(ref.K[:, 50:100]))
     sc_cal = c.scene_max
     out = {"workload": f"train {n_train} / calibrate {n_cal} / evaluate {n_eval} perturbed scenes (T GP 4 K, "
-                       f"H2O x lognormal 0.6, O3 x lognormal 0.2, CO2 390-430 ppm, scan 0-48.3 deg), full IASI "
+                       f"H2O x lognormal 0.6, O3 x lognormal 0.2, CO2 390-430 ppm, scan 0-48.3 deg; training "
+                       f"scenes also ln q_H2O GP {sigma_lnq} per level), full IASI "
                        f"8461 channels, windows variant of the line lists; targets (t, g, ILS moment) x 49 layers, "
                        f"64 PCs per layer, per-layer quadratic + context regression on 13 layer features "
                        f"(incl. Curtis-Godson means of the path above), MLP 13 -> 3x1024 GELU -> 64 "
@@ -623,6 +624,8 @@ def main():
     ap.add_argument("--emu-cal", type=int, default=1024)
     ap.add_argument("--emu-eval", type=int, default=2048)
     ap.add_argument("--emu-steps", type=int, default=96000)
+    ap.add_argument("--emu-sigma-lnq", type=float, default=0.0,
+                    help="level-by-level ln q_H2O perturbation (GP sigma) of the emulator's training scenes")
     ap.add_argument("--emulator-only", action="store_true",
                     help="the headline steps and the emulator figure only (the other figures off)")
     args = ap.parse_args()
@@ -693,7 +696,7 @@ def main():
         out["par_ingest"] = par_figure(ctx, lls)
     if not args.no_emulator:
         out["emulator"] = emulator_figure(ctx, lls_w, snd, args.emu_train, args.emu_cal, args.emu_eval,
-                                          args.emu_steps, round(value, 4))
+                                          args.emu_steps, round(value, 4), args.emu_sigma_lnq)
     if not args.no_accuracy:
         log("bench: accuracy samples and config 0 (GPU side)")
         accuracy_sample(ctx, prof, lls, lls_w)

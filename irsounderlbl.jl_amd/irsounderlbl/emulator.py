@@ -139,25 +139,32 @@ def layer_features(prof, mu: float = 1.0, T_method: str = "logp_at_pcg") -> np.n
     the H2O-weighted mean ln p and T, the O3-weighted mean T and the mass-weighted
     mean T of the layers above (u-weighted means sum u_j x_j / sum u_j, which fall
     back to the layer's own x where nothing is above)."""
-    lp = layer_properties(prof, T_method)
-    L = prof.n_levels - 1
-    def v(s):
-        return lp["vmr_cg"].get(GasSpecies(s), np.zeros(L))
-    dp = lp["dp"]
+    return _features_from_layers(*_layer_quantities(layer_properties(prof, T_method), prof.n_levels - 1), mu)
+
+
+def _layer_quantities(lp, L):
+    """(p_mid, dp, T_ave, vmr_cg H2O, O3, CO2) of layer_properties, zero for an absent species."""
+    def v(sp):
+        return lp["vmr_cg"].get(GasSpecies(sp), np.zeros(L))
+    return lp["p_mid"], lp["dp"], lp["T_ave"], v(1), v(3), v(2)
+
+
+def _features_from_layers(p_mid, dp, Ta, vh, vo, vc, mu):
+    """layer_features from the layer quantities, vectorised over any leading axes
+    (the last axis is the layer)."""
+    p_mid, dp, Ta, vh, vo, vc = np.broadcast_arrays(*(np.asarray(x, np.float64) for x in (p_mid, dp, Ta, vh, vo, vc)))
     def above(q):
-        a = np.cumsum(q[::-1])[::-1]                         # sum_{j>=k}
-        return np.concatenate([a[1:], [0.0]])                # sum_{j>k}
+        a = np.cumsum(q[..., ::-1], axis=-1)[..., ::-1]            # sum_{j>=k}
+        return np.concatenate([a[..., 1:], np.zeros(a.shape[:-1] + (1,))], axis=-1)   # sum_{j>k}
     def cg_mean(u, x, eps):
         return (above(u * x) + eps * x) / (above(u) + eps)
-    Ta = lp["T_ave"]
-    lnp = np.log(lp["p_mid"])
-    T_up = np.concatenate([Ta[1:], Ta[-1:]])
-    uh, uo = v(1) * dp, v(3) * dp
-    return np.stack([lnp, Ta, np.log(np.maximum(v(1), 1e-30)),
-                     np.log(np.maximum(v(3), 1e-30)), v(2) * 1e6, np.full(L, 1.0 / mu),
-                     np.log(above(uh) + 1e-9), np.log(above(uo) + 1e-12), T_up,
+    lnp = np.log(p_mid)
+    T_up = np.concatenate([Ta[..., 1:], Ta[..., -1:]], axis=-1)
+    uh, uo = vh * dp, vo * dp
+    return np.stack([lnp, Ta, np.log(np.maximum(vh, 1e-30)), np.log(np.maximum(vo, 1e-30)), vc * 1e6,
+                     np.full(Ta.shape, 1.0 / mu), np.log(above(uh) + 1e-9), np.log(above(uo) + 1e-12), T_up,
                      cg_mean(uh, lnp, 1e-9), cg_mean(uh, Ta, 1e-9), cg_mean(uo, Ta, 1e-12),
-                     cg_mean(dp, Ta, 1e-6)], axis=1)
+                     cg_mean(dp, Ta, 1e-6)], axis=-1)
 
 
 def _randomized_pca(dev: np.ndarray, k: int, oversample: int = 32, power: int = 2, seed: int = 0):
@@ -776,32 +783,36 @@ def correct_spectra(BT, bt_spot, spot, W, nu):
 def _feature_level_jacobian(prof, T_sfc, mu, context: ProfileContext, T_method, hT=1e-2, hq=1e-3):
     """d features / d (level T), d features / d (level ln q_H2O), d T_ave / d (level T),
     d features / d T_sfc: central differences of the host layer code for the layer
-    features, analytic for the (linear) profile context."""
+    features, analytic for the (linear) profile context.  A level bounds two layers
+    and the layer quantities of layer_properties depend on a layer's own two levels
+    only, so each perturbed level recomputes those layers from its three-level
+    neighbourhood; the features of every perturbed profile (the columns and
+    Curtis-Godson means above each layer) are then evaluated together."""
     nl = prof.n_levels
-    def feats(T=None, vmr=None):
-        p = AtmosphericProfile(prof.pressure, prof.temperature if T is None else T, prof.altitude,
-                               prof.vmr if vmr is None else vmr)
-        return layer_features(p, mu, T_method), layer_properties(p, T_method)["T_ave"]
-    X0, _ = feats()
-    L, nf = X0.shape
-    dXdT = np.zeros((nl, L, N_FEATURES))
-    dAdT = np.zeros((nl, L))
-    dXdq = np.zeros((nl, L, N_FEATURES))
+    L = nl - 1
+    q0 = np.stack(_layer_quantities(layer_properties(prof, T_method), L))          # [6, L]
+    has_q = GasSpecies.H2O in prof.vmr
+    steps = [("T", hT), ("T", -hT)] + ([("q", hq), ("q", -hq)] if has_q else [])
+    Q = np.repeat(q0[None, None], nl, axis=0).repeat(len(steps), axis=1)           # [nl, n_steps, 6, L]
     for j in range(nl):
-        Tp, Tm = prof.temperature.copy(), prof.temperature.copy()
-        Tp[j] += hT
-        Tm[j] -= hT
-        (xp, ap), (xm, am) = feats(T=Tp), feats(T=Tm)
-        dXdT[j, :, :nf] = (xp - xm) / (2 * hT)
-        dAdT[j] = (ap - am) / (2 * hT)
-        if GasSpecies.H2O in prof.vmr:
-            vp, vm = dict(prof.vmr), dict(prof.vmr)
-            q = prof.vmr[GasSpecies.H2O]
-            qp, qm = q.copy(), q.copy()
-            qp[j] *= np.exp(hq)
-            qm[j] *= np.exp(-hq)
-            vp[GasSpecies.H2O], vm[GasSpecies.H2O] = qp, qm
-            dXdq[j, :, :nf] = (feats(vmr=vp)[0] - feats(vmr=vm)[0]) / (2 * hq)
+        lo, hi = max(j - 1, 0), min(j + 1, nl - 1)
+        for m, (kind, h) in enumerate(steps):
+            T = prof.temperature[lo:hi + 1].copy()
+            vmr = {sp: v[lo:hi + 1].copy() for sp, v in prof.vmr.items()}
+            if kind == "T":
+                T[j - lo] += h
+            else:
+                vmr[GasSpecies.H2O][j - lo] *= np.exp(h)
+            sub = AtmosphericProfile(prof.pressure[lo:hi + 1], T, prof.altitude[lo:hi + 1], vmr)
+            Q[j, m, :, lo:hi] = np.stack(_layer_quantities(layer_properties(sub, T_method), hi - lo))
+    X = _features_from_layers(*(Q[:, :, i] for i in range(6)), mu)                 # [nl, n_steps, L, 13]
+    nf = X.shape[-1]
+    dXdT = np.zeros((nl, L, N_FEATURES))
+    dXdq = np.zeros((nl, L, N_FEATURES))
+    dXdT[:, :, :nf] = (X[:, 0] - X[:, 1]) / (2 * hT)
+    dAdT = (Q[:, 0, 2] - Q[:, 1, 2]) / (2 * hT)
+    if has_q:
+        dXdq[:, :, :nf] = (X[:, 2] - X[:, 3]) / (2 * hq)
     # the context is linear in [T levels, ln q_H2O levels, ln q_O3 levels, T_sfc, 1/mu]
     P = context.proj
     dXdT[:, :, nf:] = P[:, :nl].T[:, None, :]
@@ -841,11 +852,12 @@ def _emu_jacobian(emu, scenes, eps):
     out = []
     for q, (prof, ts, mu) in enumerate(scenes):
         dXdT, dAdT, dXdq, dXdTs = _feature_level_jacobian(prof, ts, mu, emu.context, emu.T_method)
-        F = dfeat[q].astype(np.float64)
-        KT = np.einsum("kic,jki->cj", F, dXdT) + np.einsum("kc,jk->cj", dTave[q].astype(np.float64), dAdT)
+        F = dfeat[q].astype(np.float64).reshape(L * N_FEATURES, nch)                 # [(k, i), c]
+        nl = dXdT.shape[0]
+        KT = (dXdT.reshape(nl, -1) @ F).T + (dAdT @ dTave[q].astype(np.float64)).T
         KT += dTlev[q].T.astype(np.float64)
-        Kq = np.einsum("kic,jki->cj", F, dXdq)
-        Ks = np.einsum("kic,ki->c", F, dXdTs)
+        Kq = (dXdq.reshape(nl, -1) @ F).T
+        Ks = dXdTs.reshape(-1) @ F
         if ts is None:                      # T_sfc is the bottom level temperature: its Planck term goes there
             KT[:, 0] += dTsfc[q].astype(np.float64)
         else:

@@ -119,11 +119,14 @@ def _matern52(z, L):
 
 def perturbed_profiles(n_prof: int, seed: int = 2002, sigma_T: float = 3.0, sigma_h2o: float = 0.3,
                        sigma_o3: float = 0.0, co2_range=None, dT_sfc: float = 10.0,
-                       corr_km: float = 2.0, with_o3: bool = True, max_scan_deg: float = 48.3):
+                       corr_km: float = 2.0, with_o3: bool = True, max_scan_deg: float = 48.3,
+                       sigma_lnq: float = 0.0):
     """Scan-line profiles: base profile with T + GP(0, sigma_T^2, Matern-5/2, L = corr_km),
     H2O x lognormal(sigma_h2o), O3 x lognormal(sigma_o3), CO2 ~ U(co2_range),
-    T_sfc = T_1 + U(-dT_sfc, dT_sfc), scan ~ U(0, max_scan_deg) -> mu.
-    Returns a list of (AtmosphericProfile, T_sfc, mu)."""
+    T_sfc = T_1 + U(-dT_sfc, dT_sfc), scan ~ U(0, max_scan_deg) -> mu.  sigma_lnq > 0
+    also multiplies H2O level by level by exp(GP(0, sigma_lnq^2, Matern-5/2, L = corr_km))
+    (drawn after the other perturbations of the profile, so sigma_lnq = 0 keeps every
+    seed's profiles).  Returns a list of (AtmosphericProfile, T_sfc, mu)."""
     rng = np.random.default_rng(seed)
     base = base_profile(with_o3=with_o3)
     z = base.altitude
@@ -140,6 +143,8 @@ def perturbed_profiles(n_prof: int, seed: int = 2002, sigma_T: float = 3.0, sigm
             vmr[GasSpecies.CO2] = np.full(len(z), rng.uniform(*co2_range))
         T_sfc = T[0] + rng.uniform(-dT_sfc, dT_sfc)
         scan = rng.uniform(0.0, max_scan_deg)
+        if sigma_lnq > 0.0:
+            vmr[GasSpecies.H2O] = vmr[GasSpecies.H2O] * np.exp((sigma_lnq / sigma_T) * (Lc @ rng.standard_normal(len(z))))
         mu = math.cos(math.radians(scan_angle_to_local_zenith(scan)))
         out.append((AtmosphericProfile(base.pressure.copy(), T, z.copy(), vmr), float(T_sfc), float(mu)))
     return out
