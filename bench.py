@@ -316,8 +316,9 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value, sig
     cal = syn.perturbed_profiles(n_cal, seed=5005, **kw)
     log(f"bench: emulator calibration ({n_cal} held-out scenes vs GPU LBL, spot blocks, correction, threshold)")
     t0 = time.perf_counter()
+    margin = 0.3
     c = emu.calibrate([p[0] for p in cal], lls, sounder=snd, geom=geoms(cal), T_sfc=[p[1] for p in cal],
-                      budget=0.05, margin=0.2, n_blocks=4, block=64, ctx=ctx, correct=True)
+                      budget=0.05, margin=margin, n_blocks=4, block=64, ctx=ctx, correct=True)
     t_cal = time.perf_counter() - t0
     ev = syn.perturbed_profiles(n_eval, seed=4004, **kw)
     profs, gs, tsf = [p[0] for p in ev], geoms(ev), [p[1] for p in ev]
@@ -392,8 +393,10 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value, sig
            "calibration": {"seconds": round(t_cal, 1),
                            "spot_blocks_cm-1": [[round(float(nu[c0]), 2), round(float(nu[c0 + n - 1]), 2)]
                                                 for c0, n in c.spot_blocks],
-                           "statistic": ["largest spot error", "leave-one-block-out residual"][c.stat],
-                           "threshold_K": round(c.threshold, 4), "budget_K": c.budget, "margin": 0.2,
+                           "kept_when": "largest spot error <= t_raw and leave-one-block-out residual <= t_lbo",
+                           "t_raw_K": None if c.w_raw == 0 else round(1.0 / c.w_raw, 4),
+                           "t_lbo_K": None if c.w_lbo == 0 else round(1.0 / c.w_lbo, 4),
+                           "budget_K": c.budget, "margin": margin,
                            "calibration_fallback_rate": round(c.fallback_rate, 4)},
            "emulator_only_spectra_per_s": round(n_eval / t_emu, 1),
            "emulator_plus_fallback_spectra_per_s": round(n_eval / t_full, 3),

@@ -522,12 +522,12 @@ int         irs_forward_emulated(irs_ctx *ctx, irs_emu *emu, const irs_profile *
  * errors at the spot channels, e = BT_emulator[S] - BT_LBL[S] over the n_blocks blocks
  * [block_c0[b], block_c0[b] + block_n[b]) (irs_forward_emulated must get the same blocks
  * in the same order; at most 1024 channels), move every channel by the linear map fitted on calibration scenes,
- * BT[c] -= sum_s e[s] W[s][c] (W [n_spot][n_channels]).  stat selects the fallback
- * statistic: 0, max |e|; 1, the largest leave-one-block-out residual
+ * BT[c] -= sum_s e[s] W[s][c] (W [n_spot][n_channels]).  The fallback statistic is
+ * max(w_raw max|e|, w_lbo r), r the largest leave-one-block-out residual
  * max_b max |e_b - e_{S without b} M_b| with lbo_W the maps M_b [n_spot - n_b][n_b],
- * concatenated over b.  n_blocks = 0 removes the correction. */
+ * concatenated over b (may be NULL when w_lbo = 0).  n_blocks = 0 removes the correction. */
 int         irs_emu_set_correction(irs_emu *emu, int n_blocks, const int64_t *block_c0, const int32_t *block_n,
-                                   const float *W, const float *lbo_W, int stat);
+                                   const float *W, const float *lbo_W, double w_raw, double w_lbo);
 
 /* Emulator Jacobians (chain rule through the MLP input Jacobian, the regression,
  * the decode and the adjoint of the channel RTE; the reflected term of eps < 1 held fixed):

@@ -79,7 +79,8 @@ struct irs_emu {
     // W [n_spot][n_ch] on the device, the leave-one-block-out maps on the host
     std::vector<int> corr_bn;
     std::vector<int64_t> corr_c0;
-    int corr_ns = 0, corr_stat = 0;
+    int corr_ns = 0;
+    double corr_w_raw = 1.0, corr_w_lbo = 0.0;   // fallback statistic max(w_raw max|e|, w_lbo residual)
     float *corr_W = nullptr, *corr_e = nullptr;
     int64_t cap_corr_e = 0;
     std::vector<float> corr_lbo;
@@ -881,7 +882,7 @@ int emu_scene_features(const irs_emu *e, const irs_profile &pr, int T_method, fl
 }  // namespace
 
 extern "C" int irs_emu_set_correction(irs_emu *e, int n_blocks, const int64_t *block_c0, const int32_t *block_n,
-                                      const float *W, const float *lbo_W, int stat) {
+                                      const float *W, const float *lbo_W, double w_raw, double w_lbo) {
     if (!e || n_blocks < 0) return IRS_EINVAL;
     irs_mlp *m = e->mlp;
     (void)hipSetDevice(m->device);
@@ -892,8 +893,9 @@ extern "C" int irs_emu_set_correction(irs_emu *e, int n_blocks, const int64_t *b
     e->corr_lbo.clear();
     e->corr_ns = 0;
     if (n_blocks == 0) return IRS_OK;
-    if (!block_c0 || !block_n || !W || (stat != 0 && stat != 1) || (stat == 1 && (!lbo_W || n_blocks < 2))) {
-        e->err = "irs_emu_set_correction: block sizes and W are required; stat is 0 or 1 (1 needs lbo_W, 2+ blocks)";
+    if (!block_c0 || !block_n || !W || !(w_raw >= 0.0) || !(w_lbo >= 0.0) || (w_lbo > 0.0 && (!lbo_W || n_blocks < 2))) {
+        e->err = "irs_emu_set_correction: blocks and W are required, the weights are >= 0, and a leave-one-block-out "
+                 "weight needs lbo_W and two or more blocks";
         return IRS_EINVAL;
     }
     int64_t ns = 0, n_lbo = 0;
@@ -917,44 +919,44 @@ extern "C" int irs_emu_set_correction(irs_emu *e, int n_blocks, const int64_t *b
     }
     e->corr_bn.assign(block_n, block_n + n_blocks);
     e->corr_c0.assign(block_c0, block_c0 + n_blocks);
-    if (stat == 1) e->corr_lbo.assign(lbo_W, lbo_W + n_lbo);
+    if (w_lbo > 0.0) e->corr_lbo.assign(lbo_W, lbo_W + n_lbo);
     e->corr_ns = (int)ns;
-    e->corr_stat = stat;
+    e->corr_w_raw = w_raw;
+    e->corr_w_lbo = w_lbo;
     return IRS_OK;
 }
 
 namespace {
 
-// Decision statistic of the corrected emulator for one scene from its spot errors
-// es[n_spot] (blocks in order): stat 0, the largest |error|; stat 1, the largest
+// Fallback statistic of the corrected emulator for one scene from its spot errors
+// es[n_spot] (blocks in order): max(w_raw max|e|, w_lbo r) with r the largest
 // leave-one-block-out residual |e_b - e_{S\b} M_b| (M_b [n_spot - n_b][n_b] row-major,
 // the map fitted on calibration scenes from the other blocks' errors to block b's).
 double emu_spot_stat(const irs_emu *e, const float *es) {
     const int nb = (int)e->corr_bn.size(), ns = e->corr_ns;
-    double st = 0.0;
-    if (e->corr_stat == 0) {
-        for (int i = 0; i < ns; ++i) st = std::max(st, (double)std::fabs(es[i]));
-        return st;
-    }
-    std::vector<double> pred;
-    const float *M = e->corr_lbo.data();
-    int off = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int n = e->corr_bn[b], no = ns - n;
-        pred.assign((size_t)n, 0.0);
-        int j = 0;
-        for (int i = 0; i < ns; ++i) {
-            if (i >= off && i < off + n) continue;
-            const double x = es[i];
-            const float *row = M + (size_t)j * n;
-            for (int c = 0; c < n; ++c) pred[c] += x * (double)row[c];
-            ++j;
+    double raw = 0.0, res = 0.0;
+    for (int i = 0; i < ns; ++i) raw = std::max(raw, (double)std::fabs(es[i]));
+    if (e->corr_w_lbo > 0.0) {
+        std::vector<double> pred;
+        const float *M = e->corr_lbo.data();
+        int off = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int n = e->corr_bn[b], no = ns - n;
+            pred.assign((size_t)n, 0.0);
+            int j = 0;
+            for (int i = 0; i < ns; ++i) {
+                if (i >= off && i < off + n) continue;
+                const double x = es[i];
+                const float *row = M + (size_t)j * n;
+                for (int c = 0; c < n; ++c) pred[c] += x * (double)row[c];
+                ++j;
+            }
+            for (int c = 0; c < n; ++c) res = std::max(res, std::fabs((double)es[off + c] - pred[c]));
+            M += (size_t)no * n;
+            off += n;
         }
-        for (int c = 0; c < n; ++c) st = std::max(st, std::fabs((double)es[off + c] - pred[c]));
-        M += (size_t)no * n;
-        off += n;
     }
-    return st;
+    return std::max(e->corr_w_raw * raw, e->corr_w_lbo * res);
 }
 
 }  // namespace
@@ -982,6 +984,14 @@ extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile 
         e->err = "sounder and emulator have different channels";
         return IRS_EINVAL;
     }
+    if (e->corr_ns > 0) {
+        bool ok = n_blocks == (int)e->corr_bn.size();
+        for (int b = 0; ok && b < n_blocks; ++b) ok = block_n[b] == e->corr_bn[b] && block_c0[b] == e->corr_c0[b];
+        if (!ok) {
+            e->err = "the spot blocks differ from those of the emulator's correction (irs_emu_set_correction)";
+            return IRS_EINVAL;
+        }
+    }
     // 1. emulator
     std::vector<float> X((size_t)n_prof * L * e->n_feat);
     std::vector<double> tlev((size_t)n_prof * (L + 1)), tave((size_t)n_prof * L), scal((size_t)n_prof * 3);
@@ -1006,14 +1016,6 @@ extern "C" int irs_forward_emulated(irs_ctx *ctx, irs_emu *e, const irs_profile 
     std::vector<irs_profile> sub(profiles, profiles + n_prof);
     const bool corr = e->corr_ns > 0;
     const int ns_spot = e->corr_ns;
-    if (corr) {
-        bool ok = n_blocks == (int)e->corr_bn.size();
-        for (int b = 0; ok && b < n_blocks; ++b) ok = block_n[b] == e->corr_bn[b] && block_c0[b] == e->corr_c0[b];
-        if (!ok) {
-            e->err = "the spot blocks differ from those of the emulator's correction (irs_emu_set_correction)";
-            return IRS_EINVAL;
-        }
-    }
     std::vector<float> es(corr ? (size_t)n_prof * ns_spot : 0);
     std::vector<double> spot_R(corr ? (size_t)n_prof * ns_spot : 0), spot_BT(spot_R.size());
     std::vector<int64_t> spot_ch;

@@ -226,7 +226,7 @@ def load_library(path: str = LIB_PATH):
     L.irs_lm_perturbation.argtypes = [P, C.c_int32, P, f64, f64, i64, f64, f64, f64, f64, f64, P]
     L.irs_relmat_terms.argtypes = [C.POINTER(IrsRelmat), P, f64, f64, f64, i64, P, C.POINTER(i64)]
     L.irs_emu_set_context.argtypes = [P, i32, i32, P, P]
-    L.irs_emu_set_correction.argtypes = [P, i32, P, P, P, P, i32]
+    L.irs_emu_set_correction.argtypes = [P, i32, P, P, P, P, f64, f64]
     L.irs_forward_emulated.argtypes = [P, P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
                                        C.POINTER(IrsOptions), i32, P, P, P, f64, P, P, P, P]
     for name in EXPORTS:

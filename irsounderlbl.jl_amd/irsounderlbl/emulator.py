@@ -291,7 +291,8 @@ class Calibration:
     # spot-check correction (None: the emulator's spectra are used as they come)
     W: Optional[np.ndarray] = field(repr=False, default=None)          # [n_spot, n_ch]
     lbo_W: Optional[np.ndarray] = field(repr=False, default=None)      # leave-one-block-out maps, concatenated
-    stat: int = 0                                                      # 0: max |e_S|, 1: leave-one-block-out residual
+    w_raw: float = 1.0                 # fallback statistic max(w_raw max|e_S|, w_lbo leave-one-block-out residual)
+    w_lbo: float = 0.0
     scene_max_corrected: Optional[np.ndarray] = field(repr=False, default=None)   # out-of-fold, per scene
     fallback_rate: float = float("nan")                                # calibration scenes over the threshold
 
@@ -461,7 +462,8 @@ class ChannelODEmulator:
         extra = {}
         if cal is not None:
             extra = dict(cal_spot=cal.spot_channels, cal_blocks=np.array(cal.spot_blocks, np.int64),
-                         cal_thr=np.array([cal.threshold, cal.budget, cal.n_scenes, cal.stat, cal.fallback_rate]),
+                         cal_thr=np.array([cal.threshold, cal.budget, cal.n_scenes, cal.w_raw, cal.w_lbo,
+                                           cal.fallback_rate]),
                          cal_ch_rms=cal.ch_rms, cal_ch_max=cal.ch_max, cal_scene_max=cal.scene_max,
                          cal_scene_spot=cal.scene_spot)
             if cal.W is not None:
@@ -482,13 +484,14 @@ class ChannelODEmulator:
                                 z["mean"], z["inv_w"], z["ch_nu"],
                                 ProfileContext(z["ctx_mean"], z["ctx_std"], z["ctx_proj"]))
         if "cal_spot" in z:
-            thr, budget, n, stat, rate = z["cal_thr"]
+            thr, budget, n, w_raw, w_lbo, rate = z["cal_thr"]
             corr = "cal_W" in z
             emu.calibration = Calibration(z["cal_spot"], [tuple(int(v) for v in b) for b in z["cal_blocks"]],
                                           float(thr), float(budget), z["cal_ch_rms"], z["cal_ch_max"],
                                           z["cal_scene_max"], z["cal_scene_spot"], int(n),
                                           W=z["cal_W"] if corr else None, lbo_W=z["cal_lbo"] if corr else None,
-                                          stat=int(stat), scene_max_corrected=z["cal_scene_max_corr"] if corr else None,
+                                          w_raw=float(w_raw), w_lbo=float(w_lbo),
+                                          scene_max_corrected=z["cal_scene_max_corr"] if corr else None,
                                           fallback_rate=float(rate))
             emu._set_correction(emu.calibration)
         return emu
@@ -551,12 +554,13 @@ class ChannelODEmulator:
         correct=True: the emulator's errors are low-rank across channels, so the errors
         measured at the spot channels predict them everywhere.  n_blocks spot blocks of
         `block` channels are picked greedily (select_correction_blocks), a ridge map W from
-        the spot errors to every channel's error is fitted on the calibration scenes, and
-        the fallback statistic is either the largest spot error or the largest
-        leave-one-block-out residual of the map, whichever needs fewer fallbacks; its
-        threshold is the largest value under which every calibration scene is within cap
-        after correction, judged out of fold (`folds` folds).  correct=False: spot blocks
-        by select_spot_blocks and the threshold on the uncorrected errors."""
+        the spot errors to every channel's error is fitted on the calibration scenes, and a
+        scene is kept when both its largest spot error and the largest leave-one-block-out
+        residual of the map are within their thresholds, chosen together
+        (fallback_thresholds) to keep the most calibration scenes with none above cap after
+        correction, judged out of fold (`folds` folds); the library tests
+        max(raw / t_raw, residual / t_lbo) against 1.  correct=False: spot blocks by
+        select_spot_blocks and the threshold on the uncorrected errors."""
         scenes, eps = self._scenes(list(profs), geom, T_sfc, eps_sfc)
         _, bt = self.forward([s[0] for s in scenes], geom=[_geom(s[2]) for s in scenes], T_sfc=[s[1] for s in scenes],
                              eps_sfc=eps)
@@ -584,16 +588,18 @@ class ChannelODEmulator:
             spot = np.concatenate([np.arange(c0, c0 + n) for c0, n in blocks])
             bn = [n for _, n in blocks]
             res, st = _out_of_fold(E, spot, bn, lam, folds)
-            best = None
-            for k in (0, 1):
-                t = spot_threshold(st[k], res, cap)
-                rate = float(np.mean(st[k] > t))
-                if best is None or rate < best[2]:
-                    best = (k, t, rate)
-            k, thr, rate = best
+            t_raw, t_lbo = fallback_thresholds(st[0], st[1], res > cap)
+            if t_raw < 0.0:                     # no calibration scene can be kept
+                thr, w_raw, w_lbo = -1.0, 1.0, 0.0
+            else:
+                thr = 1.0
+                w_raw = 0.0 if np.isinf(t_raw) else 1.0 / max(t_raw, 1e-300)
+                w_lbo = 0.0 if np.isinf(t_lbo) else 1.0 / max(t_lbo, 1e-300)
+            stat = np.maximum(w_raw * st[0], w_lbo * st[1])
             W, lbo = fit_spot_correction(E, spot, bn, lam)
-            self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, st[k], len(scenes),
-                                           W=W, lbo_W=lbo, stat=k, scene_max_corrected=res, fallback_rate=rate)
+            self.calibration = Calibration(spot, blocks, thr, budget, ch_rms, ch_max, fe, stat, len(scenes),
+                                           W=W, lbo_W=lbo, w_raw=w_raw, w_lbo=w_lbo, scene_max_corrected=res,
+                                           fallback_rate=float(np.mean(stat > thr)))
             self._set_correction(self.calibration)
         self._spot = None
         return self.calibration
@@ -607,7 +613,8 @@ class ChannelODEmulator:
             W = np.ascontiguousarray(cal.W, np.float32)
             lbo = np.ascontiguousarray(cal.lbo_W if cal.lbo_W is not None and len(cal.lbo_W) else np.zeros(1),
                                        np.float32)
-            rc = self.lib.irs_emu_set_correction(self.h, len(bn), _p(c0), _p(bn), _p(W), _p(lbo), int(cal.stat))
+            rc = self.lib.irs_emu_set_correction(self.h, len(bn), _p(c0), _p(bn), _p(W), _p(lbo), float(cal.w_raw),
+                                                 float(cal.w_lbo))
         if rc != IRS_OK:
             raise IRSError(rc, self.lib.irs_emu_last_error(self.h).decode(errors="replace"))
 
@@ -632,6 +639,29 @@ def spot_threshold(se, fe, cap) -> float:
         else:
             break
     return thr
+
+
+def fallback_thresholds(s_raw, s_lbo, bad):
+    """Thresholds (t_raw, t_lbo) of the two fallback statistics, chosen together: a
+    scene is kept when s_raw <= t_raw and s_lbo <= t_lbo.  Among the rules that keep
+    no bad scene, the one that keeps the most scenes, with each threshold at the
+    largest value of a kept scene (inf: that statistic unused).  (-1, -1) when no
+    scene can be kept."""
+    s_raw, s_lbo, bad = np.asarray(s_raw, float), np.asarray(s_lbo, float), np.asarray(bad, bool)
+    best, best_n, best_t = None, 0, None
+    for t1 in np.concatenate([np.unique(s_raw), [np.inf]]):
+        cand = s_raw <= t1
+        m = bad & cand
+        lim = s_lbo[m].min() if m.any() else np.inf
+        keep = cand & (s_lbo < lim)
+        n = int(keep.sum())
+        if n > best_n:
+            best, best_n, best_t = keep, n, (t1, lim)
+    if best is None:
+        return -1.0, -1.0
+    t_raw = np.inf if np.isinf(best_t[0]) else float(s_raw[best].max())
+    t_lbo = np.inf if np.isinf(best_t[1]) else float(s_lbo[best].max())
+    return t_raw, t_lbo
 
 
 def select_spot_blocks(d, ch_rms, n_blocks: int, block: int, cap: float, stride: int = 4, q: float = 0.9):
@@ -924,16 +954,19 @@ def forward_emulated(profs, linelists, emulator: ChannelODEmulator, *, sounder: 
                      geom=None, T_sfc=None, eps_sfc=1.0, threshold: Optional[float] = None,
                      spot: Optional[SpotChecker] = None, ctx: Optional[Context] = None,
                      line_mixing="vpy", apply_continuum: bool = False, internal_dnu: float = 0.001,
-                     cutoff: float = 25.0, dptmn: float = 1e-6, T_method: str = "logp_at_pcg") -> EmulatedResult:
-    """Emulator + per-scene LBL spot check + full-LBL fallback in one library call
-    (irs_forward_emulated).  The spot channels and the spot-error threshold come from
-    the emulator's calibration (calibrate) unless `spot` / `threshold` are given; a
-    scene whose largest spot error exceeds the threshold is recomputed with full LBL.
-    apply_continuum as in training."""
+                     cutoff: float = 25.0, dptmn: float = 1e-6, T_method: str = "logp_at_pcg",
+                     correction: Optional[bool] = None) -> EmulatedResult:
+    """Emulator + per-scene LBL spot check (+ the calibration's correction of every
+    channel) + full-LBL fallback in one library call (irs_forward_emulated).  The spot
+    channels and the threshold come from the emulator's calibration (calibrate) unless
+    `spot` / `threshold` are given; a scene whose fallback statistic exceeds the
+    threshold is recomputed with full LBL.  correction=None applies the calibration's
+    correction when the spot check runs on the calibration's blocks, False never, True
+    always (the library then refuses other blocks).  apply_continuum as in training."""
     scenes, eps = ChannelODEmulator._scenes(list(profs), geom, T_sfc, eps_sfc)
     ctx = ctx or api.default_context()
     cal = emulator.calibration
-    use_corr = _uses_correction(emulator, spot)
+    use_corr = _uses_correction(emulator, spot) if correction is None else bool(correction)
     if spot is None:
         spot = emulator.spot_checker(linelists, sounder)
     if threshold is None:
@@ -998,7 +1031,8 @@ def forward_emulated_steps(profs, linelists, emulator: ChannelODEmulator, *, sou
     cal = emulator.calibration
     if _uses_correction(emulator, spot):
         R, BT, es = correct_spectra(BT, bt_spot, spot.channels, cal.W, emulator.ch_nu)
-        err = spot_statistics(es, [len(b[0]) for b in spot.blocks], cal.lbo_W)[cal.stat]
+        raw, res = spot_statistics(es, [len(b[0]) for b in spot.blocks], cal.lbo_W)
+        err = np.maximum(cal.w_raw * raw, cal.w_lbo * res)
     t2 = time.perf_counter()
     fb = err > threshold
     if np.any(fb):

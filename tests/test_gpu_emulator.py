@@ -269,7 +269,8 @@ def test_emulator_calibrated_fallback_bounds_every_channel(ctx, small_emu):
         assert (c.W is not None) == correct
         r = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, ctx=ctx)
         d = np.abs(r.BT - bt_lbl).max(axis=1)
-        print(f"correct={correct}: threshold {c.threshold:.4f} K (stat {c.stat}) on blocks {c.spot_blocks}: "
+        print(f"correct={correct}: threshold {c.threshold:.4f} (weights {c.w_raw:.3g}, {c.w_lbo:.3g}) on blocks "
+              f"{c.spot_blocks}: "
               f"fallback {r.fell_back.mean():.2f}, max |dBT| after fallback {d.max():.4f} K "
               f"(emulated scenes {np.sum(~r.fell_back)}; uncorrected max {np.abs(bt_raw - bt_lbl).max():.4f} K)")
         assert np.all(d[~r.fell_back] <= 0.05)
@@ -281,6 +282,12 @@ def test_emulator_calibrated_fallback_bounds_every_channel(ctx, small_emu):
         if correct:
             # the spot channels carry their LBL values; the others moved by the fitted map
             np.testing.assert_allclose(r.BT[:, c.spot_channels], bt_lbl[:, c.spot_channels], rtol=0, atol=1e-4)
+            # the library refuses spot blocks other than the correction's
+            other = SpotChecker(sub, snd, blocks=[(float(snd.nu_min + (c0 - 1 if c0 > 0 else c0 + 1) * snd.dnu), n)
+                                                  for c0, n in c.spot_blocks])
+            with pytest.raises(ValueError, match="differ"):
+                forward_emulated(profs[:2], sub, emu, sounder=snd, geom=geoms[:2], T_sfc=tsf[:2], spot=other,
+                                 threshold=c.threshold, ctx=ctx, correction=True)
 
 
 def test_emulator_spot_check_and_fallback(ctx, small_emu):

@@ -98,3 +98,23 @@ def test_correct_spectra_sets_spot_channels_and_radiances():
     np.testing.assert_allclose(np.delete(BTc, spot, axis=1), np.delete(bt, spot, axis=1) - 0.05, atol=1e-6)
     assert np.array_equal(BTc[:, spot], bt_spot)
     np.testing.assert_allclose(brightness_temperature(nu, R[0]), BTc[0], rtol=0, atol=1e-9)
+
+
+def test_joint_thresholds_keep_the_most_scenes_without_a_bad_one():
+    from irsounderlbl.emulator import fallback_thresholds
+    # scene: (raw, lbo, bad).  The bad scenes are excluded by raw (scene 4) and by lbo (scene 5).
+    s_raw = np.array([0.01, 0.02, 0.03, 0.04, 0.05, 0.02])
+    s_lbo = np.array([0.01, 0.01, 0.02, 0.01, 0.01, 0.05])
+    bad = np.array([False, False, False, False, True, True])
+    t1, t2 = fallback_thresholds(s_raw, s_lbo, bad)
+    assert (t1, t2) == (0.04, 0.02)
+    keep = (s_raw <= t1) & (s_lbo <= t2)
+    assert keep.tolist() == [True, True, True, True, False, False]
+    # either statistic alone would keep fewer: raw <= 0.01 (scene 5 has raw 0.02), lbo < 0.01 (none)
+    # no bad scene: every scene kept, the raw threshold at the largest calibration value
+    assert fallback_thresholds(s_raw, s_lbo, np.zeros(6, bool)) == (0.05, np.inf)
+    assert fallback_thresholds(s_raw, s_lbo, np.ones(6, bool)) == (-1.0, -1.0)
+    # no bad scene among the small raw values: the lbo statistic is unused
+    t1, t2 = fallback_thresholds(np.array([0.01, 0.02, 0.09]), np.array([0.5, 0.4, 0.1]),
+                                 np.array([False, False, True]))
+    assert (t1, t2) == (0.02, np.inf)
