@@ -606,7 +606,7 @@ class ChannelODEmulator:
 
     def _set_correction(self, cal: Optional[Calibration]):
         if cal is None or cal.W is None:
-            rc = self.lib.irs_emu_set_correction(self.h, 0, None, None, None, None, 0)
+            rc = self.lib.irs_emu_set_correction(self.h, 0, None, None, None, None, 0.0, 0.0)
         else:
             c0 = np.ascontiguousarray([c for c, _ in cal.spot_blocks], np.int64)
             bn = np.ascontiguousarray([n for _, n in cal.spot_blocks], np.int32)

@@ -60,3 +60,47 @@ def test_errors_without_device_or_bad_input():
     if not has_gpu:
         with pytest.raises(IRSError):
             Context(0)
+
+
+def declared_arity():
+    """Parameter count of every function declared in irs.h."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(irs_[a-z_0-9]+)\s*\(([^;{)]*)\)\s*;", text):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_ctypes_bindings_match_the_header_arity():
+    """Every ctypes prototype that declares argtypes has as many as the header's
+    declaration, so a call with the wrong count fails in ctypes, not in the library."""
+    from irsounderlbl.api import load_library
+    lib = load_library()
+    ar = declared_arity()
+    assert len(ar) >= 40
+    checked = 0
+    for name, n in ar.items():
+        at = getattr(lib, name).argtypes
+        if at is not None:
+            assert len(at) == n, (name, len(at), n)
+            checked += 1
+    assert checked >= 40
+
+
+def test_emulator_correction_call_sites_match_the_binding():
+    """The emulator's calls of irs_emu_set_correction reach the library (which
+    rejects the null handle) with the argument list the binding declares."""
+    from irsounderlbl.api import IRSError, load_library
+    from irsounderlbl.emulator import Calibration, ChannelODEmulator
+
+    class Fake:
+        lib = load_library()
+        h = None
+
+    with pytest.raises(IRSError):
+        ChannelODEmulator._set_correction(Fake(), None)
+    cal = Calibration(np.arange(4), [(0, 2), (2, 2)], 1.0, 0.05, np.zeros(4), np.zeros(4), np.zeros(1), np.zeros(1), 1,
+                      W=np.zeros((4, 4), np.float32), lbo_W=np.zeros(8, np.float32), w_raw=1.0, w_lbo=2.0)
+    with pytest.raises(IRSError):
+        ChannelODEmulator._set_correction(Fake(), cal)
