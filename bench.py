@@ -316,9 +316,9 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value, sig
     cal = syn.perturbed_profiles(n_cal, seed=5005, **kw)
     log(f"bench: emulator calibration ({n_cal} held-out scenes vs GPU LBL, spot blocks, correction, threshold)")
     t0 = time.perf_counter()
-    margin = 0.3
+    margin = 0.2
     c = emu.calibrate([p[0] for p in cal], lls, sounder=snd, geom=geoms(cal), T_sfc=[p[1] for p in cal],
-                      budget=0.05, margin=margin, n_blocks=4, block=64, ctx=ctx, correct=True)
+                      budget=0.05, margin=margin, n_blocks=6, block=64, ctx=ctx, correct=True)
     t_cal = time.perf_counter() - t0
     ev = syn.perturbed_profiles(n_eval, seed=4004, **kw)
     profs, gs, tsf = [p[0] for p in ev], geoms(ev), [p[1] for p in ev]
@@ -372,7 +372,7 @@ def emulator_figure(ctx, lls, snd, n_train, n_cal, n_eval, steps, lbl_value, sig
                        f"8461 channels, windows variant of the line lists; targets (t, g, ILS moment) x 49 layers, "
                        f"64 PCs per layer, per-layer quadratic + context regression on 13 layer features "
                        f"(incl. Curtis-Godson means of the path above), MLP 13 -> 3x1024 GELU -> 64 "
-                       f"(BF16 MFMA) on its residual; spot check 4 x 64 channels, spot-error correction of "
+                       f"(BF16 MFMA) on its residual; spot check 6 x 64 channels, spot-error correction of "
                        f"every channel (ridge map fitted on the calibration scenes)",
            "training": {"seconds": round(t_train, 1), "lbl_targets_seconds": round(rep.seconds_lbl, 1),
                         "pca_seconds": round(rep.seconds_pca, 1), "regression_seconds": round(rep.seconds_fit, 1),
@@ -627,7 +627,7 @@ def main():
     ap.add_argument("--emu-cal", type=int, default=1024)
     ap.add_argument("--emu-eval", type=int, default=2048)
     ap.add_argument("--emu-steps", type=int, default=96000)
-    ap.add_argument("--emu-sigma-lnq", type=float, default=0.0,
+    ap.add_argument("--emu-sigma-lnq", type=float, default=0.05,
                     help="level-by-level ln q_H2O perturbation (GP sigma) of the emulator's training scenes")
     ap.add_argument("--emulator-only", action="store_true",
                     help="the headline steps and the emulator figure only (the other figures off)")

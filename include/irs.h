@@ -216,7 +216,10 @@ typedef struct {
     int use_graph;              /* 1: capture each profile group's device sequence in a hipGraph
                                    and replay it on later calls of the same shape */
     int autotune;               /* 1: time the (tile_points, persistent) candidates on the first
-                                   profile of each new shape and keep the fastest */
+                                   profile of each new shape and keep the fastest; with the
+                                   environment variable IRS_TUNE_FILE naming a file, results are
+                                   appended there (keyed by device, grid and line-list content)
+                                   and later contexts and processes reuse them untimed */
     double far_wing_cm;         /* > 0: far-wing sums of lines at least this far (cm^-1) from a
                                    tile are evaluated on a coarse grid (<= 0.032 cm^-1) and
                                    interpolated (cubic), and lines far from a 64-point slice

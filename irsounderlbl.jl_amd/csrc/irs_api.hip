@@ -503,6 +503,71 @@ std::string shape_key(const std::vector<const ItemDev *> &per_species, int64_t n
     return k;
 }
 
+// Key of a tuned schedule that outlives the process: the device (architecture and
+// compute-unit count), the grid, and per species the line list's content (count and
+// an FNV-1a hash of the line centres, not the session's list handle), line mixing and
+// the window widths.
+std::string persistent_tune_key(const irs_ctx *ctx, const std::vector<const ItemDev *> &per_species, int64_t n_hi,
+                                double nu_lo, double dnu) {
+    hipDeviceProp_t pr{};
+    std::string dev = "unknown";
+    if (hipGetDeviceProperties(&pr, ctx->device) == hipSuccess)
+        dev = std::string(pr.gcnArchName) + ":" + std::to_string(pr.multiProcessorCount);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s|%lld|%.17g|%.17g", dev.c_str(), (long long)n_hi, nu_lo, dnu);
+    std::string k(buf);
+    for (const ItemDev *it : per_species) {
+        const ListHost &lh = ctx->lists[it->list];
+        uint64_t h = 1469598103934665603ull;
+        const unsigned char *b = (const unsigned char *)lh.nu_host.data();
+        for (size_t i = 0; i < lh.nu_host.size() * sizeof(double); ++i) h = (h ^ b[i]) * 1099511628211ull;
+        std::snprintf(buf, sizeof buf, "|%lld:%016llx:%d:%.9g:%.9g", (long long)lh.n, (unsigned long long)h, it->lm,
+                      (double)it->C, (double)it->Clm);
+        k += buf;
+    }
+    for (char &c : k)
+        if (c == '\t' || c == '\n') c = ' ';
+    return k;
+}
+
+// Tuned schedules kept in the file named by IRS_TUNE_FILE (one "key<TAB>P<TAB>persistent"
+// line per shape), so a new process skips the timing runs.
+bool tune_file_lookup(const std::string &key, int &P, int &pers) {
+    const char *path = std::getenv("IRS_TUNE_FILE");
+    if (!path || !*path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    bool found = false;
+    std::string line;
+    int c;
+    while (true) {
+        line.clear();
+        while ((c = std::fgetc(f)) != EOF && c != '\n') line.push_back((char)c);
+        const size_t t1 = line.find('\t');
+        const size_t t2 = t1 == std::string::npos ? t1 : line.find('\t', t1 + 1);
+        if (t2 != std::string::npos && line.compare(0, t1, key) == 0 && t1 == key.size()) {
+            const int p = std::atoi(line.c_str() + t1 + 1), q = std::atoi(line.c_str() + t2 + 1);
+            if ((p == 4 || p == 8 || p == 16) && (q == 0 || q == 1)) {
+                P = p;
+                pers = q;
+                found = true;          // the last entry for a key wins
+            }
+        }
+        if (c == EOF) break;
+    }
+    std::fclose(f);
+    return found;
+}
+
+void tune_file_store(const std::string &key, int P, int pers) {
+    const char *path = std::getenv("IRS_TUNE_FILE");
+    if (!path || !*path) return;
+    if (FILE *f = std::fopen(path, "a")) {
+        std::fprintf(f, "%s\t%d\t%d\n", key.c_str(), P, pers);
+        std::fclose(f);
+    }
+}
+
 // Heaviest-first order of the spectral tiles: the cost of a tile is the number
 // of lines whose window reaches it, weighted 1.6 on line-mixing lists (the
 // dispersive term), from host copies of the line centres.
@@ -2108,6 +2173,12 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                 if (proto[s].active) per_species.push_back(&proto[s]);
             if (ctx->sched.autotune) {
                 const std::string key = shape_key(per_species, n, nu_lo, dnu, 0);
+                std::string pkey;
+                if (!ctx->tuned.count(key) && std::getenv("IRS_TUNE_FILE")) {
+                    pkey = persistent_tune_key(ctx, per_species, n, nu_lo, dnu);
+                    int fP = 0, fPers = 0;
+                    if (tune_file_lookup(pkey, fP, fPers)) ctx->tuned[key] = {fP, fPers};
+                }
                 if (!ctx->tuned.count(key)) {
                     GroupPlan tp;
                     std::memset(&tp, 0, sizeof tp);
@@ -2124,6 +2195,7 @@ static int run_forward(irs_ctx *ctx, const irs_profile *profiles, int n_prof, co
                     rc = autotune_group(ctx, tp, per_species, key);
                     if (rc) return rc;
                     tuned_now = true;
+                    if (!pkey.empty()) tune_file_store(pkey, ctx->tuned[key].first, ctx->tuned[key].second);
                 }
                 P = ctx->tuned[key].first;
                 persistent = ctx->tuned[key].second;

@@ -116,3 +116,27 @@ def test_autotune_keeps_parity(bctx, scene):
     prof, T_sfc, mu = profs[2]
     _, bto, _ = o.forward(prof, sub, snd, line_mixing=1, T_sfc=T_sfc, mu=mu)
     assert np.max(np.abs(BTa[2] - bto)) < 0.005
+
+
+def test_autotune_persists_in_tune_file(scene, tmp_path, monkeypatch):
+    """With IRS_TUNE_FILE set, the first context times the candidates and appends the
+    result (keyed by device, grid and line-list content); a second context reuses it
+    without timing and produces the same spectra; an unset variable writes nothing."""
+    path = tmp_path / "tune.tsv"
+    monkeypatch.setenv("IRS_TUNE_FILE", str(path))
+    a = Context(0)
+    a.set_schedule(4, autotune=True)
+    _, Ra, _ = _run(a, scene)
+    sa = a.stats()
+    assert sa["autotuned"] == 1
+    lines = path.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].split("\t")[1:] == [str(sa["tile_points"]), str(sa["persistent"])]
+    b = Context(0)
+    b.set_schedule(4, autotune=True)
+    _, Rb, _ = _run(b, scene)
+    sb = b.stats()
+    assert sb["autotuned"] == 0 and (sb["tile_points"], sb["persistent"]) == (sa["tile_points"], sa["persistent"])
+    assert np.array_equal(Ra, Rb)
+    assert len(path.read_text().splitlines()) == 1
+    a.close()
+    b.close()
