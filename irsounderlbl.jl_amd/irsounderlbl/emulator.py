@@ -544,7 +544,7 @@ class ChannelODEmulator:
 
     # ---- calibration --------------------------------------------------------
     def calibrate(self, profs, linelists, *, sounder: Sounder, geom=None, T_sfc=None, eps_sfc=1.0,
-                  budget: float = 0.05, margin: float = 0.2, n_blocks: int = 4, block: int = 64,
+                  budget: float = 0.05, margin: float = 0.2, n_blocks: int = 6, block: int = 64,
                   ctx: Optional[Context] = None, line_mixing="vpy", apply_continuum: bool = False,
                   bt_lbl=None, correct: bool = True, lam: float = 1e-3, folds: int = 4, **kw) -> Calibration:
         """Hold-out statistics on calibration scenes against GPU LBL (the per-channel
