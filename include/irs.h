@@ -532,6 +532,15 @@ int         irs_forward_emulated(irs_ctx *ctx, irs_emu *emu, const irs_profile *
 int         irs_emu_set_correction(irs_emu *emu, int n_blocks, const int64_t *block_c0, const int32_t *block_n,
                                    const float *W, const float *lbo_W, double w_raw, double w_lbo);
 
+/* Loads a trained and calibrated emulator from the file written by the Python package
+ * (ChannelODEmulator.export_binary): creates its MLP on `device` (max_batch rows per
+ * launch) and the emulator with its profile context and, when the file holds one, its
+ * spot-check correction; returns the calibration's threshold and spot blocks (at most
+ * max_blocks) for irs_forward_emulated.  Destroy the emulator, then the MLP.  No
+ * counterpart in the reference (the learned path's loader for Julia and C callers). */
+int         irs_emu_load(const char *path, int device, int max_batch, irs_mlp **mlp, irs_emu **emu,
+                         double *threshold, int max_blocks, int64_t *block_c0, int32_t *block_n, int *n_blocks);
+
 /* Emulator Jacobians (chain rule through the MLP input Jacobian, the regression,
  * the decode and the adjoint of the channel RTE; the reflected term of eps < 1 held fixed):
  * BT [n_prof][n_ch]; dfeat [n_prof][L][n_feat][n_ch] = dBT/d(raw feature);

@@ -926,6 +926,111 @@ extern "C" int irs_emu_set_correction(irs_emu *e, int n_blocks, const int64_t *b
     return IRS_OK;
 }
 
+// Trained emulator file (written by the Python package, ChannelODEmulator.export_binary),
+// little-endian, no padding:
+//   "IRSEMU01", int32 n_in n_hidden n_out L n_pc n_ch n_feat n_local n_ctx n_vec n_blocks
+//   corrected, float coef_scale, int64 n_params, float params[n_params],
+//   float x_mean[L n_feat] x_std[L n_feat] A[L n_phi n_pc] basis[L n_pc 3 n_ch]
+//   mean[L 3 n_ch] inv_w[L 3 n_ch], double ch_nu[n_ch] ctx_mean[n_vec] ctx_proj[n_ctx n_vec],
+//   double threshold w_raw w_lbo, int64 block_c0[n_blocks], int32 block_n[n_blocks],
+//   and when corrected: float W[n_spot n_ch] lbo[sum_b (n_spot - n_b) n_b].
+extern "C" int irs_emu_load(const char *path, int device, int max_batch, irs_mlp **mlp_out, irs_emu **emu_out,
+                            double *threshold, int max_blocks, int64_t *block_c0, int32_t *block_n,
+                            int *n_blocks_out) {
+    if (!path || !mlp_out || !emu_out || !threshold || !n_blocks_out || max_blocks < 0 ||
+        (max_blocks > 0 && (!block_c0 || !block_n)))
+        return IRS_EINVAL;
+    *mlp_out = nullptr;
+    *emu_out = nullptr;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return IRS_ENOTFOUND;
+    bool ok = true;
+    auto rd = [&](void *dst, size_t bytes) { ok = ok && std::fread(dst, 1, bytes, f) == bytes; };
+    char magic[8] = {};
+    rd(magic, 8);
+    ok = ok && std::memcmp(magic, "IRSEMU01", 8) == 0;
+    int32_t h[12] = {};
+    rd(h, sizeof h);
+    const int n_in = h[0], n_hidden = h[1], n_out = h[2], L = h[3], n_pc = h[4], n_ch = h[5], n_feat = h[6],
+              n_local = h[7], n_ctx = h[8], n_vec = h[9], nb = h[10], corrected = h[11];
+    ok = ok && n_in > 0 && n_hidden > 0 && n_out == n_pc && L > 0 && L <= 64 && n_pc > 0 && n_ch > 0 &&
+         n_local >= 0 && n_local <= n_feat && n_feat <= 24 && n_ctx == n_feat - n_local && n_vec >= 0 &&
+         nb >= 0 && nb <= max_blocks && (corrected == 0 || corrected == 1) && (corrected == 0 || nb > 0);
+    float coef_scale = 0.0f;
+    int64_t n_params = 0;
+    rd(&coef_scale, sizeof coef_scale);
+    rd(&n_params, sizeof n_params);
+    const int n_phi = 1 + n_local + n_local * (n_local + 1) / 2 + (n_feat - n_local);
+    const size_t W3 = ok ? (size_t)3 * n_ch : 0;
+    std::vector<float> params, xm, xs, A, basis, mean, inv_w, Wc, lbo;
+    std::vector<double> nu, cm, cp;
+    double thr[3] = {};
+    std::vector<int64_t> c0;
+    std::vector<int32_t> bn;
+    auto grab = [&](auto &v, size_t n) {
+        if (!ok || n > ((size_t)1 << 34)) { ok = false; return; }
+        v.resize(n);
+        rd(v.data(), n * sizeof(v[0]));
+    };
+    if (ok) {
+        grab(params, (size_t)n_params);
+        grab(xm, (size_t)L * n_feat);
+        grab(xs, (size_t)L * n_feat);
+        grab(A, (size_t)L * n_phi * n_pc);
+        grab(basis, (size_t)L * n_pc * W3);
+        grab(mean, (size_t)L * W3);
+        grab(inv_w, (size_t)L * W3);
+        grab(nu, (size_t)n_ch);
+        grab(cm, (size_t)n_vec);
+        grab(cp, (size_t)n_ctx * n_vec);
+        rd(thr, sizeof thr);
+        grab(c0, (size_t)nb);
+        grab(bn, (size_t)nb);
+    }
+    int64_t ns = 0, n_lbo = 0;
+    for (int b = 0; ok && b < nb; ++b) {
+        ok = bn[b] > 0 && c0[b] >= 0 && c0[b] + bn[b] <= n_ch;
+        ns += ok ? bn[b] : 0;
+    }
+    for (int b = 0; ok && b < nb; ++b) n_lbo += (ns - bn[b]) * bn[b];
+    if (ok && corrected) {
+        grab(Wc, (size_t)ns * n_ch);
+        grab(lbo, (size_t)n_lbo);
+    }
+    std::fclose(f);
+    if (!ok) return IRS_EINVAL;
+    irs_mlp *m = nullptr;
+    int rc = irs_mlp_create(device, n_in, n_hidden, n_out, max_batch, 0, &m);
+    if (rc) return rc;
+    if (irs_mlp_n_params(m) != n_params) {
+        irs_mlp_destroy(m);
+        return IRS_EINVAL;
+    }
+    irs_emu *e = nullptr;
+    rc = irs_mlp_set_params(m, params.data());
+    if (!rc)
+        rc = irs_emu_create(m, L, n_pc, n_ch, n_feat, n_local, xm.data(), xs.data(), A.data(), coef_scale,
+                            basis.data(), mean.data(), inv_w.data(), nu.data(), &e);
+    if (!rc) rc = irs_emu_set_context(e, n_ctx, n_vec, cm.data(), cp.data());
+    if (!rc && corrected)
+        rc = irs_emu_set_correction(e, nb, c0.data(), bn.data(), Wc.data(), lbo.empty() ? nullptr : lbo.data(),
+                                    thr[1], thr[2]);
+    if (rc) {
+        if (e) irs_emu_destroy(e);
+        irs_mlp_destroy(m);
+        return rc;
+    }
+    *threshold = thr[0];
+    for (int b = 0; b < nb; ++b) {
+        block_c0[b] = c0[b];
+        block_n[b] = bn[b];
+    }
+    *n_blocks_out = nb;
+    *mlp_out = m;
+    *emu_out = e;
+    return IRS_OK;
+}
+
 namespace {
 
 // Fallback statistic of the corrected emulator for one scene from its spot errors

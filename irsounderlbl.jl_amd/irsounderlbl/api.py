@@ -147,7 +147,7 @@ EXPORTS = ("irs_version", "irs_device_info", "irs_create", "irs_destroy", "irs_l
            "irs_cia_load_file", "irs_continuum_layer", "irs_xsec_layer_noalloc",
            "irs_channel_od_moments", "irs_par_parse", "irs_par_load", "irs_relmat_upload",
            "irs_relmat_free", "irs_lm_perturbation", "irs_relmat_terms", "irs_emu_set_context",
-           "irs_forward_emulated", "irs_emu_set_correction")
+           "irs_forward_emulated", "irs_emu_set_correction", "irs_emu_load")
 
 _lib = None
 
@@ -227,6 +227,8 @@ def load_library(path: str = LIB_PATH):
     L.irs_relmat_terms.argtypes = [C.POINTER(IrsRelmat), P, f64, f64, f64, i64, P, C.POINTER(i64)]
     L.irs_emu_set_context.argtypes = [P, i32, i32, P, P]
     L.irs_emu_set_correction.argtypes = [P, i32, P, P, P, P, f64, f64]
+    L.irs_emu_load.argtypes = [C.c_char_p, i32, i32, C.POINTER(P), C.POINTER(P), C.POINTER(f64), i32, P, P,
+                               C.POINTER(i32)]
     L.irs_forward_emulated.argtypes = [P, P, C.POINTER(IrsProfile), i32, C.POINTER(IrsSounder),
                                        C.POINTER(IrsOptions), i32, P, P, P, f64, P, P, P, P]
     for name in EXPORTS:

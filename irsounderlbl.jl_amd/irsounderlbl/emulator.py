@@ -474,6 +474,37 @@ class ChannelODEmulator:
                             ctx_std=self.context.p_std, ctx_proj=self.context.proj,
                             dims=np.array([self.mlp.n_in, self.mlp.n_hidden, self.mlp.n_out]), **extra)
 
+    def export_binary(self, path: str):
+        """The emulator, its profile context and calibration (with the spot-check
+        correction, when fitted) in the library's file format (irs_emu_load), for callers
+        of the C ABI such as the Julia package."""
+        cal = self.calibration
+        if cal is None:
+            raise ValueError("calibrate the emulator before exporting it")
+        L, n_feat = self.x_mean.shape
+        cx = self.context
+        corrected = cal.W is not None
+        blocks = list(cal.spot_blocks)
+        w_raw, w_lbo = (cal.w_raw, cal.w_lbo) if corrected else (1.0, 0.0)
+        with open(path, "wb") as f:
+            f.write(b"IRSEMU01")
+            f.write(np.array([self.mlp.n_in, self.mlp.n_hidden, self.mlp.n_out, L, self.n_pc, self.n_ch, n_feat,
+                              N_LOCAL, cx.proj.shape[0], cx.proj.shape[1], len(blocks), int(corrected)],
+                             np.int32).tobytes())
+            params = self.mlp.get_params()
+            f.write(np.float32(self.coef_scale).tobytes())
+            f.write(np.int64(params.size).tobytes())
+            for a in (params, self.x_mean, self.x_std, self.A, self.basis, self.mean, self.inv_w):
+                f.write(np.ascontiguousarray(a, np.float32).tobytes())
+            for a in (self.ch_nu, cx.p_mean, cx.proj):
+                f.write(np.ascontiguousarray(a, np.float64).tobytes())
+            f.write(np.array([cal.threshold, w_raw, w_lbo], np.float64).tobytes())
+            f.write(np.array([c for c, _ in blocks], np.int64).tobytes())
+            f.write(np.array([n for _, n in blocks], np.int32).tobytes())
+            if corrected:
+                f.write(np.ascontiguousarray(cal.W, np.float32).tobytes())
+                f.write(np.ascontiguousarray(cal.lbo_W, np.float32).tobytes())
+
     @staticmethod
     def load(path: str, max_batch: int = 4096):
         z = np.load(path, allow_pickle=False)

@@ -428,4 +428,106 @@ function faddeeva_voigt32(x::AbstractVector{<:Real}, y::AbstractVector{<:Real}; 
     return wr
 end
 
+"""
+    Emulator
+
+A trained and calibrated channel-transmittance emulator loaded by `irs_emu_load`
+from the file the Python package writes (`ChannelODEmulator.export_binary`).  It
+serves `forward_emulated`: the emulated spectra, an LBL spot check of the
+calibration's channel blocks, the correction of every channel from the scene's
+spot errors, and full LBL for the scenes the spot check flags.  No counterpart
+in the reference.
+"""
+mutable struct Emulator
+    mlp::Ptr{Cvoid}
+    emu::Ptr{Cvoid}
+    threshold::Float64
+    block_c0::Vector{Int64}
+    block_n::Vector{Int32}
+end
+
+function load_emulator(path::AbstractString; device::Integer = parse(Int, get(ENV, "IRS_DEVICE", "0")),
+                       max_batch::Integer = 4096)
+    mlp = Ref{Ptr{Cvoid}}(C_NULL); emu = Ref{Ptr{Cvoid}}(C_NULL)
+    thr = Ref{Float64}(0.0); nb = Ref{Cint}(0)
+    c0 = zeros(Int64, 64); bn = zeros(Int32, 64)
+    rc = ccall((:irs_emu_load, libirs), Cint,
+               (Cstring, Cint, Cint, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}, Ref{Float64}, Cint, Ptr{Int64}, Ptr{Int32},
+                Ref{Cint}),
+               path, device, max_batch, mlp, emu, thr, 64, c0, bn, nb)
+    rc == 0 || error("irs_emu_load($path) failed ($rc)")
+    e = Emulator(mlp[], emu[], thr[], c0[1:nb[]], bn[1:nb[]])
+    finalizer(e) do x
+        ccall((:irs_emu_destroy, libirs), Cvoid, (Ptr{Cvoid},), x.emu)
+        ccall((:irs_mlp_destroy, libirs), Cvoid, (Ptr{Cvoid},), x.mlp)
+    end
+    return e
+end
+
+"""
+    forward_emulated(prof, linelists, em::Emulator; sounder, geom, T_sfc, ε_sfc, line_mixing,
+                     apply_continuum=false, threshold=em.threshold) -> (WavenumberGrid, R, BT, fell_back)
+
+`forward_model`'s call with the learned path (`irs_forward_emulated`): the spot blocks
+are the emulator's calibration blocks, each recomputed by LBL on its own sub-grid with
+the lists filtered to the block ± 42 cm⁻¹; `fell_back[q]` marks the scenes recomputed
+with full LBL.  `apply_continuum` and `line_mixing` as in the emulator's training
+spectra.
+"""
+function forward_emulated(prof, linelists::Dict{GasSpecies, HITRANLinelist}, em::Emulator;
+                          sounder::Sounder = IASIInstrument(), iasi::Union{Sounder, Nothing} = nothing,
+                          geom = nadir_geometry(), T_sfc = nothing, ε_sfc = 1.0, line_mixing = nothing,
+                          apply_continuum::Bool = false, threshold::Float64 = em.threshold,
+                          cutoff::Float64 = 25.0, dptmn::Float64 = 1e-6)
+    snd = iasi === nothing ? sounder : iasi
+    profs = prof isa AbstractVector ? prof : [prof]
+    nb = length(profs)
+    pick(x, q) = x isa AbstractVector ? x[q] : x
+    ctx = context()
+    o = options(; internal_dnu = 0.001, high_res_factor = nothing, cutoff, lm_cutoff = cutoff, dptmn,
+                with_ils = true, apodization = :gaussian, line_mixing, T_method = :logp_at_pcg,
+                source_function = :cim, apply_continuum, continua = (:h2o, :co2, :co2_cia, :n2, :o2))
+    keep = Any[]
+    cps = Vector{CProfile}(undef, nb)
+    species = GasSpecies[]
+    for q in 1:nb
+        cps[q], species = cprofile(ctx, profs[q], linelists, line_mixing, pick(T_sfc, q), pick(ε_sfc, q),
+                                   pick(geom, q).μ, keep)
+    end
+    y = line_mixing isa AbstractVector ? line_mixing : nothing
+    nblk = length(em.block_c0)
+    lists = Matrix{Int32}(undef, length(species), nblk)        # [block][species] in C order
+    for b in 1:nblk
+        lo = snd.ν_min + em.block_c0[b] * snd.Δν - 42.0
+        hi = snd.ν_min + (em.block_c0[b] + em.block_n[b] - 1) * snd.Δν + 42.0
+        for (k, sp) in enumerate(species)
+            if !haskey(linelists, sp)
+                lists[k, b] = -1
+                continue
+            end
+            ll = linelists[sp]
+            sel = [lo <= l.wavenumber <= hi for l in ll.lines]
+            sub = IRSounderLBL.filter_linelist(ll, lo, hi)
+            yb = (sp == CO2 && y !== nothing) ? y[sel] : nothing
+            lists[k, b] = isempty(sub.lines) ? Int32(-1) : upload!(ctx, sub, yb)
+            push!(keep, sub)
+        end
+    end
+    grid = wavenumber_grid(snd.ν_min, snd.ν_max, snd.Δν)
+    R = zeros(Float64, grid.n, nb); BT = zeros(Float64, grid.n, nb)
+    fb = zeros(UInt8, nb); err = zeros(Float64, nb)
+    cs = csounder(snd)
+    GC.@preserve keep cps lists R BT fb err begin
+        rc = ccall((:irs_forward_emulated, libirs), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{CProfile}, Cint, Ref{CSounder}, Ref{COptions}, Cint, Ptr{Int64},
+                    Ptr{Int32}, Ptr{Int32}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64}),
+                   ctx.h, em.emu, cps, nb, cs, o, nblk, em.block_c0, em.block_n, lists, threshold, R, BT, fb, err)
+        rc == 0 || error(unsafe_string(ccall((:irs_emu_last_error, libirs), Cstring, (Ptr{Cvoid},), em.emu)) *
+                         " / " * unsafe_string(ccall((:irs_last_error, libirs), Cstring, (Ptr{Cvoid},), ctx.h)))
+    end
+    fell_back = fb .!= 0
+    prof isa AbstractVector && return grid, R, BT, fell_back
+    return grid, R[:, 1], BT[:, 1], fell_back[1]
+end
+
 end # module

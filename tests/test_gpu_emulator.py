@@ -387,3 +387,51 @@ def test_emulator_jacobian_matches_finite_differences(ctx, small_emu):
     relq_lbl = np.linalg.norm(jac.K_lnq_H2O - ref.K[:, 50:100]) / np.linalg.norm(ref.K[:, 50:100])
     print(f"emulator vs LBL analytic Jacobian: relative Frobenius error T {rel:.3f}, ln q_H2O {relq_lbl:.3f}")
     assert rel < 0.5
+
+
+def test_emulator_file_loaded_by_the_library_matches(ctx, small_emu, tmp_path):
+    """export_binary -> irs_emu_load (the learned path's loader for C and Julia callers):
+    the loaded emulator gives the same emulated spectra and the same corrected, spot-checked
+    result of irs_forward_emulated as the emulator it was written from."""
+    import ctypes as C
+    from irsounderlbl import api
+    from irsounderlbl.api import _p
+    sub, snd, emu, rep, hold = small_emu
+    profs, geoms, tsf = _hold(hold[:32])
+    c = emu.calibrate(profs, sub, sounder=snd, geom=geoms, T_sfc=tsf, budget=0.05, n_blocks=2, block=8, ctx=ctx)
+    path = str(tmp_path / "emu.bin")
+    emu.export_binary(path)
+    lib = emu.lib
+    hm, he, thr, nbk = C.c_void_p(), C.c_void_p(), C.c_double(), C.c_int()
+    c0 = np.zeros(8, np.int64)
+    bn = np.zeros(8, np.int32)
+    assert lib.irs_emu_load(path.encode(), 0, 4096, C.byref(hm), C.byref(he), C.byref(thr), 8, _p(c0), _p(bn),
+                            C.byref(nbk)) == 0
+    try:
+        assert thr.value == c.threshold
+        assert [(int(a), int(b)) for a, b in zip(c0[:nbk.value], bn[:nbk.value])] == \
+            [(int(a), int(b)) for a, b in c.spot_blocks]
+        tp, tg, tt = _hold(hold[32:48])
+        scenes, eps = emu._scenes(tp, tg, tt, 1.0)
+        X, tlev, tave, scal = emu._inputs(scenes, eps)
+        n = len(tp)
+        outs = []
+        for h in (emu.h, he):
+            R, BT = np.zeros((n, emu.n_ch)), np.zeros((n, emu.n_ch))
+            assert lib.irs_emu_forward(h, _p(X), _p(tlev), _p(tave), _p(scal), n, api.SOURCE_FUNCTION["cim"],
+                                       _p(R), _p(BT), None) == 0
+            outs.append(BT)
+        assert np.array_equal(outs[0], outs[1])
+        r1 = forward_emulated(tp, sub, emu, sounder=snd, geom=tg, T_sfc=tt, ctx=ctx)
+
+        class Loaded:                       # the library handle in the emulator's place
+            h, lib, n_ch, calibration = he, emu.lib, emu.n_ch, emu.calibration
+
+            def spot_checker(self, ll, s):
+                return emu.spot_checker(ll, s)
+
+        r2 = forward_emulated(tp, sub, Loaded(), sounder=snd, geom=tg, T_sfc=tt, ctx=ctx, correction=True)
+        assert np.array_equal(r1.BT, r2.BT) and np.array_equal(r1.fell_back, r2.fell_back)
+    finally:
+        lib.irs_emu_destroy(he)
+        lib.irs_mlp_destroy(hm)

@@ -120,3 +120,44 @@ def test_python_keyword_precedence():
     sig = inspect.signature(api.forward_model)
     assert sig.parameters["apply_continuum"].default is True
     assert sig.parameters["continua"].default == ("h2o", "co2", "co2_cia", "n2", "o2")
+
+
+def _jl_ccall_arity():
+    """(name, number of argument types) of every ccall into libirs in the shim."""
+    text = open(SHIM).read()
+    out = []
+    for m in re.finditer(r"ccall\(\(:(irs_\w+),\s*libirs\),\s*\w+,\s*\(", text):
+        i, depth, start = m.end(), 1, m.end()
+        while depth:
+            ch = text[i]
+            depth += ch in "({"
+            depth -= ch in ")}"
+            i += 1
+        inner = text[start:i - 1]
+        parts, depth, cur = [], 0, ""
+        for ch in inner:
+            if ch in "({":
+                depth += 1
+            elif ch in ")}":
+                depth -= 1
+            if ch == "," and depth == 0:
+                parts.append(cur)
+                cur = ""
+            else:
+                cur += ch
+        parts.append(cur)
+        out.append((m.group(1), len([p for p in parts if p.strip()])))
+    return out
+
+
+def test_julia_ccalls_match_header_arity():
+    """Every ccall of the shim passes as many argument types as the header's
+    declaration has parameters (the learned path's loader and call included)."""
+    from test_abi import declared_arity
+    ar = declared_arity()
+    calls = _jl_ccall_arity()
+    names = {n for n, _ in calls}
+    assert {"irs_forward", "irs_jacobian", "irs_emu_load", "irs_forward_emulated"} <= names
+    for name, n in calls:
+        assert name in ar, name
+        assert n == ar[name], (name, n, ar[name])
