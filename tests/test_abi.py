@@ -104,3 +104,33 @@ def test_emulator_correction_call_sites_match_the_binding():
                       W=np.zeros((4, 4), np.float32), lbo_W=np.zeros(8, np.float32), w_raw=1.0, w_lbo=2.0)
     with pytest.raises(IRSError):
         ChannelODEmulator._set_correction(Fake(), cal)
+
+
+def test_emulator_file_loader_rejects_bad_files(tmp_path):
+    """irs_emu_load checks the whole file before it touches a device: a missing file is
+    IRS_ENOTFOUND, a wrong magic, inconsistent dimensions or a truncated file IRS_EINVAL."""
+    from irsounderlbl.api import IRS_EINVAL, IRS_ENOTFOUND, load_library
+    lib = load_library()
+
+    def load(path):
+        hm, he, thr, nb = C.c_void_p(), C.c_void_p(), C.c_double(), C.c_int()
+        c0 = np.zeros(4, np.int64)
+        bn = np.zeros(4, np.int32)
+        rc = lib.irs_emu_load(str(path).encode(), 0, 64, C.byref(hm), C.byref(he), C.byref(thr), 4,
+                              c0.ctypes.data, bn.ctypes.data, C.byref(nb))
+        assert not hm.value and not he.value
+        return rc
+
+    assert load(tmp_path / "none.bin") == IRS_ENOTFOUND
+    p = tmp_path / "bad.bin"
+    p.write_bytes(b"NOTANEMU" + bytes(64))
+    assert load(p) == IRS_EINVAL
+    # n_in n_hidden n_out L n_pc n_ch n_feat n_local n_ctx n_vec n_blocks corrected
+    hdr = np.array([13, 32, 8, 3, 8, 5, 21, 13, 8, 11, 1, 0], np.int32)
+    bad = hdr.copy()
+    bad[2] = 7                                           # n_out must equal n_pc
+    p.write_bytes(b"IRSEMU01" + bad.tobytes() + np.float32(1.0).tobytes() + np.int64(10).tobytes())
+    assert load(p) == IRS_EINVAL
+    p.write_bytes(b"IRSEMU01" + hdr.tobytes() + np.float32(1.0).tobytes() + np.int64(10).tobytes() +
+                  np.zeros(10, np.float32).tobytes())     # the arrays after the parameters are missing
+    assert load(p) == IRS_EINVAL
