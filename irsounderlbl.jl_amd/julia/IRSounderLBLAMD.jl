@@ -444,6 +444,7 @@ mutable struct Emulator
     threshold::Float64
     block_c0::Vector{Int64}
     block_n::Vector{Int32}
+    lists::IdDict{Any, Matrix{Int32}}     # (linelists, Y) => spot-block list handles, uploaded once
 end
 
 function load_emulator(path::AbstractString; device::Integer = parse(Int, get(ENV, "IRS_DEVICE", "0")),
@@ -456,7 +457,7 @@ function load_emulator(path::AbstractString; device::Integer = parse(Int, get(EN
                 Ref{Cint}),
                path, device, max_batch, mlp, emu, thr, 64, c0, bn, nb)
     rc == 0 || error("irs_emu_load($path) failed ($rc)")
-    e = Emulator(mlp[], emu[], thr[], c0[1:nb[]], bn[1:nb[]])
+    e = Emulator(mlp[], emu[], thr[], c0[1:nb[]], bn[1:nb[]], IdDict{Any, Matrix{Int32}}())
     finalizer(e) do x
         ccall((:irs_emu_destroy, libirs), Cvoid, (Ptr{Cvoid},), x.emu)
         ccall((:irs_mlp_destroy, libirs), Cvoid, (Ptr{Cvoid},), x.mlp)
@@ -496,22 +497,26 @@ function forward_emulated(prof, linelists::Dict{GasSpecies, HITRANLinelist}, em:
     end
     y = line_mixing isa AbstractVector ? line_mixing : nothing
     nblk = length(em.block_c0)
-    lists = Matrix{Int32}(undef, length(species), nblk)        # [block][species] in C order
-    for b in 1:nblk
-        lo = snd.ν_min + em.block_c0[b] * snd.Δν - 42.0
-        hi = snd.ν_min + (em.block_c0[b] + em.block_n[b] - 1) * snd.Δν + 42.0
-        for (k, sp) in enumerate(species)
-            if !haskey(linelists, sp)
-                lists[k, b] = -1
-                continue
+    # the caller's lists filtered to each block +- 42 cm^-1, uploaded on the first call with
+    # these lists (and Y) and kept with the emulator: [block][species] in C order
+    lists = get!(em.lists, (linelists, y, Tuple(species), snd.ν_min, snd.Δν)) do   # compared with ===
+        m = Matrix{Int32}(undef, length(species), nblk)
+        for b in 1:nblk
+            lo = snd.ν_min + em.block_c0[b] * snd.Δν - 42.0
+            hi = snd.ν_min + (em.block_c0[b] + em.block_n[b] - 1) * snd.Δν + 42.0
+            for (k, sp) in enumerate(species)
+                if !haskey(linelists, sp)
+                    m[k, b] = -1
+                    continue
+                end
+                ll = linelists[sp]
+                sel = [lo <= l.wavenumber <= hi for l in ll.lines]
+                sub = IRSounderLBL.filter_linelist(ll, lo, hi)
+                yb = (sp == CO2 && y !== nothing) ? y[sel] : nothing
+                m[k, b] = isempty(sub.lines) ? Int32(-1) : upload!(ctx, sub, yb)
             end
-            ll = linelists[sp]
-            sel = [lo <= l.wavenumber <= hi for l in ll.lines]
-            sub = IRSounderLBL.filter_linelist(ll, lo, hi)
-            yb = (sp == CO2 && y !== nothing) ? y[sel] : nothing
-            lists[k, b] = isempty(sub.lines) ? Int32(-1) : upload!(ctx, sub, yb)
-            push!(keep, sub)
         end
+        m
     end
     grid = wavenumber_grid(snd.ν_min, snd.ν_max, snd.Δν)
     R = zeros(Float64, grid.n, nb); BT = zeros(Float64, grid.n, nb)
