@@ -288,6 +288,21 @@ def test_emulator_calibrated_fallback_bounds_every_channel(ctx, small_emu):
             with pytest.raises(ValueError, match="differ"):
                 forward_emulated(profs[:2], sub, emu, sounder=snd, geom=geoms[:2], T_sfc=tsf[:2], spot=other,
                                  threshold=c.threshold, ctx=ctx, correction=True)
+            # both statistics weighted (the leave-one-block-out residual computed by the
+            # library): the library's statistic and decisions equal the host restatement's
+            w_raw, w_lbo = c.w_raw, c.w_lbo
+            c.w_raw, c.w_lbo = 1.0 / 0.03, 1.0 / 0.01
+            emu._set_correction(c)
+            try:
+                r3 = forward_emulated(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=1.0, ctx=ctx)
+                r4 = forward_emulated_steps(profs, sub, emu, sounder=snd, geom=geoms, T_sfc=tsf, threshold=1.0,
+                                            ctx=ctx)
+                np.testing.assert_allclose(r3.spot_err, r4.spot_err, rtol=1e-4, atol=1e-4)
+                assert np.array_equal(r3.fell_back, r4.fell_back)
+                assert np.any(r3.spot_err > 0)
+            finally:
+                c.w_raw, c.w_lbo = w_raw, w_lbo
+                emu._set_correction(c)
 
 
 def test_emulator_spot_check_and_fallback(ctx, small_emu):
