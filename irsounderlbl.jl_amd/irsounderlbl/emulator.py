@@ -24,10 +24,16 @@ per-layer least-squares fits, which are FP64/FP32 host linear algebra):
      MFMA kernels (FP32 accumulation, Adam) on the regression's residual;
   4. inference: features -> coefficients (regression + MLP) -> per-layer decode ->
      channel RTE (irs_emu_forward), batched on the device;
-  5. calibration on held-out scenes (per-channel hold-out BT error, the spot
-     channels where it is largest, the spot-error threshold under which every
-     calibration scene is within the budget everywhere) and the per-scene LBL
-     spot check: scenes above the threshold are recomputed with full LBL.
+  5. calibration on held-out scenes against GPU LBL: the per-channel hold-out BT
+     error; spot blocks whose measured errors predict every channel's error (the
+     emulator's errors are low-rank across channels); a ridge map from the spot
+     errors to every channel's error; and the fallback rule (thresholds on the
+     largest spot error and on the map's leave-one-block-out residual) under which
+     every calibration scene is within (1 - margin) x budget after correction,
+     judged out of fold;
+  6. the per-scene LBL spot check (irs_forward_emulated): every channel corrected
+     from the scene's spot errors, the spot channels set to their LBL values, and
+     the scenes the rule flags recomputed with full LBL.
 """
 from __future__ import annotations
 
