@@ -347,3 +347,22 @@ def test_channel_od_helper_cim_closed_forms():
     s = np.array([0.0599999, 0.0600001])
     assert abs(_fcim(s)[1] - _fcim(s)[0]) < 1e-5           # branch continuity
     assert abs(float(_fcim(200.0)) - (1.0 - 2.0 / 200.0)) < 1e-12
+
+
+@pytest.mark.parametrize("T_method,tm_code", [("logp_at_pcg", 0), ("mass_weighted", 1)])
+def test_library_layer_properties_match_oracle(T_method, tm_code):
+    """The library's host layer code (irs_layer_properties, behind forward_model and
+    the emulator features) against the oracle's restatement of layers.jl: p_mid, dp,
+    TBAR and the Curtis-Godson vmr, p and T per species, on the config-0 profile with
+    O3 and on perturbed profiles (level-by-level H2O included)."""
+    from irsounderlbl.api import layer_properties
+    profs = [syn.base_profile(True)] + [p for p, _, _ in syn.perturbed_profiles(
+        6, seed=17, sigma_T=4.0, sigma_h2o=0.6, sigma_o3=0.2, co2_range=(390e-6, 430e-6), sigma_lnq=0.2)]
+    for prof in profs:
+        got = layer_properties(prof, T_method)
+        ref = o.layer_properties(prof, tm_code)
+        for k in ("p_mid", "dp", "T_ave"):
+            np.testing.assert_allclose(got[k], ref[k], rtol=1e-13, atol=0, err_msg=k)
+        for sp in prof.vmr:
+            for k in ("vmr_cg", "p_cg", "T_cg"):
+                np.testing.assert_allclose(got[k][sp], ref[k][sp], rtol=1e-12, atol=0, err_msg=f"{k} {sp}")
